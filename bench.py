@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Benchmark: the DG1 transport operator apply on one MI355X.
+
+Workload (the headline configuration): 2D DG1 on a seeded Delaunay
+triangulation of the unit square (20,000 triangles), S8 (40 directions), one
+group, sigma_t ~ U[0.1, 10], scattering ratio ~ U[0.5, 0.99], unit source,
+vacuum boundaries. A step is one FP32 operator apply y = A psi (streaming +
+removal + within-group scatter) over the whole phase space, psi resident in
+HBM; value = degrees of freedom per second over the timed steps.
+
+Prints ONE JSON line (the driver's contract) with a roofline block for the
+apply kernel (algorithmic bytes = psi read once + y written once, 8 B per DoF
+in FP32, over the kernel's HIP-event time) and the CPU baseline (the FP64
+oracle restatement of the reference apply, OpenMP on the host cores, a bounded
+sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(prob, seconds):
+    """The oracle (FP64 C restatement of the reference shell mult: assembled
+    streaming + removal rows, then the matrix-free scatter), timed on the
+    host cores over a bounded sample of repeated applies of the same workload."""
+    root = os.path.dirname(os.path.abspath(__file__))
+    sys.path[:0] = [root, os.path.join(root, "tests")]
+    import oracle
+    from helpers import oracle_rows, xs_arrays
+
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    oracle.set_threads(cores)
+    R = oracle_rows(prob)
+    sigs = xs_arrays(prob)[1][0, 0]
+    x = np.random.default_rng(0).standard_normal(prob.disc.n_rows)
+    oracle.shell_mult(R, prob.disc.nb, prob.quad.w, prob.quad.sum_weights, sigs, x)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.shell_mult(R, prob.disc.nb, prob.quad.w, prob.quad.sum_weights, sigs, x)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": prob.disc.n_rows * n / el / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "port",
+            "sample": f"{n} FP64 oracle applies of the same DG1 workload ({prob.disc.n_rows} DoF) in {el:.1f} s, "
+                      f"{cores} OpenMP threads; restated reference semantics, not the reference binary"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+                                   "(the multi-GPU sharding is a design note, not a measured path)"}))
+        sys.exit(2)
+    import torch
+    from ubolt_amd import problems as P
+    from ubolt_amd import core
+
+    prob = P.dg1_delaunay(n_side=100, sn=8, seed=2)
+    op = prob.operator()
+    n = prob.disc.n_rows
+    x = torch.randn(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x64 = x.double()
+    y64 = torch.empty_like(x64)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        op.mult(x, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        op.mult(x, y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    per = ms / args.steps * 1e-3  # s per apply
+    gdofs = n / per / 1e9
+
+    # FP64 apply, same protocol, for the record
+    for _ in range(args.warmup):
+        op.mult(x64, y64)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        op.mult(x64, y64)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    per64 = e0.elapsed_time(e1) / args.steps * 1e-3
+
+    alg_bytes = 2 * 4 * n
+    achieved = alg_bytes / per / 1e9
+    out = {
+        "metric": "DG1 transport matvec GDoF/s + time-to-1e-8 residual, mixed vs FP64 GPU",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Delaunay mesh and cross sections)",
+        "config": {"workload": "dg1_delaunay_unit_square_20000tri_s8_g1", "n_cells": prob.disc.n_cells,
+                   "n_angles": prob.disc.n_angles, "n_basis": prob.disc.nb, "n_dof": n,
+                   "apply": "fp32 streaming+removal+scatter, fused"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "dg_apply<float,2,3,3,true>", "alg_bytes_per_launch": alg_bytes},
+        "fp64_apply_gdofs": round(n / per64 / 1e9, 3),
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

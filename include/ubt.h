@@ -1,0 +1,236 @@
+/*
+ * ubt.h - the C ABI of libubolt_amd, an MI355X-native implementation of
+ * UBOLT's matrix-free linear-Boltzmann transport operator and its
+ * preconditioner applies.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status: UBT_OK (0) or one of the ubt_status classes below,
+ * which mirror the PETSc error classes the reference raises, so a PETSc
+ * adapter (MatShell / PCShell callbacks) can return them unchanged. The
+ * message of the last failure on the calling thread is read back with
+ * ubt_last_error(). No C++ exception crosses this boundary.
+ *
+ * Vectors are caller-owned DEVICE pointers in the reference row order of one
+ * energy group, row = (cell * n_basis + basis) * n_angles + angle (angle
+ * fastest; reference include/ubolt/phase_space.hpp:20-50). That order is also
+ * the native HBM layout: a (cell, basis) node owns a contiguous run of
+ * n_angles values, which is what lets the upwind gathers of the unstructured
+ * kernels read whole contiguous runs, so no pack/unpack step exists. A
+ * vector's element type is double for UBT_FP64 and float for UBT_FP32.
+ *
+ * Every device call is stream-ordered on the caller's stream (a hipStream_t
+ * passed as void*, NULL for the default stream), allocates nothing and
+ * synchronises nothing, so it can be captured into a hipGraph. A handle is not
+ * thread-safe: calls on one handle from several host threads must be ordered
+ * by the caller.
+ *
+ * Reference: stevendargaville/UBOLT. Each declaration cites the interface it
+ * replaces (paths relative to the reference root).
+ */
+#ifndef UBT_H
+#define UBT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status classes - the numeric values of the corresponding PETSC_ERR_* codes */
+typedef enum {
+  UBT_OK = 0,
+  UBT_ERR_MEM = 55,             /* PETSC_ERR_MEM */
+  UBT_ERR_SUP = 56,             /* PETSC_ERR_SUP: unsupported order / shape / option */
+  UBT_ERR_ARG_SIZ = 60,         /* PETSC_ERR_ARG_SIZ */
+  UBT_ERR_ARG_IDN = 61,         /* PETSC_ERR_ARG_IDN: two arguments must not alias */
+  UBT_ERR_ARG_WRONG = 62,       /* PETSC_ERR_ARG_WRONG */
+  UBT_ERR_ARG_OUTOFRANGE = 63,  /* PETSC_ERR_ARG_OUTOFRANGE */
+  UBT_ERR_MAT_LU_ZRPVT = 71,    /* PETSC_ERR_MAT_LU_ZRPVT: a singular element block */
+  UBT_ERR_ARG_WRONGSTATE = 73,  /* PETSC_ERR_ARG_WRONGSTATE */
+  UBT_ERR_ARG_INCOMP = 75,      /* PETSC_ERR_ARG_INCOMP */
+  UBT_ERR_PLIB = 77,            /* PETSC_ERR_PLIB: an internal invariant failed */
+  UBT_ERR_ARG_NULL = 85,        /* PETSC_ERR_ARG_NULL */
+  UBT_ERR_NOT_CONVERGED = 91,   /* PETSC_ERR_NOT_CONVERGED */
+  UBT_ERR_GPU = 97              /* PETSC_ERR_GPU: a HIP runtime call failed */
+} ubt_status;
+
+typedef enum { UBT_FP64 = 0, UBT_FP32 = 1 } ubt_prec;
+
+/* Boundary families (reference include/ubolt/bc_spec.hpp:22). Only the
+ * ghost-flux vacuum treatment (the reference default) exists here. */
+typedef enum { UBT_BC_VACUUM = 0, UBT_BC_REFLECT = 1 } ubt_bc_type;
+
+/* One labelled face's boundary condition (reference BCFace,
+ * include/ubolt/bc_spec.hpp:68-119). inflow is the isotropic angle-integrated
+ * strength, shared over the ordinates as inflow / sum_weights; window holds
+ * n_window_pairs tangential [lo, hi] pairs in ascending axis order. Labels are
+ * PETSc's box "Face Sets" ids: 2D 1 bottom, 2 right, 3 top, 4 left; 3D 1
+ * bottom (z-min), 2 top, 3 front (y-min), 4 back, 5 right (x-max), 6 left. */
+typedef struct {
+  int label;
+  int type;
+  double inflow;
+  int n_window_pairs;
+  double window[4];
+} ubt_bc;
+
+typedef struct ubt_quad_s ubt_quad;
+typedef struct ubt_mesh_s ubt_mesh;
+typedef struct ubt_disc_s ubt_disc;
+typedef struct ubt_op_s ubt_op;
+typedef struct ubt_blockinv_s ubt_blockinv;
+
+/* ------------------------------------------------------------------ errors */
+
+/* Copies the last failure's message (NUL-terminated, truncated to len) and
+ * returns its status; UBT_OK and an empty string when nothing failed.
+ * Replaces PetscCheck/PetscError reporting (AGENTS.md:38). */
+int ubt_last_error(char *buf, size_t len);
+
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int ubt_device_count(int *count);
+
+/* -------------------------------------------------------------- quadrature */
+
+/* The SN set of a dimension and order: 1D Gauss-Legendre (any even order), 2D
+ * level-symmetric (xi > 0 half, weights doubled) and 3D level-symmetric (all
+ * octants), S2-S12. Replaces SNQuadrature{,2D,3D}::create
+ * (src/sn_quadraturek.kokkos.cxx:181,286,344). */
+int ubt_quad_create(int dim, int sn_order, ubt_quad **quad);
+int ubt_quad_size(const ubt_quad *quad, int *n_angles, double *sum_weights);
+/* omega: 3 per ordinate (zero past dim); w: one per ordinate; reflect: 3 per
+ * ordinate, reflect[3*a+d] the ordinate with cosine d negated (-1 past dim).
+ * Replaces AngularQuadrature::{omega_host, w_d, reflect_host}
+ * (include/ubolt/sn_quadrature.hpp:21-68). Any output may be NULL. */
+int ubt_quad_get(const ubt_quad *quad, double *omega, double *w, int *reflect);
+int ubt_quad_destroy(ubt_quad *quad);
+
+/* The principal octant of the level-symmetric set of an even order: its level
+ * cosines mu (n_levels of them), and per point its 1-based level triple and
+ * its weight as a fraction of the whole sphere, points in lexicographic triple
+ * order. Buffers hold at least 6 levels and 21 points. Replaces the generated
+ * table of src/sn_lqn_table.hpp (the data, regenerated from its defining
+ * moment conditions). */
+int ubt_lqn_octant(int sn_order, int *n_levels, double *mu, int *n_points, int *triples,
+                   double *weights);
+
+/* -------------------------------------------------------------------- mesh */
+
+/* An unstructured mesh of one cell type: 2D triangles (3 vertices) or quads
+ * (4, cyclic), 3D tetrahedra (4) or hexahedra (8, VTK order: 0-3 one face
+ * cyclic, 4-7 the opposite face). coords holds dim values per vertex. Faces
+ * are matched between cells. Replaces the DMPlex mesh the reference builds in
+ * UnstructuredDG::create_mesh (src/unstructured_dgk.kokkos.cxx:12). */
+int ubt_mesh_create(int dim, int n_vertices, const double *coords, int n_cells,
+                    int verts_per_cell, const int *cell_verts, ubt_mesh **mesh);
+/* Label the boundary faces lying on the bounding box's sides with PETSc's box
+ * "Face Sets" ids (see ubt_bc). Faces elsewhere keep their label. */
+int ubt_mesh_label_box(ubt_mesh *mesh);
+/* Label boundary faces explicitly (a Gmsh file's physical tags): each face is
+ * verts_per_face vertex indices, any order. */
+int ubt_mesh_set_boundary_labels(ubt_mesh *mesh, int n_faces, int verts_per_face,
+                                 const int *face_verts, const int *labels);
+int ubt_mesh_size(const ubt_mesh *mesh, int *n_cells, int *faces_per_cell, int *n_boundary_faces);
+int ubt_mesh_destroy(ubt_mesh *mesh);
+
+/* ---------------------------------------------------------- discretisation */
+
+/* Cell-centred upwind finite differences on the box [0, lengths]^dim with
+ * n_cells[d] cells per axis (dim 2 or 3), cells x fastest. bcs: one entry per
+ * labelled face, unlisted faces cold vacuum. Replaces StructuredFD2D/3D::create
+ * and the row classification of src/structured_fd_commonk.hpp:213-397. */
+int ubt_disc_create_structured(const ubt_quad *quad, int dim, const int *n_cells,
+                               const double *lengths, const ubt_bc *bcs, int n_bcs,
+                               ubt_disc **disc);
+/* Discontinuous Galerkin of order 0 or 1 on a mesh (modal orthonormal P1
+ * basis, n_basis = dim + 1 at order 1). Replaces UnstructuredDG::create and
+ * build_dg1_geometry (src/unstructured_dgk.kokkos.cxx:224-821). */
+int ubt_disc_create_dg(const ubt_quad *quad, const ubt_mesh *mesh, int order, const ubt_bc *bcs,
+                       int n_bcs, ubt_disc **disc);
+/* Sizes of the phase space (reference PhaseSpace, include/ubolt/phase_space.hpp:34-87). */
+int ubt_disc_size(const ubt_disc *disc, int *dim, int *n_cells, int *n_basis, int *n_angles,
+                  int *faces_per_cell);
+/* Copy one host geometry table out, by name, in the reference's layouts:
+ * "volume", "centroid" (3/cell), "face_nb" (int: >=0 neighbour cell, -1 vacuum,
+ * -2-axis reflective), "face_label" (int), "face_nA" (3/slot), "face_area",
+ * "face_centroid" (3/slot), "face_distance" (2/slot), "face_inflow" (per slot:
+ * inflow / sum_weights if vacuum and in window), "face_own", "face_up"
+ * (nb*nb/slot), "basis_grad" (nb*3/cell), "face_basis_value" (nb/slot),
+ * "face_nb_grad" (nb*3/slot), "h" (3, structured), "n" (int 3, structured),
+ * "bc_face" (structured: per axis side type, inflow/W, window pairs, 4 window
+ * values: 7 doubles x 6). A slot is cell * faces_per_cell + local face.
+ * *nbytes is set to the table's size; out may be NULL to query it. */
+int ubt_disc_get(const ubt_disc *disc, const char *name, void *out, size_t *nbytes);
+int ubt_disc_destroy(ubt_disc *disc);
+
+/* ---------------------------------------------------------------- operator */
+
+/* The transport operator on the current GPU: uploads the discretisation's
+ * device tables for n_groups energy groups. Replaces TransportOperator::create
+ * + add_term (src/transport_operatork.kokkos.cxx:84-120) with the streaming,
+ * removal and within-group scattering terms fused into one apply. */
+int ubt_op_create(const ubt_disc *disc, int n_groups, ubt_op **op);
+/* Host tables, copied to the device once: sigma_t[g][cell],
+ * sigma_s[g_from][g_to][cell], source[g][cell] (isotropic, angle-integrated;
+ * may be NULL). Replaces GroupXSections::set_from_materials and
+ * MaterialSourceTable (src/multigroupk.kokkos.cxx:46-169,
+ * src/material_speck.kokkos.cxx:164-213). */
+int ubt_op_set_xs(ubt_op *op, const double *sigma_t, const double *sigma_s, const double *source);
+/* Point the operator at group g: no allocation, no transfer. Replaces
+ * TransportOperator::set_group + assemble (src/transport_operatork.kokkos.cxx:165-206). */
+int ubt_op_set_group(ubt_op *op, int g);
+int ubt_op_size(const ubt_op *op, int64_t *n_rows, int64_t *n_nodes, int *n_groups);
+/* y = A x, A = streaming + removal - within-group scatter of the current
+ * group. x and y must not alias. Replaces ShellMatMultApply
+ * (src/transport_operatork.kokkos.cxx:49-61), the MATOP_MULT of the MatShell. */
+int ubt_op_mult(ubt_op *op, int prec, const void *x, void *y, void *stream);
+/* y = L x, the operator without the scatter. Replaces UnscatteredMatMultApply
+ * (src/transport_operatork.kokkos.cxx:66-78). */
+int ubt_op_mult_unscattered(ubt_op *op, int prec, const void *x, void *y, void *stream);
+/* d = diag(A) composed from the terms (streaming outflow + sigma_t). Replaces
+ * TransportOperator::diagonal (src/transport_operatork.kokkos.cxx:240-256). */
+int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
+int ubt_op_destroy(ubt_op *op);
+
+/* ------------------------------------------------------ moments and sources */
+
+/* phi[node * n_moments + m] = sum_a w_a m_a psi[node, a], m_a = 1 (m = 0) and
+ * Omega_a[d] (m = 1 + d), n_moments 1 or 1 + dim, on the matrix cores.
+ * Replaces UboltAngularIntegral (src/sn_quadraturek.kokkos.cxx:400-427). */
+int ubt_moments(ubt_op *op, int prec, const void *psi, void *phi, int n_moments, void *stream);
+/* Cache group g's scalar flux from its angular flux psi (FP64 cache). Replaces
+ * GroupTransfer::set_scalar_flux (src/multigroupk.kokkos.cxx:266-299). */
+int ubt_set_scalar_flux(ubt_op *op, int g, int prec, const void *psi, void *stream);
+/* b += sigma_s(g_from -> g_to) phi_{g_from} / sum_weights on every row.
+ * Replaces GroupTransfer::add_transfer (src/multigroupk.kokkos.cxx:306-364). */
+int ubt_scatter_source(ubt_op *op, int g_from, int g_to, int prec, void *b, void *stream);
+/* b = ghost inflow + isotropic source (basis 0) + the transfers from every
+ * group g' < g, in one launch: the driver's rhs for group g
+ * (tests/transportk.kokkos.cxx:1058-1065; UboltFillInflow
+ * src/discretisationk.kokkos.cxx:187-206; add_isotropic
+ * src/material_speck.kokkos.cxx:164-187). */
+int ubt_rhs(ubt_op *op, int g, int prec, void *b, void *stream);
+
+/* ---------------------------------------------------------- block inverse */
+
+/* The element blocks of the current group's operator, (cell, angle) n_basis x
+ * n_basis blocks (streaming own-cell couplings + removal; reflective
+ * couplings excluded). Replaces ElementBlockInverse
+ * (include/ubolt/block_inverse.hpp:35-81). */
+int ubt_blockinv_create(ubt_op *op, ubt_blockinv **bi);
+/* Invert the current group's blocks by Gauss-Jordan with partial pivoting;
+ * UBT_ERR_MAT_LU_ZRPVT if any block is singular. Replaces
+ * ElementBlockInverse::setup (src/block_inversek.kokkos.cxx:83-150,249-277). */
+int ubt_blockinv_setup(ubt_blockinv *bi, void *stream);
+/* y = D^{-1} x block by block; x and y must not alias. Replaces
+ * ElementBlockInverse::apply (src/block_inversek.kokkos.cxx:206-222,281-298),
+ * the RemovalPCApply body (src/transport_solverk.kokkos.cxx:84-94). */
+int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream);
+int ubt_blockinv_destroy(ubt_blockinv *bi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UBT_H */

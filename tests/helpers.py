@@ -1,0 +1,72 @@
+"""Test helpers: hand a product discretisation's host tables to the oracle and
+compare device results with it."""
+import numpy as np
+import torch
+
+import oracle
+
+
+def xs_arrays(prob):
+    G, nc = prob.n_groups, prob.disc.n_cells
+    st = np.broadcast_to(np.asarray(prob.sigma_t, dtype=np.float64).reshape(G, -1), (G, nc))
+    ss = np.broadcast_to(np.asarray(prob.sigma_s, dtype=np.float64).reshape(G, G, -1), (G, G, nc))
+    q = None
+    if prob.source is not None:
+        q = np.broadcast_to(np.asarray(prob.source, dtype=np.float64).reshape(G, -1), (G, nc))
+    return st, ss, q
+
+
+def oracle_rows(prob, g=0, sigt=None):
+    d, q = prob.disc, prob.quad
+    st, _, _ = xs_arrays(prob)
+    sig = st[g] if sigt is None else sigt
+    if d.kind == "fd":
+        bcf = d.get("bc_face").reshape(6, 7)
+        return oracle.fd_assemble(d.dim, d.get("n"), d.get("h"), q.omega, q.reflect, bcf[:, 0].astype(int), sig)
+    return oracle.dg_assemble(d.nb, d.nf, q.omega, q.reflect, d.get("face_nb"), d.get("face_nA"), d.get("volume"),
+                              d.get("face_own"), d.get("face_up"), d.get("basis_grad"), sig)
+
+
+def oracle_mult(prob, x, g=0, scatter=True, rows=None):
+    R = rows if rows is not None else oracle_rows(prob, g)
+    _, ss, _ = xs_arrays(prob)
+    return oracle.shell_mult(R, prob.disc.nb, prob.quad.w, prob.quad.sum_weights, ss[g, g], x, scatter)
+
+
+def oracle_ghost_inflow(prob):
+    d, q = prob.disc, prob.quad
+    b = np.zeros(d.n_rows)
+    if d.kind == "fd":
+        return oracle.fd_ghost_inflow(d.dim, d.get("n"), d.get("h"), q.omega, d.get("bc_face"), b)
+    return oracle.dg_ghost_inflow(d.nb, d.nf, q.omega, d.get("face_nb"), d.get("face_nA"), d.get("volume"),
+                                  d.get("face_basis_value"), d.get("face_inflow"), b)
+
+
+def oracle_rhs(prob, g, phi_cache):
+    """ghost inflow + source/W on basis 0 + sum_{g' < g} sigma_s(g'->g)/W phi_g'"""
+    d, q = prob.disc, prob.quad
+    st, ss, src = xs_arrays(prob)
+    b = oracle_ghost_inflow(prob)
+    W = q.sum_weights
+    if src is not None:
+        ind = np.zeros(d.n_nodes)
+        ind[::d.nb] = 1.0
+        oracle.broadcast_add(d.nb, d.n_angles, src[g] / W, ind, b)
+    for gf in range(g):
+        oracle.broadcast_add(d.nb, d.n_angles, ss[gf, g] / W, phi_cache[gf], b)
+    return b
+
+
+def rel_max(a, b):
+    """max_r |a - b| / max_r |b| - the reference's -check_matfree measure"""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def to_dev(x, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device="cuda")
+
+
+def to_host(t):
+    return t.detach().double().cpu().numpy()

@@ -1,0 +1,133 @@
+"""ctypes binding of libubolt_amd's C ABI (include/ubt.h).
+
+The shared library is built in place by __graft_entry__.build() (or `make -C
+ubolt_amd/csrc`). Loading it imports torch first, so the library binds to the
+HIP runtime torch already loaded; a missing library raises instead of falling
+back to anything.
+"""
+import ctypes as C
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime the library links against)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libubolt_amd.so")
+
+FP64, FP32 = 0, 1
+BC_VACUUM, BC_REFLECT = 0, 1
+
+STATUS = {0: "OK", 55: "ERR_MEM", 56: "ERR_SUP", 60: "ERR_ARG_SIZ", 61: "ERR_ARG_IDN", 62: "ERR_ARG_WRONG",
+          63: "ERR_ARG_OUTOFRANGE", 71: "ERR_MAT_LU_ZRPVT", 73: "ERR_ARG_WRONGSTATE", 75: "ERR_ARG_INCOMP",
+          77: "ERR_PLIB", 85: "ERR_ARG_NULL", 91: "ERR_NOT_CONVERGED", 97: "ERR_GPU"}
+
+
+class UbtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+        self.status = STATUS.get(code, str(code))
+
+
+class Bc(C.Structure):
+    _fields_ = [("label", C.c_int), ("type", C.c_int), ("inflow", C.c_double), ("n_window_pairs", C.c_int),
+                ("window", C.c_double * 4)]
+
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+PI = C.POINTER(C.c_int)
+PD = C.POINTER(C.c_double)
+PP = C.POINTER(C.c_void_p)
+I64 = C.c_int64
+PI64 = C.POINTER(C.c_int64)
+
+# name -> argtypes (every function returns int status)
+SIGNATURES = {
+    "ubt_last_error": [C.c_char_p, C.c_size_t],
+    "ubt_device_count": [PI],
+    "ubt_quad_create": [I, I, PP],
+    "ubt_quad_size": [P, PI, PD],
+    "ubt_quad_get": [P, PD, PD, PI],
+    "ubt_quad_destroy": [P],
+    "ubt_lqn_octant": [I, PI, PD, PI, PI, PD],
+    "ubt_mesh_create": [I, I, PD, I, I, PI, PP],
+    "ubt_mesh_label_box": [P],
+    "ubt_mesh_set_boundary_labels": [P, I, I, PI, PI],
+    "ubt_mesh_size": [P, PI, PI, PI],
+    "ubt_mesh_destroy": [P],
+    "ubt_disc_create_structured": [P, I, PI, PD, C.POINTER(Bc), I, PP],
+    "ubt_disc_create_dg": [P, P, I, C.POINTER(Bc), I, PP],
+    "ubt_disc_size": [P, PI, PI, PI, PI, PI],
+    "ubt_disc_get": [P, C.c_char_p, P, C.POINTER(C.c_size_t)],
+    "ubt_disc_destroy": [P],
+    "ubt_op_create": [P, I, PP],
+    "ubt_op_set_xs": [P, PD, PD, PD],
+    "ubt_op_set_group": [P, I],
+    "ubt_op_size": [P, PI64, PI64, PI],
+    "ubt_op_mult": [P, I, P, P, P],
+    "ubt_op_mult_unscattered": [P, I, P, P, P],
+    "ubt_op_diagonal": [P, I, P, P],
+    "ubt_op_destroy": [P],
+    "ubt_moments": [P, I, P, P, I, P],
+    "ubt_set_scalar_flux": [P, I, I, P, P],
+    "ubt_scatter_source": [P, I, I, I, P, P],
+    "ubt_rhs": [P, I, I, P, P],
+    "ubt_blockinv_create": [P, PP],
+    "ubt_blockinv_setup": [P, P],
+    "ubt_blockinv_apply": [P, I, P, P, P],
+    "ubt_blockinv_destroy": [P],
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libubolt_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    code = lib().ubt_last_error(buf, 1024)
+    return code, buf.value.decode()
+
+
+def call(name, *args):
+    """Call an entry point and raise UbtError on a non-zero status."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        _, msg = last_error()
+        raise UbtError(rc, msg)
+    return rc
+
+
+def dptr(t):
+    """Device (or host) address of a torch tensor / numpy array as c_void_p."""
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    return t.ctypes.data_as(C.c_void_p)
+
+
+def as_pd(a):
+    return a.ctypes.data_as(PD)
+
+
+def as_pi(a):
+    return a.ctypes.data_as(PI)
+
+
+def stream_ptr(stream=None):
+    """The raw hipStream_t of a torch stream (the current one by default)."""
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)

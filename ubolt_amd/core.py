@@ -1,0 +1,221 @@
+"""Python handles over the C ABI: quadrature, mesh, discretisation, operator,
+block inverse. Thin owners of the C handles; every compute call goes through
+libubolt_amd and fails loudly (UbtError) on a non-zero status."""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import call, dptr, stream_ptr, as_pd, as_pi
+
+FP64, FP32 = capi.FP64, capi.FP32
+
+
+def torch_dtype(prec):
+    return torch.float64 if prec == FP64 else torch.float32
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h and self._destroy:
+                getattr(capi.lib(), self._destroy)(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Quadrature(_Handle):
+    """SN quadrature (reference AngularQuadrature / SNQuadrature{,2D,3D})."""
+    _destroy = "ubt_quad_destroy"
+
+    def __init__(self, dim, sn_order):
+        super().__init__()
+        call("ubt_quad_create", dim, sn_order, C.byref(self.h))
+        n = C.c_int()
+        sw = C.c_double()
+        call("ubt_quad_size", self.h, C.byref(n), C.byref(sw))
+        self.dim, self.sn_order, self.n_angles, self.sum_weights = dim, sn_order, n.value, sw.value
+        self.omega = np.zeros((self.n_angles, 3))
+        self.w = np.zeros(self.n_angles)
+        self.reflect = np.zeros((self.n_angles, 3), dtype=np.int32)
+        call("ubt_quad_get", self.h, as_pd(self.omega), as_pd(self.w), as_pi(self.reflect))
+
+
+def lqn_octant(sn_order):
+    nl, npt = C.c_int(), C.c_int()
+    mu = np.zeros(9)
+    tr = np.zeros(3 * 45, dtype=np.int32)
+    wt = np.zeros(45)
+    call("ubt_lqn_octant", sn_order, C.byref(nl), as_pd(mu), C.byref(npt), as_pi(tr), as_pd(wt))
+    return mu[:nl.value].copy(), tr[:3 * npt.value].reshape(-1, 3).copy(), wt[:npt.value].copy()
+
+
+class Mesh(_Handle):
+    """An unstructured mesh of one cell type (triangles, quads, tets, hexes)."""
+    _destroy = "ubt_mesh_destroy"
+
+    def __init__(self, coords, cells, label_box=True):
+        super().__init__()
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.dim = coords.shape[1]
+        self.coords, self.cells = coords, cells
+        call("ubt_mesh_create", self.dim, coords.shape[0], as_pd(coords), cells.shape[0], cells.shape[1],
+             as_pi(cells), C.byref(self.h))
+        if label_box:
+            call("ubt_mesh_label_box", self.h)
+
+    def set_boundary_labels(self, face_verts, labels):
+        fv = np.ascontiguousarray(face_verts, dtype=np.int32)
+        lb = np.ascontiguousarray(labels, dtype=np.int32)
+        call("ubt_mesh_set_boundary_labels", self.h, fv.shape[0], fv.shape[1], as_pi(fv), as_pi(lb))
+
+    def sizes(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("ubt_mesh_size", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+
+def bc_array(bcs):
+    """bcs: {label: "vacuum" | "reflect" | dict(type=..., inflow=..., window=[...])}"""
+    items = list((bcs or {}).items())
+    arr = (capi.Bc * max(1, len(items)))()
+    for i, (label, spec) in enumerate(items):
+        if isinstance(spec, str):
+            spec = {"type": spec}
+        arr[i].label = int(label)
+        arr[i].type = capi.BC_REFLECT if spec["type"] == "reflect" else capi.BC_VACUUM
+        arr[i].inflow = float(spec.get("inflow", 0.0))
+        win = list(spec.get("window", []))
+        arr[i].n_window_pairs = len(win) // 2
+        for k, v in enumerate(win):
+            arr[i].window[k] = v
+    return arr, len(items)
+
+
+class Disc(_Handle):
+    """A discretisation: structured FD box or unstructured DG0/DG1."""
+    _destroy = "ubt_disc_destroy"
+
+    @classmethod
+    def structured(cls, quad, n_cells, lengths, bcs=None):
+        self = cls()
+        n = np.ascontiguousarray(n_cells, dtype=np.int32)
+        L = np.ascontiguousarray(lengths, dtype=np.float64)
+        arr, nb = bc_array(bcs)
+        call("ubt_disc_create_structured", quad.h, len(n), as_pi(n), as_pd(L), arr, nb, C.byref(self.h))
+        self.kind, self.quad = "fd", quad
+        self._sizes()
+        return self
+
+    @classmethod
+    def dg(cls, quad, mesh, order, bcs=None):
+        self = cls()
+        arr, nb = bc_array(bcs)
+        call("ubt_disc_create_dg", quad.h, mesh.h, order, arr, nb, C.byref(self.h))
+        self.kind, self.quad, self.mesh, self.order = "dg", quad, mesh, order
+        self._sizes()
+        return self
+
+    def _sizes(self):
+        v = [C.c_int() for _ in range(5)]
+        call("ubt_disc_size", self.h, *[C.byref(x) for x in v])
+        self.dim, self.n_cells, self.nb, self.n_angles, self.nf = [x.value for x in v]
+        self.n_nodes = self.n_cells * self.nb
+        self.n_rows = self.n_nodes * self.n_angles
+
+    INT_TABLES = {"face_nb", "face_label", "n"}
+
+    def get(self, name):
+        nbytes = C.c_size_t(0)
+        call("ubt_disc_get", self.h, name.encode(), None, C.byref(nbytes))
+        dt = np.int32 if name in self.INT_TABLES else np.float64
+        out = np.zeros(nbytes.value // np.dtype(dt).itemsize, dtype=dt)
+        call("ubt_disc_get", self.h, name.encode(), dptr(out), C.byref(nbytes))
+        return out
+
+
+class Operator(_Handle):
+    """The device transport operator (reference TransportOperator + terms)."""
+    _destroy = "ubt_op_destroy"
+
+    def __init__(self, disc, n_groups=1):
+        super().__init__()
+        self.disc, self.n_groups = disc, n_groups
+        call("ubt_op_create", disc.h, n_groups, C.byref(self.h))
+        self.n_rows, self.n_nodes = disc.n_rows, disc.n_nodes
+
+    def set_xs(self, sigma_t, sigma_s, source=None):
+        """sigma_t [G, cells], sigma_s [G_from, G_to, cells], source [G, cells]"""
+        G, nc = self.n_groups, self.disc.n_cells
+        st = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma_t, dtype=np.float64).reshape(G, -1), (G, nc)))
+        ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma_s, dtype=np.float64).reshape(G, G, -1), (G, G, nc)))
+        q = None
+        if source is not None:
+            q = np.ascontiguousarray(np.broadcast_to(np.asarray(source, dtype=np.float64).reshape(G, -1), (G, nc)))
+        self.sigma_t, self.sigma_s, self.source = st, ss, q
+        call("ubt_op_set_xs", self.h, as_pd(st), as_pd(ss), as_pd(q) if q is not None else None)
+
+    def set_group(self, g):
+        call("ubt_op_set_group", self.h, g)
+        self.group = g
+
+    def vec(self, prec=FP64, fill=None):
+        t = torch.zeros(self.n_rows, dtype=torch_dtype(prec), device="cuda")
+        if fill is not None:
+            t.copy_(torch.as_tensor(fill))
+        return t
+
+    @staticmethod
+    def _prec(t):
+        return FP64 if t.dtype == torch.float64 else FP32
+
+    def mult(self, x, y, stream=None):
+        call("ubt_op_mult", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def mult_unscattered(self, x, y, stream=None):
+        call("ubt_op_mult_unscattered", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def diagonal(self, d, stream=None):
+        call("ubt_op_diagonal", self.h, self._prec(d), dptr(d), stream_ptr(stream))
+        return d
+
+    def moments(self, psi, phi, n_moments=1, stream=None):
+        call("ubt_moments", self.h, self._prec(psi), dptr(psi), dptr(phi), n_moments, stream_ptr(stream))
+        return phi
+
+    def set_scalar_flux(self, g, psi, stream=None):
+        call("ubt_set_scalar_flux", self.h, g, self._prec(psi), dptr(psi), stream_ptr(stream))
+
+    def scatter_source(self, g_from, g_to, b, stream=None):
+        call("ubt_scatter_source", self.h, g_from, g_to, self._prec(b), dptr(b), stream_ptr(stream))
+
+    def rhs(self, g, b, stream=None):
+        call("ubt_rhs", self.h, g, self._prec(b), dptr(b), stream_ptr(stream))
+        return b
+
+
+class BlockInverse(_Handle):
+    """Element block inverse (reference ElementBlockInverse)."""
+    _destroy = "ubt_blockinv_destroy"
+
+    def __init__(self, op):
+        super().__init__()
+        self.op = op
+        call("ubt_blockinv_create", op.h, C.byref(self.h))
+
+    def setup(self, stream=None):
+        call("ubt_blockinv_setup", self.h, stream_ptr(stream))
+
+    def apply(self, x, y, stream=None):
+        call("ubt_blockinv_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y

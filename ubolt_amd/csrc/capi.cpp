@@ -1,0 +1,377 @@
+// The extern "C" boundary: every entry point catches what the C++ below throws
+// and turns it into a ubt_status plus a message for ubt_last_error().
+#include <cstring>
+
+#include "operator.hpp"
+#include "ubt_internal.hpp"
+
+namespace ubt {
+int last_error_copy(char *buf, size_t len);
+}
+
+struct ubt_quad_s {
+  ubt::Quadrature q;
+};
+struct ubt_mesh_s {
+  ubt::Mesh m;
+};
+struct ubt_disc_s {
+  std::shared_ptr<ubt::Disc> d;
+};
+struct ubt_op_s {
+  std::unique_ptr<ubt::Operator> op;
+};
+struct ubt_blockinv_s {
+  ubt::BlockInverse bi;
+};
+
+namespace {
+
+template <class F> int guard(F &&f) {
+  try {
+    f();
+    ubt::set_last_error(UBT_OK, "");
+    return UBT_OK;
+  } catch (const ubt::Error &e) {
+    ubt::set_last_error(e.code, e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    ubt::set_last_error(UBT_ERR_MEM, "host allocation failed");
+    return UBT_ERR_MEM;
+  } catch (const std::exception &e) {
+    ubt::set_last_error(UBT_ERR_PLIB, e.what());
+    return UBT_ERR_PLIB;
+  }
+}
+
+#define NEED(p) UBT_CHECK((p) != nullptr, UBT_ERR_ARG_NULL, "%s is NULL", #p)
+
+ubt::BCTable bc_table(const ubt_bc *bcs, int n) {
+  ubt::BCTable t;
+  UBT_CHECK(n >= 0 && (n == 0 || bcs), UBT_ERR_ARG_NULL, "boundary conditions: %d entries but no array", n);
+  for (int i = 0; i < n; i++) {
+    ubt::BCFace f;
+    f.type = bcs[i].type;
+    f.inflow = bcs[i].inflow;
+    f.n_window_pairs = bcs[i].n_window_pairs;
+    for (int k = 0; k < 4; k++) f.window[k] = bcs[i].window[k];
+    UBT_CHECK(f.n_window_pairs >= 0 && f.n_window_pairs <= 2, UBT_ERR_ARG_WRONG, "a face takes at most 2 window pairs");
+    for (auto &e : t.faces)
+      UBT_CHECK(e.first != bcs[i].label, UBT_ERR_ARG_WRONG, "label %d is given two boundary conditions", bcs[i].label);
+    t.faces.push_back({bcs[i].label, f});
+  }
+  return t;
+}
+
+template <class V> void copy_out(const std::vector<V> &v, void *out, size_t *nbytes) {
+  const size_t n = v.size() * sizeof(V);
+  if (out) {
+    UBT_CHECK(*nbytes >= n, UBT_ERR_ARG_SIZ, "buffer of %zu bytes, the table needs %zu", *nbytes, n);
+    if (n) std::memcpy(out, v.data(), n);
+  }
+  *nbytes = n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ubt_last_error(char *buf, size_t len) { return ubt::last_error_copy(buf, len); }
+
+int ubt_device_count(int *count) {
+  return guard([&] {
+    NEED(count);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+// quadrature ---------------------------------------------------------------
+int ubt_quad_create(int dim, int sn_order, ubt_quad **quad) {
+  return guard([&] {
+    NEED(quad);
+    auto q = std::make_unique<ubt_quad_s>();
+    q->q = ubt::make_quadrature(dim, sn_order);
+    *quad = q.release();
+  });
+}
+int ubt_quad_size(const ubt_quad *quad, int *n_angles, double *sum_weights) {
+  return guard([&] {
+    NEED(quad);
+    if (n_angles) *n_angles = quad->q.n_angles;
+    if (sum_weights) *sum_weights = quad->q.sum_weights;
+  });
+}
+int ubt_quad_get(const ubt_quad *quad, double *omega, double *w, int *reflect) {
+  return guard([&] {
+    NEED(quad);
+    const auto &q = quad->q;
+    if (omega) std::memcpy(omega, q.omega.data(), q.omega.size() * sizeof(double));
+    if (w) std::memcpy(w, q.w.data(), q.w.size() * sizeof(double));
+    if (reflect)
+      for (int a = 0; a < q.n_angles; a++)
+        for (int d = 0; d < 3; d++) reflect[3 * a + d] = q.reflect[d][a];
+  });
+}
+int ubt_quad_destroy(ubt_quad *quad) {
+  delete quad;
+  return UBT_OK;
+}
+int ubt_lqn_octant(int sn_order, int *n_levels, double *mu, int *n_points, int *triples, double *weights) {
+  return guard([&] {
+    auto o = ubt::lqn_octant(sn_order);
+    if (n_levels) *n_levels = (int)o.mu.size();
+    if (n_points) *n_points = (int)o.weight.size();
+    if (mu) std::memcpy(mu, o.mu.data(), o.mu.size() * sizeof(double));
+    if (triples) std::memcpy(triples, o.triple.data(), o.triple.size() * sizeof(int));
+    if (weights) std::memcpy(weights, o.weight.data(), o.weight.size() * sizeof(double));
+  });
+}
+
+// mesh ---------------------------------------------------------------------
+int ubt_mesh_create(int dim, int n_vertices, const double *coords, int n_cells, int verts_per_cell,
+                    const int *cell_verts, ubt_mesh **mesh) {
+  return guard([&] {
+    NEED(coords);
+    NEED(cell_verts);
+    NEED(mesh);
+    auto m = std::make_unique<ubt_mesh_s>();
+    m->m = ubt::make_mesh(dim, n_vertices, coords, n_cells, verts_per_cell, cell_verts);
+    *mesh = m.release();
+  });
+}
+int ubt_mesh_label_box(ubt_mesh *mesh) {
+  return guard([&] {
+    NEED(mesh);
+    ubt::mesh_label_box(mesh->m);
+  });
+}
+int ubt_mesh_set_boundary_labels(ubt_mesh *mesh, int n_faces, int verts_per_face, const int *face_verts,
+                                 const int *labels) {
+  return guard([&] {
+    NEED(mesh);
+    UBT_CHECK(n_faces >= 0, UBT_ERR_ARG_OUTOFRANGE, "negative face count");
+    if (n_faces == 0) return;
+    NEED(face_verts);
+    NEED(labels);
+    ubt::mesh_set_boundary_labels(mesh->m, n_faces, verts_per_face, face_verts, labels);
+  });
+}
+int ubt_mesh_size(const ubt_mesh *mesh, int *n_cells, int *faces_per_cell, int *n_boundary_faces) {
+  return guard([&] {
+    NEED(mesh);
+    if (n_cells) *n_cells = mesh->m.n_cells;
+    if (faces_per_cell) *faces_per_cell = mesh->m.faces_per_cell;
+    if (n_boundary_faces) {
+      int nb = 0;
+      for (int v : mesh->m.face_neighbour) nb += v < 0;
+      *n_boundary_faces = nb;
+    }
+  });
+}
+int ubt_mesh_destroy(ubt_mesh *mesh) {
+  delete mesh;
+  return UBT_OK;
+}
+
+// discretisation -------------------------------------------------------------
+int ubt_disc_create_structured(const ubt_quad *quad, int dim, const int *n_cells, const double *lengths,
+                               const ubt_bc *bcs, int n_bcs, ubt_disc **disc) {
+  return guard([&] {
+    NEED(quad);
+    NEED(n_cells);
+    NEED(lengths);
+    NEED(disc);
+    int n[3] = {1, 1, 1};
+    double len[3] = {1, 1, 1};
+    UBT_CHECK(dim == 2 || dim == 3, UBT_ERR_SUP, "the structured backend is 2D or 3D, got %d", dim);
+    for (int d = 0; d < dim; d++) {
+      n[d] = n_cells[d];
+      len[d] = lengths[d];
+    }
+    auto h = std::make_unique<ubt_disc_s>();
+    h->d = ubt::make_structured(quad->q, dim, n, len, bc_table(bcs, n_bcs));
+    *disc = h.release();
+  });
+}
+int ubt_disc_create_dg(const ubt_quad *quad, const ubt_mesh *mesh, int order, const ubt_bc *bcs, int n_bcs,
+                       ubt_disc **disc) {
+  return guard([&] {
+    NEED(quad);
+    NEED(mesh);
+    NEED(disc);
+    auto h = std::make_unique<ubt_disc_s>();
+    h->d = ubt::make_unstructured(quad->q, mesh->m, order, bc_table(bcs, n_bcs));
+    *disc = h.release();
+  });
+}
+int ubt_disc_size(const ubt_disc *disc, int *dim, int *n_cells, int *n_basis, int *n_angles, int *faces_per_cell) {
+  return guard([&] {
+    NEED(disc);
+    const auto &D = *disc->d;
+    if (dim) *dim = D.dim;
+    if (n_cells) *n_cells = D.n_cells;
+    if (n_basis) *n_basis = D.nb;
+    if (n_angles) *n_angles = D.n_angles;
+    if (faces_per_cell) *faces_per_cell = D.kind == ubt::DISC_FD ? 2 * D.dim : D.nf;
+  });
+}
+int ubt_disc_get(const ubt_disc *disc, const char *name, void *out, size_t *nbytes) {
+  return guard([&] {
+    NEED(disc);
+    NEED(name);
+    NEED(nbytes);
+    const auto &D = *disc->d;
+    const std::string n(name);
+    if (n == "volume") copy_out(D.volume, out, nbytes);
+    else if (n == "centroid") copy_out(D.centroid, out, nbytes);
+    else if (n == "face_nb") copy_out(D.face_nb, out, nbytes);
+    else if (n == "face_label") copy_out(D.face_label, out, nbytes);
+    else if (n == "face_nA") copy_out(D.face_nA, out, nbytes);
+    else if (n == "face_area") copy_out(D.face_area, out, nbytes);
+    else if (n == "face_centroid") copy_out(D.face_centroid, out, nbytes);
+    else if (n == "face_distance") copy_out(D.face_distance, out, nbytes);
+    else if (n == "face_inflow") copy_out(D.face_inflow, out, nbytes);
+    else if (n == "face_own") copy_out(D.face_own, out, nbytes);
+    else if (n == "face_up") copy_out(D.face_up, out, nbytes);
+    else if (n == "basis_grad") copy_out(D.basis_grad, out, nbytes);
+    else if (n == "face_basis_value") copy_out(D.face_basis_value, out, nbytes);
+    else if (n == "face_nb_grad") copy_out(D.face_nb_grad, out, nbytes);
+    else if (n == "h") copy_out(std::vector<double>(D.h, D.h + 3), out, nbytes);
+    else if (n == "n") copy_out(std::vector<int>(D.n, D.n + 3), out, nbytes);
+    else if (n == "bc_face") {
+      std::vector<double> v;
+      for (int d = 0; d < 3; d++)
+        for (int s = 0; s < 2; s++) {
+          const auto &f = D.face_bc[d][s];
+          v.push_back(D.face_type[d][s]);
+          v.push_back(f.inflow / D.quad.sum_weights);
+          v.push_back(f.n_window_pairs);
+          for (int k = 0; k < 4; k++) v.push_back(f.window[k]);
+        }
+      copy_out(v, out, nbytes);
+    } else {
+      ubt::fail(UBT_ERR_ARG_WRONG, "no geometry table called \"%s\"", name);
+    }
+  });
+}
+int ubt_disc_destroy(ubt_disc *disc) {
+  delete disc;
+  return UBT_OK;
+}
+
+// operator -------------------------------------------------------------------
+int ubt_op_create(const ubt_disc *disc, int n_groups, ubt_op **op) {
+  return guard([&] {
+    NEED(disc);
+    NEED(op);
+    auto h = std::make_unique<ubt_op_s>();
+    h->op = ubt::make_operator(disc->d, n_groups);
+    *op = h.release();
+  });
+}
+int ubt_op_set_xs(ubt_op *op, const double *sigma_t, const double *sigma_s, const double *source) {
+  return guard([&] {
+    NEED(op);
+    ubt::operator_set_xs(*op->op, sigma_t, sigma_s, source);
+  });
+}
+int ubt_op_set_group(ubt_op *op, int g) {
+  return guard([&] {
+    NEED(op);
+    UBT_CHECK(g >= 0 && g < op->op->n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g,
+              op->op->n_groups);
+    op->op->group = g;
+  });
+}
+int ubt_op_size(const ubt_op *op, int64_t *n_rows, int64_t *n_nodes, int *n_groups) {
+  return guard([&] {
+    NEED(op);
+    if (n_rows) *n_rows = op->op->disc->n_rows();
+    if (n_nodes) *n_nodes = op->op->disc->n_nodes();
+    if (n_groups) *n_groups = op->op->n_groups;
+  });
+}
+int ubt_op_mult(ubt_op *op, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(op);
+    op->op->mult(prec, x, y, true, (hipStream_t)stream);
+  });
+}
+int ubt_op_mult_unscattered(ubt_op *op, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(op);
+    op->op->mult(prec, x, y, false, (hipStream_t)stream);
+  });
+}
+int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream) {
+  return guard([&] {
+    NEED(op);
+    NEED(d);
+    op->op->diagonal(prec, d, (hipStream_t)stream);
+  });
+}
+int ubt_op_destroy(ubt_op *op) {
+  delete op;
+  return UBT_OK;
+}
+
+// moments and sources ----------------------------------------------------------
+int ubt_moments(ubt_op *op, int prec, const void *psi, void *phi, int n_moments, void *stream) {
+  return guard([&] {
+    NEED(op);
+    op->op->moments(prec, psi, phi, n_moments, (hipStream_t)stream);
+  });
+}
+int ubt_set_scalar_flux(ubt_op *op, int g, int prec, const void *psi, void *stream) {
+  return guard([&] {
+    NEED(op);
+    NEED(psi);
+    op->op->set_scalar_flux(g, prec, psi, (hipStream_t)stream);
+  });
+}
+int ubt_scatter_source(ubt_op *op, int g_from, int g_to, int prec, void *b, void *stream) {
+  return guard([&] {
+    NEED(op);
+    NEED(b);
+    op->op->scatter_source(g_from, g_to, prec, b, (hipStream_t)stream);
+  });
+}
+int ubt_rhs(ubt_op *op, int g, int prec, void *b, void *stream) {
+  return guard([&] {
+    NEED(op);
+    NEED(b);
+    op->op->rhs(g, prec, b, (hipStream_t)stream);
+  });
+}
+
+// block inverse ------------------------------------------------------------------
+int ubt_blockinv_create(ubt_op *op, ubt_blockinv **bi) {
+  return guard([&] {
+    NEED(op);
+    NEED(bi);
+    UBT_CHECK(op->op->disc->nb <= 4, UBT_ERR_SUP, "element blocks of up to 4 basis functions");
+    auto h = std::make_unique<ubt_blockinv_s>();
+    h->bi.op = op->op.get();
+    *bi = h.release();
+  });
+}
+int ubt_blockinv_setup(ubt_blockinv *bi, void *stream) {
+  return guard([&] {
+    NEED(bi);
+    bi->bi.setup((hipStream_t)stream);
+  });
+}
+int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(bi);
+    bi->bi.apply(prec, x, y, (hipStream_t)stream);
+  });
+}
+int ubt_blockinv_destroy(ubt_blockinv *bi) {
+  delete bi;
+  return UBT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// The device-resident transport operator: the tables its kernels read, per
+// precision, and the host methods the C ABI calls.
+#pragma once
+
+#include "ubt_internal.hpp"
+
+namespace ubt {
+
+// Kernel arguments, by value. One struct serves every discretisation; the
+// kernels read only their own fields.
+template <class T> struct OpArgs {
+  int dim, nb, nf, A, n_cells, cpb, red_lanes;
+  T inv_sum_w;
+  const T *omega;      // 3A
+  const T *w;          // A
+  const int *reflect;  // 3A (-1 past dim)
+  const T *sigt;       // current group, per cell
+  const T *sigs;       // current group's within-group scatter, per cell
+  // structured
+  int n[3];
+  T invh[3];
+  int face_type[3][2];
+  // unstructured
+  const int *face_nb;  // slots
+  const T *face_s;     // 3 per slot: nA (DG1) or nA / V (DG0)
+  const T *face_own;   // slots * nb * nb
+  const T *face_up;
+  const T *grad;       // cells * nb * 3
+};
+
+// Per precision copies of the geometry and cross sections
+struct PrecTables {
+  DevBuf omega, w, face_s, face_own, face_up, grad;
+  DevBuf wext;        // [Apad][16]: w_a {1, Omega_x, Omega_y, Omega_z}, the moments' B operand
+  DevBuf sigt, sigs;  // G*cells, G*G*cells
+};
+
+struct Operator {
+  std::shared_ptr<Disc> disc;
+  int n_groups = 1, group = 0, device = 0;
+  bool xs_set = false;
+  PrecTables t64, t32;
+  DevBuf reflect, face_nb;
+  DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow)
+  DevBuf inv_volume;          // double, cells (rhs inflow)
+  DevBuf face_inflow;         // double, slots
+  DevBuf face_basis_value;    // double, slots * nb
+  DevBuf source;              // double, G*cells
+  DevBuf sigs_d;              // double, G*G*cells (transfer sources)
+  DevBuf phi_cache;           // double, G*nodes
+  std::vector<char> phi_set;
+  DevBuf flag;                // int scratch for setup checks
+
+  template <class T> OpArgs<T> args() const;
+  void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
+  void diagonal(int prec, void *d, hipStream_t s) const;
+  void moments(int prec, const void *psi, void *phi, int n_moments, hipStream_t s) const;
+  void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s);
+  void scatter_source(int g_from, int g_to, int prec, void *b, hipStream_t s) const;
+  void rhs(int g, int prec, void *b, hipStream_t s) const;
+};
+
+std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups);
+void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s, const double *source);
+
+struct BlockInverse {
+  Operator *op = nullptr;
+  int group_ready = -1;
+  void setup(hipStream_t s);
+  void apply(int prec, const void *x, void *y, hipStream_t s) const;
+};
+
+}  // namespace ubt
