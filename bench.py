@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="also run a plain 64 MiB device copy 5 times: the known-size reference the HBM "
+                         "traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) are scaled by")
     return ap.parse_args()
 
 
@@ -133,6 +136,12 @@ def main():
         "fp64_apply_gdofs": round(n / per64 / 1e9, 3),
         "wall_s": round(wall, 3),
     }
+    if args.copy_ref:
+        src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
     print(json.dumps(out))

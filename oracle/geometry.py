@@ -44,7 +44,10 @@ def simplex_moments(P):
 
 
 def polygon_moments_2d(X):
-    """Area, centroid and central second moment of a simple polygon (Green)."""
+    """Area, centroid and central second moment of a simple polygon (Green),
+    about the vertex average so the central moment does not cancel."""
+    o = X.mean(axis=0)
+    X = X - o
     x, y = X[:, 0], X[:, 1]
     xn, yn = np.roll(x, -1), np.roll(y, -1)
     cr = x * yn - xn * y
@@ -60,19 +63,21 @@ def polygon_moments_2d(X):
     C[0, 0] = Ixx - A * cx * cx
     C[1, 1] = Iyy - A * cy * cy
     C[0, 1] = C[1, 0] = Ixy - A * cx * cy
-    return A, np.array([cx, cy, 0.0]), C
+    return A, np.array([cx + o[0], cy + o[1], 0.0]), C
 
 
 def cell_moments(X, dim):
     if dim == 2:
         return polygon_moments_2d(X)
     tets = HEX_TETS if X.shape[0] == 8 else [(0, 1, 2, 3)]
+    o = X.mean(axis=0)
+    X = X - o
     V, m1, m2 = 0.0, np.zeros(3), np.zeros((3, 3))
     for t in tets:
         v, a, b = simplex_moments(X[list(t)])
         V, m1, m2 = V + v, m1 + a, m2 + b
     c = m1 / V
-    return V, c, m2 - V * np.outer(c, c)
+    return V, c + o, m2 - V * np.outer(c, c)
 
 
 def face_moments(X, dim):
@@ -83,13 +88,15 @@ def face_moments(X, dim):
         n = np.array([e[1], -e[0], 0.0]) / L
         return L, 0.5 * (X[0] + X[1]), np.outer(e, e) * L / 12.0, n
     tris = [(0, 1, 2)] if X.shape[0] == 3 else [(0, 1, 2), (0, 2, 3)]
+    o = X.mean(axis=0)
+    Y = X - o
     A, m1, m2 = 0.0, np.zeros(3), np.zeros((3, 3))
     for t in tris:
-        a, b, c = simplex_moments(X[list(t)])
+        a, b, c = simplex_moments(Y[list(t)])
         A, m1, m2 = A + a, m1 + b, m2 + c
     xf = m1 / A
     n = np.cross(X[1] - X[0], X[2] - X[0])
-    return A, xf, m2 - A * np.outer(xf, xf), n / np.linalg.norm(n)
+    return A, xf + o, m2 - A * np.outer(xf, xf), n / np.linalg.norm(n)
 
 
 def local_faces(dim, nv):

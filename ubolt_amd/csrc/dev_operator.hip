@@ -40,6 +40,19 @@ __device__ inline void seg_sum(const T *red, int nseg, int len, T *out, int S) {
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup -> XCD-contiguous tile. The dispatcher deals consecutive
+// workgroup ids round-robin over the 8 XCDs; remapping id b to tile
+// (b % 8) * per + b / 8 hands each XCD one contiguous run of cells, so the
+// upwind neighbours a tile gathers were mostly just read through the same
+// XCD's L2. The grid is launched with 8 * per workgroups; ids past the last
+// tile return at once.
+// ---------------------------------------------------------------------------
+__device__ inline int64_t xcd_tile(int64_t n_tiles) {
+  const int64_t per = (n_tiles + 7) / 8;
+  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+// ---------------------------------------------------------------------------
 // Structured FD apply (reference src/termsk.kokkos.cxx:70-98 streaming,
 // :480-509 removal, :584-649 scatter; upwind/BC classification
 // src/structured_fd_commonk.hpp:213-397 under the ghost-flux treatment)
@@ -50,8 +63,10 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
   T *red = reinterpret_cast<T *>(smem);
   T *phi = red + g.cpb * g.A;
   const int A = g.A;
+  const int64_t tile = xcd_tile(((int64_t)g.n_cells + g.cpb - 1) / g.cpb);
+  if (tile * g.cpb >= g.n_cells) return;
   const int lc = threadIdx.x / A, a = threadIdx.x - lc * A;
-  const int64_t c = (int64_t)blockIdx.x * g.cpb + lc;
+  const int64_t c = tile * g.cpb + lc;
   const bool live = lc < g.cpb && c < g.n_cells;
   T psi = T(0), acc = T(0);
   if (live) {
@@ -91,17 +106,58 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
 // ---------------------------------------------------------------------------
 // Unstructured DG0 / DG1 apply (reference src/termsk.kokkos.cxx:197-235 DG0,
 // :328-385 DG1, removal and scatter as above). NB basis functions and NF faces
-// per cell are compile-time; a thread owns (cell, angle) and all NB rows.
+// per cell are compile-time; a thread owns (cell, angle) and all NB rows. The
+// tile's geometry (contiguous per-cell records in HBM) is staged into LDS with
+// coalesced loads once, then every angle of a cell reads it from LDS.
 // ---------------------------------------------------------------------------
 template <class T, int DIM, int NB, int NF, bool SCATTER>
 __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem[];
-  T *red = reinterpret_cast<T *>(smem);
-  T *phi = red + g.cpb * NB * g.A;
-  const int A = g.A;
-  const int lc = threadIdx.x / A, a = threadIdx.x - lc * A;
-  const int64_t c = (int64_t)blockIdx.x * g.cpb + lc;
-  const bool live = lc < g.cpb && c < g.n_cells;
+  constexpr int FM = NB * NB;
+  const int cpb = g.cpb, A = g.A;
+  const int64_t n_tiles = ((int64_t)g.n_cells + cpb - 1) / cpb;
+  const int64_t tile = xcd_tile(n_tiles);
+  if (tile >= n_tiles) return;
+  const int64_t c0 = tile * cpb;
+  const int ncell = (int)((int64_t)g.n_cells - c0 < cpb ? (int64_t)g.n_cells - c0 : cpb);
+  // LDS: F_own, F_up, face s (DIM per slot), grad (NB-1 x DIM per cell),
+  // sigma_t, sigma_s, then the reduction buffers; face_nb after the reals
+  T *s_own = reinterpret_cast<T *>(smem);
+  T *s_up = s_own + (NB > 1 ? cpb * NF * FM : 0);
+  T *s_fs = s_up + (NB > 1 ? cpb * NF * FM : 0);
+  T *s_gr = s_fs + cpb * NF * DIM;
+  T *s_st = s_gr + (NB > 1 ? cpb * (NB - 1) * DIM : 0);
+  T *s_ss = s_st + cpb;
+  T *red = s_ss + cpb;
+  T *phi = red + (SCATTER ? cpb * NB * A : 0);
+  int *s_nb = reinterpret_cast<int *>(phi + (SCATTER ? cpb * NB : 0));
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (NB > 1) {
+    const T *go = g.face_own + c0 * NF * FM, *gu = g.face_up + c0 * NF * FM;
+    for (int e = tid; e < ncell * NF * FM; e += nt) {
+      s_own[e] = go[e];
+      s_up[e] = gu[e];
+    }
+    for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
+      const int lc = e / ((NB - 1) * DIM), r = e - lc * ((NB - 1) * DIM);
+      const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
+      s_gr[e] = g.grad[((c0 + lc) * NB + i) * 3 + d];
+    }
+  }
+  for (int e = tid; e < ncell * NF; e += nt) {
+    s_nb[e] = g.face_nb[c0 * NF + e];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[(c0 * NF + e) * 3 + d];
+  }
+  for (int e = tid; e < ncell; e += nt) {
+    s_st[e] = g.sigt[c0 + e];
+    s_ss[e] = g.sigs[c0 + e];
+  }
+  __syncthreads();
+
+  const int lc = tid / A, a = tid - lc * A;
+  const int64_t c = c0 + lc;
+  const bool live = lc < ncell;
   T psi[NB], acc[NB];
 #pragma unroll
   for (int j = 0; j < NB; j++) psi[j] = acc[j] = T(0);
@@ -109,31 +165,34 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
     const int64_t base = c * NB * A + a;
 #pragma unroll
     for (int j = 0; j < NB; j++) psi[j] = x[base + (int64_t)j * A];
-    const T sig = g.sigt[c];
+    const T sig = s_st[lc];
 #pragma unroll
     for (int i = 0; i < NB; i++) acc[i] = sig * psi[i];
-    const T ox = g.omega[3 * a], oy = g.omega[3 * a + 1], oz = DIM == 3 ? g.omega[3 * a + 2] : T(0);
+    T om[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) om[d] = g.omega[3 * a + d];
     if (NB > 1) {
       // volume term -(Omega . grad phi_i) psi_0; grad phi_0 = 0
 #pragma unroll
       for (int i = 1; i < NB; i++) {
-        const T *gr = g.grad + (c * NB + i) * 3;
-        T od = ox * gr[0] + oy * gr[1];
-        if (DIM == 3) od += oz * gr[2];
+        const T *gr = s_gr + (lc * (NB - 1) + i - 1) * DIM;
+        T od = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
         acc[i] -= od * psi[0];
       }
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      const int64_t k = c * NF + f;
-      const T *sv = g.face_s + 3 * k;
-      T s = ox * sv[0] + oy * sv[1];
-      if (DIM == 3) s += oz * sv[2];
+      const int kl = lc * NF + f;
+      T s = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) s += om[d] * s_fs[kl * DIM + d];
       if (s > T(0)) {
         if (NB == 1) {
           acc[0] += s * psi[0];
         } else {
-          const T *F = g.face_own + k * NB * NB;
+          const T *F = s_own + kl * FM;
 #pragma unroll
           for (int i = 0; i < NB; i++) {
             T v = T(0);
@@ -143,7 +202,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
           }
         }
       } else {
-        const int nbk = g.face_nb[k];
+        const int nbk = s_nb[kl];
         if (nbk == -1) continue;  // vacuum: the inflow lives in the rhs
         int64_t ub;
         int ua;
@@ -160,7 +219,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
         if (NB == 1) {
           acc[0] += s * pu[0];
         } else {
-          const T *F = g.face_up + k * NB * NB;
+          const T *F = s_up + kl * FM;
 #pragma unroll
           for (int i = 0; i < NB; i++) {
             T v = T(0);
@@ -173,16 +232,16 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
     }
   }
   if (SCATTER) {
-    if (lc < g.cpb) {
+    if (lc < cpb) {
       const T wa = live ? g.w[a] : T(0);
 #pragma unroll
       for (int j = 0; j < NB; j++) red[(lc * NB + j) * A + a] = wa * psi[j];
     }
     __syncthreads();
-    seg_sum(red, g.cpb * NB, A, phi, g.red_lanes);
+    seg_sum(red, cpb * NB, A, phi, g.red_lanes);
     __syncthreads();
     if (live) {
-      const T ss = g.sigs[c] * g.inv_sum_w;
+      const T ss = s_ss[lc] * g.inv_sum_w;
 #pragma unroll
       for (int i = 0; i < NB; i++) acc[i] -= ss * phi[lc * NB + i];
     }
@@ -517,7 +576,17 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
   Launch L = apply_launch<T>(D.n_angles, D.nb);
   a.cpb = L.cpb;
   a.red_lanes = L.lanes;
-  const unsigned grid = (unsigned)((D.n_cells + L.cpb - 1) / L.cpb);
+  const int64_t n_tiles = (D.n_cells + L.cpb - 1) / L.cpb;
+  // XCD-contiguous tiles: 8 * ceil(tiles / 8) workgroups
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+  if (D.kind == DISC_DG) {
+    // the staged geometry in LDS
+    const int nb = D.nb, nf = D.nf, dim = D.dim;
+    const size_t reals = (size_t)L.cpb * ((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) +
+                         (size_t)L.cpb * nb * D.n_angles + (size_t)L.cpb * nb;
+    L.lds = reals * sizeof(T) + sizeof(int) * (size_t)L.cpb * nf;
+    UBT_CHECK(L.lds <= 65536, UBT_ERR_SUP, "the DG tile needs %zu bytes of LDS", L.lds);
+  }
   if (D.kind == DISC_FD) {
 #define FD_LAUNCH(DIM)                                                                            \
   if (scatter)                                                                                    \
