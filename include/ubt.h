@@ -80,6 +80,24 @@ typedef struct ubt_mesh_s ubt_mesh;
 typedef struct ubt_disc_s ubt_disc;
 typedef struct ubt_op_s ubt_op;
 typedef struct ubt_blockinv_s ubt_blockinv;
+typedef struct ubt_dsa_s ubt_dsa;
+typedef struct ubt_sweep_s ubt_sweep;
+typedef struct ubt_solver_s ubt_solver;
+
+/* Solver options (ubt_solver_create). The preconditioner is the reference's
+ * multiplicative composite: stage 0 the element block inverse (pc_blocks),
+ * stage 1 the streaming stage (pc_sweep: the transport sweep), stage 2 DSA
+ * (pc_dsa, when a DSA handle is given); later stages correct the residual with
+ * the full operator when DSA is on and use_amat is set, else with L. */
+typedef struct {
+  double rtol;        /* relative true residual ||b - A x|| / ||b||, e.g. 1e-8 */
+  int max_it;         /* Krylov iterations in total */
+  int restart;        /* GMRES restart length, 1..31 */
+  int mixed;          /* 0: FP64 FGMRES; 1: FP32 inner FGMRES inside FP64 refinement */
+  double inner_rtol;  /* inner tolerance in mixed mode, e.g. 1e-4 */
+  int max_outer;      /* refinement steps in mixed mode */
+  int pc_blocks, pc_sweep, pc_dsa, use_amat;
+} ubt_solver_options;
 
 /* ------------------------------------------------------------------ errors */
 
@@ -228,6 +246,70 @@ int ubt_blockinv_setup(ubt_blockinv *bi, void *stream);
  * the RemovalPCApply body (src/transport_solverk.kokkos.cxx:84-94). */
 int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream);
 int ubt_blockinv_destroy(ubt_blockinv *bi);
+
+/* --------------------------------------------------------------------- DSA */
+
+/* The DSA preconditioner stage on the operator's discretisation: structured
+ * star, DG0 two-point flux or DG1 MIP diffusion operator, assembled on the
+ * device per group, with an FP64 block-Jacobi CG inner solve. Replaces
+ * DSAPrecon::create (src/dsak.kokkos.cxx:83-149, include/ubolt/dsa.hpp:63-149).
+ * The handle is the caller's, like the reference's DSAPrecon. */
+int ubt_dsa_create(ubt_op *op, ubt_dsa **dsa);
+/* -dsa_consistent_d[_power], -dsa_mip_penalty and the inner solve's
+ * iteration cap and relative tolerance (the reference's inner KSP, one GAMG
+ * application, src/dsak.kokkos.cxx:197-212). */
+int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power, double mip_penalty,
+                        int inner_max_it, double inner_rtol);
+/* Assemble group g's diffusion operator from the operator's cross sections:
+ * D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g, g). Replaces
+ * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345). Void cells
+ * (sigma_t <= 0) are refused with UBT_ERR_SUP. */
+int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream);
+/* y = P D^{-1} R x: the 0th moment (times V on the plex), the inner solve, and
+ * the isotropic correction / sum_weights on every row. Replaces
+ * DSAPrecon::apply (src/dsak.kokkos.cxx:350-391), the DSAPCApply body. */
+int ubt_dsa_apply(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream);
+/* y = D_diff x on node vectors, FP64 (the diffusion matvec). */
+int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream);
+/* x = D_diff^{-1} b by the inner solve (x starts at zero). */
+int ubt_dsa_solve(ubt_dsa *dsa, const double *b, double *x, void *stream);
+/* Inner iterations the last solve took. */
+int ubt_dsa_info(ubt_dsa *dsa, int *inner_iterations, int *n_nodes);
+int ubt_dsa_destroy(ubt_dsa *dsa);
+
+/* ------------------------------------------------------------------- sweep */
+
+/* The transport sweep psi = L^{-1} r (streaming + removal, reflective and
+ * cycle-closing couplings lagged): the GPU streaming stage standing in for
+ * the reference's PCAIR stage (src/transport_solverk.kokkos.cxx:238), which
+ * approximates the same inverse. Level schedule built once. */
+int ubt_sweep_create(ubt_op *op, ubt_sweep **sw);
+int ubt_sweep_info(const ubt_sweep *sw, int *n_levels, int64_t *n_lagged);
+int ubt_sweep_apply(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream);
+int ubt_sweep_destroy(ubt_sweep *sw);
+
+/* ------------------------------------------------------------------ solver */
+
+/* Right-preconditioned flexible GMRES on the operator with the composite
+ * preconditioner; dsa may be NULL. Replaces TransportSolver::create
+ * (src/transport_solverk.kokkos.cxx:196-318). options NULL = defaults. */
+int ubt_solver_create(ubt_op *op, ubt_dsa *dsa, const ubt_solver_options *options, ubt_solver **solver);
+/* Re-check the element blocks after the group changed. Replaces
+ * TransportSolver::refresh (src/transport_solverk.kokkos.cxx:324-331). */
+int ubt_solver_refresh(ubt_solver *solver, void *stream);
+/* x = A^{-1} b (FP64 device vectors, zero initial guess). Replaces
+ * TransportSolver::solve (src/transport_solverk.kokkos.cxx:335-343).
+ * Returns UBT_OK also when not converged: read ubt_solver_result. */
+int ubt_solver_solve(ubt_solver *solver, const double *b, double *x, void *stream);
+/* y = M^{-1} r, the composite preconditioner alone (prec of r and y). */
+int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, void *stream);
+/* iterations: Krylov iterations; outer: refinement steps (1 in FP64 mode);
+ * rel_residual: the final FP64 true relative residual; reason: 1 converged,
+ * -1 iteration limit (the sign convention of KSPConvergedReason). history
+ * (may be NULL) receives up to max_history per-iteration relative residuals. */
+int ubt_solver_result(const ubt_solver *solver, int *iterations, int *outer, double *rel_residual, int *reason,
+                      double *history, int max_history, int *n_history);
+int ubt_solver_destroy(ubt_solver *solver);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,134 @@
+"""DSA diffusion operators restated with scipy - TEST INFRASTRUCTURE ONLY.
+
+The reference's three DSA operators, assembled as sparse matrices cell by cell
+from the same formulas (no voids), and the DSA apply y = P D^{-1} R x with an
+exact inner solve:
+
+* structured star, per unit volume (src/dsa_structuredk.kokkos.cxx:183-285):
+  harmonic face D, consistent-D blend with the axis half-range current,
+  Marshak vacuum faces, zero Neumann reflective faces;
+* DG0 two-point flux, volume-weighted (src/dsa_plexk.kokkos.cxx:288-348);
+* DG1 MIP interior penalty, volume-weighted (src/dsa_plexk.kokkos.cxx:365-475);
+helpers: src/dsa_operatork.hpp:110-147 (half-range current, blend).
+D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+
+def half_range(omega, w, W, n):
+    c = omega @ n
+    return float((w * c)[c > 0].sum() / W)
+
+
+def blend(d, m, p):
+    return d + m if p == 1.0 else (d ** p + m ** p) ** (1.0 / p)
+
+
+def structured(dim, n, h, face_type, omega, w, W, sigt, sigs, cons=True, p=1.5):
+    """face_type[d][side]: 0 vacuum, 1 reflect"""
+    n = list(n[:dim])
+    cells = int(np.prod(n))
+    hr = [half_range(omega, w, W, np.eye(3)[d]) for d in range(dim)]
+    rows, cols, vals = [], [], []
+    strides = [1, n[0], n[0] * (n[1] if dim > 1 else 1)]
+    for c in range(cells):
+        idx, cc = [], c
+        for d in range(dim):
+            idx.append(cc % n[d])
+            cc //= n[d]
+        dc = 1.0 / (3.0 * sigt[c])
+        diag = sigt[c] - sigs[c]
+        for d in range(dim):
+            hd = h[d]
+            for side, step in ((0, -1), (1, 1)):
+                m = idx[d] + step
+                if 0 <= m < n[d]:
+                    nb = c + step * strides[d]
+                    dn = 1.0 / (3.0 * sigt[nb])
+                    df = 2.0 * dc * dn / (dc + dn)
+                    if cons:
+                        df = blend(df, hr[d] * hd, p)
+                    rows.append(c), cols.append(nb), vals.append(-df / hd ** 2)
+                    diag += df / hd ** 2
+                elif face_type[d][side] == 0:
+                    db = blend(dc, hr[d] * hd, p) if cons else dc
+                    diag += 1.0 / (hd * (2.0 + hd / (2.0 * db)))
+        rows.append(c), cols.append(c), vals.append(diag)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(cells, cells))
+
+
+def dg0(nf, face_nb, face_nA, area, dist, volume, omega, w, W, sigt, sigs, cons=True, p=1.5):
+    cells = volume.shape[0]
+    nA = face_nA.reshape(-1, 3)
+    dist = dist.reshape(-1, 2)
+    rows, cols, vals = [], [], []
+    for c in range(cells):
+        dc = 1.0 / (3.0 * sigt[c])
+        diag = volume[c] * (sigt[c] - sigs[c])
+        for f in range(nf):
+            k = c * nf + f
+            A = area[k]
+            m = half_range(omega, w, W, nA[k] / A)
+            d_own = dist[k, 0]
+            if face_nb[k] >= 0:
+                nb = face_nb[k]
+                dn = 1.0 / (3.0 * sigt[nb])
+                tf = A / (d_own / dc + dist[k, 1] / dn)
+                if cons:
+                    L = d_own + dist[k, 1]
+                    tf = A / L * blend(tf * L / A, m * L, p)
+                rows.append(c), cols.append(nb), vals.append(-tf)
+                diag += tf
+            elif face_nb[k] == -1:
+                db = blend(dc, 2.0 * d_own * m, p) if cons else dc
+                diag += A / (2.0 + d_own / db)
+        rows.append(c), cols.append(c), vals.append(diag)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(cells, cells))
+
+
+def dg1(nb, nf, face_nb, face_nA, area, dist, volume, face_own, face_up, basis_grad, sigt, sigs, C=4.0):
+    cells = volume.shape[0]
+    nA = face_nA.reshape(-1, 3)
+    dist = dist.reshape(-1, 2)
+    Fo = face_own.reshape(-1, nb, nb)
+    Fu = face_up.reshape(-1, nb, nb)
+    G = basis_grad.reshape(cells, nb, 3)
+    M = sp.lil_matrix((cells * nb, cells * nb))
+    for c in range(cells):
+        dc = 1.0 / (3.0 * sigt[c])
+        sa = sigt[c] - sigs[c]
+        V = volume[c]
+        own = V * (dc * (G[c] @ G[c].T) + sa * np.eye(nb))
+        for f in range(nf):
+            k = c * nf + f
+            n = face_nb[k]
+            if n < -1:
+                continue
+            A = area[k]
+            s = V * A
+            nrm = nA[k] / A
+            gc = G[c] @ nrm
+            hc = 2.0 * dist[k, 0]
+            m = s * Fo[k][:, 0]  # int_f phi_i
+            if n >= 0:
+                dn = 1.0 / (3.0 * sigt[n])
+                hn = 2.0 * dist[k, 1]
+                gn = G[n] @ nrm
+                kap = max(0.5 * C * (dc / hc + dn / hn), 0.25)
+                own += kap * s * Fo[k] - 0.5 * dc * np.outer(m, gc) - 0.5 * dc * np.outer(gc, s * Fo[k][0, :])
+                blk = -kap * s * Fu[k] - 0.5 * dn * np.outer(m, gn) + 0.5 * dc * np.outer(gc, s * Fu[k][0, :])
+                M[c * nb:(c + 1) * nb, n * nb:(n + 1) * nb] = M[c * nb:(c + 1) * nb, n * nb:(n + 1) * nb] + blk
+            else:
+                kap = max(C * dc / hc, 0.25)
+                own += kap * s * Fo[k] - 0.5 * dc * np.outer(m, gc) - 0.5 * dc * np.outer(gc, s * Fo[k][0, :])
+        M[c * nb:(c + 1) * nb, c * nb:(c + 1) * nb] = own
+    return M.tocsr()
+
+
+def apply(Dmat, weight, A, w, W, x):
+    """y = P D^{-1} R x, exact inner solve"""
+    phi = x.reshape(-1, A) @ w
+    sol = spl.spsolve(Dmat.tocsc(), phi * weight)
+    return np.repeat(sol / W, A)

@@ -70,3 +70,43 @@ def to_dev(x, dtype=torch.float64):
 
 def to_host(t):
     return t.detach().double().cpu().numpy()
+
+
+def oracle_dsa_matrix(prob, g=0, cons=True, p=1.5, C=4.0):
+    """the reference DSA diffusion matrix of group g (oracle/dsa.py) and the
+    restriction weight per node"""
+    from oracle import dsa as odsa
+    d, q = prob.disc, prob.quad
+    st, ss, _ = xs_arrays(prob)
+    if d.kind == "fd":
+        ft = d.get("bc_face").reshape(3, 2, 7)[:, :, 0].astype(int)
+        M = odsa.structured(d.dim, d.get("n"), d.get("h"), ft, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p)
+        return M, np.ones(d.n_nodes)
+    args = (d.nf, d.get("face_nb"), d.get("face_nA"), d.get("face_area"), d.get("face_distance"), d.get("volume"))
+    if d.nb == 1:
+        M = odsa.dg0(*args, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p)
+    else:
+        M = odsa.dg1(d.nb, *args, d.get("face_own"), d.get("face_up"), d.get("basis_grad"), st[g], ss[g, g], C)
+    return M, np.repeat(d.get("volume"), d.nb)
+
+
+def oracle_dsa_apply(prob, x, g=0):
+    from oracle import dsa as odsa
+    M, wgt = oracle_dsa_matrix(prob, g)
+    return odsa.apply(M, wgt, prob.quad.n_angles, prob.quad.w, prob.quad.sum_weights, x)
+
+
+def oracle_full(prob, g=0):
+    """the whole group operator A = streaming + removal - scatter as a sparse matrix"""
+    import scipy.sparse as sp
+    d, q = prob.disc, prob.quad
+    _, ss, _ = xs_arrays(prob)
+    R = oracle_rows(prob, g).to_scipy()
+    A = d.n_angles
+    S = sp.kron(sp.diags(np.repeat(ss[g, g], d.nb) / q.sum_weights), sp.csr_matrix(np.ones((A, 1)) @ q.w[None, :]))
+    return (R - S).tocsc()
+
+
+def oracle_solve(prob, b, g=0):
+    import scipy.sparse.linalg as spl
+    return spl.spsolve(oracle_full(prob, g), b)

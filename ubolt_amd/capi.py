@@ -27,6 +27,12 @@ class UbtError(RuntimeError):
         self.status = STATUS.get(code, str(code))
 
 
+class SolverOptions(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("max_it", C.c_int), ("restart", C.c_int), ("mixed", C.c_int),
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("pc_blocks", C.c_int), ("pc_sweep", C.c_int),
+                ("pc_dsa", C.c_int), ("use_amat", C.c_int)]
+
+
 class Bc(C.Structure):
     _fields_ = [("label", C.c_int), ("type", C.c_int), ("inflow", C.c_double), ("n_window_pairs", C.c_int),
                 ("window", C.c_double * 4)]
@@ -76,6 +82,24 @@ SIGNATURES = {
     "ubt_blockinv_setup": [P, P],
     "ubt_blockinv_apply": [P, I, P, P, P],
     "ubt_blockinv_destroy": [P],
+    "ubt_dsa_create": [P, PP],
+    "ubt_dsa_set_options": [P, I, D, D, I, D],
+    "ubt_dsa_set_group": [P, I, P],
+    "ubt_dsa_apply": [P, I, P, P, P],
+    "ubt_dsa_matvec": [P, P, P, P],
+    "ubt_dsa_solve": [P, P, P, P],
+    "ubt_dsa_info": [P, PI, PI],
+    "ubt_dsa_destroy": [P],
+    "ubt_sweep_create": [P, PP],
+    "ubt_sweep_info": [P, PI, PI64],
+    "ubt_sweep_apply": [P, I, P, P, P],
+    "ubt_sweep_destroy": [P],
+    "ubt_solver_create": [P, P, C.POINTER(SolverOptions), PP],
+    "ubt_solver_refresh": [P, P],
+    "ubt_solver_solve": [P, P, P, P],
+    "ubt_solver_apply_pc": [P, I, P, P, P],
+    "ubt_solver_result": [P, PI, PI, PD, PI, PD, I, PI],
+    "ubt_solver_destroy": [P],
 }
 
 _lib = None

@@ -219,3 +219,89 @@ class BlockInverse(_Handle):
     def apply(self, x, y, stream=None):
         call("ubt_blockinv_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
         return y
+
+
+class DSA(_Handle):
+    """DSA preconditioner stage (reference DSAPrecon): caller-owned."""
+    _destroy = "ubt_dsa_destroy"
+
+    def __init__(self, op, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200,
+                 inner_rtol=1e-6):
+        super().__init__()
+        self.op = op
+        call("ubt_dsa_create", op.h, C.byref(self.h))
+        self.set_options(consistent_d, consistent_power, mip_penalty, inner_max_it, inner_rtol)
+
+    def set_options(self, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200, inner_rtol=1e-6):
+        call("ubt_dsa_set_options", self.h, int(consistent_d), consistent_power, mip_penalty, inner_max_it, inner_rtol)
+
+    def set_group(self, g, stream=None):
+        call("ubt_dsa_set_group", self.h, g, stream_ptr(stream))
+
+    def apply(self, x, y, stream=None):
+        call("ubt_dsa_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def matvec(self, x, y, stream=None):
+        call("ubt_dsa_matvec", self.h, dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def solve(self, b, x, stream=None):
+        call("ubt_dsa_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
+        return x
+
+    def info(self):
+        it, nn = C.c_int(), C.c_int()
+        call("ubt_dsa_info", self.h, C.byref(it), C.byref(nn))
+        return it.value, nn.value
+
+
+class Sweep(_Handle):
+    """Transport sweep psi = L^{-1} r (the streaming stage)."""
+    _destroy = "ubt_sweep_destroy"
+
+    def __init__(self, op):
+        super().__init__()
+        self.op = op
+        call("ubt_sweep_create", op.h, C.byref(self.h))
+        nl, nlag = C.c_int(), C.c_int64()
+        call("ubt_sweep_info", self.h, C.byref(nl), C.byref(nlag))
+        self.n_levels, self.n_lagged = nl.value, nlag.value
+
+    def apply(self, r, psi, stream=None):
+        call("ubt_sweep_apply", self.h, Operator._prec(r), dptr(r), dptr(psi), stream_ptr(stream))
+        return psi
+
+
+class Solver(_Handle):
+    """Right-preconditioned FGMRES with the composite preconditioner
+    (reference TransportSolver)."""
+    _destroy = "ubt_solver_destroy"
+
+    def __init__(self, op, dsa=None, rtol=1e-8, max_it=500, restart=30, mixed=False, inner_rtol=1e-4, max_outer=20,
+                 pc_blocks=True, pc_sweep=True, pc_dsa=True, use_amat=True):
+        super().__init__()
+        self.op, self.dsa = op, dsa
+        o = capi.SolverOptions(rtol, max_it, restart, int(mixed), inner_rtol, max_outer, int(pc_blocks), int(pc_sweep),
+                               int(pc_dsa), int(use_amat))
+        call("ubt_solver_create", op.h, dsa.h if dsa is not None else None, C.byref(o), C.byref(self.h))
+
+    def refresh(self, stream=None):
+        call("ubt_solver_refresh", self.h, stream_ptr(stream))
+
+    def solve(self, b, x, stream=None):
+        call("ubt_solver_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
+        return self.result()
+
+    def apply_pc(self, r, y, stream=None):
+        call("ubt_solver_apply_pc", self.h, Operator._prec(r), dptr(r), dptr(y), stream_ptr(stream))
+        return y
+
+    def result(self):
+        it, outer, reason, nh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        rr = C.c_double()
+        hist = np.zeros(4096)
+        call("ubt_solver_result", self.h, C.byref(it), C.byref(outer), C.byref(rr), C.byref(reason), as_pd(hist), 4096,
+             C.byref(nh))
+        return {"iterations": it.value, "outer": outer.value, "rel_residual": rr.value, "reason": reason.value,
+                "history": hist[:min(nh.value, 4096)].copy()}

@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "operator.hpp"
+#include "solver.hpp"
 #include "ubt_internal.hpp"
 
 namespace ubt {
@@ -23,6 +24,15 @@ struct ubt_op_s {
 };
 struct ubt_blockinv_s {
   ubt::BlockInverse bi;
+};
+struct ubt_dsa_s {
+  ubt::DSA d;
+};
+struct ubt_sweep_s {
+  ubt::Sweep s;
+};
+struct ubt_solver_s {
+  ubt::Solver s;
 };
 
 namespace {
@@ -371,6 +381,175 @@ int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void 
 }
 int ubt_blockinv_destroy(ubt_blockinv *bi) {
   delete bi;
+  return UBT_OK;
+}
+
+// DSA -----------------------------------------------------------------------------
+int ubt_dsa_create(ubt_op *op, ubt_dsa **dsa) {
+  return guard([&] {
+    NEED(op);
+    NEED(dsa);
+    auto h = std::make_unique<ubt_dsa_s>();
+    h->d.create(op->op.get());
+    *dsa = h.release();
+  });
+}
+int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power, double mip_penalty, int inner_max_it,
+                        double inner_rtol) {
+  return guard([&] {
+    NEED(dsa);
+    UBT_CHECK(consistent_power >= 1.0, UBT_ERR_ARG_OUTOFRANGE, "the consistent-D power must be at least 1, got %g",
+              consistent_power);
+    UBT_CHECK(mip_penalty > 0.0, UBT_ERR_ARG_OUTOFRANGE, "the MIP penalty must be positive, got %g", mip_penalty);
+    UBT_CHECK(inner_max_it >= 1 && inner_rtol >= 0.0, UBT_ERR_ARG_OUTOFRANGE, "inner solve needs max_it >= 1, rtol >= 0");
+    dsa->d.opt.consistent_d = consistent_d;
+    dsa->d.opt.consistent_power = consistent_power;
+    dsa->d.opt.mip_penalty = mip_penalty;
+    dsa->d.opt.max_it = inner_max_it;
+    dsa->d.opt.rtol = inner_rtol;
+  });
+}
+int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    dsa->d.set_group(g, (hipStream_t)stream);
+  });
+}
+int ubt_dsa_apply(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    dsa->d.apply(prec, x, y, (hipStream_t)stream);
+  });
+}
+int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    NEED(x);
+    NEED(y);
+    dsa->d.matvec(x, y, (hipStream_t)stream);
+  });
+}
+int ubt_dsa_solve(ubt_dsa *dsa, const double *b, double *x, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    NEED(b);
+    NEED(x);
+    dsa->d.solve(b, x, (hipStream_t)stream);
+  });
+}
+int ubt_dsa_info(ubt_dsa *dsa, int *inner_iterations, int *n_nodes) {
+  return guard([&] {
+    NEED(dsa);
+    if (inner_iterations) {
+      double it = 0.0;
+      UBT_HIP(hipMemcpy(&it, dsa->d.scal.as<double>() + 5, sizeof(double), hipMemcpyDeviceToHost));
+      *inner_iterations = (int)it;
+    }
+    if (n_nodes) *n_nodes = (int)dsa->d.n_nodes;
+  });
+}
+int ubt_dsa_destroy(ubt_dsa *dsa) {
+  delete dsa;
+  return UBT_OK;
+}
+
+// sweep ---------------------------------------------------------------------------
+int ubt_sweep_create(ubt_op *op, ubt_sweep **sw) {
+  return guard([&] {
+    NEED(op);
+    NEED(sw);
+    auto h = std::make_unique<ubt_sweep_s>();
+    h->s.create(op->op.get());
+    *sw = h.release();
+  });
+}
+int ubt_sweep_info(const ubt_sweep *sw, int *n_levels, int64_t *n_lagged) {
+  return guard([&] {
+    NEED(sw);
+    if (n_levels) *n_levels = sw->s.n_levels;
+    if (n_lagged) *n_lagged = sw->s.n_lagged;
+  });
+}
+int ubt_sweep_apply(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream) {
+  return guard([&] {
+    NEED(sw);
+    sw->s.apply(prec, r, psi, (hipStream_t)stream);
+  });
+}
+int ubt_sweep_destroy(ubt_sweep *sw) {
+  delete sw;
+  return UBT_OK;
+}
+
+// solver --------------------------------------------------------------------------
+int ubt_solver_create(ubt_op *op, ubt_dsa *dsa, const ubt_solver_options *options, ubt_solver **solver) {
+  return guard([&] {
+    NEED(op);
+    NEED(solver);
+    ubt::SolverOptions o;
+    if (options) {
+      o.rtol = options->rtol;
+      o.max_it = options->max_it;
+      o.restart = options->restart;
+      o.mixed = options->mixed;
+      o.inner_rtol = options->inner_rtol;
+      o.max_outer = options->max_outer;
+      o.pc_blocks = options->pc_blocks;
+      o.pc_sweep = options->pc_sweep;
+      o.pc_dsa = options->pc_dsa;
+      o.use_amat = options->use_amat;
+    }
+    UBT_CHECK(!o.mixed || (o.inner_rtol > 0.0 && o.inner_rtol < 1.0 && o.max_outer >= 1), UBT_ERR_ARG_OUTOFRANGE,
+              "mixed mode needs 0 < inner_rtol < 1 and max_outer >= 1");
+    auto h = std::make_unique<ubt_solver_s>();
+    h->s.create(op->op.get(), dsa ? &dsa->d : nullptr, o);
+    *solver = h.release();
+  });
+}
+int ubt_solver_refresh(ubt_solver *solver, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.refresh((hipStream_t)stream);
+  });
+}
+int ubt_solver_solve(ubt_solver *solver, const double *b, double *x, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.solve(b, x, (hipStream_t)stream);
+  });
+}
+int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    NEED(r);
+    NEED(y);
+    UBT_CHECK(r != y, UBT_ERR_ARG_IDN, "the preconditioner apply needs distinct r and y");
+    auto &s = solver->s;
+    UBT_CHECK(prec == UBT_FP64 ? s.V64.ptr != nullptr : s.V32.ptr != nullptr, UBT_ERR_ARG_INCOMP,
+              "this solver holds %s work vectors", s.opt.mixed ? "FP32" : "FP64");
+    if (prec == UBT_FP64)
+      s.apply_pc<double>((const double *)r, (double *)y, (hipStream_t)stream);
+    else
+      s.apply_pc<float>((const float *)r, (float *)y, (hipStream_t)stream);
+  });
+}
+int ubt_solver_result(const ubt_solver *solver, int *iterations, int *outer, double *rel_residual, int *reason,
+                      double *history, int max_history, int *n_history) {
+  return guard([&] {
+    NEED(solver);
+    const auto &s = solver->s;
+    if (iterations) *iterations = s.iterations;
+    if (outer) *outer = s.outer;
+    if (rel_residual) *rel_residual = s.rel_residual;
+    if (reason) *reason = s.reason;
+    const int nh = (int)std::min<size_t>(s.history.size(), (size_t)std::max(0, max_history));
+    if (history)
+      for (int i = 0; i < nh; i++) history[i] = s.history[i];
+    if (n_history) *n_history = (int)s.history.size();
+  });
+}
+int ubt_solver_destroy(ubt_solver *solver) {
+  delete solver;
   return UBT_OK;
 }
 

@@ -1,0 +1,611 @@
+// DSA on gfx950: per-group assembly of the diffusion operator by a kernel (no
+// host copy of the cross sections), its block-sparse FP64 matvec, and the
+// block-Jacobi preconditioned CG inner solve whose scalars never leave the
+// device. Restriction = the matrix-core moments kernel times the node weight;
+// prolongation = the isotropic broadcast / sum_weights.
+//
+// Operators (reference src/dsa_structuredk.kokkos.cxx:183-285,
+// src/dsa_plexk.kokkos.cxx:288-475, helpers src/dsa_operatork.hpp:110-147):
+//  * structured: cell-centred star per unit volume, harmonic face D, Marshak
+//    vacuum faces 1/(h (2 + h/(2 D_b))), zero Neumann reflective faces;
+//  * DG0 plex: two-point flux A/(d_c/D_c + d_n/D_n), Marshak A/(2 + d_c/D_b),
+//    volume-weighted;
+//  * DG1 plex: the MIP interior-penalty form on the modal basis,
+//    volume-weighted;
+// with D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g) and the consistent-D
+// blend (D^p + (m h)^p)^(1/p), p = 1.5, on FD and DG0 faces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dsa.hpp"
+
+namespace ubt {
+
+namespace {
+
+struct AsmArgs {
+  int kind, dim, nb, nf, ns, n_cells, cons;
+  double p, C;
+  const double *sigt, *sigs;  // current group
+  const int *nbr;             // cells * ns
+  // structured
+  double h[3], hr[3];
+  int vac[3][2];
+  // unstructured
+  const double *volume, *area, *dist, *nA, *slot_hr, *face_own, *face_up, *grad;
+  const int *face_nb;
+  double *own, *off;
+};
+
+__device__ inline double blend(double d, double m, double p) {
+  return p == 1.0 ? d + m : pow(pow(d, p) + pow(m, p), 1.0 / p);
+}
+
+__global__ void asm_structured(AsmArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_cells) return;
+  const double st = a.sigt[c];
+  const double dc = 1.0 / (3.0 * st);
+  double diag = st - a.sigs[c];
+  for (int d = 0; d < a.dim; d++) {
+    const double h = a.h[d];
+    for (int side = 0; side < 2; side++) {
+      const int s = 2 * d + side;
+      const int n = a.nbr[c * a.ns + s];
+      double o = 0.0;
+      if (n >= 0) {
+        const double dn = 1.0 / (3.0 * a.sigt[n]);
+        double df = 2.0 * dc * dn / (dc + dn);
+        if (a.cons) df = blend(df, a.hr[d] * h, a.p);
+        o = -df / (h * h);
+        diag += df / (h * h);
+      } else if (a.vac[d][side]) {
+        const double db = a.cons ? blend(dc, a.hr[d] * h, a.p) : dc;
+        diag += 1.0 / (h * (2.0 + h / (2.0 * db)));
+      }
+      a.off[c * a.ns + s] = o;
+    }
+  }
+  a.own[c] = diag;
+}
+
+__global__ void asm_dg0(AsmArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_cells) return;
+  const double st = a.sigt[c];
+  const double dc = 1.0 / (3.0 * st);
+  double diag = a.volume[c] * (st - a.sigs[c]);
+  for (int f = 0; f < a.nf; f++) {
+    const int64_t k = c * a.nf + f;
+    const int n = a.face_nb[k];
+    const double A = a.area[k], d_own = a.dist[2 * k];
+    double o = 0.0;
+    if (n >= 0) {
+      const double dn = 1.0 / (3.0 * a.sigt[n]), d_nb = a.dist[2 * k + 1];
+      double tf = A / (d_own / dc + d_nb / dn);
+      if (a.cons) {
+        const double len = d_own + d_nb;
+        tf = A / len * blend(tf * len / A, a.slot_hr[k] * len, a.p);
+      }
+      o = -tf;
+      diag += tf;
+    } else if (n == -1) {
+      const double db = a.cons ? blend(dc, 2.0 * d_own * a.slot_hr[k], a.p) : dc;
+      diag += A / (2.0 + d_own / db);
+    }
+    a.off[k] = o;
+  }
+  a.own[c] = diag;
+}
+
+template <int NB>
+__global__ void asm_dg1(AsmArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.n_cells) return;
+  const double st = a.sigt[c];
+  const double dc = 1.0 / (3.0 * st), sa = st - a.sigs[c];
+  const double V = a.volume[c];
+  double own[NB][NB];
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) {
+      double gd = 0.0;
+      for (int d = 0; d < 3; d++) gd += a.grad[(c * NB + i) * 3 + d] * a.grad[(c * NB + j) * 3 + d];
+      own[i][j] = V * (dc * gd + (i == j ? sa : 0.0));
+    }
+  for (int f = 0; f < a.nf; f++) {
+    const int64_t k = c * a.nf + f;
+    const int n = a.face_nb[k];
+    double *nbk = a.off + k * NB * NB;
+    for (int e = 0; e < NB * NB; e++) nbk[e] = 0.0;
+    if (n < -1) continue;  // reflective: zero Neumann
+    const double A = a.area[k];
+    const double scale = V * A;
+    const double *Fo = a.face_own + k * NB * NB, *Fu = a.face_up + k * NB * NB;
+    double nrm[3];
+    for (int d = 0; d < 3; d++) nrm[d] = a.nA[3 * k + d] / A;
+    double gc[NB], gn[NB];
+    for (int j = 0; j < NB; j++) {
+      gc[j] = gn[j] = 0.0;
+      for (int d = 0; d < 3; d++) {
+        gc[j] += a.grad[(c * NB + j) * 3 + d] * nrm[d];
+        if (n >= 0) gn[j] += a.grad[((int64_t)n * NB + j) * 3 + d] * nrm[d];
+      }
+    }
+    const double hc = 2.0 * a.dist[2 * k];
+    if (n >= 0) {
+      const double dn = 1.0 / (3.0 * a.sigt[n]), hn = 2.0 * a.dist[2 * k + 1];
+      const double kappa = fmax(0.5 * a.C * (dc / hc + dn / hn), 0.25);
+      for (int i = 0; i < NB; i++) {
+        const double mi = scale * Fo[i * NB];
+        for (int j = 0; j < NB; j++) {
+          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gc[j] * mi - 0.5 * dc * gc[i] * scale * Fo[j];
+          nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dc * gc[i] * scale * Fu[j];
+        }
+      }
+    } else {
+      const double kappa = fmax(a.C * dc / hc, 0.25);
+      for (int i = 0; i < NB; i++) {
+        const double mi = scale * Fo[i * NB];
+        for (int j = 0; j < NB; j++)
+          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gc[j] * mi - 0.5 * dc * gc[i] * scale * Fo[j];
+      }
+    }
+  }
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) a.own[(c * NB + i) * NB + j] = own[i][j];
+}
+
+// block-Jacobi: invert each cell's own block (Gauss-Jordan, partial pivoting)
+template <int NB>
+__global__ void jacobi_invert(const double *own, double *jinv, int64_t n_cells, int *n_bad) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  double m[NB][NB], w[NB][NB];
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) {
+      m[i][j] = own[(c * NB + i) * NB + j];
+      w[i][j] = i == j ? 1.0 : 0.0;
+    }
+  bool ok = true;
+  for (int k = 0; k < NB && ok; k++) {
+    int piv = k;
+    for (int i = k + 1; i < NB; i++)
+      if (fabs(m[i][k]) > fabs(m[piv][k])) piv = i;
+    if (!(fabs(m[piv][k]) > 0.0)) {
+      ok = false;
+      break;
+    }
+    if (piv != k)
+      for (int j = 0; j < NB; j++) {
+        double t = m[k][j];
+        m[k][j] = m[piv][j];
+        m[piv][j] = t;
+        t = w[k][j];
+        w[k][j] = w[piv][j];
+        w[piv][j] = t;
+      }
+    const double ip = 1.0 / m[k][k];
+    for (int j = 0; j < NB; j++) {
+      m[k][j] *= ip;
+      w[k][j] *= ip;
+    }
+    for (int i = 0; i < NB; i++) {
+      if (i == k) continue;
+      const double f = m[i][k];
+      for (int j = 0; j < NB; j++) {
+        m[i][j] -= f * m[k][j];
+        w[i][j] -= f * w[k][j];
+      }
+    }
+  }
+  if (!ok) atomicAdd(n_bad, 1);
+  for (int i = 0; i < NB; i++)
+    for (int j = 0; j < NB; j++) jinv[(c * NB + i) * NB + j] = ok ? w[i][j] : 0.0;
+}
+
+struct MvArgs {
+  int nb, ns;
+  int64_t n_cells;
+  const int *nbr;
+  const double *own, *off, *jinv;
+};
+
+// y = D x for node (c, i); NB compile-time
+template <int NB>
+__device__ inline double mv_row(const MvArgs &m, const double *__restrict__ x, int64_t c, int i) {
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) v += m.own[(c * NB + i) * NB + j] * x[c * NB + j];
+  for (int s = 0; s < m.ns; s++) {
+    const int n = m.nbr[c * m.ns + s];
+    if (n < 0) continue;
+    const double *o = m.off + ((c * m.ns + s) * NB + i) * NB;
+#pragma unroll
+    for (int j = 0; j < NB; j++) v += o[j] * x[(int64_t)n * NB + j];
+  }
+  return v;
+}
+
+template <int NB>
+__global__ void dsa_matvec(MvArgs m, const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= m.n_cells * NB) return;
+  y[node] = mv_row<NB>(m, x, node / NB, (int)(node % NB));
+}
+
+// ---------------------------------------------------------------------------
+// CG with device scalars. scal: 0 rz, 1 rz0, 2 alpha, 3 beta, 4 done, 5 its,
+// 6 rtol^2. Partial sums per block, reduced in fixed order by one block.
+// ---------------------------------------------------------------------------
+constexpr int kCgThreads = 256;
+
+__device__ inline double block_sum(double v) {
+  __shared__ double sh[kCgThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += sh[i];
+  return t;
+}
+
+// z = M r (block Jacobi) for node (c, i)
+template <int NB>
+__device__ inline double prec_row(const MvArgs &m, const double *__restrict__ r, int64_t c, int i) {
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) v += m.jinv[(c * NB + i) * NB + j] * r[c * NB + j];
+  return v;
+}
+
+// cg_init and cg_update run a thread per CELL, so the block-Jacobi step reads
+// only the r values its own thread just wrote
+template <int NB>
+__global__ void cg_init(MvArgs m, const double *__restrict__ b, const double *__restrict__ wgt, double *x, double *r,
+                        double *z, double *p, double *partial) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < m.n_cells) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int64_t n = c * NB + i;
+      x[n] = 0.0;
+      r[n] = wgt ? b[n] * wgt[n] : b[n];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int64_t n = c * NB + i;
+      const double zz = prec_row<NB>(m, r, c, i);
+      z[n] = zz;
+      p[n] = zz;
+      v += r[n] * zz;
+    }
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void cg_reduce_init(const double *partial, int nblk, double *scal, double rtol2) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < nblk; i++) s += partial[i];
+  scal[0] = s;
+  scal[1] = s;
+  scal[4] = (s == 0.0) ? 1.0 : 0.0;
+  scal[5] = 0.0;
+  scal[6] = rtol2;
+}
+
+template <int NB>
+__global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ p, double *__restrict__ q, double *partial,
+                              const double *scal) {
+  if (scal[4] != 0.0) return;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (n < m.n_cells * NB) {
+    const double qq = mv_row<NB>(m, p, n / NB, (int)(n % NB));
+    q[n] = qq;
+    v = p[n] * qq;
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void cg_alpha(const double *partial, int nblk, double *scal) {
+  if (threadIdx.x != 0 || scal[4] != 0.0) return;
+  double s = 0.0;
+  for (int i = 0; i < nblk; i++) s += partial[i];
+  scal[2] = s != 0.0 ? scal[0] / s : 0.0;
+  if (s == 0.0) scal[4] = 1.0;
+}
+
+template <int NB>
+__global__ void cg_update(MvArgs m, double *x, double *r, double *z, const double *p, const double *q, double *partial,
+                          const double *scal) {
+  if (scal[4] != 0.0) return;
+  const double alpha = scal[2];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < m.n_cells) {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int64_t n = c * NB + i;
+      x[n] += alpha * p[n];
+      r[n] -= alpha * q[n];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int64_t n = c * NB + i;
+      const double zz = prec_row<NB>(m, r, c, i);
+      z[n] = zz;
+      v += r[n] * zz;
+    }
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void cg_beta(const double *partial, int nblk, double *scal) {
+  if (threadIdx.x != 0 || scal[4] != 0.0) return;
+  double s = 0.0;
+  for (int i = 0; i < nblk; i++) s += partial[i];
+  scal[3] = scal[0] != 0.0 ? s / scal[0] : 0.0;
+  scal[0] = s;
+  scal[5] += 1.0;
+  if (s <= scal[6] * scal[1]) scal[4] = 1.0;
+}
+
+__global__ void cg_direction(int64_t N, const double *z, double *p, const double *scal) {
+  if (scal[4] != 0.0) return;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) p[n] = z[n] + scal[3] * p[n];
+}
+
+template <class T>
+__global__ void prolong(const double *__restrict__ sol, T *__restrict__ y, int64_t rows, int A, double inv_w) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) y[r] = (T)(sol[r / A] * inv_w);
+}
+
+double half_range_current(const Quadrature &q, const double *n) {
+  double s = 0.0;
+  for (int a = 0; a < q.n_angles; a++) {
+    double c = 0.0;
+    for (int d = 0; d < 3; d++) c += q.omega[3 * a + d] * n[d];
+    if (c > 0.0) s += q.w[a] * c;
+  }
+  return s / q.sum_weights;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void DSA::create(Operator *o) {
+  op = o;
+  const Disc &D = *o->disc;
+  nb = D.nb;
+  n_cells = D.n_cells;
+  n_nodes = D.n_nodes();
+  std::vector<int> nb_tab;
+  std::vector<double> w(n_nodes, 1.0);
+  if (D.kind == DISC_FD) {
+    ns = 2 * D.dim;
+    nb_tab.assign((size_t)n_cells * ns, -1);
+    const int64_t stride[3] = {1, D.n[0], (int64_t)D.n[0] * D.n[1]};
+    for (int64_t c = 0; c < n_cells; c++) {
+      int64_t cc = c;
+      int idx[3];
+      idx[0] = (int)(cc % D.n[0]);
+      cc /= D.n[0];
+      idx[1] = (int)(cc % D.n[1]);
+      idx[2] = (int)(cc / D.n[1]);
+      for (int d = 0; d < D.dim; d++) {
+        if (idx[d] > 0) nb_tab[c * ns + 2 * d] = (int)(c - stride[d]);
+        if (idx[d] < D.n[d] - 1) nb_tab[c * ns + 2 * d + 1] = (int)(c + stride[d]);
+      }
+    }
+    for (int d = 0; d < D.dim; d++) {
+      double e[3] = {0, 0, 0};
+      e[d] = 1.0;
+      half_range[d] = half_range_current(D.quad, e);
+      for (int s = 0; s < 2; s++) {
+        vacuum[d][s] = D.face_type[d][s] == UBT_BC_VACUUM;
+        any_vacuum = any_vacuum || vacuum[d][s];
+      }
+    }
+  } else {
+    ns = D.nf;
+    nb_tab.resize(D.face_nb.size());
+    std::vector<double> hr(D.face_nb.size(), 0.0);
+    for (size_t k = 0; k < D.face_nb.size(); k++) {
+      nb_tab[k] = D.face_nb[k] >= 0 ? D.face_nb[k] : -1;
+      any_vacuum = any_vacuum || D.face_nb[k] == -1;
+      double nrm[3];
+      for (int d = 0; d < 3; d++) nrm[d] = D.face_nA[3 * k + d] / D.face_area[k];
+      hr[k] = half_range_current(D.quad, nrm);
+    }
+    slot_hr.upload(hr);
+    for (int64_t c = 0; c < n_cells; c++)
+      for (int i = 0; i < nb; i++) w[c * nb + i] = D.volume[c];
+  }
+  nbr.upload(nb_tab);
+  weight.upload(w);
+  own.alloc(sizeof(double) * n_cells * nb * nb);
+  off.alloc(sizeof(double) * n_cells * ns * nb * nb);
+  jinv.alloc(sizeof(double) * n_cells * nb * nb);
+  const size_t nodes = sizeof(double) * n_nodes;
+  rhs.alloc(nodes);
+  sol.alloc(nodes);
+  r.alloc(nodes);
+  z.alloc(nodes);
+  p.alloc(nodes);
+  q.alloc(nodes);
+  const int64_t nblk = (n_nodes + kCgThreads - 1) / kCgThreads;
+  partial.alloc(sizeof(double) * nblk);
+  scal.alloc(sizeof(double) * 8);
+}
+
+void DSA::set_group(int g, hipStream_t s) {
+  Operator &o = *op;
+  const Disc &D = *o.disc;
+  UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections before the DSA group");
+  UBT_CHECK(g >= 0 && g < o.n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range", g);
+  // the void census: this operator has no void masking or bridging, so a
+  // non-positive sigma_t is refused, and a pure-Neumann absorption-free group
+  const double *st = o.t64.sigt.as<double>() + (int64_t)g * n_cells;
+  {
+    std::vector<double> h(n_cells), hs(n_cells);
+    UBT_HIP(hipMemcpy(h.data(), st, sizeof(double) * n_cells, hipMemcpyDeviceToHost));
+    UBT_HIP(hipMemcpy(hs.data(), o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells,
+                      sizeof(double) * n_cells, hipMemcpyDeviceToHost));
+    double max_sa = 0.0;
+    for (int64_t c = 0; c < n_cells; c++) {
+      UBT_CHECK(h[c] > 0.0, UBT_ERR_SUP, "cell %lld is a void in group %d (sigma_t %g): DSA void treatment is not "
+                "supported", (long long)c, g, h[c]);
+      max_sa = std::max(max_sa, h[c] - hs[c]);
+    }
+    UBT_CHECK(any_vacuum || max_sa > 0.0, UBT_ERR_ARG_WRONGSTATE,
+              "the DSA diffusion operator is singular: every face is reflective and sigma_a is zero everywhere");
+  }
+  AsmArgs a{};
+  a.kind = D.kind;
+  a.dim = D.dim;
+  a.nb = nb;
+  a.nf = D.nf;
+  a.ns = ns;
+  a.n_cells = (int)n_cells;
+  a.cons = opt.consistent_d;
+  a.p = opt.consistent_power;
+  a.C = opt.mip_penalty;
+  a.sigt = st;
+  a.sigs = o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells;
+  a.nbr = nbr.as<int>();
+  for (int d = 0; d < 3; d++) {
+    a.h[d] = D.h[d];
+    a.hr[d] = half_range[d];
+    a.vac[d][0] = vacuum[d][0];
+    a.vac[d][1] = vacuum[d][1];
+  }
+  a.own = own.as<double>();
+  a.off = off.as<double>();
+  const unsigned grid = (unsigned)((n_cells + 255) / 256);
+  if (D.kind == DISC_FD) {
+    hipLaunchKernelGGL(asm_structured, dim3(grid), dim3(256), 0, s, a);
+  } else {
+    a.volume = o.volume_d.as<double>();
+    a.area = o.face_area_d.as<double>();
+    a.dist = o.face_dist_d.as<double>();
+    a.nA = o.face_nA_raw.as<double>();
+    a.slot_hr = slot_hr.as<double>();
+    a.face_own = o.t64.face_own.as<double>();
+    a.face_up = o.t64.face_up.as<double>();
+    a.grad = o.t64.grad.as<double>();
+    a.face_nb = o.face_nb.as<int>();
+    if (nb == 1)
+      hipLaunchKernelGGL(asm_dg0, dim3(grid), dim3(256), 0, s, a);
+    else if (nb == 3)
+      hipLaunchKernelGGL(asm_dg1<3>, dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(asm_dg1<4>, dim3(grid), dim3(256), 0, s, a);
+  }
+  UBT_HIP(hipGetLastError());
+  int *bad = o.flag.as<int>();
+  UBT_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  if (nb == 1)
+    hipLaunchKernelGGL(jacobi_invert<1>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
+  else if (nb == 3)
+    hipLaunchKernelGGL(jacobi_invert<3>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
+  else
+    hipLaunchKernelGGL(jacobi_invert<4>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
+  UBT_HIP(hipGetLastError());
+  int hbad = 0;
+  UBT_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  UBT_HIP(hipStreamSynchronize(s));
+  UBT_CHECK(hbad == 0, UBT_ERR_MAT_LU_ZRPVT, "%d diffusion diagonal blocks are singular", hbad);
+  group = g;
+}
+
+static MvArgs mv_args(const DSA &d) {
+  MvArgs m;
+  m.nb = d.nb;
+  m.ns = d.ns;
+  m.n_cells = d.n_cells;
+  m.nbr = d.nbr.as<int>();
+  m.own = d.own.as<double>();
+  m.off = d.off.as<double>();
+  m.jinv = d.jinv.as<double>();
+  return m;
+}
+
+void DSA::matvec(const double *x, double *y, hipStream_t s) const {
+  UBT_CHECK(group >= 0, UBT_ERR_ARG_WRONGSTATE, "set the DSA group first");
+  const MvArgs m = mv_args(*this);
+  const unsigned grid = (unsigned)((n_nodes + 255) / 256);
+  if (nb == 1)
+    hipLaunchKernelGGL(dsa_matvec<1>, dim3(grid), dim3(256), 0, s, m, x, y);
+  else if (nb == 3)
+    hipLaunchKernelGGL(dsa_matvec<3>, dim3(grid), dim3(256), 0, s, m, x, y);
+  else
+    hipLaunchKernelGGL(dsa_matvec<4>, dim3(grid), dim3(256), 0, s, m, x, y);
+  UBT_HIP(hipGetLastError());
+}
+
+template <int NB> static void cg_run(DSA &d, const double *b, const double *wgt, double *x, hipStream_t s) {
+  const MvArgs m = mv_args(d);
+  const int64_t N = d.n_nodes;
+  const unsigned gnode = (unsigned)((N + kCgThreads - 1) / kCgThreads);
+  const unsigned gcell = (unsigned)((d.n_cells + kCgThreads - 1) / kCgThreads);
+  double *sc = d.scal.as<double>(), *part = d.partial.as<double>();
+  double *r = d.r.as<double>(), *z = d.z.as<double>(), *p = d.p.as<double>(), *q = d.q.as<double>();
+  hipLaunchKernelGGL(cg_init<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, b, wgt, x, r, z, p, part);
+  hipLaunchKernelGGL(cg_reduce_init, dim3(1), dim3(64), 0, s, part, (int)gcell, sc, d.opt.rtol * d.opt.rtol);
+  for (int it = 0; it < d.opt.max_it; it++) {
+    hipLaunchKernelGGL(cg_matvec_dot<NB>, dim3(gnode), dim3(kCgThreads), 0, s, m, p, q, part, sc);
+    hipLaunchKernelGGL(cg_alpha, dim3(1), dim3(64), 0, s, part, (int)gnode, sc);
+    hipLaunchKernelGGL(cg_update<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, x, r, z, p, q, part, sc);
+    hipLaunchKernelGGL(cg_beta, dim3(1), dim3(64), 0, s, part, (int)gcell, sc);
+    hipLaunchKernelGGL(cg_direction, dim3(gnode), dim3(kCgThreads), 0, s, N, z, p, sc);
+  }
+  UBT_HIP(hipGetLastError());
+}
+
+void DSA::solve(const double *b, double *x, hipStream_t s) {
+  UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
+            "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
+  if (nb == 1)
+    cg_run<1>(*this, b, nullptr, x, s);
+  else if (nb == 3)
+    cg_run<3>(*this, b, nullptr, x, s);
+  else
+    cg_run<4>(*this, b, nullptr, x, s);
+}
+
+void DSA::apply(int prec, const void *x, void *y, hipStream_t s) {
+  UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
+            "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
+  UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
+  UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the DSA apply needs distinct x and y");
+  // R: the 0th moment per node (matrix cores), weighted inside the CG init
+  op->moments_f64(prec, x, rhs.as<double>(), s);
+  double *xs = sol.as<double>();
+  const double *w = weight.as<double>();
+  if (nb == 1)
+    cg_run<1>(*this, rhs.as<double>(), w, xs, s);
+  else if (nb == 3)
+    cg_run<3>(*this, rhs.as<double>(), w, xs, s);
+  else
+    cg_run<4>(*this, rhs.as<double>(), w, xs, s);
+  const int64_t rows = op->disc->n_rows();
+  const unsigned grid = (unsigned)((rows + 255) / 256);
+  const double inv_w = 1.0 / op->disc->quad.sum_weights;
+  if (prec == UBT_FP64)
+    hipLaunchKernelGGL(prolong<double>, dim3(grid), dim3(256), 0, s, xs, (double *)y, rows, op->disc->n_angles, inv_w);
+  else
+    hipLaunchKernelGGL(prolong<float>, dim3(grid), dim3(256), 0, s, xs, (float *)y, rows, op->disc->n_angles, inv_w);
+  UBT_HIP(hipGetLastError());
+}
+
+}  // namespace ubt

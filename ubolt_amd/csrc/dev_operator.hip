@@ -713,6 +713,9 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
   if (D.kind == DISC_DG) {
     op->face_nb.upload(D.face_nb);
     op->face_nA_raw.upload(D.face_nA);
+    op->volume_d.upload(D.volume);
+    op->face_area_d.upload(D.face_area);
+    op->face_dist_d.upload(D.face_distance);
     std::vector<double> iv(D.n_cells);
     for (int c = 0; c < D.n_cells; c++) iv[c] = 1.0 / D.volume[c];
     op->inv_volume.upload(iv);
@@ -832,6 +835,14 @@ void Operator::moments(int prec, const void *psi, void *phi, int M, hipStream_t 
     launch_moments<double, double>(*this, t64, (const double *)psi, (double *)phi, M, M, s);
   else
     launch_moments<float, float>(*this, t32, (const float *)psi, (float *)phi, M, M, s);
+}
+
+void Operator::moments_f64(int prec, const void *psi, double *phi, hipStream_t s) const {
+  UBT_CHECK(psi && phi, UBT_ERR_ARG_NULL, "psi and phi are required");
+  if (prec == UBT_FP64)
+    launch_moments<double, double>(*this, t64, (const double *)psi, phi, 1, 1, s);
+  else
+    launch_moments<float, double>(*this, t32, (const float *)psi, phi, 1, 1, s);
 }
 
 void Operator::set_scalar_flux(int g, int prec, const void *psi, hipStream_t s) {

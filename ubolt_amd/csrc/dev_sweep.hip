@@ -1,0 +1,341 @@
+// The transport sweep: psi = L^{-1} r for L = streaming + removal (no
+// scatter), the streaming stage of the preconditioner. The reference hands
+// this stage to PFLARE's PCAIR, a reduction-based AMG that approximates
+// exactly this inverse; on the GPU the inverse itself is affordable, in
+// wavefronts of cells whose upwind neighbours are all done.
+//
+// * Unstructured DG0/DG1: per angle, the cells are levelled by a topological
+//   sort of the upwind graph (host, once per discretisation). A level step
+//   processes, for every angle at once, the (cell, angle) pairs of that level:
+//   gather the upwind traces, build the cell's n_basis x n_basis block (the
+//   same block the element block inverse solves) and solve it. An edge left
+//   on a cycle (possible on a general mesh, not on a Delaunay or box mesh) is
+//   lagged: its inflow is taken as zero, so the stage stays a preconditioner.
+// * Structured FD: the levels are the diagonal hyperplanes i' + j' + k' = l of
+//   each octant, no table needed.
+// Reflective couplings are lagged the same way. Every kernel is a plain
+// launch per level on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "sweep.hpp"
+
+namespace ubt {
+
+namespace {
+
+template <class T, int DIM, int NB, int NF>
+__global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, const int *__restrict__ cell,
+                               const int *__restrict__ angle, const unsigned char *__restrict__ lag, int64_t begin,
+                               int64_t count) {
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= count) return;
+  const int64_t item = begin + it;
+  const int64_t c = cell[item];
+  const int a = angle[item];
+  const unsigned char skip = lag[item];
+  const int A = g.A;
+  T om[3] = {g.omega[3 * a], g.omega[3 * a + 1], DIM == 3 ? g.omega[3 * a + 2] : T(0)};
+  T B[NB][NB], v[NB];
+  const int64_t base = c * NB * A + a;
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    v[i] = r[base + (int64_t)i * A];
+#pragma unroll
+    for (int j = 0; j < NB; j++) B[i][j] = T(0);
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int64_t k = c * NF + f;
+    T s = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s += om[d] * g.face_s[3 * k + d];
+    if (s > T(0)) {
+      if (NB == 1) {
+        B[0][0] += s;
+      } else {
+        const T *F = g.face_own + k * NB * NB;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+          for (int j = 0; j < NB; j++) B[i][j] += s * F[i * NB + j];
+      }
+    } else if (s < T(0)) {
+      const int n = g.face_nb[k];
+      if (n < 0 || ((skip >> f) & 1)) continue;  // vacuum / reflective / lagged: no upwind trace
+      const int64_t ub = (int64_t)n * NB * A + a;
+      if (NB == 1) {
+        v[0] -= s * psi[ub];
+      } else {
+        T pu[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) pu[j] = psi[ub + (int64_t)j * A];
+        const T *F = g.face_up + k * NB * NB;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+          T t = T(0);
+#pragma unroll
+          for (int j = 0; j < NB; j++) t += F[i * NB + j] * pu[j];
+          v[i] -= s * t;
+        }
+      }
+    }
+  }
+  if (NB > 1) {
+#pragma unroll
+    for (int i = 1; i < NB; i++) {
+      const T *gr = g.grad + (c * NB + i) * 3;
+      T od = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
+      B[i][0] -= od;
+    }
+  }
+  const T sig = g.sigt[c];
+#pragma unroll
+  for (int i = 0; i < NB; i++) B[i][i] += sig;
+  // Gauss-Jordan with partial pivoting (the blocks are non-singular by setup)
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    int piv = k;
+#pragma unroll
+    for (int i = k + 1; i < NB; i++)
+      if (fabs(B[i][k]) > fabs(B[piv][k])) piv = i;
+    if (piv != k) {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        T t = B[k][j];
+        B[k][j] = B[piv][j];
+        B[piv][j] = t;
+      }
+      T t = v[k];
+      v[k] = v[piv];
+      v[piv] = t;
+    }
+    const T inv = T(1) / B[k][k];
+#pragma unroll
+    for (int j = 0; j < NB; j++) B[k][j] *= inv;
+    v[k] *= inv;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      if (i == k) continue;
+      const T f = B[i][k];
+#pragma unroll
+      for (int j = 0; j < NB; j++) B[i][j] -= f * B[k][j];
+      v[i] -= f * v[k];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = v[i];
+}
+
+// Structured: thread = (angle, j', k'); the level fixes i' = l - j' - k'
+template <class T, int DIM>
+__global__ void sweep_fd_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, int level) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = g.n[0], ny = g.n[1], nz = DIM == 3 ? g.n[2] : 1;
+  const int64_t per_angle = (int64_t)ny * nz;
+  if (t >= per_angle * g.A) return;
+  const int a = (int)(t / per_angle);
+  const int64_t jk = t - (int64_t)a * per_angle;
+  const int jp = (int)(jk % ny), kp = (int)(jk / ny);
+  const int ip = level - jp - kp;
+  if (ip < 0 || ip >= nx) return;
+  const T cs[3] = {g.omega[3 * a], g.omega[3 * a + 1], DIM == 3 ? g.omega[3 * a + 2] : T(0)};
+  int idx[3];
+  idx[0] = cs[0] >= T(0) ? ip : nx - 1 - ip;
+  idx[1] = cs[1] >= T(0) ? jp : ny - 1 - jp;
+  idx[2] = DIM == 3 ? (cs[2] >= T(0) ? kp : nz - 1 - kp) : 0;
+  const int64_t stride[3] = {1, nx, (int64_t)nx * ny};
+  const int64_t c = idx[0] + (int64_t)nx * (idx[1] + (int64_t)ny * idx[2]);
+  const int A = g.A;
+  T num = r[c * A + a], den = g.sigt[c];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const T cd = fabs(cs[d]) * g.invh[d];
+    den += cd;
+    if (cs[d] == T(0)) continue;
+    const int up = cs[d] > T(0) ? idx[d] - 1 : idx[d] + 1;
+    const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+    if (up >= 0 && up < nd) num += cd * psi[(c + (cs[d] > T(0) ? -stride[d] : stride[d])) * A + a];
+  }
+  psi[c * A + a] = num / den;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host: the DG level schedule
+// ---------------------------------------------------------------------------
+void Sweep::create(Operator *o) {
+  op = o;
+  const Disc &D = *o->disc;
+  if (D.kind == DISC_FD) {
+    n_levels = 0;
+    for (int d = 0; d < D.dim; d++) n_levels += D.n[d] - 1;
+    n_levels += 1;
+    return;
+  }
+  const int nc = D.n_cells, nf = D.nf, A = D.n_angles;
+  std::vector<int> level((size_t)nc), indeg((size_t)nc);
+  std::vector<std::vector<int>> by_level;
+  // per angle: indegree = interior inflow faces; Kahn by levels; leftovers
+  // on a cycle get their remaining inflow edges lagged
+  std::vector<std::vector<int64_t>> buckets;  // level -> item ids in the per-angle order
+  std::vector<int> item_cell, item_angle;
+  std::vector<unsigned char> item_lag;
+  std::vector<int64_t> item_level;
+  item_cell.reserve((size_t)nc * A);
+  item_angle.reserve((size_t)nc * A);
+  item_lag.reserve((size_t)nc * A);
+  item_level.reserve((size_t)nc * A);
+  std::vector<int> frontier, next;
+  std::vector<char> done((size_t)nc);
+  // rev[k]: the neighbour's slot for the same face, so a dependency is always
+  // judged by the downstream cell's own inflow test (the two sides' normals
+  // are negatives of each other only up to rounding)
+  std::vector<int64_t> rev((size_t)nc * nf, -1);
+  for (int64_t k = 0; k < (int64_t)nc * nf; k++) {
+    const int n = D.face_nb[k];
+    if (n < 0) continue;
+    const int c = (int)(k / nf);
+    for (int f = 0; f < nf; f++)
+      if (D.face_nb[(int64_t)n * nf + f] == c) rev[k] = (int64_t)n * nf + f;
+    UBT_CHECK(rev[k] >= 0, UBT_ERR_PLIB, "face %lld of cell %d has no matching face in its neighbour", (long long)k, c);
+  }
+  std::vector<unsigned char> lagmask((size_t)nc);
+  n_lagged = 0;
+  int max_level = 0;
+  for (int a = 0; a < A; a++) {
+    const double *om = &D.quad.omega[3 * a];
+    auto flux = [&](int64_t k) { return om[0] * D.face_nA[3 * k] + om[1] * D.face_nA[3 * k + 1] + om[2] * D.face_nA[3 * k + 2]; };
+    frontier.clear();
+    for (int c = 0; c < nc; c++) {
+      int in = 0;
+      for (int f = 0; f < nf; f++) {
+        const int64_t k = (int64_t)c * nf + f;
+        if (D.face_nb[k] >= 0 && flux(k) < 0.0) in++;
+      }
+      indeg[c] = in;
+      done[c] = 0;
+      lagmask[c] = 0;
+      if (in == 0) frontier.push_back(c);
+    }
+    int lvl = 0, n_done = 0;
+    while (n_done < nc) {
+      if (frontier.empty()) {
+        // a cycle: lag every remaining inflow edge of the not-yet-levelled
+        // cell with the fewest, and release it
+        int best = -1;
+        for (int c = 0; c < nc; c++)
+          if (!done[c] && (best < 0 || indeg[c] < indeg[best])) best = c;
+        for (int f = 0; f < nf; f++) {
+          const int64_t k = (int64_t)best * nf + f;
+          const int n = D.face_nb[k];
+          if (n >= 0 && flux(k) < 0.0 && !done[n]) {
+            lagmask[best] |= (unsigned char)(1u << f);
+            n_lagged++;
+          }
+        }
+        indeg[best] = 0;
+        frontier.push_back(best);
+      }
+      next.clear();
+      for (int c : frontier) {
+        done[c] = 1;
+        level[c] = lvl;
+        n_done++;
+        item_cell.push_back(c);
+        item_angle.push_back(a);
+        item_lag.push_back(lagmask[c]);
+        item_level.push_back(lvl);
+      }
+      for (int c : frontier)
+        for (int f = 0; f < nf; f++) {
+          const int64_t k = (int64_t)c * nf + f;
+          const int n = D.face_nb[k];
+          // n counted c as upwind iff its own test on the face says inflow
+          if (n >= 0 && !done[n] && flux(rev[k]) < 0.0 && !((lagmask[n] >> (rev[k] - (int64_t)n * nf)) & 1)) {
+            if (--indeg[n] == 0) next.push_back(n);
+          }
+        }
+      std::sort(next.begin(), next.end());
+      frontier.swap(next);
+      lvl++;
+    }
+    max_level = std::max(max_level, lvl);
+  }
+  // order the items by level (stable: angle, then cell within a level)
+  const int64_t n_items = (int64_t)item_cell.size();
+  std::vector<int64_t> count(max_level + 1, 0);
+  for (int64_t i = 0; i < n_items; i++) count[item_level[i] + 1]++;
+  for (int l = 0; l < max_level; l++) count[l + 1] += count[l];
+  level_off.assign(count.begin(), count.end());
+  std::vector<int> oc(n_items), oa(n_items);
+  std::vector<unsigned char> ol(n_items);
+  std::vector<int64_t> pos(count.begin(), count.end() - 1);
+  for (int64_t i = 0; i < n_items; i++) {
+    const int64_t p = pos[item_level[i]]++;
+    oc[p] = item_cell[i];
+    oa[p] = item_angle[i];
+    ol[p] = item_lag[i];
+  }
+  n_levels = max_level;
+  cells.upload(oc);
+  angles.upload(oa);
+  lag.upload(ol);
+}
+
+void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s) const {
+  const Operator &o = *op;
+  UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections before sweeping");
+  UBT_CHECK(r && psi, UBT_ERR_ARG_NULL, "r and psi are required");
+  UBT_CHECK(r != psi, UBT_ERR_ARG_IDN, "the sweep needs distinct r and psi");
+  const Disc &D = *o.disc;
+  if (D.kind == DISC_FD) {
+    const int64_t per = (int64_t)D.n[1] * (D.dim == 3 ? D.n[2] : 1) * D.n_angles;
+    const unsigned grid = (unsigned)((per + 255) / 256);
+    for (int l = 0; l < n_levels; l++) {
+      if (prec == UBT_FP64) {
+        if (D.dim == 2)
+          hipLaunchKernelGGL((sweep_fd_level<double, 2>), dim3(grid), dim3(256), 0, s, (const double *)r, (double *)psi, o.args<double>(), l);
+        else
+          hipLaunchKernelGGL((sweep_fd_level<double, 3>), dim3(grid), dim3(256), 0, s, (const double *)r, (double *)psi, o.args<double>(), l);
+      } else {
+        if (D.dim == 2)
+          hipLaunchKernelGGL((sweep_fd_level<float, 2>), dim3(grid), dim3(256), 0, s, (const float *)r, (float *)psi, o.args<float>(), l);
+        else
+          hipLaunchKernelGGL((sweep_fd_level<float, 3>), dim3(grid), dim3(256), 0, s, (const float *)r, (float *)psi, o.args<float>(), l);
+      }
+    }
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  const int *ci = cells.as<int>(), *ai = angles.as<int>();
+  const unsigned char *lg = lag.as<unsigned char>();
+  for (int l = 0; l < n_levels; l++) {
+    const int64_t b = level_off[l], n = level_off[l + 1] - level_off[l];
+    if (n == 0) continue;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    bool launched = false;
+#define SW(T, DIM, NB, NF)                                                                                     \
+  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                 \
+    hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
+                       o.args<T>(), ci, ai, lg, b, n);                                                         \
+    launched = true;                                                                                           \
+  }
+#define SW_ALL(T) SW(T, 2, 1, 3) SW(T, 2, 1, 4) SW(T, 3, 1, 4) SW(T, 3, 1, 6) SW(T, 2, 3, 3) SW(T, 2, 3, 4) SW(T, 3, 4, 4) SW(T, 3, 4, 6)
+    if (prec == UBT_FP64) {
+      SW_ALL(double)
+    } else {
+      SW_ALL(float)
+    }
+#undef SW_ALL
+#undef SW
+    UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for dim %d, %d basis, %d faces", D.dim, D.nb, D.nf);
+  }
+  UBT_HIP(hipGetLastError());
+}
+
+}  // namespace ubt

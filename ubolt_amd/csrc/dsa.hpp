@@ -1,0 +1,45 @@
+// Diffusion synthetic acceleration on the device: the diffusion operator of
+// each backend assembled per group by a kernel, its FP64 matvec, a block-Jacobi
+// preconditioned CG inner solve with device-resident scalars (no host round
+// trip, so a stream can be captured), and the restriction / prolongation.
+#pragma once
+
+#include "operator.hpp"
+
+namespace ubt {
+
+struct DSAOptions {
+  int consistent_d = 1;      // blend the upwind numerical diffusion m h into face D
+  double consistent_power = 1.5;
+  double mip_penalty = 4.0;  // C in kappa = max(C/2 (D_c/h_c + D_n/h_n), 1/4)
+  int max_it = 200;          // inner CG iterations at most
+  double rtol = 1e-6;        // inner CG relative residual (preconditioned norm)
+};
+
+struct DSA {
+  Operator *op = nullptr;
+  DSAOptions opt;
+  int group = -1;
+  int nb = 1, ns = 0;        // block size, neighbour slots per cell
+  int64_t n_cells = 0, n_nodes = 0;
+  double half_range[3] = {0, 0, 0};
+  int vacuum[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  bool any_vacuum = false;
+  DevBuf nbr;                // int, cells * ns: neighbour cell per slot, -1 none
+  DevBuf own, off, jinv;     // double: cells*nb*nb, cells*ns*nb*nb, cells*nb*nb
+  DevBuf weight;             // double, nodes: V (plex) or 1 (structured)
+  DevBuf slot_hr;            // double, cells*ns: half-range current per face (DG0)
+  DevBuf rhs, sol, r, z, p, q, partial, scal;
+  int last_iterations = 0;
+
+  void create(Operator *o);
+  void set_group(int g, hipStream_t s);
+  // y = P D^{-1} R x
+  void apply(int prec, const void *x, void *y, hipStream_t s);
+  // y = D_diff x on node vectors (FP64)
+  void matvec(const double *x, double *y, hipStream_t s) const;
+  // sol = D^{-1} rhs by the inner CG (x0 = 0)
+  void solve(const double *b, double *x, hipStream_t s);
+};
+
+}  // namespace ubt

@@ -41,7 +41,8 @@ struct Operator {
   bool xs_set = false;
   PrecTables t64, t32;
   DevBuf reflect, face_nb;
-  DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow)
+  DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
+  DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
   DevBuf face_inflow;         // double, slots
   DevBuf face_basis_value;    // double, slots * nb
@@ -55,6 +56,8 @@ struct Operator {
   void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
   void diagonal(int prec, void *d, hipStream_t s) const;
   void moments(int prec, const void *psi, void *phi, int n_moments, hipStream_t s) const;
+  // phi (FP64, one moment) from psi of either precision
+  void moments_f64(int prec, const void *psi, double *phi, hipStream_t s) const;
   void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s);
   void scatter_source(int g_from, int g_to, int prec, void *b, hipStream_t s) const;
   void rhs(int g, int prec, void *b, hipStream_t s) const;
