@@ -257,9 +257,11 @@ int ubt_blockinv_destroy(ubt_blockinv *bi);
 int ubt_dsa_create(ubt_op *op, ubt_dsa **dsa);
 /* -dsa_consistent_d[_power], -dsa_mip_penalty and the inner solve's
  * iteration cap and relative tolerance (the reference's inner KSP, one GAMG
- * application, src/dsak.kokkos.cxx:197-212). */
+ * application, src/dsak.kokkos.cxx:197-212). inner_check_every > 0 reads the
+ * inner solve's converged flag back every that many iterations and stops
+ * early (a host synchronisation); 0 never synchronises (capturable). */
 int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power, double mip_penalty,
-                        int inner_max_it, double inner_rtol);
+                        int inner_max_it, double inner_rtol, int inner_check_every);
 /* Assemble group g's diffusion operator from the operator's cross sections:
  * D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g, g). Replaces
  * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345). Void cells

@@ -226,14 +226,16 @@ class DSA(_Handle):
     _destroy = "ubt_dsa_destroy"
 
     def __init__(self, op, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200,
-                 inner_rtol=1e-6):
+                 inner_rtol=1e-6, inner_check_every=8):
         super().__init__()
         self.op = op
         call("ubt_dsa_create", op.h, C.byref(self.h))
-        self.set_options(consistent_d, consistent_power, mip_penalty, inner_max_it, inner_rtol)
+        self.set_options(consistent_d, consistent_power, mip_penalty, inner_max_it, inner_rtol, inner_check_every)
 
-    def set_options(self, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200, inner_rtol=1e-6):
-        call("ubt_dsa_set_options", self.h, int(consistent_d), consistent_power, mip_penalty, inner_max_it, inner_rtol)
+    def set_options(self, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200, inner_rtol=1e-6,
+                    inner_check_every=8):
+        call("ubt_dsa_set_options", self.h, int(consistent_d), consistent_power, mip_penalty, inner_max_it, inner_rtol,
+             inner_check_every)
 
     def set_group(self, g, stream=None):
         call("ubt_dsa_set_group", self.h, g, stream_ptr(stream))

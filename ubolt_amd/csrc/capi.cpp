@@ -395,18 +395,20 @@ int ubt_dsa_create(ubt_op *op, ubt_dsa **dsa) {
   });
 }
 int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power, double mip_penalty, int inner_max_it,
-                        double inner_rtol) {
+                        double inner_rtol, int inner_check_every) {
   return guard([&] {
     NEED(dsa);
     UBT_CHECK(consistent_power >= 1.0, UBT_ERR_ARG_OUTOFRANGE, "the consistent-D power must be at least 1, got %g",
               consistent_power);
     UBT_CHECK(mip_penalty > 0.0, UBT_ERR_ARG_OUTOFRANGE, "the MIP penalty must be positive, got %g", mip_penalty);
-    UBT_CHECK(inner_max_it >= 1 && inner_rtol >= 0.0, UBT_ERR_ARG_OUTOFRANGE, "inner solve needs max_it >= 1, rtol >= 0");
+    UBT_CHECK(inner_max_it >= 1 && inner_rtol >= 0.0 && inner_check_every >= 0, UBT_ERR_ARG_OUTOFRANGE,
+              "inner solve needs max_it >= 1, rtol >= 0, check_every >= 0");
     dsa->d.opt.consistent_d = consistent_d;
     dsa->d.opt.consistent_power = consistent_power;
     dsa->d.opt.mip_penalty = mip_penalty;
     dsa->d.opt.max_it = inner_max_it;
     dsa->d.opt.rtol = inner_rtol;
+    dsa->d.opt.check_every = inner_check_every;
   });
 }
 int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream) {

@@ -446,6 +446,11 @@ void DSA::create(Operator *o) {
   const int64_t nblk = (n_nodes + kCgThreads - 1) / kCgThreads;
   partial.alloc(sizeof(double) * nblk);
   scal.alloc(sizeof(double) * 8);
+  UBT_HIP(hipHostMalloc((void **)&flag_host, sizeof(double), hipHostMallocDefault));
+}
+
+DSA::~DSA() {
+  if (flag_host) (void)hipHostFree(flag_host);
 }
 
 void DSA::set_group(int g, hipStream_t s) {
@@ -568,6 +573,11 @@ template <int NB> static void cg_run(DSA &d, const double *b, const double *wgt,
     hipLaunchKernelGGL(cg_update<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, x, r, z, p, q, part, sc);
     hipLaunchKernelGGL(cg_beta, dim3(1), dim3(64), 0, s, part, (int)gcell, sc);
     hipLaunchKernelGGL(cg_direction, dim3(gnode), dim3(kCgThreads), 0, s, N, z, p, sc);
+    if (d.opt.check_every > 0 && (it + 1) % d.opt.check_every == 0 && it + 1 < d.opt.max_it) {
+      UBT_HIP(hipMemcpyAsync(d.flag_host, sc + 4, sizeof(double), hipMemcpyDeviceToHost, s));
+      UBT_HIP(hipStreamSynchronize(s));
+      if (*d.flag_host != 0.0) break;
+    }
   }
   UBT_HIP(hipGetLastError());
 }

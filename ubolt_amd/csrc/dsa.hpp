@@ -14,6 +14,9 @@ struct DSAOptions {
   double mip_penalty = 4.0;  // C in kappa = max(C/2 (D_c/h_c + D_n/h_n), 1/4)
   int max_it = 200;          // inner CG iterations at most
   double rtol = 1e-6;        // inner CG relative residual (preconditioned norm)
+  int check_every = 8;       // read the converged flag on the host every this many
+                             // iterations and stop launching (0: never - no host
+                             // synchronisation, so the solve can be graph-captured)
 };
 
 struct DSA {
@@ -30,6 +33,8 @@ struct DSA {
   DevBuf weight;             // double, nodes: V (plex) or 1 (structured)
   DevBuf slot_hr;            // double, cells*ns: half-range current per face (DG0)
   DevBuf rhs, sol, r, z, p, q, partial, scal;
+  double *flag_host = nullptr;  // pinned: the converged flag read back every check_every iterations
+  ~DSA();
   int last_iterations = 0;
 
   void create(Operator *o);
