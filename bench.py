@@ -162,8 +162,10 @@ def main():
                    "apply": "fp32 streaming+removal+scatter, fused"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "dg_apply<float,2,3,3,true>", "alg_bytes_per_launch": alg_bytes},
+                     "kernel": ("dg_apply_wave<float,2,3,3,1,true>" if op.tuning()["variant_fp32"] == 1
+                                else "dg_apply<float,2,3,3,true>"), "alg_bytes_per_launch": alg_bytes},
         "fp64_apply_gdofs": round(n / per64 / 1e9, 3),
+        "apply_tuning": op.tuning(),
         "wall_s": round(wall, 3),
     }
     if not args.no_solve:

@@ -206,9 +206,30 @@ int ubt_op_mult(ubt_op *op, int prec, const void *x, void *y, void *stream);
 /* y = L x, the operator without the scatter. Replaces UnscatteredMatMultApply
  * (src/transport_operatork.kokkos.cxx:66-78). */
 int ubt_op_mult_unscattered(ubt_op *op, int prec, const void *x, void *y, void *stream);
+/* y = A^T x, the adjoint operator: one fused kernel that streams in the
+ * reversed direction (A^T = V P A P V^{-1} for the upwind part, P reversing
+ * every angle, V the cell volumes) and transposes the scatter (unweighted
+ * angle sum, w_a on the output). No reference counterpart: the reference has
+ * no adjoint; this is the drop-in MATOP_MULT_TRANSPOSE a PETSc MatShell would
+ * register next to ShellMatMultApply (src/transport_operatork.kokkos.cxx:49-61).
+ * Exact for vacuum faces and for reflective faces normal to an axis. */
+int ubt_op_mult_transpose(ubt_op *op, int prec, const void *x, void *y, void *stream);
+/* y = L^T x (no scatter), the transpose of UnscatteredMatMultApply
+ * (src/transport_operatork.kokkos.cxx:66-78). */
+int ubt_op_mult_unscattered_transpose(ubt_op *op, int prec, const void *x, void *y, void *stream);
 /* d = diag(A) composed from the terms (streaming outflow + sigma_t). Replaces
  * TransportOperator::diagonal (src/transport_operatork.kokkos.cxx:240-256). */
 int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
+/* The apply kernel variant per precision: 0 the LDS-staged tile kernel (whole
+ * cells, every angle, per workgroup), 1 the wavefront-per-cell kernel
+ * (per-cell data through the scalar cache, no barrier); -1 re-runs the
+ * autotuner, which times both on scratch vectors when the cross sections are
+ * first set and keeps the faster. No reference counterpart (the reference has
+ * one Kokkos SpMV); exposed so the tests can check both kernels. */
+int ubt_op_set_variant(ubt_op *op, int prec, int variant);
+/* variant[2] (FP64, FP32) in use and us[4] the tuned times (FP64 tile, FP64
+ * wave, FP32 tile, FP32 wave; 0 where not applicable). */
+int ubt_op_tuning(const ubt_op *op, int *variant, float *us);
 int ubt_op_destroy(ubt_op *op);
 
 /* ------------------------------------------------------ moments and sources */
@@ -245,6 +266,9 @@ int ubt_blockinv_setup(ubt_blockinv *bi, void *stream);
  * ElementBlockInverse::apply (src/block_inversek.kokkos.cxx:206-222,281-298),
  * the RemovalPCApply body (src/transport_solverk.kokkos.cxx:84-94). */
 int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream);
+/* y = D^{-T} x: the element blocks of A^T (the forward blocks at -Omega), the
+ * PCApplyTranspose of the RemovalPCApply shell (src/transport_solverk.kokkos.cxx:84-94). */
+int ubt_blockinv_apply_transpose(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream);
 int ubt_blockinv_destroy(ubt_blockinv *bi);
 
 /* --------------------------------------------------------------------- DSA */
@@ -271,6 +295,10 @@ int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream);
  * the isotropic correction / sum_weights on every row. Replaces
  * DSAPrecon::apply (src/dsak.kokkos.cxx:350-391), the DSAPCApply body. */
 int ubt_dsa_apply(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream);
+/* y = R^T W D^{-1} P^T x, the transpose of ubt_dsa_apply (the diffusion
+ * operators are symmetric): the PCApplyTranspose of the DSAPCApply shell
+ * (src/transport_solverk.kokkos.cxx:102-112). */
+int ubt_dsa_apply_transpose(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream);
 /* y = D_diff x on node vectors, FP64 (the diffusion matvec). */
 int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream);
 /* x = D_diff^{-1} b by the inner solve (x starts at zero). */
@@ -288,6 +316,9 @@ int ubt_dsa_destroy(ubt_dsa *dsa);
 int ubt_sweep_create(ubt_op *op, ubt_sweep **sw);
 int ubt_sweep_info(const ubt_sweep *sw, int *n_levels, int64_t *n_lagged);
 int ubt_sweep_apply(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream);
+/* psi = L^{-T} r: the adjoint sweep, the same level schedule run on the
+ * reversed angles' data with the volume scaling of L^T = V P L P V^{-1}. */
+int ubt_sweep_apply_transpose(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream);
 int ubt_sweep_destroy(ubt_sweep *sw);
 
 /* ------------------------------------------------------------------ solver */
@@ -303,6 +334,10 @@ int ubt_solver_refresh(ubt_solver *solver, void *stream);
  * TransportSolver::solve (src/transport_solverk.kokkos.cxx:335-343).
  * Returns UBT_OK also when not converged: read ubt_solver_result. */
 int ubt_solver_solve(ubt_solver *solver, const double *b, double *x, void *stream);
+/* x = A^{-T} b, the adjoint solve: the same FGMRES / refinement with
+ * ubt_op_mult_transpose and every preconditioner stage transposed (KSPSolveTranspose
+ * of the reference's KSP, src/transport_solverk.kokkos.cxx:335-343). */
+int ubt_solver_solve_transpose(ubt_solver *solver, const double *b, double *x, void *stream);
 /* y = M^{-1} r, the composite preconditioner alone (prec of r and y). */
 int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, void *stream);
 /* iterations: Krylov iterations; outer: refinement steps (1 in FP64 mode);

@@ -35,6 +35,9 @@ def _cases():
         "dg1_hex_reflect": lambda: P.dg_box_3d(3, 6, order=1, bcs={5: "reflect", 2: "reflect"}),
         "dg1_tet": lambda: _tets(),
         "dg1_delaunay": lambda: P.dg1_delaunay(n_side=12, sn=8, seed=7),
+        "dg1_quad_s12": lambda: P.dg_box_2d(4, 3, 12, order=1, bcs={2: "reflect"}),
+        "dg0_hex_s8": lambda: P.dg_box_3d(3, 8, order=0, bcs={1: "reflect"}),
+        "fd3d_s8_reflect": lambda: _fd3d(4, 8, {5: "reflect", 4: "reflect"}),
     }
 
 
@@ -71,38 +74,59 @@ def built(request):
     return request.param, prob, op
 
 
-def test_mult_fp64(gpu, built):
+VARIANTS = [0, 1]  # the tile kernel and the wavefront-per-cell kernel
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_mult_fp64(gpu, built, variant):
     name, prob, op = built
+    op.set_variant(core.FP64, variant)
     x = np.random.default_rng(1).standard_normal(prob.disc.n_rows)
     want = oracle_mult(prob, x)
     y = op.vec()
     op.mult(to_dev(x), y)
     torch.cuda.synchronize()
     err = rel_max(to_host(y), want)
-    print(f"{name}: fp64 apply rel err {err:.3e}")
+    print(f"{name}: fp64 apply (variant {variant}) rel err {err:.3e}")
     assert err <= FP64_TOL
 
 
-def test_mult_fp32(gpu, built):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_mult_fp32(gpu, built, variant):
     name, prob, op = built
+    op.set_variant(core.FP32, variant)
     x = np.random.default_rng(2).standard_normal(prob.disc.n_rows)
     want = oracle_mult(prob, x)
     y = op.vec(core.FP32)
     op.mult(to_dev(x, torch.float32), y)
     torch.cuda.synchronize()
     err = rel_max(to_host(y), want)
-    print(f"{name}: fp32 apply rel err vs fp64 oracle {err:.3e}")
+    print(f"{name}: fp32 apply (variant {variant}) rel err vs fp64 oracle {err:.3e}")
     assert err <= FP32_TOL
 
 
-def test_mult_unscattered(gpu, built):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_mult_unscattered(gpu, built, variant):
     name, prob, op = built
+    op.set_variant(core.FP64, variant)
+    op.set_variant(core.FP32, variant)
     x = np.random.default_rng(3).standard_normal(prob.disc.n_rows)
     want = oracle_mult(prob, x, scatter=False)
     y = op.vec()
     op.mult_unscattered(to_dev(x), y)
+    y32 = op.vec(core.FP32)
+    op.mult_unscattered(to_dev(x, torch.float32), y32)
     torch.cuda.synchronize()
     assert rel_max(to_host(y), want) <= FP64_TOL
+    assert rel_max(to_host(y32), want) <= FP32_TOL
+
+
+def test_tuner_reports(gpu, built):
+    name, prob, op = built
+    op.set_variant(core.FP64, -1)
+    t = op.tuning()
+    print(f"{name}: tuned {t}")
+    assert t["variant_fp64"] in (0, 1) and t["variant_fp32"] in (0, 1) and t["us"][0] > 0
 
 
 def test_diagonal(gpu, built):

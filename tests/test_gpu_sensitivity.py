@@ -1,0 +1,170 @@
+"""The sensitivity harness: gradients of a flux response with respect to
+per-cell sigma_t, sigma_s(g' -> g) and source, from the adjoint solves behind
+the torch.autograd.Function, checked
+
+* against central finite differences of the GPU forward solve (<= 1e-5
+  relative, the gradient's own scale setting the floor for cells whose
+  gradient is tiny), and
+* against central finite differences of the oracle's forward solve (direct
+  sparse solves of the oracle's assembled group operators, group by group),
+  an independent forward model.
+
+Response: J = sum_g sum_c r_c phi_g(c, basis 0) + 0.1 (sum_g sum_c r_c
+phi_g(c, 0))^2 / sum_c r_c with r the box weights (cell volume inside
+[0.55, 0.9]^dim), so dJ/dphi is not constant."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import ubolt_amd as u
+from helpers import oracle_rhs, oracle_solve
+from ubolt_amd import problems as P
+from ubolt_amd import sensitivity as S
+
+pytestmark = pytest.mark.gpu
+
+
+def fd3_mg():
+    q = u.Quadrature(3, 4)
+    d = u.Disc.structured(q, [4, 5, 3], [1.0, 1.2, 0.9], P.VACUUM_3D)
+    rng = np.random.default_rng(41)
+    st, ss = P.downscatter_xs(rng, 3, d.n_cells)
+    src = np.zeros((3, d.n_cells))
+    src[0] = rng.uniform(0.5, 1.5, d.n_cells)
+    return P.Problem("fd3_mg", q, d, st, ss, src, n_groups=3)
+
+
+def delaunay_mg():
+    pts, cells = P.delaunay_square(6, 9)
+    q = u.Quadrature(2, 4)
+    d = u.Disc.dg(q, u.Mesh(pts, cells), 1, P.VACUUM_2D)
+    rng = np.random.default_rng(42)
+    st, ss = P.downscatter_xs(rng, 2, d.n_cells, 0.1, 10.0, 0.95)
+    src = np.zeros((2, d.n_cells))
+    src[0] = 1.0
+    return P.Problem("dg1_delaunay_mg", q, d, st, ss, src, n_groups=2)
+
+
+CASES = {
+    "fd3_mg": fd3_mg,
+    "dg1_hex_mg": lambda: P.dg_box_3d(3, 4, order=1, G=2),
+    "dg0_tri_mg": lambda: P.dg_box_2d(5, 4, 4, order=0, tris=True, G=2),
+    "dg1_delaunay_mg": delaunay_mg,
+}
+
+
+def response(phi, r, nb):
+    rt = torch.as_tensor(r, dtype=torch.float64, device="cuda")
+    lin = (phi[:, ::nb] * rt).sum()
+    return lin + 0.1 * lin ** 2 / float(r.sum())
+
+
+def response_np(phis, r, nb):
+    lin = sum(float(r @ p[::nb]) for p in phis)
+    return lin + 0.1 * lin ** 2 / float(r.sum())
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def setup(request):
+    prob = CASES[request.param]()
+    model = S.TransportModel(prob.disc, prob.n_groups, rtol=1e-13)
+    r = S.box_response_weights(prob.disc, [0.55] * 3, [0.9] * 3)
+    assert r.sum() > 0
+    G, nc = prob.n_groups, prob.disc.n_cells
+    xs = [torch.tensor(np.broadcast_to(np.asarray(a, dtype=np.float64).reshape(shape), shape).copy(),
+                       dtype=torch.float64, requires_grad=True)
+          for a, shape in ((prob.sigma_t, (G, nc)), (prob.sigma_s, (G, G, nc)), (prob.source, (G, nc)))]
+    phi = S.scalar_flux(model, *xs)
+    J = response(phi, r, prob.disc.nb)
+    J.backward()
+    grads = [x.grad.detach().cpu().numpy() for x in xs]
+    print(f"[case] {request.param}: rows/group {prob.disc.n_rows}, J = {float(J):.10e}, solves "
+          f"{[(w, g, res['iterations']) for w, g, res in model.last_results]}", flush=True)
+    return request.param, prob, model, r, [x.detach().numpy().copy() for x in xs], grads
+
+
+def picks(prob, seed, k=5):
+    """(kind, index) entries to check: sigma_t and source per group, sigma_s on
+    the self and the first downscatter transfer"""
+    rng = np.random.default_rng(seed)
+    G, nc = prob.n_groups, prob.disc.n_cells
+    out = []
+    for _ in range(k):
+        g = int(rng.integers(G))
+        c = int(rng.integers(nc))
+        out.append(("sigma_t", (g, c)))
+        out.append(("source", (g, c)))
+        gt = min(g + int(rng.integers(2)), G - 1)
+        out.append(("sigma_s", (g, gt, c)))
+    return out
+
+
+def perturbed(base, kind, idx, delta):
+    st, ss, q = [a.copy() for a in base]
+    a = {"sigma_t": st, "sigma_s": ss, "source": q}[kind]
+    a[idx] += delta
+    return st, ss, q
+
+
+def step(base, kind, idx):
+    a = {"sigma_t": base[0], "sigma_s": base[1], "source": base[2]}[kind]
+    return 1e-3 * max(abs(a[idx]), 1e-2 * float(np.abs(a).max()))
+
+
+def check(name, got, fd, scale, tag):
+    err = abs(got - fd) / max(abs(fd), 1e-3 * scale)
+    print(f"{name} {tag}: adjoint {got:+.10e} fd {fd:+.10e} rel {err:.2e}")
+    return err
+
+
+def test_gradient_vs_gpu_finite_differences(gpu, setup):
+    name, prob, model, r, base, grads = setup
+    nb = prob.disc.nb
+    gmap = {"sigma_t": grads[0], "sigma_s": grads[1], "source": grads[2]}
+    worst = 0.0
+    for kind, idx in picks(prob, 7):
+        h = step(base, kind, idx)
+        J = []
+        for sgn in (1, -1):
+            st, ss, q = perturbed(base, kind, idx, sgn * h)
+            with torch.no_grad():
+                _, phi = model.forward_solve(*(torch.as_tensor(a) for a in (st, ss, q)))
+            J.append(float(response(phi, r, nb)))
+        fd = (J[0] - J[1]) / (2 * h)
+        worst = max(worst, check(name, float(gmap[kind][idx]), fd, float(np.abs(gmap[kind]).max()), f"{kind}{idx}"))
+    assert worst <= 1e-5
+
+
+def oracle_forward(prob, st, ss, q):
+    p = P.Problem(prob.name, prob.quad, prob.disc, st, ss, q, n_groups=prob.n_groups)
+    phis = []
+    for g in range(prob.n_groups):
+        b = oracle_rhs(p, g, phis)
+        psi = oracle_solve(p, b, g)
+        phis.append(oracle.moments(psi, prob.quad.w))
+    return phis
+
+
+def test_gradient_vs_oracle_finite_differences(gpu, setup):
+    name, prob, model, r, base, grads = setup
+    nb = prob.disc.nb
+    gmap = {"sigma_t": grads[0], "sigma_s": grads[1], "source": grads[2]}
+    worst = 0.0
+    for kind, idx in picks(prob, 8, k=2):
+        h = step(base, kind, idx)
+        J = [response_np(oracle_forward(prob, *perturbed(base, kind, idx, sgn * h)), r, nb) for sgn in (1, -1)]
+        fd = (J[0] - J[1]) / (2 * h)
+        worst = max(worst, check(name, float(gmap[kind][idx]), fd, float(np.abs(gmap[kind]).max()),
+                                 f"oracle {kind}{idx}"))
+    assert worst <= 1e-5
+
+
+def test_upscatter_refused(gpu):
+    prob = P.dg_box_2d(3, 3, 2, order=0, G=2)
+    model = S.TransportModel(prob.disc, 2)
+    st = torch.tensor(prob.sigma_t)
+    ss = torch.tensor(prob.sigma_s).clone()
+    ss[1, 0] = 0.1
+    with pytest.raises(ValueError):
+        S.scalar_flux(model, st, ss, torch.tensor(prob.source))

@@ -72,7 +72,11 @@ SIGNATURES = {
     "ubt_op_size": [P, PI64, PI64, PI],
     "ubt_op_mult": [P, I, P, P, P],
     "ubt_op_mult_unscattered": [P, I, P, P, P],
+    "ubt_op_mult_transpose": [P, I, P, P, P],
+    "ubt_op_mult_unscattered_transpose": [P, I, P, P, P],
     "ubt_op_diagonal": [P, I, P, P],
+    "ubt_op_set_variant": [P, I, I],
+    "ubt_op_tuning": [P, PI, C.POINTER(C.c_float)],
     "ubt_op_destroy": [P],
     "ubt_moments": [P, I, P, P, I, P],
     "ubt_set_scalar_flux": [P, I, I, P, P],
@@ -81,11 +85,13 @@ SIGNATURES = {
     "ubt_blockinv_create": [P, PP],
     "ubt_blockinv_setup": [P, P],
     "ubt_blockinv_apply": [P, I, P, P, P],
+    "ubt_blockinv_apply_transpose": [P, I, P, P, P],
     "ubt_blockinv_destroy": [P],
     "ubt_dsa_create": [P, PP],
     "ubt_dsa_set_options": [P, I, D, D, I, D, I],
     "ubt_dsa_set_group": [P, I, P],
     "ubt_dsa_apply": [P, I, P, P, P],
+    "ubt_dsa_apply_transpose": [P, I, P, P, P],
     "ubt_dsa_matvec": [P, P, P, P],
     "ubt_dsa_solve": [P, P, P, P],
     "ubt_dsa_info": [P, PI, PI],
@@ -93,10 +99,12 @@ SIGNATURES = {
     "ubt_sweep_create": [P, PP],
     "ubt_sweep_info": [P, PI, PI64],
     "ubt_sweep_apply": [P, I, P, P, P],
+    "ubt_sweep_apply_transpose": [P, I, P, P, P],
     "ubt_sweep_destroy": [P],
     "ubt_solver_create": [P, P, C.POINTER(SolverOptions), PP],
     "ubt_solver_refresh": [P, P],
     "ubt_solver_solve": [P, P, P, P],
+    "ubt_solver_solve_transpose": [P, P, P, P],
     "ubt_solver_apply_pc": [P, I, P, P, P],
     "ubt_solver_result": [P, PI, PI, PD, PI, PD, I, PI],
     "ubt_solver_destroy": [P],

@@ -167,6 +167,16 @@ class Operator(_Handle):
         call("ubt_op_set_group", self.h, g)
         self.group = g
 
+    def set_variant(self, prec, variant):
+        """apply kernel: 0 tile, 1 wavefront per cell, -1 re-tune"""
+        call("ubt_op_set_variant", self.h, prec, variant)
+
+    def tuning(self):
+        v = (C.c_int * 2)()
+        us = (C.c_float * 4)()
+        call("ubt_op_tuning", self.h, v, us)
+        return {"variant_fp64": v[0], "variant_fp32": v[1], "us": [round(x, 2) for x in us]}
+
     def vec(self, prec=FP64, fill=None):
         t = torch.zeros(self.n_rows, dtype=torch_dtype(prec), device="cuda")
         if fill is not None:
@@ -183,6 +193,15 @@ class Operator(_Handle):
 
     def mult_unscattered(self, x, y, stream=None):
         call("ubt_op_mult_unscattered", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def mult_transpose(self, x, y, stream=None):
+        """y = A^T x, the adjoint operator"""
+        call("ubt_op_mult_transpose", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def mult_unscattered_transpose(self, x, y, stream=None):
+        call("ubt_op_mult_unscattered_transpose", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
         return y
 
     def diagonal(self, d, stream=None):
@@ -220,6 +239,10 @@ class BlockInverse(_Handle):
         call("ubt_blockinv_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
         return y
 
+    def apply_transpose(self, x, y, stream=None):
+        call("ubt_blockinv_apply_transpose", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
 
 class DSA(_Handle):
     """DSA preconditioner stage (reference DSAPrecon): caller-owned."""
@@ -242,6 +265,10 @@ class DSA(_Handle):
 
     def apply(self, x, y, stream=None):
         call("ubt_dsa_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def apply_transpose(self, x, y, stream=None):
+        call("ubt_dsa_apply_transpose", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
         return y
 
     def matvec(self, x, y, stream=None):
@@ -274,6 +301,10 @@ class Sweep(_Handle):
         call("ubt_sweep_apply", self.h, Operator._prec(r), dptr(r), dptr(psi), stream_ptr(stream))
         return psi
 
+    def apply_transpose(self, r, psi, stream=None):
+        call("ubt_sweep_apply_transpose", self.h, Operator._prec(r), dptr(r), dptr(psi), stream_ptr(stream))
+        return psi
+
 
 class Solver(_Handle):
     """Right-preconditioned FGMRES with the composite preconditioner
@@ -293,6 +324,11 @@ class Solver(_Handle):
 
     def solve(self, b, x, stream=None):
         call("ubt_solver_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
+        return self.result()
+
+    def solve_transpose(self, b, x, stream=None):
+        """x = A^{-T} b, the adjoint solve"""
+        call("ubt_solver_solve_transpose", self.h, dptr(b), dptr(x), stream_ptr(stream))
         return self.result()
 
     def apply_pc(self, r, y, stream=None):

@@ -315,11 +315,45 @@ int ubt_op_mult_unscattered(ubt_op *op, int prec, const void *x, void *y, void *
     op->op->mult(prec, x, y, false, (hipStream_t)stream);
   });
 }
+int ubt_op_mult_transpose(ubt_op *op, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(op);
+    op->op->mult_transpose(prec, x, y, true, (hipStream_t)stream);
+  });
+}
+int ubt_op_mult_unscattered_transpose(ubt_op *op, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(op);
+    op->op->mult_transpose(prec, x, y, false, (hipStream_t)stream);
+  });
+}
 int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream) {
   return guard([&] {
     NEED(op);
     NEED(d);
     op->op->diagonal(prec, d, (hipStream_t)stream);
+  });
+}
+int ubt_op_set_variant(ubt_op *op, int prec, int variant) {
+  return guard([&] {
+    NEED(op);
+    UBT_CHECK(prec == UBT_FP64 || prec == UBT_FP32, UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
+    UBT_CHECK(variant >= -1 && variant <= 1, UBT_ERR_ARG_OUTOFRANGE, "apply variant must be -1 (tune), 0 or 1");
+    UBT_CHECK(op->op->xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
+    if (variant == -1)
+      op->op->tune();
+    else
+      op->op->variant[prec == UBT_FP64 ? 0 : 1] = variant;
+  });
+}
+int ubt_op_tuning(const ubt_op *op, int *variant, float *us) {
+  return guard([&] {
+    NEED(op);
+    if (variant)
+      for (int p = 0; p < 2; p++) variant[p] = op->op->variant[p];
+    if (us)
+      for (int p = 0; p < 2; p++)
+        for (int v = 0; v < 2; v++) us[2 * p + v] = op->op->tuned_us[p][v];
   });
 }
 int ubt_op_destroy(ubt_op *op) {
@@ -379,6 +413,12 @@ int ubt_blockinv_apply(ubt_blockinv *bi, int prec, const void *x, void *y, void 
     bi->bi.apply(prec, x, y, (hipStream_t)stream);
   });
 }
+int ubt_blockinv_apply_transpose(ubt_blockinv *bi, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(bi);
+    bi->bi.apply(prec, x, y, (hipStream_t)stream, true);
+  });
+}
 int ubt_blockinv_destroy(ubt_blockinv *bi) {
   delete bi;
   return UBT_OK;
@@ -421,6 +461,12 @@ int ubt_dsa_apply(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream) 
   return guard([&] {
     NEED(dsa);
     dsa->d.apply(prec, x, y, (hipStream_t)stream);
+  });
+}
+int ubt_dsa_apply_transpose(ubt_dsa *dsa, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    dsa->d.apply(prec, x, y, (hipStream_t)stream, true);
   });
 }
 int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream) {
@@ -478,6 +524,12 @@ int ubt_sweep_apply(ubt_sweep *sw, int prec, const void *r, void *psi, void *str
     sw->s.apply(prec, r, psi, (hipStream_t)stream);
   });
 }
+int ubt_sweep_apply_transpose(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream) {
+  return guard([&] {
+    NEED(sw);
+    sw->s.apply(prec, r, psi, (hipStream_t)stream, true);
+  });
+}
 int ubt_sweep_destroy(ubt_sweep *sw) {
   delete sw;
   return UBT_OK;
@@ -520,6 +572,12 @@ int ubt_solver_solve(ubt_solver *solver, const double *b, double *x, void *strea
     solver->s.solve(b, x, (hipStream_t)stream);
   });
 }
+int ubt_solver_solve_transpose(ubt_solver *solver, const double *b, double *x, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.solve(b, x, (hipStream_t)stream, true);
+  });
+}
 int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, void *stream) {
   return guard([&] {
     NEED(solver);
@@ -527,6 +585,7 @@ int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, vo
     NEED(y);
     UBT_CHECK(r != y, UBT_ERR_ARG_IDN, "the preconditioner apply needs distinct r and y");
     auto &s = solver->s;
+    s.transpose = false;
     UBT_CHECK(prec == UBT_FP64 ? s.V64.ptr != nullptr : s.V32.ptr != nullptr, UBT_ERR_ARG_INCOMP,
               "this solver holds %s work vectors", s.opt.mixed ? "FP32" : "FP64");
     if (prec == UBT_FP64)

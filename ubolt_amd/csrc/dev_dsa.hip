@@ -370,6 +370,17 @@ __global__ void prolong(const double *__restrict__ sol, T *__restrict__ y, int64
   if (r < rows) y[r] = (T)(sol[r / A] * inv_w);
 }
 
+// the adjoint prolongation: y(node, a) = w_a * weight(node) * sol(node) * inv_w
+template <class T>
+__global__ void prolong_t(const double *__restrict__ sol, const double *__restrict__ weight,
+                          const double *__restrict__ w, T *__restrict__ y, int64_t rows, int A, double inv_w) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) {
+    const int64_t node = r / A;
+    y[r] = (T)(w[r - node * A] * weight[node] * sol[node] * inv_w);
+  }
+}
+
 double half_range_current(const Quadrature &q, const double *n) {
   double s = 0.0;
   for (int a = 0; a < q.n_angles; a++) {
@@ -593,15 +604,16 @@ void DSA::solve(const double *b, double *x, hipStream_t s) {
     cg_run<4>(*this, b, nullptr, x, s);
 }
 
-void DSA::apply(int prec, const void *x, void *y, hipStream_t s) {
+void DSA::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose) {
   UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
             "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
   UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
   UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the DSA apply needs distinct x and y");
-  // R: the 0th moment per node (matrix cores), weighted inside the CG init
-  op->moments_f64(prec, x, rhs.as<double>(), s);
+  // R: the 0th moment per node (matrix cores), weighted inside the CG init;
+  // the adjoint sums unweighted and moves the node weight to the prolongation
+  op->moments_f64(prec, x, rhs.as<double>(), s, transpose);
   double *xs = sol.as<double>();
-  const double *w = weight.as<double>();
+  const double *w = transpose ? nullptr : weight.as<double>();
   if (nb == 1)
     cg_run<1>(*this, rhs.as<double>(), w, xs, s);
   else if (nb == 3)
@@ -611,7 +623,15 @@ void DSA::apply(int prec, const void *x, void *y, hipStream_t s) {
   const int64_t rows = op->disc->n_rows();
   const unsigned grid = (unsigned)((rows + 255) / 256);
   const double inv_w = 1.0 / op->disc->quad.sum_weights;
-  if (prec == UBT_FP64)
+  if (transpose) {
+    const double *wa = op->t64.w.as<double>(), *wn = weight.as<double>();
+    if (prec == UBT_FP64)
+      hipLaunchKernelGGL(prolong_t<double>, dim3(grid), dim3(256), 0, s, xs, wn, wa, (double *)y, rows,
+                         op->disc->n_angles, inv_w);
+    else
+      hipLaunchKernelGGL(prolong_t<float>, dim3(grid), dim3(256), 0, s, xs, wn, wa, (float *)y, rows,
+                         op->disc->n_angles, inv_w);
+  } else if (prec == UBT_FP64)
     hipLaunchKernelGGL(prolong<double>, dim3(grid), dim3(256), 0, s, xs, (double *)y, rows, op->disc->n_angles, inv_w);
   else
     hipLaunchKernelGGL(prolong<float>, dim3(grid), dim3(256), 0, s, xs, (float *)y, rows, op->disc->n_angles, inv_w);

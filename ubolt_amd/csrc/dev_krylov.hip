@@ -171,7 +171,10 @@ void Solver::refresh(hipStream_t s) {
 }
 
 template <class T> void Solver::apply_op(const T *x, T *y, bool scatter, hipStream_t s) {
-  op->mult(prec_of<T>(), x, y, scatter, s);
+  if (transpose)
+    op->mult_transpose(prec_of<T>(), x, y, scatter, s);
+  else
+    op->mult(prec_of<T>(), x, y, scatter, s);
 }
 
 template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s) {
@@ -192,9 +195,9 @@ template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s) {
     fn((const T *)t, u);
     hipLaunchKernelGGL(k_axpby<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, 1.0, (const T *)u, 1.0, y);
   };
-  if (opt.pc_blocks) stage([&](const T *a, T *b) { blocks.apply(prec, a, b, s); });
-  if (opt.pc_sweep) stage([&](const T *a, T *b) { sweep.apply(prec, a, b, s); });
-  if (with_dsa) stage([&](const T *a, T *b) { dsa->apply(prec, a, b, s); });
+  if (opt.pc_blocks) stage([&](const T *a, T *b) { blocks.apply(prec, a, b, s, transpose); });
+  if (opt.pc_sweep) stage([&](const T *a, T *b) { sweep.apply(prec, a, b, s, transpose); });
+  if (with_dsa) stage([&](const T *a, T *b) { dsa->apply(prec, a, b, s, transpose); });
   if (first) hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, 1.0, r, y);
   UBT_HIP(hipGetLastError());
 }
@@ -277,7 +280,7 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
   }
 }
 
-void Solver::solve(const double *b, double *x, hipStream_t s) {
+void Solver::solve(const double *b, double *x, hipStream_t s, bool adj) {
   UBT_CHECK(b && x, UBT_ERR_ARG_NULL, "b and x are required");
   UBT_CHECK(b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
   UBT_CHECK(op->xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
@@ -288,6 +291,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s) {
   history.clear();
   iterations = outer = 0;
   reason = 0;
+  transpose = adj;
   const unsigned gv = grid_for(n);
   double *part = partial.as<double>(), *hc = hcol.as<double>();
   hipLaunchKernelGGL((k_convert<double, double>), dim3(gv), dim3(kThreads), 0, s, n, 0.0, b, x);  // x = 0
@@ -312,7 +316,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s) {
   float *d = d32.as<float>(), *bf = b32.as<float>();
   std::vector<double> hist;
   for (outer = 0; outer <= opt.max_outer; outer++) {
-    op->mult(UBT_FP64, x, r, true, s);
+    apply_op<double>(x, r, true, s);
     hipLaunchKernelGGL(k_axpby<double>, dim3(gv), dim3(kThreads), 0, s, n, 1.0, b, -1.0, r);
     const double rn = host_norm<double>(r, n, hc, part, s);
     rel_residual = rn / bnorm;

@@ -57,7 +57,10 @@ __device__ inline int64_t xcd_tile(int64_t n_tiles) {
 // :480-509 removal, :584-649 scatter; upwind/BC classification
 // src/structured_fd_commonk.hpp:213-397 under the ghost-flux treatment)
 // ---------------------------------------------------------------------------
-template <class T, int DIM, bool SCATTER>
+// ADJ: the adjoint A^T - direction reversed (the upwind neighbour becomes the
+// downwind one), the scatter transposed (unweighted moment, w_a on output).
+// On a grid every cell has the same volume, so no volume ratio appears.
+template <class T, int DIM, bool SCATTER, bool ADJ = false>
 __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem[];
   T *red = reinterpret_cast<T *>(smem);
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
     const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-      const T cs = g.omega[3 * a + d];
+      const T cs = ADJ ? -g.omega[3 * a + d] : g.omega[3 * a + d];
       const T cd = fabs(cs) * g.invh[d];
       acc += cd * psi;
       if (cs != T(0)) {
@@ -94,11 +97,11 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
     }
   }
   if (SCATTER) {
-    if (lc < g.cpb) red[lc * A + a] = live ? g.w[a] * psi : T(0);
+    if (lc < g.cpb) red[lc * A + a] = live ? (ADJ ? psi : g.w[a] * psi) : T(0);
     __syncthreads();
     seg_sum(red, g.cpb, A, phi, g.red_lanes);
     __syncthreads();
-    if (live) acc -= g.sigs[c] * g.inv_sum_w * phi[lc];
+    if (live) acc -= g.sigs[c] * g.inv_sum_w * (ADJ ? g.w[a] : T(1)) * phi[lc];
   }
   if (live) y[c * A + a] = acc;
 }
@@ -110,7 +113,10 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
 // tile's geometry (contiguous per-cell records in HBM) is staged into LDS with
 // coalesced loads once, then every angle of a cell reads it from LDS.
 // ---------------------------------------------------------------------------
-template <class T, int DIM, int NB, int NF, bool SCATTER>
+// ADJ: the adjoint A^T = V P L P V^{-1} - S^T (P reverses every angle): the
+// same kernel with Omega -> -Omega, each neighbour's trace scaled by V_c / V_n,
+// and the scatter transposed.
+template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ = false>
 __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int FM = NB * NB;
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
     for (int i = 0; i < NB; i++) acc[i] = sig * psi[i];
     T om[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) om[d] = g.omega[3 * a + d];
+    for (int d = 0; d < DIM; d++) om[d] = ADJ ? -g.omega[3 * a + d] : g.omega[3 * a + d];
     if (NB > 1) {
       // volume term -(Omega . grad phi_i) psi_0; grad phi_0 = 0
 #pragma unroll
@@ -214,8 +220,9 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
           ua = g.reflect[3 * a + (-2 - nbk)];
         }
         T pu[NB];
+        const T vr = (ADJ && nbk >= 0) ? g.vol[c] / g.vol[nbk] : T(1);
 #pragma unroll
-        for (int j = 0; j < NB; j++) pu[j] = x[ub + (int64_t)j * A + ua];
+        for (int j = 0; j < NB; j++) pu[j] = vr * x[ub + (int64_t)j * A + ua];
         if (NB == 1) {
           acc[0] += s * pu[0];
         } else {
@@ -233,7 +240,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
   }
   if (SCATTER) {
     if (lc < cpb) {
-      const T wa = live ? g.w[a] : T(0);
+      const T wa = live ? (ADJ ? T(1) : g.w[a]) : T(0);
 #pragma unroll
       for (int j = 0; j < NB; j++) red[(lc * NB + j) * A + a] = wa * psi[j];
     }
@@ -241,7 +248,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
     seg_sum(red, cpb * NB, A, phi, g.red_lanes);
     __syncthreads();
     if (live) {
-      const T ss = s_ss[lc] * g.inv_sum_w;
+      const T ss = s_ss[lc] * g.inv_sum_w * (ADJ ? g.w[a] : T(1));
 #pragma unroll
       for (int i = 0; i < NB; i++) acc[i] -= ss * phi[lc * NB + i];
     }
@@ -250,6 +257,188 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
     const int64_t base = c * NB * A + a;
 #pragma unroll
     for (int i = 0; i < NB; i++) y[base + (int64_t)i * A] = acc[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-per-cell applies. A wavefront owns one cell and all its angles
+// (lane l takes angles l, l + 64, ...: APL per lane), so everything per cell -
+// cross sections, face normals and matrices, neighbour ids - sits at a
+// wave-uniform address and is read through the scalar cache, and the
+// scatter's moment is a butterfly across the wave. No LDS, no barrier: the
+// latency chain of a launch is the own-psi load, then the upwind gather.
+// Used where the angle count keeps most lanes busy (A > 32); the tile kernels
+// above serve the small sets. The autotuner picks per operator.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class T, int DIM, int NB, int NF, int APL, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_wave(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  constexpr int FM = NB * NB;
+  const int64_t n_tiles = ((int64_t)g.n_cells + 3) / 4;
+  const int64_t tile = xcd_tile(n_tiles);
+  const int lane = threadIdx.x & 63;
+  const int64_t c = __builtin_amdgcn_readfirstlane((int)(tile * 4 + (threadIdx.x >> 6)));
+  if (tile >= n_tiles || c >= g.n_cells) return;
+  const int A = g.A;
+  T psi[APL][NB], acc[APL][NB], om[APL][3];
+  bool on[APL];
+  const T sig = g.sigt[c];
+#pragma unroll
+  for (int q = 0; q < APL; q++) {
+    const int a = lane + 64 * q;
+    on[q] = a < A;
+    const int ac = on[q] ? a : 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) om[q][d] = (d < DIM) ? g.omega[3 * ac + d] : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; j++) psi[q][j] = on[q] ? x[(c * NB + j) * A + a] : T(0);
+#pragma unroll
+    for (int i = 0; i < NB; i++) acc[q][i] = sig * psi[q][i];
+  }
+  if (NB > 1) {
+#pragma unroll
+    for (int i = 1; i < NB; i++) {
+      const T *gr = g.grad + (c * NB + i) * 3;
+#pragma unroll
+      for (int q = 0; q < APL; q++) {
+        T od = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) od += om[q][d] * gr[d];
+        acc[q][i] -= od * psi[q][0];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int64_t k = c * NF + f;
+    const int nbk = g.face_nb[k];
+    const T *sv = g.face_s + 3 * k;
+    const T *Fo = g.face_own + k * FM, *Fu = g.face_up + k * FM;
+    int64_t ub = 0;
+    int axis = 0;
+    if (nbk >= 0) ub = (int64_t)nbk * NB * A;
+    else if (nbk <= -2) { ub = c * NB * A; axis = -2 - nbk; }
+#pragma unroll
+    for (int q = 0; q < APL; q++) {
+      if (!on[q]) continue;
+      T s = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) s += om[q][d] * sv[d];
+      if (s > T(0)) {
+        if (NB == 1) {
+          acc[q][0] += s * psi[q][0];
+        } else {
+#pragma unroll
+          for (int i = 0; i < NB; i++) {
+            T v = T(0);
+#pragma unroll
+            for (int j = 0; j < NB; j++) v += Fo[i * NB + j] * psi[q][j];
+            acc[q][i] += s * v;
+          }
+        }
+      } else if (nbk != -1) {
+        const int a = lane + 64 * q;
+        const int ua = nbk >= 0 ? a : g.reflect[3 * a + axis];
+        T pu[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) pu[j] = x[ub + (int64_t)j * A + ua];
+        if (NB == 1) {
+          acc[q][0] += s * pu[0];
+        } else {
+#pragma unroll
+          for (int i = 0; i < NB; i++) {
+            T v = T(0);
+#pragma unroll
+            for (int j = 0; j < NB; j++) v += Fu[i * NB + j] * pu[j];
+            acc[q][i] += s * v;
+          }
+        }
+      }
+    }
+  }
+  if (SCATTER) {
+    const T ss = g.sigs[c] * g.inv_sum_w;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      T part = T(0);
+#pragma unroll
+      for (int q = 0; q < APL; q++)
+        if (on[q]) part += g.w[lane + 64 * q] * psi[q][j];
+      const T phi = wave_sum(part);
+#pragma unroll
+      for (int q = 0; q < APL; q++) acc[q][j] -= ss * phi;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < APL; q++) {
+    if (!on[q]) continue;
+    const int a = lane + 64 * q;
+#pragma unroll
+    for (int i = 0; i < NB; i++) y[(c * NB + i) * A + a] = acc[q][i];
+  }
+}
+
+template <class T, int DIM, int APL, bool SCATTER>
+__global__ void __launch_bounds__(256) fd_apply_wave(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  const int64_t n_tiles = ((int64_t)g.n_cells + 3) / 4;
+  const int64_t tile = xcd_tile(n_tiles);
+  const int lane = threadIdx.x & 63;
+  const int64_t c = __builtin_amdgcn_readfirstlane((int)(tile * 4 + (threadIdx.x >> 6)));
+  if (tile >= n_tiles || c >= g.n_cells) return;
+  const int A = g.A;
+  int idx[3];
+  {
+    int64_t cc = c;
+    idx[0] = (int)(cc % g.n[0]);
+    cc /= g.n[0];
+    idx[1] = (int)(cc % g.n[1]);
+    idx[2] = (int)(cc / g.n[1]);
+  }
+  const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
+  const T sig = g.sigt[c];
+  T psi[APL], acc[APL];
+#pragma unroll
+  for (int q = 0; q < APL; q++) {
+    const int a = lane + 64 * q;
+    psi[q] = T(0);
+    acc[q] = T(0);
+    if (a >= A) continue;
+    psi[q] = x[c * A + a];
+    T v = sig * psi[q];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      const T cs = g.omega[3 * a + d];
+      const T cd = fabs(cs) * g.invh[d];
+      v += cd * psi[q];
+      if (cs == T(0)) continue;
+      const int up = cs > T(0) ? idx[d] - 1 : idx[d] + 1;
+      if (up >= 0 && up < g.n[d])
+        v -= cd * x[(c + (cs > T(0) ? -stride[d] : stride[d])) * A + a];
+      else if (g.face_type[d][cs > T(0) ? 0 : 1] == kReflect)
+        v -= cd * x[c * A + g.reflect[3 * a + d]];
+    }
+    acc[q] = v;
+  }
+  if (SCATTER) {
+    T part = T(0);
+#pragma unroll
+    for (int q = 0; q < APL; q++)
+      if (lane + 64 * q < A) part += g.w[lane + 64 * q] * psi[q];
+    const T phi = wave_sum(part);
+    const T ss = g.sigs[c] * g.inv_sum_w;
+#pragma unroll
+    for (int q = 0; q < APL; q++) acc[q] -= ss * phi;
+  }
+#pragma unroll
+  for (int q = 0; q < APL; q++) {
+    const int a = lane + 64 * q;
+    if (a < A) y[c * A + a] = acc[q];
   }
 }
 
@@ -431,7 +620,7 @@ __global__ void __launch_bounds__(256) moments_mfma(const T *__restrict__ psi, O
 // reads NB values per row instead of NB*NB stored inverse entries.
 // ---------------------------------------------------------------------------
 template <class T, int NB>
-__device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_fd, T B[NB][NB]) {
+__device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_fd, T B[NB][NB], T dir = T(1)) {
 #pragma unroll
   for (int i = 0; i < NB; i++)
 #pragma unroll
@@ -443,7 +632,7 @@ __device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_
     B[0][0] = v + sig;
     return true;
   }
-  const T ox = g.omega[3 * a], oy = g.omega[3 * a + 1], oz = g.omega[3 * a + 2];
+  const T ox = dir * g.omega[3 * a], oy = dir * g.omega[3 * a + 1], oz = dir * g.omega[3 * a + 2];
   for (int f = 0; f < g.nf; f++) {
     const int64_t k = c * g.nf + f;
     const T s = ox * g.face_s[3 * k] + oy * g.face_s[3 * k + 1] + oz * g.face_s[3 * k + 2];
@@ -508,14 +697,15 @@ __device__ inline bool gj_solve(T B[NB][NB], T v[NB]) {
   return true;
 }
 
+// transpose: the blocks of A^T are the forward blocks at the reversed direction
 template <class T, int NB>
-__global__ void blockinv_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g, int is_fd) {
+__global__ void blockinv_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g, int is_fd, int transpose) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= (int64_t)g.n_cells * g.A) return;
   const int64_t c = b / g.A;
   const int a = (int)(b - c * g.A);
   T B[NB][NB], v[NB];
-  build_block<T, NB>(g, c, a, is_fd, B);
+  build_block<T, NB>(g, c, a, is_fd, B, transpose ? T(-1) : T(1));
   const int64_t base = c * NB * g.A + a;
 #pragma unroll
   for (int j = 0; j < NB; j++) v[j] = x[base + (int64_t)j * g.A];
@@ -569,10 +759,50 @@ template <class T> Launch apply_launch(int A, int nb) {
   return L;
 }
 
+// The wavefront kernels: one cell per wavefront, 4 per workgroup, APL angles per lane
 template <class T>
-void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, bool scatter, hipStream_t s) {
+bool launch_apply_wave(const Operator &op, const OpArgs<T> &a, const T *x, T *y, bool scatter, hipStream_t s) {
+  const Disc &D = *op.disc;
+  const int apl = (D.n_angles + 63) / 64;
+  if (apl > 4) return false;
+  const int64_t n_tiles = (D.n_cells + 3) / 4;
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+  bool launched = false;
+#define W_FD(DIM, APL)                                                                                         \
+  if (!launched && D.kind == DISC_FD && D.dim == DIM && apl == APL) {                                         \
+    if (scatter)                                                                                               \
+      hipLaunchKernelGGL((fd_apply_wave<T, DIM, APL, true>), dim3(grid), dim3(256), 0, s, x, y, a);            \
+    else                                                                                                       \
+      hipLaunchKernelGGL((fd_apply_wave<T, DIM, APL, false>), dim3(grid), dim3(256), 0, s, x, y, a);           \
+    launched = true;                                                                                           \
+  }
+#define W_DG(DIM, NB, NF, APL)                                                                                 \
+  if (!launched && D.kind == DISC_DG && D.dim == DIM && D.nb == NB && D.nf == NF && apl == APL) {             \
+    if (scatter)                                                                                               \
+      hipLaunchKernelGGL((dg_apply_wave<T, DIM, NB, NF, APL, true>), dim3(grid), dim3(256), 0, s, x, y, a);    \
+    else                                                                                                       \
+      hipLaunchKernelGGL((dg_apply_wave<T, DIM, NB, NF, APL, false>), dim3(grid), dim3(256), 0, s, x, y, a);   \
+    launched = true;                                                                                           \
+  }
+#define W_DG_ALL(APL) W_DG(2, 1, 3, APL) W_DG(2, 1, 4, APL) W_DG(3, 1, 4, APL) W_DG(3, 1, 6, APL) \
+  W_DG(2, 3, 3, APL) W_DG(2, 3, 4, APL) W_DG(3, 4, 4, APL) W_DG(3, 4, 6, APL)
+  W_FD(2, 1) W_FD(2, 2) W_FD(3, 1) W_FD(3, 2) W_FD(3, 3) W_FD(3, 4)
+  W_DG_ALL(1) W_DG_ALL(2)
+#undef W_DG_ALL
+#undef W_DG
+#undef W_FD
+  return launched;
+}
+
+template <class T>
+void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, bool scatter, hipStream_t s,
+                  bool adj = false) {
   const Disc &D = *op.disc;
   OpArgs<T> a = a0;
+  if (!adj && op.variant[sizeof(T) == 8 ? 0 : 1] == 1 && launch_apply_wave<T>(op, a, x, y, scatter, s)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
   Launch L = apply_launch<T>(D.n_angles, D.nb);
   a.cpb = L.cpb;
   a.red_lanes = L.lanes;
@@ -588,10 +818,15 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
     UBT_CHECK(L.lds <= 65536, UBT_ERR_SUP, "the DG tile needs %zu bytes of LDS", L.lds);
   }
   if (D.kind == DISC_FD) {
-#define FD_LAUNCH(DIM)                                                                            \
-  if (scatter)                                                                                    \
-    hipLaunchKernelGGL((fd_apply<T, DIM, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);  \
-  else                                                                                            \
+#define FD_LAUNCH(DIM)                                                                                       \
+  if (adj) {                                                                                                 \
+    if (scatter)                                                                                             \
+      hipLaunchKernelGGL((fd_apply<T, DIM, true, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);     \
+    else                                                                                                     \
+      hipLaunchKernelGGL((fd_apply<T, DIM, false, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);    \
+  } else if (scatter)                                                                                        \
+    hipLaunchKernelGGL((fd_apply<T, DIM, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);             \
+  else                                                                                                       \
     hipLaunchKernelGGL((fd_apply<T, DIM, false>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);
     if (D.dim == 2) {
       FD_LAUNCH(2)
@@ -600,13 +835,18 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
     }
 #undef FD_LAUNCH
   } else {
-#define DG_LAUNCH(DIM, NB, NF)                                                                        \
-  if (D.dim == DIM && D.nb == NB && D.nf == NF) {                                                     \
-    if (scatter)                                                                                      \
-      hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a); \
-    else                                                                                              \
-      hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, false>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a); \
-    launched = true;                                                                                  \
+#define DG_LAUNCH(DIM, NB, NF)                                                                                    \
+  if (D.dim == DIM && D.nb == NB && D.nf == NF) {                                                                 \
+    if (adj) {                                                                                                    \
+      if (scatter)                                                                                                \
+        hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, true, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a); \
+      else                                                                                                        \
+        hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, false, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a); \
+    } else if (scatter)                                                                                           \
+      hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, true>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((dg_apply<T, DIM, NB, NF, false>), dim3(grid), dim3(L.threads), L.lds, s, x, y, a);     \
+    launched = true;                                                                                              \
   }
     bool launched = false;
     DG_LAUNCH(2, 1, 3)
@@ -635,6 +875,8 @@ template <class T> OpArgs<T> make_args(const Operator &op, const PrecTables &t) 
   a.omega = t.omega.as<T>();
   a.w = t.w.as<T>();
   a.reflect = op.reflect.as<int>();
+  a.neg = op.neg.as<int>();
+  a.vol = t.vol.as<T>();
   a.sigt = t.sigt.as<T>() + (int64_t)op.group * D.n_cells;
   a.sigs = t.sigs.as<T>() + ((int64_t)op.group * op.n_groups + op.group) * D.n_cells;
   for (int d = 0; d < 3; d++) {
@@ -671,27 +913,32 @@ static void upload_prec(PrecTables &t, const Disc &D, bool fp32) {
     }
   }
   const int A = D.n_angles, Apad = (A + 3) / 4 * 4;
-  std::vector<double> wext((size_t)Apad * 16, 0.0);
+  std::vector<double> wext((size_t)Apad * 16, 0.0), wunit((size_t)Apad * 16, 0.0);
   for (int a = 0; a < A; a++) {
+    wunit[(size_t)a * 16] = 1.0;
     wext[(size_t)a * 16] = w[a];
     for (int d = 0; d < 3; d++) wext[(size_t)a * 16 + 1 + d] = w[a] * omega[3 * a + d];
   }
   if (fp32) {
     t.wext.upload_as<float>(wext);
+    t.wunit.upload_as<float>(wunit);
     t.omega.upload_as<float>(omega);
     t.w.upload_as<float>(w);
     t.face_s.upload_as<float>(face_s);
     t.face_own.upload_as<float>(D.face_own);
     t.face_up.upload_as<float>(D.face_up);
     t.grad.upload_as<float>(D.basis_grad);
+    t.vol.upload_as<float>(D.volume);
   } else {
     t.wext.upload(wext);
+    t.wunit.upload(wunit);
     t.omega.upload(omega);
     t.w.upload(w);
     t.face_s.upload(face_s);
     t.face_own.upload(D.face_own);
     t.face_up.upload(D.face_up);
     t.grad.upload(D.basis_grad);
+    t.vol.upload(D.volume);
   }
 }
 
@@ -710,6 +957,13 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
   for (int d = 0; d < D.dim; d++)
     for (int a = 0; a < D.n_angles; a++) refl[3 * a + d] = D.quad.reflect[d][a];
   op->reflect.upload(refl);
+  std::vector<int> neg(D.n_angles);
+  for (int a = 0; a < D.n_angles; a++) {
+    int b = a;
+    for (int d = 0; d < D.dim; d++) b = D.quad.reflect[d][b];
+    neg[a] = b;
+  }
+  op->neg.upload(neg);
   if (D.kind == DISC_DG) {
     op->face_nb.upload(D.face_nb);
     op->face_nA_raw.upload(D.face_nA);
@@ -739,6 +993,7 @@ void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s,
   op.t32.sigt.upload_as<float>(st);
   op.t32.sigs.upload_as<float>(ss);
   op.sigs_d.upload(ss);
+  const bool first = !op.xs_set;
   if (source) {
     std::vector<double> q(source, source + G * nc);
     op.source.upload(q);
@@ -746,6 +1001,7 @@ void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s,
     op.source.release();
   }
   op.xs_set = true;
+  if (first) op.tune();
 }
 
 // ---------------------------------------------------------------------------
@@ -759,6 +1015,63 @@ void Operator::mult(int prec, const void *x, void *y, bool scatter, hipStream_t 
     launch_apply<double>(*this, args<double>(), (const double *)x, (double *)y, scatter, s);
   else if (prec == UBT_FP32)
     launch_apply<float>(*this, args<float>(), (const float *)x, (float *)y, scatter, s);
+  else
+    fail(UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: time each apply variant (with scatter) on zero-filled scratch
+// vectors - 2 warm-up and 5 timed launches, HIP events - and keep the fastest
+// per precision. Runs when the cross sections are first set; the scratch is
+// freed before returning, so applies still allocate nothing.
+// ---------------------------------------------------------------------------
+void Operator::tune() {
+  const Disc &D = *disc;
+  const int64_t n = D.n_rows();
+  const bool wave_ok = (D.n_angles + 63) / 64 <= (D.kind == DISC_FD ? (D.dim == 3 ? 4 : 2) : 2);
+  hipStream_t s = nullptr;
+  UBT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  UBT_HIP(hipEventCreate(&e0));
+  UBT_HIP(hipEventCreate(&e1));
+  DevBuf xb, yb;
+  xb.alloc(sizeof(double) * n);
+  yb.alloc(sizeof(double) * n);
+  UBT_HIP(hipMemsetAsync(xb.ptr, 0, sizeof(double) * n, s));
+  for (int p = 0; p < 2; p++) {
+    float best = 1e30f;
+    int best_v = 0;
+    for (int v = 0; v < (wave_ok ? 2 : 1); v++) {
+      variant[p] = v;
+      for (int it = 0; it < 2; it++) mult(p == 0 ? UBT_FP64 : UBT_FP32, xb.ptr, yb.ptr, true, s);
+      UBT_HIP(hipEventRecord(e0, s));
+      for (int it = 0; it < 5; it++) mult(p == 0 ? UBT_FP64 : UBT_FP32, xb.ptr, yb.ptr, true, s);
+      UBT_HIP(hipEventRecord(e1, s));
+      UBT_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      UBT_HIP(hipEventElapsedTime(&ms, e0, e1));
+      tuned_us[p][v] = ms * 1000.f / 5.f;
+      if (tuned_us[p][v] < best) {
+        best = tuned_us[p][v];
+        best_v = v;
+      }
+    }
+    variant[p] = best_v;
+  }
+  UBT_HIP(hipStreamSynchronize(s));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+}
+
+void Operator::mult_transpose(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {
+  UBT_CHECK(xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections before applying the operator");
+  UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
+  UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the operator apply needs distinct x and y");
+  if (prec == UBT_FP64)
+    launch_apply<double>(*this, args<double>(), (const double *)x, (double *)y, scatter, s, true);
+  else if (prec == UBT_FP32)
+    launch_apply<float>(*this, args<float>(), (const float *)x, (float *)y, scatter, s, true);
   else
     fail(UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
 }
@@ -778,15 +1091,15 @@ void Operator::diagonal(int prec, void *d, hipStream_t s) const {
 namespace {
 template <class T, class OutT>
 void launch_moments(const Operator &op, const PrecTables &t, const T *psi, OutT *phi, int M, int stride,
-                    hipStream_t s) {
+                    hipStream_t s, bool unit = false) {
   const Disc &D = *op.disc;
   const int A = D.n_angles, Apad = (A + 3) / 4 * 4;
   const int64_t N = D.n_nodes();
   const unsigned grid = (unsigned)((N + 63) / 64);
   const size_t lds = (size_t)64 * Apad * sizeof(T);
   UBT_CHECK(lds <= 65536, UBT_ERR_SUP, "the moments tile of %d angles needs %zu bytes of LDS", A, lds);
-  hipLaunchKernelGGL((moments_mfma<T, OutT>), dim3(grid), dim3(256), lds, s, psi, phi, t.wext.as<T>(), N, A, Apad,
-                     M, stride);
+  hipLaunchKernelGGL((moments_mfma<T, OutT>), dim3(grid), dim3(256), lds, s, psi, phi,
+                     unit ? t.wunit.as<T>() : t.wext.as<T>(), N, A, Apad, M, stride);
   UBT_HIP(hipGetLastError());
 }
 
@@ -837,12 +1150,12 @@ void Operator::moments(int prec, const void *psi, void *phi, int M, hipStream_t 
     launch_moments<float, float>(*this, t32, (const float *)psi, (float *)phi, M, M, s);
 }
 
-void Operator::moments_f64(int prec, const void *psi, double *phi, hipStream_t s) const {
+void Operator::moments_f64(int prec, const void *psi, double *phi, hipStream_t s, bool unit) const {
   UBT_CHECK(psi && phi, UBT_ERR_ARG_NULL, "psi and phi are required");
   if (prec == UBT_FP64)
-    launch_moments<double, double>(*this, t64, (const double *)psi, phi, 1, 1, s);
+    launch_moments<double, double>(*this, t64, (const double *)psi, phi, 1, 1, s, unit);
   else
-    launch_moments<float, double>(*this, t32, (const float *)psi, phi, 1, 1, s);
+    launch_moments<float, double>(*this, t32, (const float *)psi, phi, 1, 1, s, unit);
 }
 
 void Operator::set_scalar_flux(int g, int prec, const void *psi, hipStream_t s) {
@@ -916,7 +1229,7 @@ void BlockInverse::setup(hipStream_t s) {
   group_ready = o.group;
 }
 
-void BlockInverse::apply(int prec, const void *x, void *y, hipStream_t s) const {
+void BlockInverse::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose) const {
   const Operator &o = *op;
   UBT_CHECK(group_ready == o.group, UBT_ERR_ARG_WRONGSTATE,
             "the block inverse was not set up for the operator's current group %d", o.group);
@@ -929,10 +1242,10 @@ void BlockInverse::apply(int prec, const void *x, void *y, hipStream_t s) const 
 #define BI(NB)                                                                                               \
   if (prec == UBT_FP64)                                                                                      \
     hipLaunchKernelGGL((blockinv_apply<double, NB>), dim3(grid), dim3(256), 0, s, (const double *)x, (double *)y, \
-                       o.args<double>(), fd);                                                                \
+                       o.args<double>(), fd, transpose ? 1 : 0);                                             \
   else                                                                                                       \
     hipLaunchKernelGGL((blockinv_apply<float, NB>), dim3(grid), dim3(256), 0, s, (const float *)x, (float *)y,   \
-                       o.args<float>(), fd);
+                       o.args<float>(), fd, transpose ? 1 : 0);
   switch (D.nb) {
     case 1: BI(1) break;
     case 3: BI(3) break;

@@ -11,10 +11,19 @@
 //   same block the element block inverse solves) and solve it. An edge left
 //   on a cycle (possible on a general mesh, not on a Delaunay or box mesh) is
 //   lagged: its inflow is taken as zero, so the stage stays a preconditioner.
+//   Which faces read an upwind trace is the schedule's own per-item mask, not
+//   a sign test in the kernel: a face tangent to the direction has flux 0 on
+//   the host but +-1 ulp on the device (FMA, FP32 rounding), and a device-side
+//   test would then read a neighbour the schedule has not ordered first.
 // * Structured FD: the levels are the diagonal hyperplanes i' + j' + k' = l of
 //   each octant, no table needed.
 // Reflective couplings are lagged the same way. Every kernel is a plain
 // launch per level on the caller's stream.
+//
+// The adjoint sweep solves L^T y = r with L^T = V P L P V^{-1} (P reverses
+// every angle, V the cell volumes): the forward schedule of angle b solves for
+// u = L^{-1} P V^{-1} r, whose angle b lives in y's angle -b, so the ADJ
+// kernels read and write the reversed angle's data and scale by the volumes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,23 +34,25 @@ namespace ubt {
 
 namespace {
 
-template <class T, int DIM, int NB, int NF>
+template <class T, int DIM, int NB, int NF, bool ADJ>
 __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, const int *__restrict__ cell,
-                               const int *__restrict__ angle, const unsigned char *__restrict__ lag, int64_t begin,
+                               const int *__restrict__ angle, const unsigned char *__restrict__ upwind, int64_t begin,
                                int64_t count) {
   const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (it >= count) return;
   const int64_t item = begin + it;
   const int64_t c = cell[item];
-  const int a = angle[item];
-  const unsigned char skip = lag[item];
+  const int b = angle[item];               // the direction swept
+  const int a = ADJ ? g.neg[b] : b;        // the angle whose data it carries
+  const unsigned char up = upwind[item];  // bit f: face f reads its neighbour's trace
   const int A = g.A;
-  T om[3] = {g.omega[3 * a], g.omega[3 * a + 1], DIM == 3 ? g.omega[3 * a + 2] : T(0)};
+  T om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
   T B[NB][NB], v[NB];
   const int64_t base = c * NB * A + a;
+  const T inv_vc = ADJ ? T(1) / g.vol[c] : T(1);
 #pragma unroll
   for (int i = 0; i < NB; i++) {
-    v[i] = r[base + (int64_t)i * A];
+    v[i] = inv_vc * r[base + (int64_t)i * A];
 #pragma unroll
     for (int j = 0; j < NB; j++) B[i][j] = T(0);
   }
@@ -51,26 +62,16 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
     T s = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) s += om[d] * g.face_s[3 * k + d];
-    if (s > T(0)) {
-      if (NB == 1) {
-        B[0][0] += s;
-      } else {
-        const T *F = g.face_own + k * NB * NB;
-#pragma unroll
-        for (int i = 0; i < NB; i++)
-#pragma unroll
-          for (int j = 0; j < NB; j++) B[i][j] += s * F[i * NB + j];
-      }
-    } else if (s < T(0)) {
+    if ((up >> f) & 1) {
       const int n = g.face_nb[k];
-      if (n < 0 || ((skip >> f) & 1)) continue;  // vacuum / reflective / lagged: no upwind trace
       const int64_t ub = (int64_t)n * NB * A + a;
+      const T inv_vn = ADJ ? T(1) / g.vol[n] : T(1);
       if (NB == 1) {
-        v[0] -= s * psi[ub];
+        v[0] -= s * inv_vn * psi[ub];
       } else {
         T pu[NB];
 #pragma unroll
-        for (int j = 0; j < NB; j++) pu[j] = psi[ub + (int64_t)j * A];
+        for (int j = 0; j < NB; j++) pu[j] = inv_vn * psi[ub + (int64_t)j * A];
         const T *F = g.face_up + k * NB * NB;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
@@ -79,6 +80,16 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
           for (int j = 0; j < NB; j++) t += F[i * NB + j] * pu[j];
           v[i] -= s * t;
         }
+      }
+    } else if (s > T(0)) {  // outflow; an inflow face without the bit is vacuum / reflective / lagged
+      if (NB == 1) {
+        B[0][0] += s;
+      } else {
+        const T *F = g.face_own + k * NB * NB;
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+          for (int j = 0; j < NB; j++) B[i][j] += s * F[i * NB + j];
       }
     }
   }
@@ -126,23 +137,26 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
       v[i] -= f * v[k];
     }
   }
+  const T vc = ADJ ? g.vol[c] : T(1);
 #pragma unroll
-  for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = v[i];
+  for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
 }
 
-// Structured: thread = (angle, j', k'); the level fixes i' = l - j' - k'
-template <class T, int DIM>
+// Structured: thread = (angle, j', k'); the level fixes i' = l - j' - k'.
+// Every cell has the same volume, so the adjoint only swaps the angle index.
+template <class T, int DIM, bool ADJ>
 __global__ void sweep_fd_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, int level) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = g.n[0], ny = g.n[1], nz = DIM == 3 ? g.n[2] : 1;
   const int64_t per_angle = (int64_t)ny * nz;
   if (t >= per_angle * g.A) return;
-  const int a = (int)(t / per_angle);
-  const int64_t jk = t - (int64_t)a * per_angle;
+  const int b = (int)(t / per_angle);
+  const int a = ADJ ? g.neg[b] : b;
+  const int64_t jk = t - (int64_t)b * per_angle;
   const int jp = (int)(jk % ny), kp = (int)(jk / ny);
   const int ip = level - jp - kp;
   if (ip < 0 || ip >= nx) return;
-  const T cs[3] = {g.omega[3 * a], g.omega[3 * a + 1], DIM == 3 ? g.omega[3 * a + 2] : T(0)};
+  const T cs[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
   int idx[3];
   idx[0] = cs[0] >= T(0) ? ip : nx - 1 - ip;
   idx[1] = cs[1] >= T(0) ? jp : ny - 1 - jp;
@@ -184,11 +198,11 @@ void Sweep::create(Operator *o) {
   // on a cycle get their remaining inflow edges lagged
   std::vector<std::vector<int64_t>> buckets;  // level -> item ids in the per-angle order
   std::vector<int> item_cell, item_angle;
-  std::vector<unsigned char> item_lag;
+  std::vector<unsigned char> item_up;  // per item: the faces that read an upwind trace
   std::vector<int64_t> item_level;
   item_cell.reserve((size_t)nc * A);
   item_angle.reserve((size_t)nc * A);
-  item_lag.reserve((size_t)nc * A);
+  item_up.reserve((size_t)nc * A);
   item_level.reserve((size_t)nc * A);
   std::vector<int> frontier, next;
   std::vector<char> done((size_t)nc);
@@ -246,9 +260,14 @@ void Sweep::create(Operator *o) {
         done[c] = 1;
         level[c] = lvl;
         n_done++;
+        unsigned char upw = 0;
+        for (int f = 0; f < nf; f++) {
+          const int64_t k = (int64_t)c * nf + f;
+          if (D.face_nb[k] >= 0 && flux(k) < 0.0 && !((lagmask[c] >> f) & 1)) upw |= (unsigned char)(1u << f);
+        }
         item_cell.push_back(c);
         item_angle.push_back(a);
-        item_lag.push_back(lagmask[c]);
+        item_up.push_back(upw);
         item_level.push_back(lvl);
       }
       for (int c : frontier)
@@ -279,15 +298,15 @@ void Sweep::create(Operator *o) {
     const int64_t p = pos[item_level[i]]++;
     oc[p] = item_cell[i];
     oa[p] = item_angle[i];
-    ol[p] = item_lag[i];
+    ol[p] = item_up[i];
   }
   n_levels = max_level;
   cells.upload(oc);
   angles.upload(oa);
-  lag.upload(ol);
+  upwind.upload(ol);
 }
 
-void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s) const {
+void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) const {
   const Operator &o = *op;
   UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections before sweeping");
   UBT_CHECK(r && psi, UBT_ERR_ARG_NULL, "r and psi are required");
@@ -296,34 +315,48 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s) const {
   if (D.kind == DISC_FD) {
     const int64_t per = (int64_t)D.n[1] * (D.dim == 3 ? D.n[2] : 1) * D.n_angles;
     const unsigned grid = (unsigned)((per + 255) / 256);
+#define SWF(T, DIM)                                                                                        \
+  if (adj)                                                                                                 \
+    hipLaunchKernelGGL((sweep_fd_level<T, DIM, true>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi,  \
+                       o.args<T>(), l);                                                                    \
+  else                                                                                                     \
+    hipLaunchKernelGGL((sweep_fd_level<T, DIM, false>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
+                       o.args<T>(), l);
     for (int l = 0; l < n_levels; l++) {
       if (prec == UBT_FP64) {
-        if (D.dim == 2)
-          hipLaunchKernelGGL((sweep_fd_level<double, 2>), dim3(grid), dim3(256), 0, s, (const double *)r, (double *)psi, o.args<double>(), l);
-        else
-          hipLaunchKernelGGL((sweep_fd_level<double, 3>), dim3(grid), dim3(256), 0, s, (const double *)r, (double *)psi, o.args<double>(), l);
+        if (D.dim == 2) {
+          SWF(double, 2)
+        } else {
+          SWF(double, 3)
+        }
       } else {
-        if (D.dim == 2)
-          hipLaunchKernelGGL((sweep_fd_level<float, 2>), dim3(grid), dim3(256), 0, s, (const float *)r, (float *)psi, o.args<float>(), l);
-        else
-          hipLaunchKernelGGL((sweep_fd_level<float, 3>), dim3(grid), dim3(256), 0, s, (const float *)r, (float *)psi, o.args<float>(), l);
+        if (D.dim == 2) {
+          SWF(float, 2)
+        } else {
+          SWF(float, 3)
+        }
       }
     }
+#undef SWF
     UBT_HIP(hipGetLastError());
     return;
   }
   const int *ci = cells.as<int>(), *ai = angles.as<int>();
-  const unsigned char *lg = lag.as<unsigned char>();
+  const unsigned char *lg = upwind.as<unsigned char>();
   for (int l = 0; l < n_levels; l++) {
     const int64_t b = level_off[l], n = level_off[l + 1] - level_off[l];
     if (n == 0) continue;
     const unsigned grid = (unsigned)((n + 255) / 256);
     bool launched = false;
-#define SW(T, DIM, NB, NF)                                                                                     \
-  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                 \
-    hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
-                       o.args<T>(), ci, ai, lg, b, n);                                                         \
-    launched = true;                                                                                           \
+#define SW(T, DIM, NB, NF)                                                                                           \
+  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                       \
+    if (adj)                                                                                                         \
+      hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF, true>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
+                         o.args<T>(), ci, ai, lg, b, n);                                                             \
+    else                                                                                                             \
+      hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF, false>), dim3(grid), dim3(256), 0, s, (const T *)r,          \
+                         (T *)psi, o.args<T>(), ci, ai, lg, b, n);                                                   \
+    launched = true;                                                                                                 \
   }
 #define SW_ALL(T) SW(T, 2, 1, 3) SW(T, 2, 1, 4) SW(T, 3, 1, 4) SW(T, 3, 1, 6) SW(T, 2, 3, 3) SW(T, 2, 3, 4) SW(T, 3, 4, 4) SW(T, 3, 4, 6)
     if (prec == UBT_FP64) {

@@ -39,8 +39,10 @@ struct DSA {
 
   void create(Operator *o);
   void set_group(int g, hipStream_t s);
-  // y = P D^{-1} R x
-  void apply(int prec, const void *x, void *y, hipStream_t s);
+  // y = P D^{-1} W R x, or (transpose) y = R^T W D^{-1} P^T x: the diffusion
+  // operators are symmetric, so the adjoint restricts with the unweighted angle
+  // sum / sum_weights and prolongs with w_a times the node weight
+  void apply(int prec, const void *x, void *y, hipStream_t s, bool transpose = false);
   // y = D_diff x on node vectors (FP64)
   void matvec(const double *x, double *y, hipStream_t s) const;
   // sol = D^{-1} rhs by the inner CG (x0 = 0)

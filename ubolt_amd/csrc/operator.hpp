@@ -14,6 +14,8 @@ template <class T> struct OpArgs {
   const T *omega;      // 3A
   const T *w;          // A
   const int *reflect;  // 3A (-1 past dim)
+  const int *neg;      // A: the ordinate with every cosine negated
+  const T *vol;        // cells: volumes (unstructured; the adjoint's neighbour ratio V_c / V_n)
   const T *sigt;       // current group, per cell
   const T *sigs;       // current group's within-group scatter, per cell
   // structured
@@ -30,7 +32,8 @@ template <class T> struct OpArgs {
 
 // Per precision copies of the geometry and cross sections
 struct PrecTables {
-  DevBuf omega, w, face_s, face_own, face_up, grad;
+  DevBuf omega, w, face_s, face_own, face_up, grad, vol;
+  DevBuf wunit;       // [Apad][16]: column 0 = 1, the unweighted angle sum (adjoint restriction)
   DevBuf wext;        // [Apad][16]: w_a {1, Omega_x, Omega_y, Omega_z}, the moments' B operand
   DevBuf sigt, sigs;  // G*cells, G*G*cells
 };
@@ -38,9 +41,13 @@ struct PrecTables {
 struct Operator {
   std::shared_ptr<Disc> disc;
   int n_groups = 1, group = 0, device = 0;
+  // apply kernel variant per precision [fp64, fp32]: 0 tile (LDS-staged, cpb
+  // cells per workgroup), 1 wavefront per cell; chosen by tune()
+  int variant[2] = {0, 0};
+  float tuned_us[2][2] = {{0, 0}, {0, 0}};
   bool xs_set = false;
   PrecTables t64, t32;
-  DevBuf reflect, face_nb;
+  DevBuf reflect, face_nb, neg;
   DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
   DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
@@ -54,13 +61,18 @@ struct Operator {
 
   template <class T> OpArgs<T> args() const;
   void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
+  // y = A^T x (scatter) or L^T x: the adjoint, reversed-direction kernel
+  void mult_transpose(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
   void diagonal(int prec, void *d, hipStream_t s) const;
   void moments(int prec, const void *psi, void *phi, int n_moments, hipStream_t s) const;
   // phi (FP64, one moment) from psi of either precision
-  void moments_f64(int prec, const void *psi, double *phi, hipStream_t s) const;
+  // unit: the unweighted sum over angles (the transposed prolongation)
+  void moments_f64(int prec, const void *psi, double *phi, hipStream_t s, bool unit = false) const;
   void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s);
   void scatter_source(int g_from, int g_to, int prec, void *b, hipStream_t s) const;
   void rhs(int g, int prec, void *b, hipStream_t s) const;
+  // time every applicable apply variant on scratch vectors and keep the fastest
+  void tune();
 };
 
 std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups);
@@ -70,7 +82,8 @@ struct BlockInverse {
   Operator *op = nullptr;
   int group_ready = -1;
   void setup(hipStream_t s);
-  void apply(int prec, const void *x, void *y, hipStream_t s) const;
+  // y = D^{-1} x, or the blocks of A^T (transpose) - the forward blocks at -Omega
+  void apply(int prec, const void *x, void *y, hipStream_t s, bool transpose = false) const;
 };
 
 }  // namespace ubt

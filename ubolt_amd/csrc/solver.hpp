@@ -40,7 +40,9 @@ struct Solver {
 
   void create(Operator *o, DSA *d, const SolverOptions &opts);
   void refresh(hipStream_t s);  // per group: block-inverse check
-  void solve(const double *b, double *x, hipStream_t s);
+  bool transpose = false;  // the current solve is with A^T (the adjoint)
+  // x = A^{-1} b, or (adj) x = A^{-T} b with every preconditioner stage transposed
+  void solve(const double *b, double *x, hipStream_t s, bool adj = false);
   template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
   template <class T> void apply_op(const T *x, T *y, bool scatter, hipStream_t s);
   template <class T> double fgmres(const T *b, T *x, double tol_abs, int max_it, int *its, hipStream_t s);

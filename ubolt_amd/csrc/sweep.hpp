@@ -11,9 +11,10 @@ struct Sweep {
   int n_levels = 0;
   int64_t n_lagged = 0;             // upwind edges lagged to break cycles
   std::vector<int64_t> level_off;   // DG: items of level l are [level_off[l], level_off[l+1])
-  DevBuf cells, angles, lag;        // DG: per item, level-ordered
+  DevBuf cells, angles, upwind;     // DG: per item, level-ordered; upwind = face bit mask
   void create(Operator *o);
-  void apply(int prec, const void *r, void *psi, hipStream_t s) const;
+  // psi = L^{-1} r, or (adj) psi = L^{-T} r
+  void apply(int prec, const void *r, void *psi, hipStream_t s, bool adj = false) const;
 };
 
 }  // namespace ubt
