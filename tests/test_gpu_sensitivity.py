@@ -79,7 +79,7 @@ def setup(request):
     J = response(phi, r, prob.disc.nb)
     J.backward()
     grads = [x.grad.detach().cpu().numpy() for x in xs]
-    print(f"[case] {request.param}: rows/group {prob.disc.n_rows}, J = {float(J):.10e}, solves "
+    print(f"[case] {request.param}: rows/group {prob.disc.n_rows}, J = {float(J.detach()):.10e}, solves "
           f"{[(w, g, res['iterations']) for w, g, res in model.last_results]}", flush=True)
     return request.param, prob, model, r, [x.detach().numpy().copy() for x in xs], grads
 
