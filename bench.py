@@ -15,6 +15,13 @@ oracle restatement of the reference apply, OpenMP on the host cores, a bounded
 sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+`--sensitivity` runs the adjoint workload instead (3D DG1 hexes 64^3, S6, 4
+downscatter groups, cross sections piecewise constant on 8^3-cell blocks):
+gradients of the volume integral of the group-summed scalar flux over
+[0.7, 0.9]^3 with respect to per-cell sigma_t, sigma_s and source by the
+adjoint solves, checked against central finite differences of the forward
+solve on 32 random cells; it prints its own JSON line.
 """
 import argparse
 import json
@@ -36,6 +43,9 @@ def parse():
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sensitivity", action="store_true", help="run the adjoint / sensitivity workload")
+    ap.add_argument("--sens-n", type=int, default=64, help="hexes per side of the sensitivity workload")
+    ap.add_argument("--sens-cells", type=int, default=32, help="finite-difference checks (one cell each)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="also run a plain 64 MiB device copy 5 times: the known-size reference the HBM "
                          "traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) are scaled by")
@@ -97,12 +107,91 @@ def solve_to_1e8(prob, mixed, reps, dsa_rtol, dsa_max_it):
     return float(np.median(times)), res, phi
 
 
+def sensitivity(args):
+    """Adjoint gradients of J = int_[0.7,0.9]^3 sum_g phi_g against central
+    differences of the forward solve. Cells are drawn at random among those
+    whose sigma_t gradient is at least 1e-2 of the largest (elsewhere J moves by
+    less than the solve tolerance under any affordable step); each check
+    perturbs one of sigma_t, sigma_s(g -> g), source in turn, by 2e-3 of its
+    value (source: of the group-0 source)."""
+    import torch
+    from ubolt_amd import problems as P
+    from ubolt_amd import sensitivity as S
+
+    n = args.sens_n
+    t0 = time.perf_counter()
+    prob = P.dg_box_3d(n, 6, order=1, G=4, block=max(n // 8, 1))
+    G, nc, nb = 4, prob.disc.n_cells, prob.disc.nb
+    model = S.TransportModel(prob.disc, G, rtol=1e-13, dsa_inner_rtol=1e-3, dsa_inner_max_it=100)
+    r = S.box_response_weights(prob.disc, [0.7] * 3, [0.9] * 3)
+    rt = torch.as_tensor(r, dtype=torch.float64, device="cuda")
+    base = [np.ascontiguousarray(a, dtype=np.float64) for a in (prob.sigma_t, prob.sigma_s, prob.source)]
+    setup_s = time.perf_counter() - t0
+
+    def J_of(phi):
+        return (phi[:, ::nb] * rt).sum()
+
+    with torch.no_grad():  # first solve builds the sweep schedule, DSA and tunes the apply
+        model.forward_solve(*(torch.as_tensor(a) for a in base))
+    xs = [torch.tensor(a, requires_grad=True) for a in base]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phi = S.scalar_flux(model, *xs)
+    J = J_of(phi)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    fwd_its = [res["iterations"] for _, _, res in model.last_results]
+    t0 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    adj_its = [res["iterations"] for w, _, res in model.last_results if w == "adjoint"]
+    g_st, g_ss, g_q = [x.grad.numpy() for x in xs]
+
+    rng = np.random.default_rng(2024)
+    strong = np.flatnonzero(np.abs(g_st).max(0) >= 1e-2 * np.abs(g_st).max())
+    cells = rng.choice(strong, size=min(args.sens_cells, strong.size), replace=False)
+    kinds = ["sigma_t", "sigma_s", "source"]
+    checks, t0 = [], time.perf_counter()
+    for i, c in enumerate(cells):
+        kind = kinds[i % 3]
+        g = int(rng.integers(G)) if kind != "source" else 0
+        idx = (g, int(c)) if kind != "sigma_s" else (g, g, int(c))
+        arr = {"sigma_t": 0, "sigma_s": 1, "source": 2}[kind]
+        h = 2e-3 * (abs(base[arr][idx]) if kind != "source" else float(np.abs(base[2][0]).max()))
+        Js = []
+        for sgn in (1, -1):
+            pert = [a.copy() for a in base]
+            pert[arr][idx] += sgn * h
+            with torch.no_grad():
+                _, ph = model.forward_solve(*(torch.as_tensor(a) for a in pert))
+                Js.append(float(J_of(ph)))
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float({"sigma_t": g_st, "sigma_s": g_ss, "source": g_q}[kind][idx])
+        checks.append({"param": kind, "index": list(idx), "adjoint": ad, "fd": fd,
+                       "rel_err": abs(ad - fd) / abs(fd)})
+        print(f"[fd {i + 1}/{len(cells)}] {kind}{idx}: adjoint {ad:+.8e} fd {fd:+.8e} rel {checks[-1]['rel_err']:.2e}",
+              file=sys.stderr, flush=True)
+    errs = np.array([c["rel_err"] for c in checks])
+    print(json.dumps({
+        "metric": "adjoint gradient vs central finite differences",
+        "workload": f"dg1_hex_{n}cube_s6_g4_blocks8", "n_dof_per_group": prob.disc.n_rows, "groups": G,
+        "response": "volume integral of the group-summed scalar flux over cells with centroid in [0.7,0.9]^3",
+        "J": float(J.detach()), "forward_s": round(fwd_s, 3), "adjoint_s": round(adj_s, 3),
+        "setup_s": round(setup_s, 3), "forward_iterations": fwd_its, "adjoint_iterations": adj_its,
+        "solve_rtol": 1e-13, "fd_checks": len(checks), "fd_seconds": round(time.perf_counter() - t0, 1),
+        "max_rel_err": float(errs.max()), "median_rel_err": float(np.median(errs)), "checks": checks}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
                                    "(the multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
+    if args.sensitivity:
+        sensitivity(args)
+        return
     import torch
     from ubolt_amd import problems as P
     from ubolt_amd import core
