@@ -32,6 +32,11 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# the FP32 2D triangle DG1 apply kernel of each autotuner variant
+APPLY_KERNELS = {0: "dg_apply<float,2,3,3,true,false>", 1: "dg_apply_wave<float,2,3,3,1,true>",
+                 2: "dg_apply_mc<float,2,3,3,2,true>", 3: "dg_apply_mc<float,2,3,3,4,true>",
+                 4: "dg_apply_pencil<float,2,3>", 5: "dg_apply_pencil<float,2,3>", 6: "dg_apply_pencil<float,2,3>",
+                 7: "dg_apply_q4<float,2,3,3,true>"}
 
 
 def parse():
@@ -251,8 +256,7 @@ def main():
                    "apply": "fp32 streaming+removal+scatter, fused"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("dg_apply_wave<float,2,3,3,1,true>" if op.tuning()["variant_fp32"] == 1
-                                else "dg_apply<float,2,3,3,true>"), "alg_bytes_per_launch": alg_bytes},
+                     "kernel": APPLY_KERNELS[op.tuning()["variant_fp32"]], "alg_bytes_per_launch": alg_bytes},
         "fp64_apply_gdofs": round(n / per64 / 1e9, 3),
         "apply_tuning": op.tuning(),
         "wall_s": round(wall, 3),

@@ -199,6 +199,11 @@ int ubt_op_set_xs(ubt_op *op, const double *sigma_t, const double *sigma_s, cons
  * TransportOperator::set_group + assemble (src/transport_operatork.kokkos.cxx:165-206). */
 int ubt_op_set_group(ubt_op *op, int g);
 int ubt_op_size(const ubt_op *op, int64_t *n_rows, int64_t *n_nodes, int *n_groups);
+/* Number of distinct DG geometry records the applies read (cells whose face
+ * normals, face matrices, basis gradients and volume agree to 1e-13 of each
+ * quantity's scale share one; a uniform box has a handful). 0 for grids. No
+ * reference counterpart: the reference stores a record per cell. */
+int ubt_op_geometry_classes(const ubt_op *op, int *n_classes);
 /* y = A x, A = streaming + removal - within-group scatter of the current
  * group. x and y must not alias. Replaces ShellMatMultApply
  * (src/transport_operatork.kokkos.cxx:49-61), the MATOP_MULT of the MatShell. */
@@ -222,13 +227,20 @@ int ubt_op_mult_unscattered_transpose(ubt_op *op, int prec, const void *x, void 
 int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
 /* The apply kernel variant per precision: 0 the LDS-staged tile kernel (whole
  * cells, every angle, per workgroup), 1 the wavefront-per-cell kernel
- * (per-cell data through the scalar cache, no barrier); -1 re-runs the
- * autotuner, which times both on scratch vectors when the cross sections are
- * first set and keeps the faster. No reference counterpart (the reference has
- * one Kokkos SpMV); exposed so the tests can check both kernels. */
+ * (per-cell data through the scalar cache, no barrier), 2 and 3 the
+ * multi-cell tile kernel (2 or 4 cells per thread, all loads before the first
+ * barrier; DG only), 4, 5 and 6 the pencil kernel (DG on a structured box:
+ * workgroups march 8, 16 or 32 cells along the last axis with the upwind
+ * traces in registers and LDS), 7 the four-angle tile kernel (a thread owns
+ * four consecutive angles of a cell: 16-byte loads, per-cell work shared;
+ * DG with A % 4 == 0); -1 re-runs the autotuner, which times every applicable
+ * kernel on scratch vectors when the cross sections are first set and keeps
+ * the fastest. A variant that does not apply falls back to 0. No reference
+ * counterpart (the reference has one Kokkos SpMV); exposed so the tests can
+ * check every kernel. */
 int ubt_op_set_variant(ubt_op *op, int prec, int variant);
-/* variant[2] (FP64, FP32) in use and us[4] the tuned times (FP64 tile, FP64
- * wave, FP32 tile, FP32 wave; 0 where not applicable). */
+/* variant[2] (FP64, FP32) in use and us[16] the tuned times (FP64 variants
+ * 0-7, then FP32 variants 0-7; 0 where not applicable). */
 int ubt_op_tuning(const ubt_op *op, int *variant, float *us);
 int ubt_op_destroy(ubt_op *op);
 

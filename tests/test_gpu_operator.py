@@ -74,7 +74,10 @@ def built(request):
     return request.param, prob, op
 
 
-VARIANTS = [0, 1]  # the tile kernel and the wavefront-per-cell kernel
+# tile, wavefront per cell, multi-cell tile (2 and 4 cells per thread), pencil on
+# a structured box (8, 16, 32 cells per workgroup), four-angle tile; a variant
+# that does not apply to a case runs the tile kernel
+VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -126,7 +129,7 @@ def test_tuner_reports(gpu, built):
     op.set_variant(core.FP64, -1)
     t = op.tuning()
     print(f"{name}: tuned {t}")
-    assert t["variant_fp64"] in (0, 1) and t["variant_fp32"] in (0, 1) and t["us"][0] > 0
+    assert 0 <= t["variant_fp64"] < 8 and 0 <= t["variant_fp32"] < 8 and t["us"][0] > 0
 
 
 def test_diagonal(gpu, built):
@@ -210,3 +213,18 @@ def test_rhs_multigroup(gpu):
     torch.cuda.synchronize()
     want = oracle.broadcast_add(d.nb, d.n_angles, ss[0, 2] / prob.quad.sum_weights, cache[0], np.zeros(d.n_rows))
     assert rel_max(to_host(b), want) <= 1e-13
+
+
+def test_geometry_classes(gpu):
+    """uniform boxes collapse to one geometry record per boundary pattern (a
+    cell's faces on each axis: low boundary, interior or high boundary, so at
+    most 3^dim); a jittered Delaunay mesh keeps one per cell. The applies above
+    agree with the oracle's per-cell geometry either way."""
+    box = P.dg_box_3d(6, 2, order=1).operator()
+    quad = P.dg_box_2d(9, 7, 2, order=1).operator()
+    dela = P.dg1_delaunay(n_side=8, sn=2, seed=1)
+    d_op = dela.operator()
+    print("classes: hex box", box.geometry_classes(), "quad box", quad.geometry_classes(), "delaunay",
+          d_op.geometry_classes(), "of", dela.disc.n_cells)
+    assert box.geometry_classes() <= 27 and quad.geometry_classes() <= 9
+    assert d_op.geometry_classes() == dela.disc.n_cells

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ubt_op_mult": [P, I, P, P, P],
     "ubt_op_mult_unscattered": [P, I, P, P, P],
     "ubt_op_mult_transpose": [P, I, P, P, P],
+    "ubt_op_geometry_classes": [P, P],
     "ubt_op_mult_unscattered_transpose": [P, I, P, P, P],
     "ubt_op_diagonal": [P, I, P, P],
     "ubt_op_set_variant": [P, I, I],

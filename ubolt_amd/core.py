@@ -168,12 +168,13 @@ class Operator(_Handle):
         self.group = g
 
     def set_variant(self, prec, variant):
-        """apply kernel: 0 tile, 1 wavefront per cell, -1 re-tune"""
+        """apply kernel: 0 tile, 1 wavefront per cell, 2/3 multi-cell tile (2/4 cells per thread),
+        4/5/6 pencil on a structured box (8/16/32 cells per workgroup), 7 four-angle tile, -1 re-tune"""
         call("ubt_op_set_variant", self.h, prec, variant)
 
     def tuning(self):
         v = (C.c_int * 2)()
-        us = (C.c_float * 4)()
+        us = (C.c_float * 16)()
         call("ubt_op_tuning", self.h, v, us)
         return {"variant_fp64": v[0], "variant_fp32": v[1], "us": [round(x, 2) for x in us]}
 
@@ -194,6 +195,11 @@ class Operator(_Handle):
     def mult_unscattered(self, x, y, stream=None):
         call("ubt_op_mult_unscattered", self.h, self._prec(x), dptr(x), dptr(y), stream_ptr(stream))
         return y
+
+    def geometry_classes(self):
+        n = C.c_int()
+        call("ubt_op_geometry_classes", self.h, C.byref(n))
+        return n.value
 
     def mult_transpose(self, x, y, stream=None):
         """y = A^T x, the adjoint operator"""

@@ -315,6 +315,13 @@ int ubt_op_mult_unscattered(ubt_op *op, int prec, const void *x, void *y, void *
     op->op->mult(prec, x, y, false, (hipStream_t)stream);
   });
 }
+int ubt_op_geometry_classes(const ubt_op *op, int *n_classes) {
+  return guard([&] {
+    NEED(op);
+    NEED(n_classes);
+    *n_classes = op->op->n_geo_classes;
+  });
+}
 int ubt_op_mult_transpose(ubt_op *op, int prec, const void *x, void *y, void *stream) {
   return guard([&] {
     NEED(op);
@@ -338,7 +345,8 @@ int ubt_op_set_variant(ubt_op *op, int prec, int variant) {
   return guard([&] {
     NEED(op);
     UBT_CHECK(prec == UBT_FP64 || prec == UBT_FP32, UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
-    UBT_CHECK(variant >= -1 && variant <= 1, UBT_ERR_ARG_OUTOFRANGE, "apply variant must be -1 (tune), 0 or 1");
+    UBT_CHECK(variant >= -1 && variant < ubt::kApplyVariants, UBT_ERR_ARG_OUTOFRANGE,
+              "apply variant must be -1 (tune) or 0 to %d", ubt::kApplyVariants - 1);
     UBT_CHECK(op->op->xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
     if (variant == -1)
       op->op->tune();
@@ -353,7 +361,7 @@ int ubt_op_tuning(const ubt_op *op, int *variant, float *us) {
       for (int p = 0; p < 2; p++) variant[p] = op->op->variant[p];
     if (us)
       for (int p = 0; p < 2; p++)
-        for (int v = 0; v < 2; v++) us[2 * p + v] = op->op->tuned_us[p][v];
+        for (int v = 0; v < ubt::kApplyVariants; v++) us[ubt::kApplyVariants * p + v] = op->op->tuned_us[p][v];
   });
 }
 int ubt_op_destroy(ubt_op *op) {

@@ -33,8 +33,9 @@ struct AsmArgs {
   double h[3], hr[3];
   int vac[3][2];
   // unstructured
-  const double *volume, *area, *dist, *nA, *slot_hr, *face_own, *face_up, *grad;
-  const int *face_nb;
+  const double *volume, *area, *dist, *nA, *slot_hr;
+  const double *face_own, *face_up, *grad;  // by geometry class
+  const int *face_nb, *gcls;
   double *own, *off;
 };
 
@@ -106,11 +107,12 @@ __global__ void asm_dg1(AsmArgs a) {
   const double st = a.sigt[c];
   const double dc = 1.0 / (3.0 * st), sa = st - a.sigs[c];
   const double V = a.volume[c];
+  const int64_t gc = a.gcls[c];
   double own[NB][NB];
   for (int i = 0; i < NB; i++)
     for (int j = 0; j < NB; j++) {
       double gd = 0.0;
-      for (int d = 0; d < 3; d++) gd += a.grad[(c * NB + i) * 3 + d] * a.grad[(c * NB + j) * 3 + d];
+      for (int d = 0; d < 3; d++) gd += a.grad[(gc * NB + i) * 3 + d] * a.grad[(gc * NB + j) * 3 + d];
       own[i][j] = V * (dc * gd + (i == j ? sa : 0.0));
     }
   for (int f = 0; f < a.nf; f++) {
@@ -121,15 +123,16 @@ __global__ void asm_dg1(AsmArgs a) {
     if (n < -1) continue;  // reflective: zero Neumann
     const double A = a.area[k];
     const double scale = V * A;
-    const double *Fo = a.face_own + k * NB * NB, *Fu = a.face_up + k * NB * NB;
+    const int64_t kg = gc * a.nf + f;
+    const double *Fo = a.face_own + kg * NB * NB, *Fu = a.face_up + kg * NB * NB;
     double nrm[3];
     for (int d = 0; d < 3; d++) nrm[d] = a.nA[3 * k + d] / A;
-    double gc[NB], gn[NB];
+    double gcv[NB], gn[NB];
     for (int j = 0; j < NB; j++) {
-      gc[j] = gn[j] = 0.0;
+      gcv[j] = gn[j] = 0.0;
       for (int d = 0; d < 3; d++) {
-        gc[j] += a.grad[(c * NB + j) * 3 + d] * nrm[d];
-        if (n >= 0) gn[j] += a.grad[((int64_t)n * NB + j) * 3 + d] * nrm[d];
+        gcv[j] += a.grad[(gc * NB + j) * 3 + d] * nrm[d];
+        if (n >= 0) gn[j] += a.grad[((int64_t)a.gcls[n] * NB + j) * 3 + d] * nrm[d];
       }
     }
     const double hc = 2.0 * a.dist[2 * k];
@@ -139,8 +142,8 @@ __global__ void asm_dg1(AsmArgs a) {
       for (int i = 0; i < NB; i++) {
         const double mi = scale * Fo[i * NB];
         for (int j = 0; j < NB; j++) {
-          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gc[j] * mi - 0.5 * dc * gc[i] * scale * Fo[j];
-          nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dc * gc[i] * scale * Fu[j];
+          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gcv[j] * mi - 0.5 * dc * gcv[i] * scale * Fo[j];
+          nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dc * gcv[i] * scale * Fu[j];
         }
       }
     } else {
@@ -148,7 +151,7 @@ __global__ void asm_dg1(AsmArgs a) {
       for (int i = 0; i < NB; i++) {
         const double mi = scale * Fo[i * NB];
         for (int j = 0; j < NB; j++)
-          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gc[j] * mi - 0.5 * dc * gc[i] * scale * Fo[j];
+          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gcv[j] * mi - 0.5 * dc * gcv[i] * scale * Fo[j];
       }
     }
   }
@@ -520,6 +523,7 @@ void DSA::set_group(int g, hipStream_t s) {
     a.face_up = o.t64.face_up.as<double>();
     a.grad = o.t64.grad.as<double>();
     a.face_nb = o.face_nb.as<int>();
+    a.gcls = o.gcls.as<int>();
     if (nb == 1)
       hipLaunchKernelGGL(asm_dg0, dim3(grid), dim3(256), 0, s, a);
     else if (nb == 3)

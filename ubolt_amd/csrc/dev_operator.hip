@@ -14,6 +14,8 @@
 // contiguous run for the angles that see it upwind.
 #include <hip/hip_runtime.h>
 
+#include <unordered_map>
+
 #include "operator.hpp"
 
 namespace ubt {
@@ -139,21 +141,24 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
   int *s_nb = reinterpret_cast<int *>(phi + (SCATTER ? cpb * NB : 0));
   const int tid = threadIdx.x, nt = blockDim.x;
   if (NB > 1) {
-    const T *go = g.face_own + c0 * NF * FM, *gu = g.face_up + c0 * NF * FM;
     for (int e = tid; e < ncell * NF * FM; e += nt) {
-      s_own[e] = go[e];
-      s_up[e] = gu[e];
+      const int lc = e / (NF * FM);
+      const int64_t ge = (int64_t)g.gcls[c0 + lc] * NF * FM + (e - lc * NF * FM);
+      s_own[e] = g.face_own[ge];
+      s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int lc = e / ((NB - 1) * DIM), r = e - lc * ((NB - 1) * DIM);
       const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
-      s_gr[e] = g.grad[((c0 + lc) * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + lc] * NB + i) * 3 + d];
     }
   }
   for (int e = tid; e < ncell * NF; e += nt) {
     s_nb[e] = g.face_nb[c0 * NF + e];
+    const int lc = e / NF;
+    const int64_t gk = (int64_t)g.gcls[c0 + lc] * NF + (e - lc * NF);
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[(c0 * NF + e) * 3 + d];
+    for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
   }
   for (int e = tid; e < ncell; e += nt) {
     s_st[e] = g.sigt[c0 + e];
@@ -220,7 +225,7 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
           ua = g.reflect[3 * a + (-2 - nbk)];
         }
         T pu[NB];
-        const T vr = (ADJ && nbk >= 0) ? g.vol[c] / g.vol[nbk] : T(1);
+        const T vr = (ADJ && nbk >= 0) ? g.vol[g.gcls[c]] / g.vol[g.gcls[nbk]] : T(1);
 #pragma unroll
         for (int j = 0; j < NB; j++) pu[j] = vr * x[ub + (int64_t)j * A + ua];
         if (NB == 1) {
@@ -261,6 +266,565 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
 }
 
 // ---------------------------------------------------------------------------
+// Multi-cell tile apply: the tile above with CPT cells per thread (thread =
+// angle a of cells r, r + R, ..., r + (CPT-1) R, R = blockDim / A), so one
+// workgroup carries CPT times the work through the same latency chain, and
+// every global load is issued before the first barrier: the own psi, and the
+// upwind traces chosen by the inflow test on the class tables (face_nb, the
+// geometry class and its normals are small and cache-resident). Only the face
+// matrices and gradients go through LDS. Same arithmetic as dg_apply.
+// ---------------------------------------------------------------------------
+template <class T, int DIM, int NB, int NF, int CPT, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int FM = NB * NB;
+  const int R = g.cpb, A = g.A, tcells = R * CPT;
+  const int64_t n_tiles = ((int64_t)g.n_cells + tcells - 1) / tcells;
+  const int64_t tile = xcd_tile(n_tiles);
+  if (tile >= n_tiles) return;
+  const int64_t c0 = tile * tcells;
+  const int ncell = (int)((int64_t)g.n_cells - c0 < tcells ? (int64_t)g.n_cells - c0 : tcells);
+  T *s_own = reinterpret_cast<T *>(smem);
+  T *s_up = s_own + (NB > 1 ? tcells * NF * FM : 0);
+  T *s_gr = s_up + (NB > 1 ? tcells * NF * FM : 0);
+  T *red = s_gr + (NB > 1 ? tcells * (NB - 1) * DIM : 0);
+  T *phi = red + (SCATTER ? tcells * NB * A : 0);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int r = tid / A, a = tid - r * A;
+  const bool lane = r < R;
+  const int ac = lane ? a : 0;
+  T om[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) om[d] = g.omega[3 * ac + d];
+
+  // 1. global loads: own psi, inflow test, upwind traces
+  T psi[CPT][NB], pu[CPT][NF][NB], sf[CPT][NF];
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    const int lc = r + k * R;
+    const bool live = lane && lc < ncell;
+    const int64_t c = c0 + (live ? lc : 0);
+    const int64_t base = c * NB * A + ac;
+#pragma unroll
+    for (int j = 0; j < NB; j++) psi[k][j] = live ? x[base + (int64_t)j * A] : T(0);
+    const int64_t gc = g.gcls[c];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const int nbk = g.face_nb[c * NF + f];
+      T sv = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) sv += om[d] * g.face_s[(gc * NF + f) * 3 + d];
+      sf[k][f] = sv;
+      const bool up = live && !(sv > T(0)) && nbk != -1;
+      int64_t ub = base;
+      if (up) ub = nbk >= 0 ? (int64_t)nbk * NB * A + ac : c * NB * A + g.reflect[3 * ac + (-2 - nbk)];
+#pragma unroll
+      for (int j = 0; j < NB; j++) pu[k][f][j] = up ? x[ub + (int64_t)j * A] : T(0);
+    }
+  }
+  // 2. stage the face matrices and gradients of the tile's cells
+  if (NB > 1) {
+    for (int e = tid; e < ncell * NF * FM; e += nt) {
+      const int lc = e / (NF * FM);
+      const int64_t ge = (int64_t)g.gcls[c0 + lc] * NF * FM + (e - lc * NF * FM);
+      s_own[e] = g.face_own[ge];
+      s_up[e] = g.face_up[ge];
+    }
+    for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
+      const int lc = e / ((NB - 1) * DIM), rr = e - lc * ((NB - 1) * DIM);
+      const int i = 1 + rr / DIM, d = rr - (i - 1) * DIM;
+      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + lc] * NB + i) * 3 + d];
+    }
+  }
+  if (SCATTER) {
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+      const int lc = r + k * R;
+      if (lane) {
+        const T wa = lc < ncell ? g.w[a] : T(0);
+#pragma unroll
+        for (int j = 0; j < NB; j++) red[(lc * NB + j) * A + a] = wa * psi[k][j];
+      }
+    }
+  }
+  __syncthreads();
+  if (SCATTER) {
+    seg_sum(red, tcells * NB, A, phi, g.red_lanes);
+    __syncthreads();
+  }
+  // 3. compute and store
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    const int lc = r + k * R;
+    if (!lane || lc >= ncell) continue;
+    const int64_t c = c0 + lc;
+    T acc[NB];
+    const T sig = g.sigt[c];
+#pragma unroll
+    for (int i = 0; i < NB; i++) acc[i] = sig * psi[k][i];
+    if (NB > 1) {
+#pragma unroll
+      for (int i = 1; i < NB; i++) {
+        const T *gr = s_gr + (lc * (NB - 1) + i - 1) * DIM;
+        T od = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
+        acc[i] -= od * psi[k][0];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const T sv = sf[k][f];
+      const bool out = sv > T(0);
+      if (NB == 1) {
+        acc[0] += sv * (out ? psi[k][0] : pu[k][f][0]);
+      } else {
+        const T *F = (out ? s_own : s_up) + (lc * NF + f) * FM;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+          T v = T(0);
+#pragma unroll
+          for (int j = 0; j < NB; j++) v += F[i * NB + j] * (out ? psi[k][j] : pu[k][f][j]);
+          acc[i] += sv * v;
+        }
+      }
+    }
+    if (SCATTER) {
+      const T ss = g.sigs[c] * g.inv_sum_w;
+#pragma unroll
+      for (int i = 0; i < NB; i++) acc[i] -= ss * phi[lc * NB + i];
+    }
+    const int64_t base = c * NB * A + a;
+#pragma unroll
+    for (int i = 0; i < NB; i++) y[base + (int64_t)i * A] = acc[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Four-angle tile apply: thread = (cell, four consecutive angles), so a
+// thread moves 16-byte vectors (FP32; two 16-byte halves in FP64) and every
+// per-cell cost - neighbour ids, normals, face matrices from LDS, address
+// arithmetic - is paid once per four angles. The upwind choice is per angle
+// and branch-free: both the outflow product F_own psi and the inflow product
+// F_up psi_nb are formed and the face flux picks one. Needs A % 4 == 0 (every
+// level-symmetric set). Same arithmetic as dg_apply, geometry staged the same
+// way.
+// ---------------------------------------------------------------------------
+template <class T> struct Vec4 { T v[4]; };
+template <class T> __device__ inline Vec4<T> ld4(const T *__restrict__ p) {
+  Vec4<T> r;
+  if constexpr (sizeof(T) == 4) {
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    r.v[0] = q.x; r.v[1] = q.y; r.v[2] = q.z; r.v[3] = q.w;
+  } else {
+    const double2 q0 = *reinterpret_cast<const double2 *>(p), q1 = *reinterpret_cast<const double2 *>(p + 2);
+    r.v[0] = q0.x; r.v[1] = q0.y; r.v[2] = q1.x; r.v[3] = q1.y;
+  }
+  return r;
+}
+template <class T> __device__ inline void st4(T *__restrict__ p, const T v[4]) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+  }
+}
+
+template <class T, int DIM, int NB, int NF, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_q4(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int FM = NB * NB;
+  const int cpb = g.cpb, A = g.A, TPC = A >> 2;
+  const int64_t n_tiles = ((int64_t)g.n_cells + cpb - 1) / cpb;
+  const int64_t tile = xcd_tile(n_tiles);
+  if (tile >= n_tiles) return;
+  const int64_t c0 = tile * cpb;
+  const int ncell = (int)((int64_t)g.n_cells - c0 < cpb ? (int64_t)g.n_cells - c0 : cpb);
+  T *s_own = reinterpret_cast<T *>(smem);
+  T *s_up = s_own + (NB > 1 ? cpb * NF * FM : 0);
+  T *s_fs = s_up + (NB > 1 ? cpb * NF * FM : 0);
+  T *s_gr = s_fs + cpb * NF * DIM;
+  T *s_st = s_gr + (NB > 1 ? cpb * (NB - 1) * DIM : 0);
+  T *s_ss = s_st + cpb;
+  T *red = s_ss + cpb;
+  T *phi = red + (SCATTER ? cpb * NB * TPC : 0);
+  int *s_nb = reinterpret_cast<int *>(phi + (SCATTER ? cpb * NB : 0));
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lc = tid / TPC, q = tid - lc * TPC, a0 = 4 * q;
+  const bool live = lc < ncell;
+  const int64_t c = c0 + (live ? lc : 0);
+  // own psi first: its loads overlap the staging below
+  Vec4<T> psi[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    if (live) psi[j] = ld4(x + (c * NB + j) * A + a0);
+    else
+#pragma unroll
+      for (int t = 0; t < 4; t++) psi[j].v[t] = T(0);
+  }
+  if (NB > 1) {
+    for (int e = tid; e < ncell * NF * FM; e += nt) {
+      const int l = e / (NF * FM);
+      const int64_t ge = (int64_t)g.gcls[c0 + l] * NF * FM + (e - l * NF * FM);
+      s_own[e] = g.face_own[ge];
+      s_up[e] = g.face_up[ge];
+    }
+    for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
+      const int l = e / ((NB - 1) * DIM), r = e - l * ((NB - 1) * DIM);
+      const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
+      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + l] * NB + i) * 3 + d];
+    }
+  }
+  for (int e = tid; e < ncell * NF; e += nt) {
+    s_nb[e] = g.face_nb[c0 * NF + e];
+    const int l = e / NF;
+    const int64_t gk = (int64_t)g.gcls[c0 + l] * NF + (e - l * NF);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
+  }
+  for (int e = tid; e < ncell; e += nt) {
+    s_st[e] = g.sigt[c0 + e];
+    s_ss[e] = g.sigs[c0 + e];
+  }
+  T om[4][DIM];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) om[t][d] = live ? g.omega[3 * (a0 + t) + d] : T(0);
+  if (SCATTER && live) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      T p = T(0);
+#pragma unroll
+      for (int t = 0; t < 4; t++) p += g.w[a0 + t] * psi[j].v[t];
+      red[(lc * NB + j) * TPC + q] = p;
+    }
+  }
+  __syncthreads();
+  T acc[NB][4];
+  if (live) {
+    const T sig = s_st[lc];
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc[j][t] = sig * psi[j].v[t];
+    if (NB > 1) {
+#pragma unroll
+      for (int i = 1; i < NB; i++) {
+        const T *gr = s_gr + (lc * (NB - 1) + i - 1) * DIM;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          T od = T(0);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) od += om[t][d] * gr[d];
+          acc[i][t] -= od * psi[0].v[t];
+        }
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const int kl = lc * NF + f;
+      const int nbk = s_nb[kl];
+      T sv[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        sv[t] = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sv[t] += om[t][d] * s_fs[kl * DIM + d];
+      }
+      Vec4<T> nbv[NB];
+      if (nbk >= 0) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) nbv[j] = ld4(x + ((int64_t)nbk * NB + j) * A + a0);
+      } else if (nbk <= -2) {  // reflective: this cell, mirrored angles
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int ra = g.reflect[3 * (a0 + t) + (-2 - nbk)];
+#pragma unroll
+          for (int j = 0; j < NB; j++) nbv[j].v[t] = x[(c * NB + j) * A + ra];
+        }
+      } else {  // vacuum: the inflow lives in the rhs
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int t = 0; t < 4; t++) nbv[j].v[t] = T(0);
+      }
+      if (NB == 1) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[0][t] += sv[t] * (sv[t] > T(0) ? psi[0].v[t] : nbv[0].v[t]);
+      } else {
+        const T *Fo = s_own + kl * FM, *Fu = s_up + kl * FM;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+          T fo[NB], fu[NB];
+#pragma unroll
+          for (int j = 0; j < NB; j++) {
+            fo[j] = Fo[i * NB + j];
+            fu[j] = Fu[i * NB + j];
+          }
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            T vo = T(0), vu = T(0);
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+              vo += fo[j] * psi[j].v[t];
+              vu += fu[j] * nbv[j].v[t];
+            }
+            acc[i][t] += sv[t] * (sv[t] > T(0) ? vo : vu);
+          }
+        }
+      }
+    }
+  }
+  if (SCATTER) {
+    // the moment of each (cell, basis): sum over the cell's TPC partials
+    for (int e = tid; e < ncell * NB; e += nt) {
+      T p = T(0);
+      for (int k = 0; k < TPC; k++) p += red[e * TPC + k];
+      phi[e] = p;
+    }
+    __syncthreads();
+    if (live) {
+      const T ss = s_ss[lc] * g.inv_sum_w;
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[j][t] -= ss * phi[lc * NB + j];
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) st4(y + (c * NB + j) * A + a0, acc[j]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pencil apply for DG on a structured box (cells numbered lexicographically,
+// x fastest, every cell's local faces in one axis/side order; detected on the
+// host). A workgroup owns W consecutive cells along x (thread = (cell, angle),
+// all NB rows) and marches along the last axis (y in 2D, z in 3D) over H
+// cells. The upwind trace along the march axis comes from the thread's own
+// register window (rows m-1, m, m+1 resident, m+2 in flight), along x from
+// the row buffer in LDS (segment ends: a halo load issued one step ahead),
+// and in 3D along y from one global load per step, also issued one step
+// ahead. Every geometry class is staged in LDS once. So a DoF costs one
+// global read (two in 3D) instead of one per face, and the loads of the next
+// step overlap the current step's barriers.
+// ---------------------------------------------------------------------------
+struct PencilArgs {
+  int n[3];        // cells per axis
+  int W, H;        // cells per segment along x, steps along the march axis
+  int fmap[3][2];  // local face of (axis, side)
+  int ncls;        // geometry classes (all staged)
+  int segs[3];     // workgroup grid: x segments, y rows (3D), march chunks
+};
+
+template <class T, int DIM, int NB>
+__global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g,
+                                                       PencilArgs pa, int scatter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NF = 2 * DIM, FM = NB * NB, MA = DIM - 1;  // march axis
+  const int A = g.A, W = pa.W, ncls = pa.ncls;
+  const int nx = pa.n[0], ny = pa.n[1], nm = pa.n[MA];
+  const int64_t nwg = (int64_t)pa.segs[0] * pa.segs[1] * pa.segs[2];
+  const int64_t wg = xcd_tile(nwg);
+  if (wg >= nwg) return;
+  const int sx = (int)(wg % pa.segs[0]);
+  const int64_t rest = wg / pa.segs[0];
+  const int jy = DIM == 3 ? (int)(rest % pa.segs[1]) : 0;  // the fixed y row in 3D
+  const int sm = (int)(DIM == 3 ? rest / pa.segs[1] : rest);
+  const int i0 = sx * W, m0 = sm * pa.H, m1 = min(nm, m0 + pa.H);
+  const int64_t sstride = DIM == 2 ? (int64_t)nx : (int64_t)nx * ny;  // march stride
+  // LDS: class tables, two row buffers, the weighted row, the moments
+  T *c_fs = reinterpret_cast<T *>(smem);
+  T *c_own = c_fs + ncls * NF * DIM;
+  T *c_up = c_own + (NB > 1 ? ncls * NF * FM : 0);
+  T *c_gr = c_up + (NB > 1 ? ncls * NF * FM : 0);
+  T *rowb = c_gr + (NB > 1 ? ncls * (NB - 1) * DIM : 0);
+  T *red = rowb + 2 * W * NB * A;
+  T *phi = red + W * NB * A;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < ncls * NF * DIM; e += nt) {
+    const int cf = e / DIM, d = e - cf * DIM;
+    c_fs[e] = g.face_s[(int64_t)cf * 3 + d];
+  }
+  if (NB > 1) {
+    for (int e = tid; e < ncls * NF * FM; e += nt) {
+      c_own[e] = g.face_own[e];
+      c_up[e] = g.face_up[e];
+    }
+    for (int e = tid; e < ncls * (NB - 1) * DIM; e += nt) {
+      const int cl = e / ((NB - 1) * DIM), r = e - cl * ((NB - 1) * DIM);
+      const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
+      c_gr[e] = g.grad[((int64_t)cl * NB + i) * 3 + d];
+    }
+  }
+  const int lc = tid / A, a = tid - lc * A;
+  const bool lane = lc < W;
+  const int ac = lane ? a : 0;
+  const int i = i0 + lc;
+  const bool col = lane && i < nx;
+  T om[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) om[d] = g.omega[3 * ac + d];
+  const T wa = col ? g.w[ac] : T(0);
+  // the x halo this lane reads from global (segment end, inflow side), else -1
+  const bool hx_lo = col && lc == 0 && i > 0 && om[0] > T(0);
+  const bool hx_hi = col && (lc == W - 1 || i == nx - 1) && i + 1 < nx && om[0] < T(0);
+  const int64_t colbase = (int64_t)(DIM == 3 ? (int64_t)jy * nx : 0) + i;  // cell at march index 0
+  auto cell_at = [&](int m) { return colbase + (int64_t)m * sstride; };
+  auto load_row = [&](int m, T v[NB]) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = T(0);
+    if (col && m >= 0 && m < nm) {
+      const int64_t b = cell_at(m) * NB * A + ac;
+#pragma unroll
+      for (int k = 0; k < NB; k++) v[k] = x[b + (int64_t)k * A];
+    }
+  };
+  // 3D: the y-upwind neighbour (one side per lane by the sign of Omega_y)
+  const int ydir = DIM == 3 ? (om[1] > T(0) ? -1 : 1) : 0;
+  const bool yin = DIM == 3 && col && jy + ydir >= 0 && jy + ydir < ny;
+  auto load_y = [&](int m, T v[NB]) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = T(0);
+    if (yin && m < nm) {
+      const int64_t b = (cell_at(m) + (int64_t)ydir * nx) * NB * A + ac;
+#pragma unroll
+      for (int k = 0; k < NB; k++) v[k] = x[b + (int64_t)k * A];
+    }
+  };
+  auto load_hx = [&](int m, T v[NB]) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = T(0);
+    if ((hx_lo || hx_hi) && m < nm) {
+      const int64_t b = (cell_at(m) + (hx_lo ? -1 : 1)) * NB * A + ac;
+#pragma unroll
+      for (int k = 0; k < NB; k++) v[k] = x[b + (int64_t)k * A];
+    }
+  };
+  T pm[NB], p0[NB], pp[NB], pn[NB], hy[NB], hyn[NB], hx[NB], hxn[NB];
+  load_row(m0 - 1, pm);
+  load_row(m0, p0);
+  load_row(m0 + 1, pp);
+  load_y(m0, hy);
+  load_hx(m0, hx);
+  int gc = col && m0 < nm ? g.gcls[cell_at(m0)] : 0;
+  T st = col && m0 < nm ? g.sigt[cell_at(m0)] : T(0), ssg = col && m0 < nm ? g.sigs[cell_at(m0)] : T(0);
+  __syncthreads();  // class tables staged
+  for (int m = m0, par = 0; m < m1; m++, par ^= 1) {
+    // next step's loads first
+    load_row(m + 2, pn);
+    load_y(m + 1, hyn);
+    load_hx(m + 1, hxn);
+    const bool nxt = col && m + 1 < nm;
+    const int gcn = nxt ? g.gcls[cell_at(m + 1)] : 0;
+    const T stn = nxt ? g.sigt[cell_at(m + 1)] : T(0), ssn = nxt ? g.sigs[cell_at(m + 1)] : T(0);
+    const int64_t c = cell_at(m);
+    int fnb[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) fnb[f] = col ? g.face_nb[c * NF + f] : -1;
+    T *rb = rowb + par * W * NB * A;
+    if (lane) {
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        rb[(lc * NB + k) * A + a] = p0[k];
+        red[(lc * NB + k) * A + a] = wa * p0[k];
+      }
+    }
+    __syncthreads();
+    if (scatter) seg_sum(red, W * NB, A, phi, g.red_lanes);
+    __syncthreads();
+    if (col) {
+      T acc[NB];
+#pragma unroll
+      for (int k = 0; k < NB; k++) acc[k] = st * p0[k];
+      if (NB > 1) {
+#pragma unroll
+        for (int k = 1; k < NB; k++) {
+          const T *gr = c_gr + (gc * (NB - 1) + k - 1) * DIM;
+          T od = T(0);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
+          acc[k] -= od * p0[0];
+        }
+      }
+#pragma unroll
+      for (int ax = 0; ax < DIM; ax++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          const int f = pa.fmap[ax][side];
+          const T *fs = c_fs + (gc * NF + f) * DIM;
+          T sv = T(0);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) sv += om[d] * fs[d];
+          T pu[NB];
+          bool have = true;
+          if (sv > T(0)) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) pu[k] = p0[k];
+          } else if (fnb[f] == -1) {
+            have = false;  // vacuum: the inflow lives in the rhs
+          } else if (fnb[f] <= -2) {  // reflective: this cell, mirrored angle (row buffer)
+            const int ra = g.reflect[3 * a + (-2 - fnb[f])];
+#pragma unroll
+            for (int k = 0; k < NB; k++) pu[k] = rb[(lc * NB + k) * A + ra];
+          } else if (ax == MA) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) pu[k] = side == 0 ? pm[k] : pp[k];
+          } else if (ax == 0) {
+            const int l2 = side == 0 ? lc - 1 : lc + 1;
+            if (l2 >= 0 && l2 < W) {
+#pragma unroll
+              for (int k = 0; k < NB; k++) pu[k] = rb[(l2 * NB + k) * A + a];
+            } else {
+#pragma unroll
+              for (int k = 0; k < NB; k++) pu[k] = hx[k];
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < NB; k++) pu[k] = hy[k];
+          }
+          if (!have) continue;
+          if (NB == 1) {
+            acc[0] += sv * pu[0];
+          } else {
+            const T *F = (sv > T(0) ? c_own : c_up) + (gc * NF + f) * FM;
+#pragma unroll
+            for (int r = 0; r < NB; r++) {
+              T v = T(0);
+#pragma unroll
+              for (int k = 0; k < NB; k++) v += F[r * NB + k] * pu[k];
+              acc[r] += sv * v;
+            }
+          }
+        }
+      }
+      if (scatter) {
+        const T ss = ssg * g.inv_sum_w;
+#pragma unroll
+        for (int k = 0; k < NB; k++) acc[k] -= ss * phi[lc * NB + k];
+      }
+      const int64_t b = c * NB * A + a;
+#pragma unroll
+      for (int k = 0; k < NB; k++) y[b + (int64_t)k * A] = acc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      pm[k] = p0[k];
+      p0[k] = pp[k];
+      pp[k] = pn[k];
+      hy[k] = hyn[k];
+      hx[k] = hxn[k];
+    }
+    gc = gcn;
+    st = stn;
+    ssg = ssn;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Wavefront-per-cell applies. A wavefront owns one cell and all its angles
 // (lane l takes angles l, l + 64, ...: APL per lane), so everything per cell -
 // cross sections, face normals and matrices, neighbour ids - sits at a
@@ -289,6 +853,7 @@ __global__ void __launch_bounds__(256) dg_apply_wave(const T *__restrict__ x, T 
   T psi[APL][NB], acc[APL][NB], om[APL][3];
   bool on[APL];
   const T sig = g.sigt[c];
+  const int gc = g.gcls[c];
 #pragma unroll
   for (int q = 0; q < APL; q++) {
     const int a = lane + 64 * q;
@@ -304,7 +869,7 @@ __global__ void __launch_bounds__(256) dg_apply_wave(const T *__restrict__ x, T 
   if (NB > 1) {
 #pragma unroll
     for (int i = 1; i < NB; i++) {
-      const T *gr = g.grad + (c * NB + i) * 3;
+      const T *gr = g.grad + ((int64_t)gc * NB + i) * 3;
 #pragma unroll
       for (int q = 0; q < APL; q++) {
         T od = T(0);
@@ -316,10 +881,10 @@ __global__ void __launch_bounds__(256) dg_apply_wave(const T *__restrict__ x, T 
   }
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    const int64_t k = c * NF + f;
+    const int64_t k = c * NF + f, kg = (int64_t)gc * NF + f;
     const int nbk = g.face_nb[k];
-    const T *sv = g.face_s + 3 * k;
-    const T *Fo = g.face_own + k * FM, *Fu = g.face_up + k * FM;
+    const T *sv = g.face_s + 3 * kg;
+    const T *Fo = g.face_own + kg * FM, *Fu = g.face_up + kg * FM;
     int64_t ub = 0;
     int axis = 0;
     if (nbk >= 0) ub = (int64_t)nbk * NB * A;
@@ -460,10 +1025,10 @@ __global__ void diag_kernel(T *__restrict__ d, OpArgs<T> g, int is_fd) {
     for (int dd = 0; dd < g.dim; dd++) v += fabs(g.omega[3 * a + dd]) * g.invh[dd];
   } else {
     for (int f = 0; f < g.nf; f++) {
-      const int64_t k = c * g.nf + f;
+      const int64_t kg = (int64_t)g.gcls[c] * g.nf + f;
       T s = T(0);
-      for (int dd = 0; dd < g.dim; dd++) s += g.omega[3 * a + dd] * g.face_s[3 * k + dd];
-      if (s > T(0)) v += g.nb == 1 ? s : s * g.face_own[(k * g.nb + i) * g.nb + i];
+      for (int dd = 0; dd < g.dim; dd++) s += g.omega[3 * a + dd] * g.face_s[3 * kg + dd];
+      if (s > T(0)) v += g.nb == 1 ? s : s * g.face_own[(kg * g.nb + i) * g.nb + i];
     }
   }
   d[r] = v + g.sigt[c];
@@ -633,8 +1198,9 @@ __device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_
     return true;
   }
   const T ox = dir * g.omega[3 * a], oy = dir * g.omega[3 * a + 1], oz = dir * g.omega[3 * a + 2];
+  const int64_t gc = g.gcls[c];
   for (int f = 0; f < g.nf; f++) {
-    const int64_t k = c * g.nf + f;
+    const int64_t k = gc * g.nf + f;
     const T s = ox * g.face_s[3 * k] + oy * g.face_s[3 * k + 1] + oz * g.face_s[3 * k + 2];
     if (s > T(0)) {
       if (NB == 1) {
@@ -651,7 +1217,7 @@ __device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_
   if (NB > 1) {
 #pragma unroll
     for (int i = 1; i < NB; i++) {
-      const T *gr = g.grad + (c * NB + i) * 3;
+      const T *gr = g.grad + (gc * NB + i) * 3;
       B[i][0] -= ox * gr[0] + oy * gr[1] + oz * gr[2];
     }
   }
@@ -744,17 +1310,22 @@ int pow2_floor(int v) {
   return p;
 }
 
-template <class T> Launch apply_launch(int A, int nb) {
+// geo_bytes: LDS per cell for staged geometry (DG); the cells per workgroup
+// shrink until the tile fits the 64 KB a workgroup may allocate
+template <class T> Launch apply_launch(int A, int nb, size_t geo_bytes = 0) {
   Launch L;
   L.threads = A <= 256 ? 256 : ((A + 63) / 64) * 64;
   UBT_CHECK(L.threads <= 1024, UBT_ERR_SUP, "%d angles per cell exceed one workgroup", A);
   L.cpb = L.threads / A;
+  auto lds_for = [&](int cpb) { return (size_t)(cpb * nb * A + cpb * nb) * sizeof(T) + (size_t)cpb * geo_bytes; };
+  while (L.cpb > 1 && lds_for(L.cpb) > 65536) L.cpb--;
+  L.threads = std::max(64, ((L.cpb * A + 63) / 64) * 64);
   const int nseg = L.cpb * nb;
   int S = pow2_floor(std::max(1, L.threads / nseg));
   S = std::min(S, 64);
   S = std::min(S, pow2_floor(A));
   L.lanes = std::max(1, S);
-  L.lds = (size_t)(L.cpb * nb * A + L.cpb * nb) * sizeof(T);
+  L.lds = lds_for(L.cpb);
   UBT_CHECK(L.lds <= 65536, UBT_ERR_SUP, "%d angles x %d basis need %zu bytes of LDS per workgroup", A, nb, L.lds);
   return L;
 }
@@ -794,29 +1365,160 @@ bool launch_apply_wave(const Operator &op, const OpArgs<T> &a, const T *x, T *y,
   return launched;
 }
 
+// The four-angle tile kernel; false where it does not apply (FD, A not a
+// multiple of 4, or a tile that overflows the LDS)
+template <class T>
+bool launch_apply_q4(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s,
+                     bool dry = false) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
+  if (D.kind != DISC_DG || A % 4 != 0 || A / 4 > 256) return false;
+  const int TPC = A / 4;
+  int cpb = 256 / TPC;
+  auto lds_for = [&](int cp) {
+    const size_t reals = (size_t)cp * ((nb > 1 ? 2 * nf * nb * nb + (nb - 1) * dim : 0) + nf * dim + 2) +
+                         (size_t)cp * nb * TPC + (size_t)cp * nb;
+    return reals * sizeof(T) + sizeof(int) * (size_t)cp * nf;
+  };
+  while (cpb > 1 && lds_for(cpb) > 65536) cpb--;
+  if (lds_for(cpb) > 65536) return false;
+  if (dry) return true;
+  a.cpb = cpb;
+  const int threads = ((cpb * TPC + 63) / 64) * 64;
+  const int64_t n_tiles = (D.n_cells + cpb - 1) / cpb;
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+  const size_t lds = lds_for(cpb);
+  bool launched = false;
+#define Q4(DIM, NB, NF)                                                                                           \
+  if (!launched && dim == DIM && nb == NB && nf == NF) {                                                          \
+    if (scatter)                                                                                                  \
+      hipLaunchKernelGGL((dg_apply_q4<T, DIM, NB, NF, true>), dim3(grid), dim3(threads), lds, s, x, y, a);         \
+    else                                                                                                          \
+      hipLaunchKernelGGL((dg_apply_q4<T, DIM, NB, NF, false>), dim3(grid), dim3(threads), lds, s, x, y, a);        \
+    launched = true;                                                                                              \
+  }
+  Q4(2, 1, 3) Q4(2, 1, 4) Q4(3, 1, 4) Q4(3, 1, 6) Q4(2, 3, 3) Q4(2, 3, 4) Q4(3, 4, 4) Q4(3, 4, 6)
+#undef Q4
+  return launched;
+}
+
+// The pencil kernel on a structured box marching H cells per workgroup; false
+// where it does not apply (not a box, more than 256 angles, too many geometry
+// classes to stage, or a row that overflows the LDS)
+template <class T>
+bool launch_apply_pencil(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int H,
+                         bool dry = false) {
+  const Disc &D = *op.disc;
+  if (!op.box || D.n_angles > 256 || op.n_geo_classes > kPencilMaxClasses) return false;
+  const int A = D.n_angles, nb = D.nb, dim = D.dim, nf = D.nf, ncls = op.n_geo_classes, FM = nb * nb;
+  PencilArgs pa;
+  pa.W = 256 / A;
+  pa.H = H;
+  pa.ncls = ncls;
+  for (int d = 0; d < 3; d++) {
+    pa.n[d] = op.box_n[d];
+    pa.fmap[d][0] = op.box_fmap[d][0];
+    pa.fmap[d][1] = op.box_fmap[d][1];
+  }
+  const size_t reals = (size_t)ncls * nf * dim + (nb > 1 ? (size_t)2 * ncls * nf * FM + (size_t)ncls * (nb - 1) * dim : 0) +
+                       (size_t)3 * pa.W * nb * A + (size_t)pa.W * nb;
+  const size_t lds = reals * sizeof(T);
+  if (lds > 65536) return false;
+  if (dry) return true;
+  pa.segs[0] = (pa.n[0] + pa.W - 1) / pa.W;
+  pa.segs[1] = dim == 3 ? pa.n[1] : 1;
+  pa.segs[2] = (pa.n[dim - 1] + H - 1) / H;
+  const int64_t nwg = (int64_t)pa.segs[0] * pa.segs[1] * pa.segs[2];
+  const unsigned grid = (unsigned)(8 * ((nwg + 7) / 8));
+  const int threads = ((pa.W * A + 63) / 64) * 64;
+  int S = pow2_floor(std::max(1, threads / (pa.W * nb)));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  a.red_lanes = std::max(1, S);
+  const int sc = scatter ? 1 : 0;
+  if (dim == 2 && nb == 1)
+    hipLaunchKernelGGL((dg_apply_pencil<T, 2, 1>), dim3(grid), dim3(threads), lds, s, x, y, a, pa, sc);
+  else if (dim == 2 && nb == 3)
+    hipLaunchKernelGGL((dg_apply_pencil<T, 2, 3>), dim3(grid), dim3(threads), lds, s, x, y, a, pa, sc);
+  else if (dim == 3 && nb == 1)
+    hipLaunchKernelGGL((dg_apply_pencil<T, 3, 1>), dim3(grid), dim3(threads), lds, s, x, y, a, pa, sc);
+  else if (dim == 3 && nb == 4)
+    hipLaunchKernelGGL((dg_apply_pencil<T, 3, 4>), dim3(grid), dim3(threads), lds, s, x, y, a, pa, sc);
+  else
+    return false;
+  return true;
+}
+
+// The multi-cell tile kernel with CPT cells per thread; false where it does
+// not apply (FD, more than 256 angles, or a tile that overflows the LDS)
+template <class T>
+bool launch_apply_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int cpt) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
+  if (D.kind != DISC_DG || A > 256) return false;
+  const int R = 256 / A, tcells = R * cpt;
+  const size_t reals = (size_t)tcells * ((nb > 1 ? 2 * nf * nb * nb + (nb - 1) * dim : 0) + nb * A + nb);
+  const size_t lds = reals * sizeof(T);
+  if (lds > 65536) return false;
+  const int threads = ((R * A + 63) / 64) * 64;
+  int S = pow2_floor(std::max(1, threads / (tcells * nb)));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  a.cpb = R;
+  a.red_lanes = std::max(1, S);
+  const int64_t n_tiles = (D.n_cells + tcells - 1) / tcells;
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+  bool launched = false;
+#define MC(DIM, NB, NF, CPT)                                                                                      \
+  if (!launched && dim == DIM && nb == NB && nf == NF && cpt == CPT) {                                          \
+    if (scatter)                                                                                                \
+      hipLaunchKernelGGL((dg_apply_mc<T, DIM, NB, NF, CPT, true>), dim3(grid), dim3(threads), lds, s, x, y, a);  \
+    else                                                                                                        \
+      hipLaunchKernelGGL((dg_apply_mc<T, DIM, NB, NF, CPT, false>), dim3(grid), dim3(threads), lds, s, x, y, a); \
+    launched = true;                                                                                            \
+  }
+#define MC_ALL(CPT) MC(2, 1, 3, CPT) MC(2, 1, 4, CPT) MC(3, 1, 4, CPT) MC(3, 1, 6, CPT) \
+                    MC(2, 3, 3, CPT) MC(2, 3, 4, CPT) MC(3, 4, 4, CPT) MC(3, 4, 6, CPT)
+  MC_ALL(2)
+  MC_ALL(4)
+#undef MC_ALL
+#undef MC
+  return launched;
+}
+
 template <class T>
 void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, bool scatter, hipStream_t s,
                   bool adj = false) {
   const Disc &D = *op.disc;
   OpArgs<T> a = a0;
-  if (!adj && op.variant[sizeof(T) == 8 ? 0 : 1] == 1 && launch_apply_wave<T>(op, a, x, y, scatter, s)) {
+  const int var = op.variant[sizeof(T) == 8 ? 0 : 1];
+  if (!adj && var == 7 && launch_apply_q4<T>(op, a, x, y, scatter, s)) {
     UBT_HIP(hipGetLastError());
     return;
   }
-  Launch L = apply_launch<T>(D.n_angles, D.nb);
+  if (!adj && var >= 4 && var <= 6 && launch_apply_pencil<T>(op, a, x, y, scatter, s, 8 << (var - 4))) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  if (!adj && (var == 2 || var == 3) && launch_apply_mc<T>(op, a, x, y, scatter, s, var == 2 ? 2 : 4)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  if (!adj && var == 1 && launch_apply_wave<T>(op, a, x, y, scatter, s)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  size_t geo = 0;
+  if (D.kind == DISC_DG) {
+    // the staged geometry in LDS, per cell
+    const int nb = D.nb, nf = D.nf, dim = D.dim;
+    geo = (size_t)((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) * sizeof(T) +
+          sizeof(int) * (size_t)nf;
+  }
+  Launch L = apply_launch<T>(D.n_angles, D.nb, geo);
   a.cpb = L.cpb;
   a.red_lanes = L.lanes;
   const int64_t n_tiles = (D.n_cells + L.cpb - 1) / L.cpb;
   // XCD-contiguous tiles: 8 * ceil(tiles / 8) workgroups
   const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
-  if (D.kind == DISC_DG) {
-    // the staged geometry in LDS
-    const int nb = D.nb, nf = D.nf, dim = D.dim;
-    const size_t reals = (size_t)L.cpb * ((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) +
-                         (size_t)L.cpb * nb * D.n_angles + (size_t)L.cpb * nb;
-    L.lds = reals * sizeof(T) + sizeof(int) * (size_t)L.cpb * nf;
-    UBT_CHECK(L.lds <= 65536, UBT_ERR_SUP, "the DG tile needs %zu bytes of LDS", L.lds);
-  }
   if (D.kind == DISC_FD) {
 #define FD_LAUNCH(DIM)                                                                                       \
   if (adj) {                                                                                                 \
@@ -877,6 +1579,7 @@ template <class T> OpArgs<T> make_args(const Operator &op, const PrecTables &t) 
   a.reflect = op.reflect.as<int>();
   a.neg = op.neg.as<int>();
   a.vol = t.vol.as<T>();
+  a.gcls = op.gcls.as<int>();
   a.sigt = t.sigt.as<T>() + (int64_t)op.group * D.n_cells;
   a.sigs = t.sigs.as<T>() + ((int64_t)op.group * op.n_groups + op.group) * D.n_cells;
   for (int d = 0; d < 3; d++) {
@@ -901,15 +1604,135 @@ template <> OpArgs<float> Operator::args<float>() const { return make_args<float
 // ---------------------------------------------------------------------------
 // Operator construction: upload every table once
 // ---------------------------------------------------------------------------
-static void upload_prec(PrecTables &t, const Disc &D, bool fp32) {
+// Geometry classes: cells whose DG geometry record (face normals, face
+// matrices, basis gradients, volume) agrees within 1e-13 of each quantity's
+// largest magnitude share a class. Records are bucketed by a hash of their
+// values rounded to 1e-9 of that scale, then compared within the bucket, so a
+// record that straddles a rounding boundary only costs an extra class.
+static void geometry_classes(const Disc &D, std::vector<int> &cls, std::vector<int> &rep) {
+  const int nc = D.n_cells, nf = D.nf, nb = D.nb, FM = nb * nb;
+  cls.assign(nc, 0);
+  rep.clear();
+  double scale[4] = {0, 0, 0, 0};  // nA, face matrices, gradients, volume
+  for (double v : D.face_nA) scale[0] = std::max(scale[0], std::fabs(v));
+  for (double v : D.face_own) scale[1] = std::max(scale[1], std::fabs(v));
+  for (double v : D.face_up) scale[1] = std::max(scale[1], std::fabs(v));
+  for (double v : D.basis_grad) scale[2] = std::max(scale[2], std::fabs(v));
+  for (double v : D.volume) scale[3] = std::max(scale[3], std::fabs(v));
+  for (double &s : scale) s = s > 0.0 ? s : 1.0;
+  auto visit = [&](int c, auto &&fn) {
+    for (int e = 0; e < 3 * nf; e++) fn(D.face_nA[(size_t)c * 3 * nf + e], 0);
+    if (nb > 1) {
+      for (int e = 0; e < nf * FM; e++) fn(D.face_own[(size_t)c * nf * FM + e], 1);
+      for (int e = 0; e < nf * FM; e++) fn(D.face_up[(size_t)c * nf * FM + e], 1);
+      for (int e = 0; e < 3 * nb; e++) fn(D.basis_grad[(size_t)c * 3 * nb + e], 2);
+    }
+    fn(D.volume[c], 3);
+  };
+  std::unordered_map<uint64_t, std::vector<int>> buckets;
+  std::vector<double> rc, rr;
+  for (int c = 0; c < nc; c++) {
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    visit(c, [&](double v, int kind) {
+      const int64_t q = (int64_t)std::llround(v / scale[kind] * 1e9);
+      h ^= (uint64_t)q + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    });
+    auto &list = buckets[h];
+    int found = -1;
+    rc.clear();
+    visit(c, [&](double v, int kind) { rc.push_back(v / scale[kind]); });
+    for (int k : list) {
+      rr.clear();
+      visit(rep[k], [&](double v, int kind) { rr.push_back(v / scale[kind]); });
+      double worst = 0.0;
+      for (size_t e = 0; e < rc.size(); e++) worst = std::max(worst, std::fabs(rc[e] - rr[e]));
+      if (worst <= 1e-13) {
+        found = k;
+        break;
+      }
+    }
+    if (found < 0) {
+      found = (int)rep.size();
+      rep.push_back(c);
+      list.push_back(found);
+    }
+    cls[c] = found;
+  }
+}
+
+// A DG mesh is a lexicographic box when every cell's faces are axis-aligned in
+// one local (axis, side) order and its neighbours are the grid neighbours of
+// cell c = (k * ny + j) * nx + i (x fastest), boundary faces exactly on the
+// grid's edges. The pencil kernel needs nothing else.
+static bool detect_box(const Disc &D, int n[3], int fmap[3][2]) {
+  if (D.kind != DISC_DG || D.nf != 2 * D.dim) return false;
+  const int dim = D.dim, nf = D.nf, nc = D.n_cells;
+  auto classify = [&](int64_t c, int f, int &ax, int &side) {
+    const double *v = &D.face_nA[3 * ((size_t)c * nf + f)];
+    ax = 0;
+    for (int d = 1; d < dim; d++)
+      if (std::fabs(v[d]) > std::fabs(v[ax])) ax = d;
+    for (int d = 0; d < 3; d++)
+      if (d != ax && std::fabs(v[d]) > 1e-12 * std::fabs(v[ax])) return false;
+    side = v[ax] > 0.0 ? 1 : 0;
+    return v[ax] != 0.0;
+  };
+  for (int d = 0; d < 3; d++) fmap[d][0] = fmap[d][1] = -1;
+  for (int f = 0; f < nf; f++) {
+    int ax, side;
+    if (!classify(0, f, ax, side) || fmap[ax][side] >= 0) return false;
+    fmap[ax][side] = f;
+  }
+  auto nbr = [&](int64_t c, int ax, int side) { return D.face_nb[(size_t)c * nf + fmap[ax][side]]; };
+  int64_t c = 0;
+  while (c + 1 < nc && nbr(c, 0, 1) == c + 1) c++;
+  n[0] = (int)(c + 1);
+  n[1] = n[2] = 1;
+  if (dim == 2) {
+    if (nc % n[0]) return false;
+    n[1] = nc / n[0];
+  } else {
+    c = 0;
+    while (c + n[0] < nc && nbr(c, 1, 1) == c + n[0]) c += n[0];
+    n[1] = (int)(c / n[0] + 1);
+    if (nc % ((int64_t)n[0] * n[1])) return false;
+    n[2] = (int)(nc / ((int64_t)n[0] * n[1]));
+  }
+  const int64_t stride[3] = {1, n[0], (int64_t)n[0] * n[1]};
+  for (c = 0; c < nc; c++) {
+    const int idx[3] = {(int)(c % n[0]), (int)((c / n[0]) % n[1]), (int)(c / stride[2])};
+    for (int f = 0; f < nf; f++) {
+      int ax, side;
+      if (!classify(c, f, ax, side) || fmap[ax][side] != f) return false;
+      const int o = idx[ax] + (side ? 1 : -1);
+      const int got = D.face_nb[(size_t)c * nf + f];
+      if (o >= 0 && o < n[ax]) {
+        if (got != c + (side ? stride[ax] : -stride[ax])) return false;
+      } else if (got >= 0) {
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
+static void upload_prec(PrecTables &t, const Disc &D, const std::vector<int> &rep, bool fp32) {
   std::vector<double> omega(D.quad.omega), w(D.quad.w);
-  std::vector<double> face_s;
+  std::vector<double> face_s, face_own, face_up, grad, vol;
   if (D.kind == DISC_DG) {
-    const size_t slots = (size_t)D.n_cells * D.nf;
-    face_s.assign(3 * slots, 0.0);
-    for (size_t k = 0; k < slots; k++) {
-      const double sc = D.order == 0 ? 1.0 / D.volume[k / D.nf] : 1.0;
-      for (int d = 0; d < 3; d++) face_s[3 * k + d] = D.face_nA[3 * k + d] * sc;
+    const int nf = D.nf, FM = D.nb * D.nb, ncls = (int)rep.size();
+    face_s.assign(3 * (size_t)ncls * nf, 0.0);
+    for (int k = 0; k < ncls; k++) {
+      const int r = rep[k];
+      const double sc = D.order == 0 ? 1.0 / D.volume[r] : 1.0;
+      for (int e = 0; e < 3 * nf; e++) face_s[(size_t)k * 3 * nf + e] = D.face_nA[(size_t)r * 3 * nf + e] * sc;
+      vol.push_back(D.volume[r]);
+      if (D.nb == 1) continue;  // DG0 has no basis gradients or face matrices
+      for (int e = 0; e < 3 * D.nb; e++) grad.push_back(D.basis_grad[(size_t)r * 3 * D.nb + e]);
+      for (int e = 0; e < nf * FM; e++) {
+        face_own.push_back(D.face_own[(size_t)r * nf * FM + e]);
+        face_up.push_back(D.face_up[(size_t)r * nf * FM + e]);
+      }
     }
   }
   const int A = D.n_angles, Apad = (A + 3) / 4 * 4;
@@ -925,20 +1748,20 @@ static void upload_prec(PrecTables &t, const Disc &D, bool fp32) {
     t.omega.upload_as<float>(omega);
     t.w.upload_as<float>(w);
     t.face_s.upload_as<float>(face_s);
-    t.face_own.upload_as<float>(D.face_own);
-    t.face_up.upload_as<float>(D.face_up);
-    t.grad.upload_as<float>(D.basis_grad);
-    t.vol.upload_as<float>(D.volume);
+    t.face_own.upload_as<float>(face_own);
+    t.face_up.upload_as<float>(face_up);
+    t.grad.upload_as<float>(grad);
+    t.vol.upload_as<float>(vol);
   } else {
     t.wext.upload(wext);
     t.wunit.upload(wunit);
     t.omega.upload(omega);
     t.w.upload(w);
     t.face_s.upload(face_s);
-    t.face_own.upload(D.face_own);
-    t.face_up.upload(D.face_up);
-    t.grad.upload(D.basis_grad);
-    t.vol.upload(D.volume);
+    t.face_own.upload(face_own);
+    t.face_up.upload(face_up);
+    t.grad.upload(grad);
+    t.vol.upload(vol);
   }
 }
 
@@ -951,8 +1774,15 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
   op->n_groups = n_groups;
   UBT_HIP(hipGetDevice(&op->device));
   const Disc &D = *disc;
-  upload_prec(op->t64, D, false);
-  upload_prec(op->t32, D, true);
+  if (D.kind == DISC_DG) {
+    std::vector<int> cls;
+    geometry_classes(D, cls, op->geo_rep);
+    op->n_geo_classes = (int)op->geo_rep.size();
+    op->gcls.upload(cls);
+    op->box = detect_box(D, op->box_n, op->box_fmap);
+  }
+  upload_prec(op->t64, D, op->geo_rep, false);
+  upload_prec(op->t32, D, op->geo_rep, true);
   std::vector<int> refl(3 * (size_t)D.n_angles, -1);
   for (int d = 0; d < D.dim; d++)
     for (int a = 0; a < D.n_angles; a++) refl[3 * a + d] = D.quad.reflect[d][a];
@@ -1029,6 +1859,21 @@ void Operator::tune() {
   const Disc &D = *disc;
   const int64_t n = D.n_rows();
   const bool wave_ok = (D.n_angles + 63) / 64 <= (D.kind == DISC_FD ? (D.dim == 3 ? 4 : 2) : 2);
+  auto applicable = [&](int v, int p) {
+    if (v == 0) return true;
+    if (v == 1) return wave_ok;
+    if (v == 7)
+      return p == 0 ? launch_apply_q4<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
+                    : launch_apply_q4<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);
+    if (v >= 4)
+      return p == 0 ? launch_apply_pencil<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, 8, true)
+                    : launch_apply_pencil<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, 8, true);
+    if (D.kind != DISC_DG || D.n_angles > 256) return false;
+    const int R = 256 / D.n_angles, tcells = R * (v == 2 ? 2 : 4);
+    const size_t reals = (size_t)tcells * ((D.nb > 1 ? 2 * D.nf * D.nb * D.nb + (D.nb - 1) * D.dim : 0) +
+                                           D.nb * D.n_angles + D.nb);
+    return reals * (p == 0 ? 8 : 4) <= 65536;
+  };
   hipStream_t s = nullptr;
   UBT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   hipEvent_t e0, e1;
@@ -1041,7 +1886,9 @@ void Operator::tune() {
   for (int p = 0; p < 2; p++) {
     float best = 1e30f;
     int best_v = 0;
-    for (int v = 0; v < (wave_ok ? 2 : 1); v++) {
+    for (int v = 0; v < kApplyVariants; v++) {
+      tuned_us[p][v] = 0.f;
+      if (!applicable(v, p)) continue;
       variant[p] = v;
       for (int it = 0; it < 2; it++) mult(p == 0 ? UBT_FP64 : UBT_FP32, xb.ptr, yb.ptr, true, s);
       UBT_HIP(hipEventRecord(e0, s));

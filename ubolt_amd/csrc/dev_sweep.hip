@@ -49,7 +49,8 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
   T om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
   T B[NB][NB], v[NB];
   const int64_t base = c * NB * A + a;
-  const T inv_vc = ADJ ? T(1) / g.vol[c] : T(1);
+  const int64_t gc = g.gcls[c];
+  const T inv_vc = ADJ ? T(1) / g.vol[gc] : T(1);
 #pragma unroll
   for (int i = 0; i < NB; i++) {
     v[i] = inv_vc * r[base + (int64_t)i * A];
@@ -58,21 +59,21 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
   }
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    const int64_t k = c * NF + f;
+    const int64_t k = c * NF + f, kg = gc * NF + f;
     T s = T(0);
 #pragma unroll
-    for (int d = 0; d < DIM; d++) s += om[d] * g.face_s[3 * k + d];
+    for (int d = 0; d < DIM; d++) s += om[d] * g.face_s[3 * kg + d];
     if ((up >> f) & 1) {
       const int n = g.face_nb[k];
       const int64_t ub = (int64_t)n * NB * A + a;
-      const T inv_vn = ADJ ? T(1) / g.vol[n] : T(1);
+      const T inv_vn = ADJ ? T(1) / g.vol[g.gcls[n]] : T(1);
       if (NB == 1) {
         v[0] -= s * inv_vn * psi[ub];
       } else {
         T pu[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++) pu[j] = inv_vn * psi[ub + (int64_t)j * A];
-        const T *F = g.face_up + k * NB * NB;
+        const T *F = g.face_up + kg * NB * NB;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
           T t = T(0);
@@ -85,7 +86,7 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
       if (NB == 1) {
         B[0][0] += s;
       } else {
-        const T *F = g.face_own + k * NB * NB;
+        const T *F = g.face_own + kg * NB * NB;
 #pragma unroll
         for (int i = 0; i < NB; i++)
 #pragma unroll
@@ -96,7 +97,7 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
   if (NB > 1) {
 #pragma unroll
     for (int i = 1; i < NB; i++) {
-      const T *gr = g.grad + (c * NB + i) * 3;
+      const T *gr = g.grad + (gc * NB + i) * 3;
       T od = T(0);
 #pragma unroll
       for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
@@ -137,7 +138,7 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
       v[i] -= f * v[k];
     }
   }
-  const T vc = ADJ ? g.vol[c] : T(1);
+  const T vc = ADJ ? g.vol[gc] : T(1);
 #pragma unroll
   for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
 }

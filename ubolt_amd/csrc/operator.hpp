@@ -15,19 +15,23 @@ template <class T> struct OpArgs {
   const T *w;          // A
   const int *reflect;  // 3A (-1 past dim)
   const int *neg;      // A: the ordinate with every cosine negated
-  const T *vol;        // cells: volumes (unstructured; the adjoint's neighbour ratio V_c / V_n)
   const T *sigt;       // current group, per cell
   const T *sigs;       // current group's within-group scatter, per cell
   // structured
   int n[3];
   T invh[3];
   int face_type[3][2];
-  // unstructured
-  const int *face_nb;  // slots
-  const T *face_s;     // 3 per slot: nA (DG1) or nA / V (DG0)
-  const T *face_own;   // slots * nb * nb
+  // unstructured. The geometry tables are indexed by geometry class: cells
+  // whose geometry agrees to 1e-13 of each quantity's scale share one record
+  // (every interior cell of a uniform box is one class), so the applies read
+  // a few KB of geometry instead of a record per cell. Neighbours stay per cell.
+  const int *face_nb;  // cells * nf (per cell)
+  const int *gcls;     // cells: geometry class
+  const T *face_s;     // classes * nf * 3: nA (DG1) or nA / V (DG0)
+  const T *face_own;   // classes * nf * nb * nb
   const T *face_up;
-  const T *grad;       // cells * nb * 3
+  const T *grad;       // classes * nb * 3
+  const T *vol;        // classes: volume (the adjoint's neighbour ratio V_c / V_n)
 };
 
 // Per precision copies of the geometry and cross sections
@@ -38,16 +42,30 @@ struct PrecTables {
   DevBuf sigt, sigs;  // G*cells, G*G*cells
 };
 
+// apply kernels the tuner chooses from: 0 tile, 1 wavefront per cell,
+// 2 and 3 the multi-cell tile with 2 and 4 cells per thread, 4-6 the pencil
+// kernel of a structured box marching 8, 16 or 32 cells per workgroup, 7 the
+// four-angle tile
+constexpr int kApplyVariants = 8;
+constexpr int kPencilMaxClasses = 64;
+
 struct Operator {
   std::shared_ptr<Disc> disc;
   int n_groups = 1, group = 0, device = 0;
   // apply kernel variant per precision [fp64, fp32]: 0 tile (LDS-staged, cpb
   // cells per workgroup), 1 wavefront per cell; chosen by tune()
   int variant[2] = {0, 0};
-  float tuned_us[2][2] = {{0, 0}, {0, 0}};
+  float tuned_us[2][kApplyVariants] = {};
   bool xs_set = false;
   PrecTables t64, t32;
   DevBuf reflect, face_nb, neg;
+  DevBuf gcls;                // int per cell: geometry class (DG)
+  int n_geo_classes = 0;
+  std::vector<int> geo_rep;   // representative cell of each class
+  // a DG mesh that is a lexicographic box (the pencil kernel's precondition)
+  bool box = false;
+  int box_n[3] = {1, 1, 1};
+  int box_fmap[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
   DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
   DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
