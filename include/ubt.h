@@ -83,6 +83,8 @@ typedef struct ubt_blockinv_s ubt_blockinv;
 typedef struct ubt_dsa_s ubt_dsa;
 typedef struct ubt_sweep_s ubt_sweep;
 typedef struct ubt_solver_s ubt_solver;
+typedef struct ubt_batch_s ubt_batch;
+typedef struct ubt_batch_solver_s ubt_batch_solver;
 
 /* Solver options (ubt_solver_create). The preconditioner is the reference's
  * multiplicative composite: stage 0 the element block inverse (pc_blocks),
@@ -359,6 +361,51 @@ int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, vo
 int ubt_solver_result(const ubt_solver *solver, int *iterations, int *outer, double *rel_residual, int *reason,
                       double *history, int max_history, int *n_history);
 int ubt_solver_destroy(ubt_solver *solver);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent problems driven as one (no reference counterpart: the
+ * reference solves one problem per process; this is how a caller runs a
+ * parameter sweep or an ensemble). A batch references n operators (which
+ * must outlive it) of one DG type (dimension, basis, faces per cell); they
+ * may differ in mesh, quadrature and group count. Batch vectors are the
+ * problems' group vectors end to end (ubt_batch_offsets). */
+int ubt_batch_create(ubt_op *const *ops, int n, ubt_batch **batch);
+int ubt_batch_size(const ubt_batch *batch, int64_t *n_rows, int *n_problems);
+/* off[n + 1]: first row of every problem in a batch vector */
+int ubt_batch_offsets(const ubt_batch *batch, int64_t *off);
+/* Switch every problem to group g; problems with fewer groups sit out (their
+ * span of a batch vector is left alone). Rebuilds the work lists. */
+int ubt_batch_set_group(ubt_batch *batch, int g);
+/* y = A x for every active problem in one persistent launch. */
+int ubt_batch_mult(ubt_batch *batch, int prec, const void *x, void *y, void *stream);
+/* body[2] (FP64, FP32: 0 tile, 1 four-angle tile) and persist[2] (workgroups
+ * per CU) chosen by the batch autotuner, us[4] the best time per body
+ * (FP64 tile, FP64 four-angle, FP32 tile, FP32 four-angle; 0 if it does not
+ * apply). */
+int ubt_batch_tuning(const ubt_batch *batch, int *body, int *persist, float *us);
+/* the right-hand side of every active problem (ubt_rhs), zero elsewhere */
+int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream);
+int ubt_batch_set_scalar_flux(ubt_batch *batch, int prec, const void *psi, void *stream);
+/* the element block inverse / the transport sweep of every active problem */
+int ubt_batch_blocks_apply(ubt_batch *batch, int prec, const void *x, void *y, void *stream);
+int ubt_batch_sweep_apply(ubt_batch *batch, int prec, const void *r, void *psi, void *stream);
+int ubt_batch_destroy(ubt_batch *batch);
+/* FGMRES over every active problem at once with one Hessenberg per problem
+ * on the device; a restart cycle is captured as a HIP graph and replayed.
+ * Preconditioner: element blocks then the transport sweep (pc_dsa is
+ * ignored). mixed: FP32 inner solves inside FP64 refinement, per problem. */
+int ubt_batch_solver_create(ubt_batch *batch, const ubt_solver_options *options, ubt_batch_solver **solver);
+/* after ubt_batch_set_group: block check and per-problem reductions */
+int ubt_batch_solver_refresh(ubt_batch_solver *solver, void *stream);
+/* x = A^{-1} b per problem (FP64 batch vectors, zero initial guess);
+ * synchronises the stream */
+int ubt_batch_solver_solve(ubt_batch_solver *solver, const double *b, double *x, void *stream);
+/* per problem (arrays of n): Krylov iterations, final FP64 true relative
+ * residual, reason (1 converged, -1 limit, 0 inactive); outer refinement
+ * steps and restart cycles in total */
+int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, double *rel_residual, int *reason,
+                            int *outer, int *cycles);
+int ubt_batch_solver_destroy(ubt_batch_solver *solver);
 
 #ifdef __cplusplus
 }

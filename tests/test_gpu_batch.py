@@ -1,0 +1,137 @@
+"""Batched problems on the GPU: one persistent apply, the batched block
+inverse and merged sweep, and the batched FGMRES (device-side per-problem
+Hessenberg, graph-replayed cycles) against the oracle, problem by problem."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+import torch
+
+import oracle
+import ubolt_amd as u
+from helpers import oracle_full, oracle_mult, oracle_rhs, oracle_rows, rel_max, to_dev, to_host
+from ubolt_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    probs = P.batch_quads(n_problems=6, seed=5, side=(3, 9), sns=(2, 4, 6), groups=(1, 3))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    print(f"[case] batch of {len(probs)}: rows {b.n_rows}, tuning {b.tuning()}", flush=True)
+    return probs, ops, b
+
+
+def test_batch_mult(gpu, ensemble):
+    probs, ops, b = ensemble
+    b.set_group(0)
+    x = np.random.default_rng(1).standard_normal(b.n_rows)
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+        y = b.vec(u.FP64 if dt == torch.float64 else u.FP32)
+        b.mult(to_dev(x, dt), y)
+        torch.cuda.synchronize()
+        yh = to_host(y)
+        for p, prob in enumerate(probs):
+            sl = slice(b.offsets[p], b.offsets[p + 1])
+            err = rel_max(yh[sl], oracle_mult(prob, x[sl]))
+            assert err <= tol, (p, dt, err)
+
+
+def test_batch_inactive_untouched(gpu, ensemble):
+    """group 2 exists only in the problems with three groups; the others' spans
+    keep what was there"""
+    probs, ops, b = ensemble
+    live = [p.n_groups > 2 for p in probs]
+    if not any(live):
+        pytest.skip("no problem with three groups in this draw")
+    b.set_group(2)
+    x = b.vec()
+    x.normal_()
+    y = b.vec()
+    y.fill_(7.0)
+    b.mult(x, y)
+    torch.cuda.synchronize()
+    yh, xh = to_host(y), to_host(x)
+    for p, prob in enumerate(probs):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        if live[p]:
+            assert rel_max(yh[sl], oracle_mult(prob, xh[sl], g=2)) <= 1e-12
+        else:
+            assert np.all(yh[sl] == 7.0)
+    b.set_group(0)
+
+
+def test_batch_blocks_and_sweep(gpu, ensemble):
+    probs, ops, b = ensemble
+    b.set_group(0)
+    x = np.random.default_rng(2).standard_normal(b.n_rows)
+    yb, ys = b.vec(), b.vec()
+    b.blocks_apply(to_dev(x), yb)
+    b.sweep_apply(to_dev(x), ys)
+    torch.cuda.synchronize()
+    for p, (prob, op) in enumerate(zip(probs, ops)):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        bi = u.BlockInverse(op)
+        bi.setup()
+        want = op.vec()
+        bi.apply(to_dev(x[sl]), want)
+        torch.cuda.synchronize()
+        assert rel_max(to_host(yb)[sl], to_host(want)) <= 1e-14
+        ref = spl.spsolve(oracle_rows(prob).to_scipy().tocsc(), x[sl])
+        assert rel_max(to_host(ys)[sl], ref) <= 1e-11
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_solve(gpu, ensemble, mixed):
+    probs, ops, b = ensemble
+    b.set_group(0)
+    sol = u.BatchSolver(b, rtol=1e-8 if mixed else 1e-11, max_it=600, restart=20, mixed=mixed)
+    sol.refresh()
+    rhs = b.vec()
+    b.rhs(rhs)
+    x = b.vec()
+    res = sol.solve(rhs, x)
+    torch.cuda.synchronize()
+    print(f"batch solve {'mixed' if mixed else 'fp64'}: {res}")
+    bh, xh = to_host(rhs), to_host(x)
+    for p, prob in enumerate(probs):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        A = oracle_full(prob)
+        true_res = np.linalg.norm(bh[sl] - A @ xh[sl]) / np.linalg.norm(bh[sl])
+        assert res["reason"][p] == 1, (p, res)
+        assert true_res <= (1.05e-8 if mixed else 1e-10), (p, true_res)
+        assert abs(true_res - res["rel_residual"][p]) <= 1e-6 * max(true_res, 1e-16) + 1e-14
+        if not mixed:
+            want = spl.spsolve(A, bh[sl])
+            phi, phw = oracle.moments(xh[sl], prob.quad.w), oracle.moments(want, prob.quad.w)
+            assert np.linalg.norm(phi - phw) / np.linalg.norm(phw) <= 1e-8
+
+
+def test_batch_multigroup(gpu, ensemble):
+    """group Gauss-Seidel over the batch: rhs with transfers from the solved
+    groups of each problem, one batched solve per group"""
+    probs, ops, b = ensemble
+    G = max(p.n_groups for p in probs)
+    caches = [[] for _ in probs]
+    for g in range(G):
+        b.set_group(g)
+        sol = u.BatchSolver(b, rtol=1e-11, max_it=600, restart=20)
+        sol.refresh()
+        rhs = b.vec()
+        b.rhs(rhs)
+        x = b.vec()
+        res = sol.solve(rhs, x)
+        b.set_scalar_flux(x)
+        torch.cuda.synchronize()
+        xh = to_host(x)
+        for p, prob in enumerate(probs):
+            if prob.n_groups <= g:
+                assert res["reason"][p] == 0
+                continue
+            sl = slice(b.offsets[p], b.offsets[p + 1])
+            want = spl.spsolve(oracle_full(prob, g), oracle_rhs(prob, g, caches[p]))
+            caches[p].append(oracle.moments(want, prob.quad.w))
+            phi = oracle.moments(xh[sl], prob.quad.w)
+            assert np.linalg.norm(phi - caches[p][-1]) / np.linalg.norm(caches[p][-1]) <= 1e-8, (p, g)
+    b.set_group(0)

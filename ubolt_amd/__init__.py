@@ -6,4 +6,5 @@ the C ABI of include/ubt.h); this package is the Python binding used by the
 tests, the benchmark and the sensitivity harness.
 """
 from .capi import UbtError, FP64, FP32  # noqa: F401
-from .core import Quadrature, Mesh, Disc, Operator, BlockInverse, DSA, Sweep, Solver, lqn_octant  # noqa: F401
+from .core import (Quadrature, Mesh, Disc, Operator, BlockInverse, DSA, Sweep, Solver, Batch, BatchSolver,  # noqa: F401
+                   lqn_octant)

@@ -109,6 +109,22 @@ SIGNATURES = {
     "ubt_solver_apply_pc": [P, I, P, P, P],
     "ubt_solver_result": [P, PI, PI, PD, PI, PD, I, PI],
     "ubt_solver_destroy": [P],
+    "ubt_batch_create": [PP, I, PP],
+    "ubt_batch_size": [P, PI64, PI],
+    "ubt_batch_offsets": [P, PI64],
+    "ubt_batch_set_group": [P, I],
+    "ubt_batch_mult": [P, I, P, P, P],
+    "ubt_batch_tuning": [P, PI, PI, C.POINTER(C.c_float)],
+    "ubt_batch_rhs": [P, I, P, P],
+    "ubt_batch_set_scalar_flux": [P, I, P, P],
+    "ubt_batch_blocks_apply": [P, I, P, P, P],
+    "ubt_batch_sweep_apply": [P, I, P, P, P],
+    "ubt_batch_destroy": [P],
+    "ubt_batch_solver_create": [P, C.POINTER(SolverOptions), PP],
+    "ubt_batch_solver_refresh": [P, P],
+    "ubt_batch_solver_solve": [P, P, P, P],
+    "ubt_batch_solver_result": [P, PI, PD, PI, PI, PI],
+    "ubt_batch_solver_destroy": [P],
 }
 
 _lib = None

@@ -349,3 +349,89 @@ class Solver(_Handle):
              C.byref(nh))
         return {"iterations": it.value, "outer": outer.value, "rel_residual": rr.value, "reason": reason.value,
                 "history": hist[:min(nh.value, 4096)].copy()}
+
+
+class Batch(_Handle):
+    """Many independent DG problems driven as one: one persistent launch per
+    apply, merged sweep levels, batched block inverse. The operators must
+    outlive the batch."""
+    _destroy = "ubt_batch_destroy"
+
+    def __init__(self, ops):
+        super().__init__()
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[o.h.value for o in self.ops])
+        call("ubt_batch_create", C.cast(arr, capi.PP), len(self.ops), C.byref(self.h))
+        nr, npb = C.c_int64(), C.c_int()
+        call("ubt_batch_size", self.h, C.byref(nr), C.byref(npb))
+        self.n_rows, self.n_problems = nr.value, npb.value
+        off = (C.c_int64 * (self.n_problems + 1))()
+        call("ubt_batch_offsets", self.h, off)
+        self.offsets = np.array(off[:], dtype=np.int64)
+
+    def set_group(self, g):
+        call("ubt_batch_set_group", self.h, g)
+
+    def vec(self, prec=FP64):
+        return torch.zeros(self.n_rows, dtype=torch_dtype(prec), device="cuda")
+
+    def part(self, v, p):
+        return v[self.offsets[p]:self.offsets[p + 1]]
+
+    def mult(self, x, y, stream=None):
+        call("ubt_batch_mult", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def blocks_apply(self, x, y, stream=None):
+        call("ubt_batch_blocks_apply", self.h, Operator._prec(x), dptr(x), dptr(y), stream_ptr(stream))
+        return y
+
+    def sweep_apply(self, r, psi, stream=None):
+        call("ubt_batch_sweep_apply", self.h, Operator._prec(r), dptr(r), dptr(psi), stream_ptr(stream))
+        return psi
+
+    def rhs(self, b, stream=None):
+        call("ubt_batch_rhs", self.h, Operator._prec(b), dptr(b), stream_ptr(stream))
+        return b
+
+    def set_scalar_flux(self, psi, stream=None):
+        call("ubt_batch_set_scalar_flux", self.h, Operator._prec(psi), dptr(psi), stream_ptr(stream))
+
+    def tuning(self):
+        body, persist = (C.c_int * 2)(), (C.c_int * 2)()
+        us = (C.c_float * 4)()
+        call("ubt_batch_tuning", self.h, body, persist, us)
+        return {"body_fp64": body[0], "body_fp32": body[1], "persist_fp64": persist[0], "persist_fp32": persist[1],
+                "us": [round(x, 2) for x in us]}
+
+
+class BatchSolver(_Handle):
+    """FGMRES over every problem of a batch at once (per-problem Hessenberg on
+    the device, restart cycles replayed as a HIP graph); element blocks then
+    the sweep as the preconditioner."""
+    _destroy = "ubt_batch_solver_destroy"
+
+    def __init__(self, batch, rtol=1e-8, max_it=500, restart=30, mixed=False, inner_rtol=1e-4, max_outer=20,
+                 pc_blocks=True, pc_sweep=True):
+        super().__init__()
+        self.batch = batch
+        o = capi.SolverOptions(rtol, max_it, restart, int(mixed), inner_rtol, max_outer, int(pc_blocks), int(pc_sweep),
+                               0, 0)
+        call("ubt_batch_solver_create", batch.h, C.byref(o), C.byref(self.h))
+
+    def refresh(self, stream=None):
+        call("ubt_batch_solver_refresh", self.h, stream_ptr(stream))
+
+    def solve(self, b, x, stream=None):
+        call("ubt_batch_solver_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
+        return self.result()
+
+    def result(self):
+        n = self.batch.n_problems
+        it = (C.c_int * n)()
+        rr = (C.c_double * n)()
+        rs = (C.c_int * n)()
+        outer, cycles = C.c_int(), C.c_int()
+        call("ubt_batch_solver_result", self.h, it, rr, rs, C.byref(outer), C.byref(cycles))
+        return {"iterations": list(it), "rel_residual": list(rr), "reason": list(rs), "outer": outer.value,
+                "cycles": cycles.value}

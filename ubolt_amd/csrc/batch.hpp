@@ -1,0 +1,101 @@
+// Many independent transport problems driven as one: every operator apply,
+// block inverse and sweep level of all problems is one launch over a work
+// list of (problem, tile) items, and the Krylov solver keeps one Hessenberg
+// per problem on the device so a whole restart cycle replays as one captured
+// graph. Problems may differ in mesh size, quadrature and group count; they
+// share the discretisation type (dimension, basis, faces per cell).
+#pragma once
+
+#include "solver.hpp"
+
+namespace ubt {
+
+template <class T> struct BatchItem {
+  OpArgs<T> g;  // the problem's current group
+  int64_t off;  // first row of the problem in the batch vector
+};
+
+// apply bodies the batch autotuner chooses from
+constexpr int kBatchBodies = 2;  // 0 tile, 1 four-angle tile
+
+struct BatchSweep {
+  int n_levels = 0;
+  std::vector<int64_t> level_off;
+  DevBuf prob, cells, angles, upwind;
+};
+
+struct Batch {
+  std::vector<Operator *> ops;
+  std::vector<std::unique_ptr<Sweep>> sweeps;  // per problem, host schedule kept
+  int dim = 0, nb = 0, nf = 0, group = 0;
+  int64_t n_rows = 0;
+  std::vector<int64_t> off;  // n + 1 row offsets
+  std::vector<char> active;  // the problem has the current group
+  // per precision, per body: work list of the active problems
+  DevBuf items[2][kBatchBodies], tiles[2][kBatchBodies];
+  int n_items[2][kBatchBodies] = {};
+  int64_t n_tiles[2][kBatchBodies] = {};
+  size_t lds[2][kBatchBodies] = {};
+  bool body_ok[2][kBatchBodies] = {};
+  int body[2] = {0, 0};
+  int persist[2] = {8, 8};     // workgroups per CU of the persistent grid
+  float tuned_us[2][kBatchBodies] = {};
+  int n_cu = 256;
+  int gen = 0;                 // bumped whenever the work lists are rebuilt
+  // block inverse and sweep work lists
+  DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems
+  int64_t n_blocks = 0;
+  BatchSweep sweep;
+
+  void create(const std::vector<Operator *> &o);
+  void set_group(int g);       // problems without group g go inactive
+  void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
+  void blocks_setup(hipStream_t s);  // singular check of every active problem's blocks
+  void blocks_apply(int prec, const void *x, void *y, hipStream_t s) const;
+  void sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const;
+  void rhs(int prec, void *b, hipStream_t s) const;
+  void set_scalar_flux(int prec, const void *psi, hipStream_t s) const;
+  void tune();
+  void build_lists();
+};
+
+// launchers (dev_operator.hip, dev_sweep.hip)
+template <class T>
+void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s);
+template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hipStream_t s);
+template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
+// per-problem tile size and LDS of a body; false where it does not apply
+bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *red_lanes, size_t *lds);
+
+struct BatchSolver {
+  Batch *batch = nullptr;
+  SolverOptions opt;
+  int64_t n = 0;
+  int np = 0;
+  // chunking of the batch vector for the per-problem reductions
+  int n_chunks = 0;
+  DevBuf chunk_start, chunk_len, chunk_prob;
+  DevBuf V64, Z64, V32, Z32, w64, t64, u64, w32, t32, u32, d32, b32, r64;
+  DevBuf cfirst;              // per problem: its first chunk (prefix, np + 1)
+  DevBuf part, H, cs, sn, gv, hc, yv, nrm, tol, res, bnorm;
+  DevBuf onrm;                // per problem: the outer residual norm (refinement scale)
+  DevBuf done;                // int per problem x 4: done, steps this cycle, iterations, live
+  int gen = -1;               // the batch work lists the graphs were captured on
+  std::vector<int> iterations, reason;
+  std::vector<double> rel_residual;
+  int outer = 0, cycles = 0;
+  bool use_graph = true;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one restart cycle per precision
+  hipStream_t cap = nullptr;
+  ~BatchSolver();
+
+  void create(Batch *b, const SolverOptions &o);
+  void refresh(hipStream_t s);
+  void solve(const double *b, double *x, hipStream_t s);
+  template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
+  template <class T> void cycle(const T *b, T *x, hipStream_t s);
+  template <class T> void arnoldi_step(int j, hipStream_t s);
+  template <class T> void fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, hipStream_t s);
+};
+
+}  // namespace ubt

@@ -2,6 +2,7 @@
 // and turns it into a ubt_status plus a message for ubt_last_error().
 #include <cstring>
 
+#include "batch.hpp"
 #include "operator.hpp"
 #include "solver.hpp"
 #include "ubt_internal.hpp"
@@ -33,6 +34,12 @@ struct ubt_sweep_s {
 };
 struct ubt_solver_s {
   ubt::Solver s;
+};
+struct ubt_batch_s {
+  ubt::Batch b;
+};
+struct ubt_batch_solver_s {
+  ubt::BatchSolver s;
 };
 
 namespace {
@@ -618,6 +625,150 @@ int ubt_solver_result(const ubt_solver *solver, int *iterations, int *outer, dou
   });
 }
 int ubt_solver_destroy(ubt_solver *solver) {
+  delete solver;
+  return UBT_OK;
+}
+
+// batches -------------------------------------------------------------------------
+int ubt_batch_create(ubt_op *const *ops, int n, ubt_batch **batch) {
+  return guard([&] {
+    NEED(ops);
+    NEED(batch);
+    UBT_CHECK(n >= 1, UBT_ERR_ARG_SIZ, "a batch needs at least one problem, got %d", n);
+    std::vector<ubt::Operator *> o;
+    for (int p = 0; p < n; p++) {
+      UBT_CHECK(ops[p], UBT_ERR_ARG_NULL, "operator %d is NULL", p);
+      o.push_back(ops[p]->op.get());
+    }
+    auto h = std::make_unique<ubt_batch_s>();
+    h->b.create(o);
+    *batch = h.release();
+  });
+}
+int ubt_batch_size(const ubt_batch *batch, int64_t *n_rows, int *n_problems) {
+  return guard([&] {
+    NEED(batch);
+    if (n_rows) *n_rows = batch->b.n_rows;
+    if (n_problems) *n_problems = (int)batch->b.ops.size();
+  });
+}
+int ubt_batch_offsets(const ubt_batch *batch, int64_t *off) {
+  return guard([&] {
+    NEED(batch);
+    NEED(off);
+    for (size_t p = 0; p < batch->b.off.size(); p++) off[p] = batch->b.off[p];
+  });
+}
+int ubt_batch_set_group(ubt_batch *batch, int g) {
+  return guard([&] {
+    NEED(batch);
+    batch->b.set_group(g);
+  });
+}
+int ubt_batch_mult(ubt_batch *batch, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(batch);
+    batch->b.mult(prec, x, y, true, (hipStream_t)stream);
+  });
+}
+int ubt_batch_tuning(const ubt_batch *batch, int *body, int *persist, float *us) {
+  return guard([&] {
+    NEED(batch);
+    for (int p = 0; p < 2; p++) {
+      if (body) body[p] = batch->b.body[p];
+      if (persist) persist[p] = batch->b.persist[p];
+      if (us)
+        for (int v = 0; v < ubt::kBatchBodies; v++) us[ubt::kBatchBodies * p + v] = batch->b.tuned_us[p][v];
+    }
+  });
+}
+int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream) {
+  return guard([&] {
+    NEED(batch);
+    NEED(b);
+    batch->b.rhs(prec, b, (hipStream_t)stream);
+  });
+}
+int ubt_batch_set_scalar_flux(ubt_batch *batch, int prec, const void *psi, void *stream) {
+  return guard([&] {
+    NEED(batch);
+    NEED(psi);
+    batch->b.set_scalar_flux(prec, psi, (hipStream_t)stream);
+  });
+}
+int ubt_batch_blocks_apply(ubt_batch *batch, int prec, const void *x, void *y, void *stream) {
+  return guard([&] {
+    NEED(batch);
+    NEED(x);
+    NEED(y);
+    UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the block inverse apply needs distinct x and y");
+    batch->b.blocks_apply(prec, x, y, (hipStream_t)stream);
+  });
+}
+int ubt_batch_sweep_apply(ubt_batch *batch, int prec, const void *r, void *psi, void *stream) {
+  return guard([&] {
+    NEED(batch);
+    NEED(r);
+    NEED(psi);
+    UBT_CHECK(r != psi, UBT_ERR_ARG_IDN, "the sweep needs distinct r and psi");
+    batch->b.sweep_apply(prec, r, psi, (hipStream_t)stream);
+  });
+}
+int ubt_batch_destroy(ubt_batch *batch) {
+  delete batch;
+  return UBT_OK;
+}
+int ubt_batch_solver_create(ubt_batch *batch, const ubt_solver_options *options, ubt_batch_solver **solver) {
+  return guard([&] {
+    NEED(batch);
+    NEED(solver);
+    ubt::SolverOptions o;
+    if (options) {
+      o.rtol = options->rtol;
+      o.max_it = options->max_it;
+      o.restart = options->restart;
+      o.mixed = options->mixed;
+      o.inner_rtol = options->inner_rtol;
+      o.max_outer = options->max_outer;
+      o.pc_blocks = options->pc_blocks;
+      o.pc_sweep = options->pc_sweep;
+      o.pc_dsa = 0;
+      o.use_amat = options->use_amat;
+    }
+    UBT_CHECK(!o.mixed || (o.inner_rtol > 0.0 && o.inner_rtol < 1.0 && o.max_outer >= 1), UBT_ERR_ARG_OUTOFRANGE,
+              "mixed mode needs 0 < inner_rtol < 1 and max_outer >= 1");
+    auto h = std::make_unique<ubt_batch_solver_s>();
+    h->s.create(&batch->b, o);
+    *solver = h.release();
+  });
+}
+int ubt_batch_solver_refresh(ubt_batch_solver *solver, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.refresh((hipStream_t)stream);
+  });
+}
+int ubt_batch_solver_solve(ubt_batch_solver *solver, const double *b, double *x, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.solve(b, x, (hipStream_t)stream);
+  });
+}
+int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, double *rel_residual, int *reason,
+                            int *outer, int *cycles) {
+  return guard([&] {
+    NEED(solver);
+    const auto &s = solver->s;
+    for (size_t p = 0; p < s.iterations.size(); p++) {
+      if (iterations) iterations[p] = s.iterations[p];
+      if (rel_residual) rel_residual[p] = s.rel_residual[p];
+      if (reason) reason[p] = s.reason[p];
+    }
+    if (outer) *outer = s.outer;
+    if (cycles) *cycles = s.cycles;
+  });
+}
+int ubt_batch_solver_destroy(ubt_batch_solver *solver) {
   delete solver;
   return UBT_OK;
 }

@@ -1,0 +1,701 @@
+// Batched problems: work lists, the batch autotuner, and FGMRES with one
+// Hessenberg per problem held on the device.
+//
+// Vectors of a batch are the problems' group vectors laid end to end (row
+// offsets fixed at creation; problems without the current group keep their
+// span and are skipped). Every reduction of the Krylov solver is segmented
+// by problem: the vector is cut into chunks that never straddle two problems,
+// a block reduces one chunk, and a second kernel sums each problem's chunks.
+// The Givens rotations, the residual estimates and the convergence flags of
+// every problem live in device arrays, so a whole restart cycle - restart
+// Arnoldi steps, each a preconditioner apply, an operator apply, two
+// classical Gram-Schmidt passes and the rotation - has no host round trip and
+// is captured once as a HIP graph and replayed; the host reads the flags once
+// per cycle. A problem that converges mid-cycle freezes its rotation state;
+// the rest of the cycle still runs over its (inert) span.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "batch.hpp"
+
+namespace ubt {
+
+namespace {
+
+constexpr int kS = 32;         // per-problem stride of the coefficient arrays (restart + 1 at most)
+constexpr int kChunk = 16384;  // rows per reduction chunk
+constexpr int kT = 256;
+
+template <class T>
+__global__ void __launch_bounds__(kT) k_bdot(int64_t n, int k, const T *__restrict__ V, const T *__restrict__ w,
+                                             const int64_t *__restrict__ cstart, const int *__restrict__ clen,
+                                             double *__restrict__ part) {
+  const int ch = blockIdx.x;
+  const int64_t s0 = cstart[ch];
+  const int len = clen[ch];
+  double acc[kS];
+#pragma unroll
+  for (int j = 0; j < kS; j++) acc[j] = 0.0;
+  for (int i = threadIdx.x; i < len; i += kT) {
+    const double wi = (double)w[s0 + i];
+#pragma unroll
+    for (int j = 0; j < kS; j++)
+      if (j < k) acc[j] += (double)V[(int64_t)j * n + s0 + i] * wi;
+  }
+  __shared__ double sh[kS][kT / 64];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < kS; j++) {
+    if (j >= k) break;
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (l == 0) sh[j][wv] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double t = 0.0;
+    for (int q = 0; q < kT / 64; q++) t += sh[threadIdx.x][q];
+    part[(int64_t)ch * kS + threadIdx.x] = t;
+  }
+}
+
+// out[p * stride + j] = (sqrt of) the sum of problem p's chunk partials
+__global__ void k_bdot_final(int k, const int *__restrict__ cfirst, const double *__restrict__ part,
+                             double *__restrict__ out, int stride, int take_sqrt) {
+  const int p = blockIdx.x, j = threadIdx.x;
+  if (j >= k) return;
+  double t = 0.0;
+  for (int ch = cfirst[p]; ch < cfirst[p + 1]; ch++) t += part[(int64_t)ch * kS + j];
+  out[(int64_t)p * stride + j] = take_sqrt ? sqrt(t) : t;
+}
+
+// w += sign * sum_j coef[p][j] V[j] over each chunk of problem p
+template <class T>
+__global__ void __launch_bounds__(kT) k_bmaxpy(int64_t n, int k, const T *__restrict__ V, const double *__restrict__ coef,
+                                               int stride, double sign, T *__restrict__ w,
+                                               const int64_t *__restrict__ cstart, const int *__restrict__ clen,
+                                               const int *__restrict__ cprob) {
+  const int ch = blockIdx.x;
+  const int64_t s0 = cstart[ch];
+  const int len = clen[ch];
+  const double *h = coef + (int64_t)cprob[ch] * stride;
+  for (int i = threadIdx.x; i < len; i += kT) {
+    double v = (double)w[s0 + i];
+    for (int j = 0; j < k; j++) v += sign * h[j] * (double)V[(int64_t)j * n + s0 + i];
+    w[s0 + i] = (T)v;
+  }
+}
+
+// v = w * (d[p] != 0 ? 1 / d[p] : 0) (inv) or v = w * d[p] (!inv), converting precision
+template <class Ti, class To>
+__global__ void __launch_bounds__(kT) k_bscale(const Ti *__restrict__ w, const double *__restrict__ d, int inv,
+                                               To *__restrict__ v, const int64_t *__restrict__ cstart,
+                                               const int *__restrict__ clen, const int *__restrict__ cprob) {
+  const int ch = blockIdx.x;
+  const int64_t s0 = cstart[ch];
+  const int len = clen[ch];
+  const double dd = d[cprob[ch]];
+  const double sc = inv ? (dd != 0.0 ? 1.0 / dd : 0.0) : dd;
+  for (int i = threadIdx.x; i < len; i += kT) v[s0 + i] = (To)(sc * (double)w[s0 + i]);
+}
+
+// x += d[p] * u over problem p's chunks (the refinement update)
+template <class T>
+__global__ void __launch_bounds__(kT) k_badd(const T *__restrict__ u, const double *__restrict__ d, double *__restrict__ x,
+                                             const int64_t *__restrict__ cstart, const int *__restrict__ clen,
+                                             const int *__restrict__ cprob) {
+  const int ch = blockIdx.x;
+  const int64_t s0 = cstart[ch];
+  const int len = clen[ch];
+  const double dd = d[cprob[ch]];
+  for (int i = threadIdx.x; i < len; i += kT) x[s0 + i] += dd * (double)u[s0 + i];
+}
+
+template <class T> __global__ void k_fill(int64_t n, T v, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
+}
+
+template <class T> __global__ void k_axpby_b(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+// cycle start: g = beta e_0, done where beta <= tol (or the problem is idle)
+__global__ void k_bstart(int np, int m, const double *__restrict__ beta, const double *__restrict__ tol,
+                         const int *__restrict__ live, double *__restrict__ g, int *__restrict__ done,
+                         int *__restrict__ jdone) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  for (int i = 0; i < kS; i++) g[(int64_t)p * kS + i] = 0.0;
+  g[(int64_t)p * kS] = beta[p];
+  const bool d = !live[p] || !(beta[p] > tol[p]);
+  done[p] = d ? 1 : 0;
+  jdone[p] = d ? 0 : m;
+}
+
+// Arnoldi column j of every live problem: assemble, rotate, test
+__global__ void k_bgivens(int np, int j, const double *__restrict__ h1, const double *__restrict__ h2,
+                          const double *__restrict__ hn, double *__restrict__ H, double *__restrict__ cs,
+                          double *__restrict__ sn, double *__restrict__ g, double *__restrict__ res,
+                          const double *__restrict__ tol, int *__restrict__ done, int *__restrict__ jdone,
+                          int *__restrict__ its) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || done[p]) return;
+  double *Hp = H + (int64_t)p * kS * kS, *c = cs + (int64_t)p * kS, *s = sn + (int64_t)p * kS,
+         *gp = g + (int64_t)p * kS;
+  for (int i = 0; i <= j; i++) Hp[i * kS + j] = h1[(int64_t)p * kS + i] + h2[(int64_t)p * kS + i];
+  const double hnext = hn[p];
+  for (int i = 0; i < j; i++) {
+    const double a = Hp[i * kS + j], b = Hp[(i + 1) * kS + j];
+    Hp[i * kS + j] = c[i] * a + s[i] * b;
+    Hp[(i + 1) * kS + j] = -s[i] * a + c[i] * b;
+  }
+  const double a = Hp[j * kS + j];
+  const double rr = hypot(a, hnext);
+  c[j] = rr != 0.0 ? a / rr : 1.0;
+  s[j] = rr != 0.0 ? hnext / rr : 0.0;
+  Hp[j * kS + j] = rr;
+  gp[j + 1] = -s[j] * gp[j];
+  gp[j] = c[j] * gp[j];
+  res[p] = fabs(gp[j + 1]);
+  its[p] += 1;
+  if (!(res[p] > tol[p]) || hnext == 0.0) {
+    done[p] = 1;
+    jdone[p] = j + 1;
+  }
+}
+
+// y = H^{-1} g on the leading jdone x jdone triangle; zero past it
+__global__ void k_btri(int np, const double *__restrict__ H, const double *__restrict__ g, const int *__restrict__ jdone,
+                       double *__restrict__ y) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double *Hp = H + (int64_t)p * kS * kS, *gp = g + (int64_t)p * kS;
+  double *yp = y + (int64_t)p * kS;
+  const int jd = jdone[p];
+  for (int i = kS - 1; i >= 0; i--) {
+    if (i >= jd) {
+      yp[i] = 0.0;
+      continue;
+    }
+    double v = gp[i];
+    for (int k = i + 1; k < jd; k++) v -= Hp[i * kS + k] * yp[k];
+    yp[i] = Hp[i * kS + i] != 0.0 ? v / Hp[i * kS + i] : 0.0;
+  }
+}
+
+unsigned grid_n(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 8192)); }
+
+template <class T> int prec_id() { return sizeof(T) == 8 ? UBT_FP64 : UBT_FP32; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+void Batch::create(const std::vector<Operator *> &o) {
+  UBT_CHECK(!o.empty(), UBT_ERR_ARG_SIZ, "a batch needs at least one problem");
+  ops = o;
+  const Disc &D0 = *o[0]->disc;
+  UBT_CHECK(D0.kind == DISC_DG, UBT_ERR_SUP, "batches hold DG problems");
+  dim = D0.dim;
+  nb = D0.nb;
+  nf = D0.nf;
+  off.assign(1, 0);
+  for (size_t p = 0; p < o.size(); p++) {
+    const Disc &D = *o[p]->disc;
+    UBT_CHECK(D.kind == DISC_DG && D.dim == dim && D.nb == nb && D.nf == nf, UBT_ERR_ARG_INCOMP,
+              "problem %zu is not a %dD DG problem with %d basis functions and %d faces per cell like problem 0", p,
+              dim, nb, nf);
+    UBT_CHECK(o[p]->xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections of problem %zu first", p);
+    UBT_CHECK(o[p]->device == o[0]->device, UBT_ERR_ARG_INCOMP, "the problems of a batch share one device");
+    off.push_back(off.back() + D.n_rows());
+  }
+  n_rows = off.back();
+  sweeps.clear();
+  for (auto *op : ops) {
+    sweeps.emplace_back(new Sweep());
+    sweeps.back()->create(op, true);
+  }
+  int dev = 0;
+  UBT_HIP(hipGetDevice(&dev));
+  UBT_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  set_group(0);
+  tune();
+}
+
+void Batch::set_group(int g) {
+  UBT_CHECK(g >= 0, UBT_ERR_ARG_OUTOFRANGE, "group %d is negative", g);
+  bool any = false;
+  active.assign(ops.size(), 0);
+  for (size_t p = 0; p < ops.size(); p++) {
+    if (ops[p]->n_groups > g) {
+      active[p] = 1;
+      ops[p]->group = g;
+      any = true;
+    }
+  }
+  UBT_CHECK(any, UBT_ERR_ARG_OUTOFRANGE, "no problem of the batch has group %d", g);
+  group = g;
+  build_lists();
+  gen++;
+}
+
+template <class T>
+static void build_items(Batch &b, int body) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  std::vector<BatchItem<T>> items;
+  std::vector<int64_t> tiles(1, 0);
+  size_t lds = 0;
+  bool ok = true;
+  for (size_t p = 0; p < b.ops.size() && ok; p++) {
+    if (!b.active[p]) continue;
+    int cpb = 1, lanes = 1;
+    size_t l = 0;
+    if (!batch_body_layout(*b.ops[p], pi == 0 ? UBT_FP64 : UBT_FP32, body, &cpb, &lanes, &l)) {
+      ok = false;
+      break;
+    }
+    BatchItem<T> it;
+    it.g = b.ops[p]->args<T>();
+    it.g.cpb = cpb;
+    it.g.red_lanes = lanes;
+    it.off = b.off[p];
+    items.push_back(it);
+    tiles.push_back(tiles.back() + (b.ops[p]->disc->n_cells + cpb - 1) / cpb);
+    lds = std::max(lds, l);
+  }
+  b.body_ok[pi][body] = ok;
+  if (!ok) return;
+  b.items[pi][body].upload(items);
+  b.tiles[pi][body].upload(tiles);
+  b.n_items[pi][body] = (int)items.size();
+  b.n_tiles[pi][body] = tiles.back();
+  b.lds[pi][body] = lds;
+}
+
+void Batch::build_lists() {
+  for (int body_i = 0; body_i < kBatchBodies; body_i++) {
+    build_items<double>(*this, body_i);
+    build_items<float>(*this, body_i);
+  }
+  UBT_CHECK(body_ok[0][0] && body_ok[1][0], UBT_ERR_SUP, "a problem of the batch has more than 256 angles");
+  for (int pi = 0; pi < 2; pi++)
+    if (!body_ok[pi][body[pi]]) body[pi] = 0;
+  // element blocks: prefix of cells x angles over the active problems
+  std::vector<int64_t> blk(1, 0);
+  for (size_t p = 0; p < ops.size(); p++)
+    if (active[p]) blk.push_back(blk.back() + (int64_t)ops[p]->disc->n_cells * ops[p]->disc->n_angles);
+  bblk.upload(blk);
+  n_blocks = blk.back();
+  // sweep: level l of the batch = level l of every active problem
+  int levels = 0;
+  for (size_t p = 0; p < ops.size(); p++)
+    if (active[p]) levels = std::max(levels, sweeps[p]->n_levels);
+  std::vector<int> pr, ce, an;
+  std::vector<unsigned char> up;
+  std::vector<int64_t> loff(1, 0);
+  for (int l = 0; l < levels; l++) {
+    int q = 0;
+    for (size_t p = 0; p < ops.size(); p++) {
+      if (!active[p]) continue;
+      const Sweep &s = *sweeps[p];
+      if (l < s.n_levels)
+        for (int64_t i = s.level_off[l]; i < s.level_off[l + 1]; i++) {
+          pr.push_back(q);
+          ce.push_back(s.h_cells[i]);
+          an.push_back(s.h_angles[i]);
+          up.push_back(s.h_up[i]);
+        }
+      q++;
+    }
+    loff.push_back((int64_t)pr.size());
+  }
+  sweep.n_levels = levels;
+  sweep.level_off = loff;
+  sweep.prob.upload(pr);
+  sweep.cells.upload(ce);
+  sweep.angles.upload(an);
+  sweep.upwind.upload(up);
+}
+
+void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {
+  UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
+  UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the batched apply needs distinct x and y");
+  if (prec == UBT_FP64)
+    batch_apply_launch<double>(*this, body[0], (const double *)x, (double *)y, scatter, s);
+  else if (prec == UBT_FP32)
+    batch_apply_launch<float>(*this, body[1], (const float *)x, (float *)y, scatter, s);
+  else
+    fail(UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
+}
+
+void Batch::blocks_setup(hipStream_t s) {
+  for (size_t p = 0; p < ops.size(); p++) {
+    if (!active[p]) continue;
+    BlockInverse bi;
+    bi.op = ops[p];
+    bi.setup(s);  // UBT_ERR_MAT_LU_ZRPVT on a singular block
+  }
+}
+
+void Batch::blocks_apply(int prec, const void *x, void *y, hipStream_t s) const {
+  if (prec == UBT_FP64)
+    batch_blocks_launch<double>(*this, (const double *)x, (double *)y, s);
+  else
+    batch_blocks_launch<float>(*this, (const float *)x, (float *)y, s);
+}
+
+void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const {
+  if (prec == UBT_FP64)
+    batch_sweep_launch<double>(*this, (const double *)r, (double *)psi, s);
+  else
+    batch_sweep_launch<float>(*this, (const float *)r, (float *)psi, s);
+}
+
+void Batch::rhs(int prec, void *b, hipStream_t s) const {
+  const size_t ts = prec == UBT_FP64 ? 8 : 4;
+  for (size_t p = 0; p < ops.size(); p++) {
+    char *bp = (char *)b + off[p] * ts;
+    const int64_t n = off[p + 1] - off[p];
+    if (active[p]) {
+      ops[p]->rhs(group, prec, bp, s);
+    } else if (prec == UBT_FP64) {
+      hipLaunchKernelGGL(k_fill<double>, dim3(grid_n(n)), dim3(kT), 0, s, n, 0.0, (double *)bp);
+    } else {
+      hipLaunchKernelGGL(k_fill<float>, dim3(grid_n(n)), dim3(kT), 0, s, n, 0.f, (float *)bp);
+    }
+  }
+  UBT_HIP(hipGetLastError());
+}
+
+void Batch::set_scalar_flux(int prec, const void *psi, hipStream_t s) const {
+  const size_t ts = prec == UBT_FP64 ? 8 : 4;
+  for (size_t p = 0; p < ops.size(); p++)
+    if (active[p]) ops[p]->set_scalar_flux(group, prec, (const char *)psi + off[p] * ts, s);
+}
+
+void Batch::tune() {
+  hipStream_t s = nullptr;
+  UBT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  UBT_HIP(hipEventCreate(&e0));
+  UBT_HIP(hipEventCreate(&e1));
+  DevBuf xb, yb;
+  xb.alloc(sizeof(double) * n_rows);
+  yb.alloc(sizeof(double) * n_rows);
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_n(n_rows)), dim3(kT), 0, s, n_rows, 0.0, xb.as<double>());
+  const int persists[3] = {2, 4, 8};
+  for (int pi = 0; pi < 2; pi++) {
+    float best = 1e30f;
+    int best_b = 0, best_p = 8;
+    for (int bo = 0; bo < kBatchBodies; bo++) {
+      tuned_us[pi][bo] = 0.f;
+      if (!body_ok[pi][bo]) continue;
+      float bb = 1e30f;
+      for (int pp : persists) {
+        body[pi] = bo;
+        persist[pi] = pp;
+        const int prec = pi == 0 ? UBT_FP64 : UBT_FP32;
+        for (int it = 0; it < 2; it++) mult(prec, xb.ptr, yb.ptr, true, s);
+        UBT_HIP(hipEventRecord(e0, s));
+        for (int it = 0; it < 5; it++) mult(prec, xb.ptr, yb.ptr, true, s);
+        UBT_HIP(hipEventRecord(e1, s));
+        UBT_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        UBT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1000.f / 5.f;
+        bb = std::min(bb, us);
+        if (us < best) {
+          best = us;
+          best_b = bo;
+          best_p = pp;
+        }
+      }
+      tuned_us[pi][bo] = bb;
+    }
+    body[pi] = best_b;
+    persist[pi] = best_p;
+  }
+  UBT_HIP(hipStreamSynchronize(s));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+}
+
+// ---------------------------------------------------------------------------
+// BatchSolver
+// ---------------------------------------------------------------------------
+BatchSolver::~BatchSolver() {
+  for (auto &gx : graph)
+    if (gx) (void)hipGraphExecDestroy(gx);
+  if (cap) (void)hipStreamDestroy(cap);
+}
+
+void BatchSolver::create(Batch *b, const SolverOptions &o) {
+  batch = b;
+  opt = o;
+  UBT_CHECK(opt.restart >= 1 && opt.restart + 1 <= kS, UBT_ERR_ARG_OUTOFRANGE, "restart must be 1 to %d, got %d", kS - 1,
+            opt.restart);
+  UBT_CHECK(opt.rtol > 0.0 && opt.max_it >= 1, UBT_ERR_ARG_OUTOFRANGE, "rtol must be positive and max_it >= 1");
+  n = b->n_rows;
+  np = (int)b->ops.size();
+  const int m = opt.restart;
+  if (!opt.mixed) {
+    V64.alloc(sizeof(double) * n * (m + 1));
+    Z64.alloc(sizeof(double) * n * m);
+    w64.alloc(sizeof(double) * n);
+    t64.alloc(sizeof(double) * n);
+    u64.alloc(sizeof(double) * n);
+    b32.alloc(sizeof(double) * n);  // the internal right-hand side (FP64 here)
+    d32.alloc(sizeof(double) * n);  // the internal solution (FP64 here)
+  } else {
+    V32.alloc(sizeof(float) * n * (m + 1));
+    Z32.alloc(sizeof(float) * n * m);
+    w32.alloc(sizeof(float) * n);
+    t32.alloc(sizeof(float) * n);
+    u32.alloc(sizeof(float) * n);
+    d32.alloc(sizeof(float) * n);
+    b32.alloc(sizeof(float) * n);
+    r64.alloc(sizeof(double) * n);
+  }
+  const size_t P = (size_t)np;
+  H.alloc(sizeof(double) * P * kS * kS);
+  cs.alloc(sizeof(double) * P * kS);
+  sn.alloc(sizeof(double) * P * kS);
+  gv.alloc(sizeof(double) * P * kS);
+  hc.alloc(sizeof(double) * P * kS * 2);
+  yv.alloc(sizeof(double) * P * kS);
+  nrm.alloc(sizeof(double) * P);
+  tol.alloc(sizeof(double) * P);
+  res.alloc(sizeof(double) * P);
+  onrm.alloc(sizeof(double) * P);
+  bnorm.alloc(sizeof(double) * P);
+  done.alloc(sizeof(int) * P * 4);  // done, jdone, its, live
+  UBT_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+}
+
+void BatchSolver::refresh(hipStream_t s) {
+  batch->blocks_setup(s);
+  // chunks of the active problems; the live flags
+  std::vector<int64_t> st;
+  std::vector<int> ln, pr, first(np + 1, 0), live(np, 0);
+  for (int p = 0; p < np; p++) {
+    first[p] = (int)st.size();
+    if (!batch->active[p]) continue;
+    live[p] = 1;
+    for (int64_t r = batch->off[p]; r < batch->off[p + 1]; r += kChunk) {
+      st.push_back(r);
+      ln.push_back((int)std::min<int64_t>(kChunk, batch->off[p + 1] - r));
+      pr.push_back(p);
+    }
+  }
+  first[np] = (int)st.size();
+  n_chunks = (int)st.size();
+  chunk_start.upload(st);
+  chunk_len.upload(ln);
+  chunk_prob.upload(pr);
+  part.alloc(sizeof(double) * (size_t)std::max(1, n_chunks) * kS);
+  cfirst.upload(first);
+  UBT_HIP(hipMemcpyAsync(done.as<int>() + 3 * np, live.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
+  UBT_HIP(hipStreamSynchronize(s));
+  for (auto &gx : graph) {
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+  }
+  gen = batch->gen;
+}
+
+// y = M^{-1} r: element blocks, then the sweep on the L-residual
+template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
+  T *t = sizeof(T) == 8 ? t64.as<T>() : t32.as<T>();
+  T *u = sizeof(T) == 8 ? u64.as<T>() : u32.as<T>();
+  const int prec = prec_id<T>();
+  const unsigned gn = grid_n(n);
+  bool first = true;
+  if (opt.pc_blocks) {
+    batch->blocks_apply(prec, r, y, s);
+    first = false;
+  }
+  if (opt.pc_sweep) {
+    if (first) {
+      batch->sweep_apply(prec, r, y, s);
+      first = false;
+    } else {
+      batch->mult(prec, y, t, false, s);
+      hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, r, -1.0, t);
+      batch->sweep_apply(prec, t, u, s);
+      hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, (const T *)u, 1.0, y);
+    }
+  }
+  if (first) hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, r, 0.0, y);
+  UBT_HIP(hipGetLastError());
+}
+
+// one restart cycle of every problem: b, x are the internal vectors
+template <class T> void BatchSolver::cycle(const T *b, T *x, hipStream_t s) {
+  const int m = opt.restart, prec = prec_id<T>();
+  T *V = sizeof(T) == 8 ? V64.as<T>() : V32.as<T>();
+  T *Z = sizeof(T) == 8 ? Z64.as<T>() : Z32.as<T>();
+  T *w = sizeof(T) == 8 ? w64.as<T>() : w32.as<T>();
+  const int64_t *cst = chunk_start.as<int64_t>();
+  const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
+  int *dn = done.as<int>(), *jd = dn + np, *its = dn + 2 * np, *live = dn + 3 * np;
+  double *h1 = hc.as<double>(), *h2 = h1 + (size_t)np * kS, *hn = nrm.as<double>(), *pt = part.as<double>();
+  const unsigned gn = grid_n(n), gp = (unsigned)((np + 63) / 64), nc = (unsigned)std::max(1, n_chunks);
+  if (n_chunks == 0) return;
+  // r0 = b - A x, beta per problem, V0 = r0 / beta
+  batch->mult(prec, x, w, true, s);
+  hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, w);
+  hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
+  hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
+  hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, V, cst, cln, cpr);
+  hipLaunchKernelGGL(k_bstart, dim3(gp), dim3(64), 0, s, np, m, (const double *)hn, (const double *)tol.as<double>(),
+                     (const int *)live, gv.as<double>(), dn, jd);
+  for (int j = 0; j < m; j++) {
+    T *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *vn = V + (int64_t)(j + 1) * n;
+    apply_pc<T>(vj, zj, s);
+    batch->mult(prec, zj, w, true, s);
+    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
+    hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h1, kS, -1.0, w,
+                       cst, cln, cpr);
+    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
+    hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0, w,
+                       cst, cln, cpr);
+    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
+    hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, vn, cst, cln, cpr);
+    hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1, (const double *)h2,
+                       (const double *)hn, H.as<double>(), cs.as<double>(), sn.as<double>(), gv.as<double>(),
+                       res.as<double>(), (const double *)tol.as<double>(), dn, jd, its);
+  }
+  hipLaunchKernelGGL(k_btri, dim3(gp), dim3(64), 0, s, np, (const double *)H.as<double>(),
+                     (const double *)gv.as<double>(), (const int *)jd, yv.as<double>());
+  hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, m, (const T *)Z, (const double *)yv.as<double>(), kS, 1.0,
+                     x, cst, cln, cpr);
+  UBT_HIP(hipGetLastError());
+}
+
+template <class T>
+void BatchSolver::fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  UBT_HIP(hipMemcpyAsync(tol.as<double>(), tol_abs, sizeof(double) * np, hipMemcpyHostToDevice, s));
+  UBT_HIP(hipStreamSynchronize(s));
+  if (use_graph && !graph[pi]) {
+    hipGraph_t gr = nullptr;
+    UBT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    cycle<T>(b, x, cap);
+    UBT_HIP(hipStreamEndCapture(cap, &gr));
+    UBT_HIP(hipGraphInstantiate(&graph[pi], gr, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(gr);
+  }
+  std::vector<int> dn(np);
+  for (int c = 0; c < max_cycles; c++) {
+    if (use_graph)
+      UBT_HIP(hipGraphLaunch(graph[pi], cap));
+    else
+      cycle<T>(b, x, cap);
+    UBT_HIP(hipMemcpyAsync(dn.data(), done.as<int>(), sizeof(int) * np, hipMemcpyDeviceToHost, cap));
+    UBT_HIP(hipStreamSynchronize(cap));
+    cycles++;
+    bool all = true;
+    for (int p = 0; p < np; p++) all = all && dn[p];
+    if (all) break;
+  }
+}
+
+void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
+  UBT_CHECK(b && x, UBT_ERR_ARG_NULL, "b and x are required");
+  UBT_CHECK(b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
+  UBT_CHECK(n_chunks > 0 && gen == batch->gen, UBT_ERR_ARG_WRONGSTATE,
+            "refresh the batch solver after setting the batch group");
+  const int m = opt.restart;
+  const int max_cycles = std::max(1, (opt.max_it + m - 1) / m);
+  const int64_t *cst = chunk_start.as<int64_t>();
+  const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
+  const unsigned gn = grid_n(n), nc = (unsigned)n_chunks;
+  double *pt = part.as<double>();
+  // the caller's stream, then the capture stream
+  UBT_HIP(hipStreamSynchronize(s));
+  cycles = outer = 0;
+  iterations.assign(np, 0);
+  reason.assign(np, 0);
+  rel_residual.assign(np, 0.0);
+  UBT_HIP(hipMemsetAsync(done.as<int>() + 2 * np, 0, sizeof(int) * np, cap));  // iteration counters
+  // ||b|| per problem
+  hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, b, b, cst, cln, pt);
+  hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, bnorm.as<double>(), 1, 1);
+  std::vector<double> bn(np), rn(np), tl(np);
+  std::vector<int> live(np);
+  UBT_HIP(hipMemcpyAsync(bn.data(), bnorm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
+  UBT_HIP(hipMemcpyAsync(live.data(), done.as<int>() + 3 * np, sizeof(int) * np, hipMemcpyDeviceToHost, cap));
+  UBT_HIP(hipStreamSynchronize(cap));
+  if (!opt.mixed) {
+    double *bi = b32.as<double>(), *xi = d32.as<double>();
+    hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, 0.0, bi);
+    hipLaunchKernelGGL(k_fill<double>, dim3(gn), dim3(kT), 0, cap, n, 0.0, xi);
+    for (int p = 0; p < np; p++) tl[p] = opt.rtol * bn[p];
+    fgmres<double>(bi, xi, tl.data(), max_cycles, cap);
+    // true residuals
+    double *w = w64.as<double>();
+    batch->mult(UBT_FP64, xi, w, true, cap);
+    hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, w);
+    hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)w, (const double *)w, cst, cln, pt);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, nrm.as<double>(), 1, 1);
+    hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, (const double *)xi, 0.0, x);
+    UBT_HIP(hipMemcpyAsync(rn.data(), nrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
+    outer = 1;
+  } else {
+    double *r = r64.as<double>();
+    float *bf = b32.as<float>(), *d = d32.as<float>();
+    hipLaunchKernelGGL(k_fill<double>, dim3(gn), dim3(kT), 0, cap, n, 0.0, x);
+    for (outer = 0; outer <= opt.max_outer; outer++) {
+      batch->mult(UBT_FP64, x, r, true, cap);
+      hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, r);
+      hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)r, (const double *)r, cst, cln, pt);
+      hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, onrm.as<double>(), 1, 1);
+      UBT_HIP(hipMemcpyAsync(rn.data(), onrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
+      UBT_HIP(hipStreamSynchronize(cap));
+      bool all = true;
+      for (int p = 0; p < np; p++) {
+        const bool conv = !live[p] || rn[p] <= opt.rtol * bn[p];
+        all = all && conv;
+        tl[p] = conv ? 1e300 : opt.inner_rtol;  // a converged problem sits the inner solve out
+      }
+      int its_max = 0;
+      {
+        std::vector<int> it(np);
+        UBT_HIP(hipMemcpyAsync(it.data(), done.as<int>() + 2 * np, sizeof(int) * np, hipMemcpyDeviceToHost, cap));
+        UBT_HIP(hipStreamSynchronize(cap));
+        for (int v : it) its_max = std::max(its_max, v);
+      }
+      if (all || outer == opt.max_outer || its_max >= opt.max_it) break;
+      // correction: A d = r / |r| per problem, FP32 storage, to inner_rtol
+      hipLaunchKernelGGL((k_bscale<double, float>), dim3(nc), dim3(kT), 0, cap, (const double *)r,
+                         (const double *)onrm.as<double>(), 1, bf, cst, cln, cpr);
+      hipLaunchKernelGGL(k_fill<float>, dim3(gn), dim3(kT), 0, cap, n, 0.f, d);
+      fgmres<float>(bf, d, tl.data(), max_cycles, cap);
+      // the inner cycles reuse nrm; the outer scale has its own buffer
+      hipLaunchKernelGGL(k_badd<float>, dim3(nc), dim3(kT), 0, cap, (const float *)d, (const double *)onrm.as<double>(),
+                         x, cst, cln, cpr);
+      UBT_HIP(hipGetLastError());
+    }
+  }
+  std::vector<int> it(np);
+  UBT_HIP(hipMemcpyAsync(it.data(), done.as<int>() + 2 * np, sizeof(int) * np, hipMemcpyDeviceToHost, cap));
+  UBT_HIP(hipStreamSynchronize(cap));
+  for (int p = 0; p < np; p++) {
+    iterations[p] = it[p];
+    rel_residual[p] = live[p] ? (bn[p] > 0.0 ? rn[p] / bn[p] : 0.0) : 0.0;
+    reason[p] = !live[p] ? 0 : (rel_residual[p] <= opt.rtol ? 1 : -1);
+  }
+}
+
+template void BatchSolver::apply_pc<double>(const double *, double *, hipStream_t);
+template void BatchSolver::apply_pc<float>(const float *, float *, hipStream_t);
+
+}  // namespace ubt

@@ -16,6 +16,7 @@
 
 #include <unordered_map>
 
+#include "batch.hpp"
 #include "operator.hpp"
 
 namespace ubt {
@@ -118,14 +119,12 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
 // ADJ: the adjoint A^T = V P L P V^{-1} - S^T (P reverses every angle): the
 // same kernel with Omega -> -Omega, each neighbour's trace scaled by V_c / V_n,
 // and the scatter transposed.
-template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ = false>
-__global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ>
+__device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
+                                             int64_t tile) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int FM = NB * NB;
   const int cpb = g.cpb, A = g.A;
-  const int64_t n_tiles = ((int64_t)g.n_cells + cpb - 1) / cpb;
-  const int64_t tile = xcd_tile(n_tiles);
-  if (tile >= n_tiles) return;
   const int64_t c0 = tile * cpb;
   const int ncell = (int)((int64_t)g.n_cells - c0 < cpb ? (int64_t)g.n_cells - c0 : cpb);
   // LDS: F_own, F_up, face s (DIM per slot), grad (NB-1 x DIM per cell),
@@ -263,6 +262,14 @@ __global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__r
 #pragma unroll
     for (int i = 0; i < NB; i++) y[base + (int64_t)i * A] = acc[i];
   }
+}
+
+template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ = false>
+__global__ void __launch_bounds__(1024) dg_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  const int64_t n_tiles = ((int64_t)g.n_cells + g.cpb - 1) / g.cpb;
+  const int64_t tile = xcd_tile(n_tiles);
+  if (tile >= n_tiles) return;
+  dg_tile_body<T, DIM, NB, NF, SCATTER, ADJ>(x, y, g, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,13 +439,11 @@ template <class T> __device__ inline void st4(T *__restrict__ p, const T v[4]) {
 }
 
 template <class T, int DIM, int NB, int NF, bool SCATTER>
-__global__ void __launch_bounds__(256) dg_apply_q4(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+__device__ __forceinline__ void dg_q4_body(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
+                                           int64_t tile) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int FM = NB * NB;
   const int cpb = g.cpb, A = g.A, TPC = A >> 2;
-  const int64_t n_tiles = ((int64_t)g.n_cells + cpb - 1) / cpb;
-  const int64_t tile = xcd_tile(n_tiles);
-  if (tile >= n_tiles) return;
   const int64_t c0 = tile * cpb;
   const int ncell = (int)((int64_t)g.n_cells - c0 < cpb ? (int64_t)g.n_cells - c0 : cpb);
   T *s_own = reinterpret_cast<T *>(smem);
@@ -597,6 +602,14 @@ __global__ void __launch_bounds__(256) dg_apply_q4(const T *__restrict__ x, T *_
 #pragma unroll
     for (int j = 0; j < NB; j++) st4(y + (c * NB + j) * A + a0, acc[j]);
   }
+}
+
+template <class T, int DIM, int NB, int NF, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_q4(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  const int64_t n_tiles = ((int64_t)g.n_cells + g.cpb - 1) / g.cpb;
+  const int64_t tile = xcd_tile(n_tiles);
+  if (tile >= n_tiles) return;
+  dg_q4_body<T, DIM, NB, NF, SCATTER>(x, y, g, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -821,6 +834,33 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
     gc = gcn;
     st = stn;
     ssg = ssn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched apply: one persistent launch over the tiles of many problems. The
+// work list holds each active problem's operator arguments and row offset;
+// tile_off is the prefix of tiles per problem. Workgroups are ranked
+// XCD-major and each walks one contiguous run of tiles, so a run stays within
+// one or two problems and its upwind gathers in one XCD's L2.
+// ---------------------------------------------------------------------------
+template <class T, int DIM, int NB, int NF, int BODY, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T *__restrict__ y,
+                                                      const BatchItem<T> *__restrict__ items,
+                                                      const int64_t *__restrict__ tile_off, int64_t total) {
+  const int64_t G = gridDim.x;
+  const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  const int64_t chunk = (total + G - 1) / G;
+  const int64_t t0 = rank * chunk, t1 = t0 + chunk < total ? t0 + chunk : total;
+  int p = 0;
+  for (int64_t t = t0; t < t1; t++) {
+    while (t >= tile_off[p + 1]) p++;
+    const BatchItem<T> &it = items[p];
+    if (BODY == 0)
+      dg_tile_body<T, DIM, NB, NF, SCATTER, false>(x + it.off, y + it.off, it.g, t - tile_off[p]);
+    else
+      dg_q4_body<T, DIM, NB, NF, SCATTER>(x + it.off, y + it.off, it.g, t - tile_off[p]);
+    __syncthreads();  // the next tile restages the LDS
   }
 }
 
@@ -1281,6 +1321,37 @@ __global__ void blockinv_apply(const T *__restrict__ x, T *__restrict__ y, OpArg
   }
 #pragma unroll
   for (int i = 0; i < NB; i++) y[base + (int64_t)i * g.A] = v[i];
+}
+
+// element blocks of many problems: block index -> problem by bisection of the
+// block prefix
+template <class T, int NB>
+__global__ void blockinv_batch(const T *__restrict__ x, T *__restrict__ y, const BatchItem<T> *__restrict__ items,
+                               const int64_t *__restrict__ blk_off, int n_items, int64_t total) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= total) return;
+  int lo = 0, hi = n_items;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blk_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const BatchItem<T> &it = items[lo];
+  const int A = it.g.A;
+  const int64_t bb = b - blk_off[lo];
+  const int64_t c = bb / A;
+  const int a = (int)(bb - c * A);
+  T B[NB][NB], v[NB];
+  build_block<T, NB>(it.g, c, a, 0, B);
+  const int64_t base = it.off + c * NB * A + a;
+#pragma unroll
+  for (int j = 0; j < NB; j++) v[j] = x[base + (int64_t)j * A];
+  if (!gj_solve<T, NB>(B, v)) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) v[j] = T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < NB; i++) y[base + (int64_t)i * A] = v[i];
 }
 
 template <int NB>
@@ -2102,5 +2173,94 @@ void BlockInverse::apply(int prec, const void *x, void *y, hipStream_t s, bool t
 #undef BI
   UBT_HIP(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------
+// Batched launchers
+// ---------------------------------------------------------------------------
+bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int *lanes_out, size_t *lds_out) {
+  const Disc &D = *op.disc;
+  if (D.kind != DISC_DG) return false;
+  const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
+  const size_t ts = prec == UBT_FP64 ? 8 : 4;
+  const size_t geo = (size_t)((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) * ts +
+                     sizeof(int) * (size_t)nf;
+  int cpb, lanes = 1;
+  size_t lds;
+  if (body == 0) {
+    if (A > 256) return false;
+    cpb = 256 / A;
+    auto lds_for = [&](int cp) { return (size_t)(cp * nb * A + cp * nb) * ts + (size_t)cp * geo; };
+    while (cpb > 1 && lds_for(cpb) > 65536) cpb--;
+    lds = lds_for(cpb);
+    int S = pow2_floor(std::max(1, 256 / (cpb * nb)));
+    S = std::min(std::min(S, 64), pow2_floor(A));
+    lanes = std::max(1, S);
+  } else {
+    if (A % 4 != 0 || A / 4 > 256) return false;
+    const int TPC = A / 4;
+    cpb = 256 / TPC;
+    auto lds_for = [&](int cp) { return (size_t)cp * geo + (size_t)(cp * nb * TPC + cp * nb) * ts; };
+    while (cpb > 1 && lds_for(cpb) > 65536) cpb--;
+    lds = lds_for(cpb);
+  }
+  if (lds > 65536) return false;
+  *cpb_out = cpb;
+  *lanes_out = lanes;
+  *lds_out = lds;
+  return true;
+}
+
+template <class T>
+void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const int64_t total = b.n_tiles[pi][body];
+  if (total == 0) return;
+  int64_t g = std::min<int64_t>(total, (int64_t)b.n_cu * b.persist[pi]);
+  g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+  const unsigned grid = (unsigned)g;
+  const BatchItem<T> *items = b.items[pi][body].as<BatchItem<T>>();
+  const int64_t *toff = b.tiles[pi][body].as<int64_t>();
+  const size_t lds = b.lds[pi][body];
+  bool launched = false;
+#define BT(DIM, NB, NF, BODY)                                                                                       \
+  if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF && body == BODY) {                                     \
+    if (scatter)                                                                                                    \
+      hipLaunchKernelGGL((dg_apply_batch<T, DIM, NB, NF, BODY, true>), dim3(grid), dim3(256), lds, s, x, y, items, \
+                         toff, total);                                                                              \
+    else                                                                                                            \
+      hipLaunchKernelGGL((dg_apply_batch<T, DIM, NB, NF, BODY, false>), dim3(grid), dim3(256), lds, s, x, y,       \
+                         items, toff, total);                                                                       \
+    launched = true;                                                                                                \
+  }
+#define BT_ALL(BODY) BT(2, 1, 3, BODY) BT(2, 1, 4, BODY) BT(3, 1, 4, BODY) BT(3, 1, 6, BODY) \
+                     BT(2, 3, 3, BODY) BT(2, 3, 4, BODY) BT(3, 4, 4, BODY) BT(3, 4, 6, BODY)
+  BT_ALL(0)
+  BT_ALL(1)
+#undef BT_ALL
+#undef BT
+  UBT_CHECK(launched, UBT_ERR_SUP, "no batched apply for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
+  UBT_HIP(hipGetLastError());
+}
+
+template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  if (b.n_blocks == 0) return;
+  const unsigned grid = (unsigned)((b.n_blocks + 255) / 256);
+  const BatchItem<T> *items = b.items[pi][0].as<BatchItem<T>>();
+  const int64_t *boff = b.bblk.as<int64_t>();
+  const int n = b.n_items[pi][0];
+  switch (b.nb) {
+    case 1: hipLaunchKernelGGL((blockinv_batch<T, 1>), dim3(grid), dim3(256), 0, s, x, y, items, boff, n, b.n_blocks); break;
+    case 3: hipLaunchKernelGGL((blockinv_batch<T, 3>), dim3(grid), dim3(256), 0, s, x, y, items, boff, n, b.n_blocks); break;
+    case 4: hipLaunchKernelGGL((blockinv_batch<T, 4>), dim3(grid), dim3(256), 0, s, x, y, items, boff, n, b.n_blocks); break;
+    default: fail(UBT_ERR_SUP, "element blocks of %d basis functions are not supported", b.nb);
+  }
+  UBT_HIP(hipGetLastError());
+}
+
+template void batch_apply_launch<double>(const Batch &, int, const double *, double *, bool, hipStream_t);
+template void batch_apply_launch<float>(const Batch &, int, const float *, float *, bool, hipStream_t);
+template void batch_blocks_launch<double>(const Batch &, const double *, double *, hipStream_t);
+template void batch_blocks_launch<float>(const Batch &, const float *, float *, hipStream_t);
 
 }  // namespace ubt

@@ -28,23 +28,18 @@
 
 #include <algorithm>
 
+#include "batch.hpp"
 #include "sweep.hpp"
 
 namespace ubt {
 
 namespace {
 
+// one (cell, direction) item of a level: gather, build the block, solve
 template <class T, int DIM, int NB, int NF, bool ADJ>
-__global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, const int *__restrict__ cell,
-                               const int *__restrict__ angle, const unsigned char *__restrict__ upwind, int64_t begin,
-                               int64_t count) {
-  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= count) return;
-  const int64_t item = begin + it;
-  const int64_t c = cell[item];
-  const int b = angle[item];               // the direction swept
-  const int a = ADJ ? g.neg[b] : b;        // the angle whose data it carries
-  const unsigned char up = upwind[item];  // bit f: face f reads its neighbour's trace
+__device__ __forceinline__ void sweep_item(const T *__restrict__ r, T *__restrict__ psi, const OpArgs<T> &g,
+                                           int64_t c, int b, unsigned char up) {
+  const int a = ADJ ? g.neg[b] : b;        // b: the direction swept; a: the angle whose data it carries
   const int A = g.A;
   T om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
   T B[NB][NB], v[NB];
@@ -143,6 +138,29 @@ __global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpA
   for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
 }
 
+template <class T, int DIM, int NB, int NF, bool ADJ>
+__global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, const int *__restrict__ cell,
+                               const int *__restrict__ angle, const unsigned char *__restrict__ upwind, int64_t begin,
+                               int64_t count) {
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= count) return;
+  const int64_t item = begin + it;
+  sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[item], angle[item], upwind[item]);
+}
+
+// a merged level of many problems: each item names its problem
+template <class T, int DIM, int NB, int NF>
+__global__ void sweep_dg_level_batch(const T *__restrict__ r, T *__restrict__ psi, const BatchItem<T> *__restrict__ items,
+                                     const int *__restrict__ prob, const int *__restrict__ cell,
+                                     const int *__restrict__ angle, const unsigned char *__restrict__ upwind,
+                                     int64_t begin, int64_t count) {
+  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= count) return;
+  const int64_t item = begin + it;
+  const BatchItem<T> &bi = items[prob[item]];
+  sweep_item<T, DIM, NB, NF, false>(r + bi.off, psi + bi.off, bi.g, cell[item], angle[item], upwind[item]);
+}
+
 // Structured: thread = (angle, j', k'); the level fixes i' = l - j' - k'.
 // Every cell has the same volume, so the adjoint only swaps the angle index.
 template <class T, int DIM, bool ADJ>
@@ -183,7 +201,7 @@ __global__ void sweep_fd_level(const T *__restrict__ r, T *__restrict__ psi, OpA
 // ---------------------------------------------------------------------------
 // Host: the DG level schedule
 // ---------------------------------------------------------------------------
-void Sweep::create(Operator *o) {
+void Sweep::create(Operator *o, bool keep_host) {
   op = o;
   const Disc &D = *o->disc;
   if (D.kind == DISC_FD) {
@@ -305,6 +323,11 @@ void Sweep::create(Operator *o) {
   cells.upload(oc);
   angles.upload(oa);
   upwind.upload(ol);
+  if (keep_host) {
+    h_cells.swap(oc);
+    h_angles.swap(oa);
+    h_up.swap(ol);
+  }
 }
 
 void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) const {
@@ -371,5 +394,32 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   }
   UBT_HIP(hipGetLastError());
 }
+
+template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const BatchSweep &w = b.sweep;
+  const BatchItem<T> *items = b.items[pi][0].as<BatchItem<T>>();
+  const int *pr = w.prob.as<int>(), *ci = w.cells.as<int>(), *ai = w.angles.as<int>();
+  const unsigned char *up = w.upwind.as<unsigned char>();
+  for (int l = 0; l < w.n_levels; l++) {
+    const int64_t beg = w.level_off[l], n = w.level_off[l + 1] - w.level_off[l];
+    if (n == 0) continue;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    bool launched = false;
+#define SWB(DIM, NB, NF)                                                                                           \
+  if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF) {                                                     \
+    hipLaunchKernelGGL((sweep_dg_level_batch<T, DIM, NB, NF>), dim3(grid), dim3(256), 0, s, r, psi, items, pr, ci, \
+                       ai, up, beg, n);                                                                            \
+    launched = true;                                                                                               \
+  }
+    SWB(2, 1, 3) SWB(2, 1, 4) SWB(3, 1, 4) SWB(3, 1, 6) SWB(2, 3, 3) SWB(2, 3, 4) SWB(3, 4, 4) SWB(3, 4, 6)
+#undef SWB
+    UBT_CHECK(launched, UBT_ERR_SUP, "no batched sweep for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
+  }
+  UBT_HIP(hipGetLastError());
+}
+
+template void batch_sweep_launch<double>(const Batch &, const double *, double *, hipStream_t);
+template void batch_sweep_launch<float>(const Batch &, const float *, float *, hipStream_t);
 
 }  // namespace ubt

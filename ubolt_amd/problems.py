@@ -191,3 +191,19 @@ def dg_box_3d(n, sn, order=1, seed=5, G=1, block=None, bcs=None):
     src = np.zeros((G, nc))
     src[0] = 1.0
     return Problem("dg%d_hex_%d_s%d_g%d" % (order, n, sn, G), q, disc, st, ss, src, n_groups=G)
+
+
+def batch_quads(n_problems=64, seed=11, side=(32, 256), sns=(4, 6, 8, 10, 12), groups=(1, 4), c_range=(0.3, 0.99)):
+    """The batched ensemble: n_problems 2D DG1 problems on quad boxes of the
+    unit square, n_x and n_y drawn independently from U{side}, the S_N order
+    from sns, the group count from U{groups}, sigma_t ~ U[0.1, 10], scattering
+    ratio ~ U[c_range] (downscatter tables when G > 1), unit source in group 0,
+    vacuum. Problem k uses seed * 1000 + k for its cross sections."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for k in range(n_problems):
+        nx, ny = (int(v) for v in rng.integers(side[0], side[1] + 1, 2))
+        sn = int(rng.choice(sns))
+        G = int(rng.integers(groups[0], groups[1] + 1))
+        probs.append(dg_box_2d(nx, ny, sn, order=1, seed=seed * 1000 + k, c_range=c_range, G=G))
+    return probs
