@@ -16,6 +16,12 @@ sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
+`--batch` runs the ensemble workload instead (64 independent 2D DG1 quad
+problems, 32..256 cells per side, S4-S12, 1-4 groups): the batched FP32 / FP64
+apply in one persistent launch against 64 separate launches, and the
+time to a 1e-8 residual of every problem and group, FP64 against mixed; it
+prints its own JSON line.
+
 `--sensitivity` runs the adjoint workload instead (3D DG1 hexes 64^3, S6, 4
 downscatter groups, cross sections piecewise constant on 8^3-cell blocks):
 gradients of the volume integral of the group-summed scalar flux over
@@ -49,6 +55,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sensitivity", action="store_true", help="run the adjoint / sensitivity workload")
+    ap.add_argument("--batch", action="store_true", help="run the batched ensemble workload")
+    ap.add_argument("--batch-n", type=int, default=64, help="problems in the ensemble")
     ap.add_argument("--sens-n", type=int, default=64, help="hexes per side of the sensitivity workload")
     ap.add_argument("--sens-cells", type=int, default=32, help="finite-difference checks (one cell each)")
     ap.add_argument("--copy-ref", action="store_true",
@@ -188,6 +196,91 @@ def sensitivity(args):
         "max_rel_err": float(errs.max()), "median_rel_err": float(np.median(errs)), "checks": checks}))
 
 
+def batch_bench(args):
+    """The ensemble: apply throughput of one persistent launch over every
+    problem, and the time to a 1e-8 true residual for every problem and group
+    (group Gauss-Seidel, one batched solve per group), FP64 against mixed."""
+    import torch
+    import ubolt_amd as u
+    from ubolt_amd import problems as P
+
+    t0 = time.perf_counter()
+    probs = P.batch_quads(n_problems=args.batch_n, seed=11)
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    setup_s = time.perf_counter() - t0
+    G = max(p.n_groups for p in probs)
+    b.set_group(0)
+    n = b.n_rows
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"metric": "batched DG1 apply GDoF/s + time-to-1e-8 of every problem, mixed vs FP64",
+           "workload": f"{args.batch_n} DG1 quad problems, 32..256 per side, S4-S12, 1-4 groups",
+           "n_problems": args.batch_n, "groups_max": G, "dof_group0": n,
+           "dof_all_groups": int(sum(p.disc.n_rows * p.n_groups for p in probs)), "setup_s": round(setup_s, 2),
+           "tuning": b.tuning()}
+    for name, dt, sz in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        x = torch.randn(n, dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            b.mult(x, y)
+        torch.cuda.synchronize()
+        steps = 50
+        e0.record()
+        for _ in range(steps):
+            b.mult(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        per = e0.elapsed_time(e1) / steps * 1e-3
+        # the same work as 64 launches, one per problem
+        xs = [b.part(x, p) for p in range(b.n_problems)]
+        ys = [b.part(y, p) for p in range(b.n_problems)]
+        e0.record()
+        for _ in range(10):
+            for op, xp, yp in zip(ops, xs, ys):
+                op.mult(xp, yp)
+        e1.record()
+        torch.cuda.synchronize()
+        per_sep = e0.elapsed_time(e1) / 10 * 1e-3
+        ach = 2 * sz * n / per / 1e9
+        out[f"apply_{name}"] = {"us": round(per * 1e6, 1), "gdofs": round(n / per / 1e9, 2),
+                                "roofline_frac": round(ach / HBM_PEAK_GBS, 4), "achieved_gbs": round(ach, 1),
+                                "separate_launches_us": round(per_sep * 1e6, 1)}
+        del x, y, xs, ys
+    torch.cuda.empty_cache()
+    for mode in ("fp64", "mixed"):
+        sol = u.BatchSolver(b, rtol=1e-8, max_it=600, restart=20, mixed=(mode == "mixed"), inner_rtol=1e-4)
+        total, first_total, its, worst, per_group = 0.0, 0.0, [], 0.0, []
+        for g in range(G):
+            b.set_group(g)
+            sol.refresh()
+            rhs = b.vec()
+            b.rhs(rhs)
+            x = b.vec()
+            times = []
+            for rep in range(2):  # the first solve after a refresh also captures the graphs
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                res = sol.solve(rhs, x)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t1)
+            b.set_scalar_flux(x)
+            total += times[1]
+            first_total += times[0]
+            per_group.append(round(times[1] * 1e3, 1))
+            live = [k for k in range(b.n_problems) if res["reason"][k] != 0]
+            its.append(max(res["iterations"][k] for k in live))
+            worst = max([worst] + [res["rel_residual"][k] for k in live])
+            del rhs, x
+        out[f"solve_{mode}"] = {"ms_all_groups": round(total * 1e3, 1),
+                                "ms_all_groups_with_capture": round(first_total * 1e3, 1), "ms_per_group": per_group,
+                                "max_iterations_per_group": its, "worst_rel_residual": worst}
+        del sol
+        torch.cuda.empty_cache()
+    out["mixed_speedup"] = round(out["solve_fp64"]["ms_all_groups"] / out["solve_mixed"]["ms_all_groups"], 3)
+    b.set_group(0)
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -196,6 +289,9 @@ def main():
         sys.exit(2)
     if args.sensitivity:
         sensitivity(args)
+        return
+    if args.batch:
+        batch_bench(args)
         return
     import torch
     from ubolt_amd import problems as P

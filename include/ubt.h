@@ -405,6 +405,9 @@ int ubt_batch_solver_solve(ubt_batch_solver *solver, const double *b, double *x,
  * steps and restart cycles in total */
 int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, double *rel_residual, int *reason,
                             int *outer, int *cycles);
+/* 1 (default): replay the restart cycles as captured HIP graphs (captured on
+ * the first solve after a refresh); 0: launch every kernel from the host */
+int ubt_batch_solver_set_graph(ubt_batch_solver *solver, int on);
 int ubt_batch_solver_destroy(ubt_batch_solver *solver);
 
 #ifdef __cplusplus

@@ -124,6 +124,7 @@ SIGNATURES = {
     "ubt_batch_solver_refresh": [P, P],
     "ubt_batch_solver_solve": [P, P, P, P],
     "ubt_batch_solver_result": [P, PI, PD, PI, PI, PI],
+    "ubt_batch_solver_set_graph": [P, I],
     "ubt_batch_solver_destroy": [P],
 }
 

@@ -422,6 +422,9 @@ class BatchSolver(_Handle):
     def refresh(self, stream=None):
         call("ubt_batch_solver_refresh", self.h, stream_ptr(stream))
 
+    def set_graph(self, on):
+        call("ubt_batch_solver_set_graph", self.h, int(on))
+
     def solve(self, b, x, stream=None):
         call("ubt_batch_solver_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
         return self.result()

@@ -85,16 +85,19 @@ struct BatchSolver {
   std::vector<double> rel_residual;
   int outer = 0, cycles = 0;
   bool use_graph = true;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one restart cycle per precision
+  static constexpr int kStepsPerCheck = 4;       // Arnoldi steps per graph between flag reads
+  std::vector<hipGraphExec_t> graphs[2];         // per precision: start, step blocks, end
   hipStream_t cap = nullptr;
   ~BatchSolver();
+  void drop_graphs();
 
   void create(Batch *b, const SolverOptions &o);
   void refresh(hipStream_t s);
   void solve(const double *b, double *x, hipStream_t s);
   template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
-  template <class T> void cycle(const T *b, T *x, hipStream_t s);
-  template <class T> void arnoldi_step(int j, hipStream_t s);
+  template <class T> void cycle_start(const T *b, const T *x, hipStream_t s);
+  template <class T> void cycle_steps(int j0, int j1, hipStream_t s);
+  template <class T> void cycle_end(T *x, hipStream_t s);
   template <class T> void fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, hipStream_t s);
 };
 

@@ -768,6 +768,13 @@ int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, dou
     if (cycles) *cycles = s.cycles;
   });
 }
+int ubt_batch_solver_set_graph(ubt_batch_solver *solver, int on) {
+  return guard([&] {
+    NEED(solver);
+    solver->s.use_graph = on != 0;
+    solver->s.drop_graphs();
+  });
+}
 int ubt_batch_solver_destroy(ubt_batch_solver *solver) {
   delete solver;
   return UBT_OK;

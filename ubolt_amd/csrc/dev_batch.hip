@@ -71,16 +71,19 @@ __global__ void k_bdot_final(int k, const int *__restrict__ cfirst, const double
   out[(int64_t)p * stride + j] = take_sqrt ? sqrt(t) : t;
 }
 
-// w += sign * sum_j coef[p][j] V[j] over each chunk of problem p
+// w += sign * sum_j coef[p][j] V[j] over each chunk of problem p, j < k (and
+// j < kcount[p] when given: basis vectors past a problem's own count may never
+// have been written this cycle)
 template <class T>
 __global__ void __launch_bounds__(kT) k_bmaxpy(int64_t n, int k, const T *__restrict__ V, const double *__restrict__ coef,
                                                int stride, double sign, T *__restrict__ w,
                                                const int64_t *__restrict__ cstart, const int *__restrict__ clen,
-                                               const int *__restrict__ cprob) {
+                                               const int *__restrict__ cprob, const int *__restrict__ kcount) {
   const int ch = blockIdx.x;
   const int64_t s0 = cstart[ch];
   const int len = clen[ch];
   const double *h = coef + (int64_t)cprob[ch] * stride;
+  if (kcount) k = min(k, kcount[cprob[ch]]);
   for (int i = threadIdx.x; i < len; i += kT) {
     double v = (double)w[s0 + i];
     for (int j = 0; j < k; j++) v += sign * h[j] * (double)V[(int64_t)j * n + s0 + i];
@@ -428,9 +431,15 @@ void Batch::tune() {
 // ---------------------------------------------------------------------------
 // BatchSolver
 // ---------------------------------------------------------------------------
+void BatchSolver::drop_graphs() {
+  for (auto &v : graphs) {
+    for (auto g : v) (void)hipGraphExecDestroy(g);
+    v.clear();
+  }
+}
+
 BatchSolver::~BatchSolver() {
-  for (auto &gx : graph)
-    if (gx) (void)hipGraphExecDestroy(gx);
+  drop_graphs();
   if (cap) (void)hipStreamDestroy(cap);
 }
 
@@ -501,10 +510,7 @@ void BatchSolver::refresh(hipStream_t s) {
   cfirst.upload(first);
   UBT_HIP(hipMemcpyAsync(done.as<int>() + 3 * np, live.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
   UBT_HIP(hipStreamSynchronize(s));
-  for (auto &gx : graph) {
-    if (gx) (void)hipGraphExecDestroy(gx);
-    gx = nullptr;
-  }
+  drop_graphs();
   gen = batch->gen;
 }
 
@@ -534,19 +540,17 @@ template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
   UBT_HIP(hipGetLastError());
 }
 
-// one restart cycle of every problem: b, x are the internal vectors
-template <class T> void BatchSolver::cycle(const T *b, T *x, hipStream_t s) {
+// A restart cycle in three parts: start (r0, beta, V0), Arnoldi steps
+// [j0, j1), end (the least-squares update of x). b, x are internal vectors.
+template <class T> void BatchSolver::cycle_start(const T *b, const T *x, hipStream_t s) {
   const int m = opt.restart, prec = prec_id<T>();
   T *V = sizeof(T) == 8 ? V64.as<T>() : V32.as<T>();
-  T *Z = sizeof(T) == 8 ? Z64.as<T>() : Z32.as<T>();
   T *w = sizeof(T) == 8 ? w64.as<T>() : w32.as<T>();
   const int64_t *cst = chunk_start.as<int64_t>();
   const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
-  int *dn = done.as<int>(), *jd = dn + np, *its = dn + 2 * np, *live = dn + 3 * np;
-  double *h1 = hc.as<double>(), *h2 = h1 + (size_t)np * kS, *hn = nrm.as<double>(), *pt = part.as<double>();
-  const unsigned gn = grid_n(n), gp = (unsigned)((np + 63) / 64), nc = (unsigned)std::max(1, n_chunks);
-  if (n_chunks == 0) return;
-  // r0 = b - A x, beta per problem, V0 = r0 / beta
+  int *dn = done.as<int>(), *jd = dn + np, *live = dn + 3 * np;
+  double *hn = nrm.as<double>(), *pt = part.as<double>();
+  const unsigned gn = grid_n(n), gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
   batch->mult(prec, x, w, true, s);
   hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, w);
   hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
@@ -554,18 +558,31 @@ template <class T> void BatchSolver::cycle(const T *b, T *x, hipStream_t s) {
   hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, V, cst, cln, cpr);
   hipLaunchKernelGGL(k_bstart, dim3(gp), dim3(64), 0, s, np, m, (const double *)hn, (const double *)tol.as<double>(),
                      (const int *)live, gv.as<double>(), dn, jd);
-  for (int j = 0; j < m; j++) {
+  UBT_HIP(hipGetLastError());
+}
+
+template <class T> void BatchSolver::cycle_steps(int j0, int j1, hipStream_t s) {
+  const int prec = prec_id<T>();
+  T *V = sizeof(T) == 8 ? V64.as<T>() : V32.as<T>();
+  T *Z = sizeof(T) == 8 ? Z64.as<T>() : Z32.as<T>();
+  T *w = sizeof(T) == 8 ? w64.as<T>() : w32.as<T>();
+  const int64_t *cst = chunk_start.as<int64_t>();
+  const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
+  int *dn = done.as<int>(), *jd = dn + np, *its = dn + 2 * np;
+  double *h1 = hc.as<double>(), *h2 = h1 + (size_t)np * kS, *hn = nrm.as<double>(), *pt = part.as<double>();
+  const unsigned gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
+  for (int j = j0; j < j1; j++) {
     T *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *vn = V + (int64_t)(j + 1) * n;
     apply_pc<T>(vj, zj, s);
     batch->mult(prec, zj, w, true, s);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
     hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h1, kS, -1.0, w,
-                       cst, cln, cpr);
+                       cst, cln, cpr, (const int *)nullptr);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
     hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0, w,
-                       cst, cln, cpr);
+                       cst, cln, cpr, (const int *)nullptr);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
     hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, vn, cst, cln, cpr);
@@ -573,38 +590,78 @@ template <class T> void BatchSolver::cycle(const T *b, T *x, hipStream_t s) {
                        (const double *)hn, H.as<double>(), cs.as<double>(), sn.as<double>(), gv.as<double>(),
                        res.as<double>(), (const double *)tol.as<double>(), dn, jd, its);
   }
-  hipLaunchKernelGGL(k_btri, dim3(gp), dim3(64), 0, s, np, (const double *)H.as<double>(),
-                     (const double *)gv.as<double>(), (const int *)jd, yv.as<double>());
-  hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, m, (const T *)Z, (const double *)yv.as<double>(), kS, 1.0,
-                     x, cst, cln, cpr);
   UBT_HIP(hipGetLastError());
 }
 
+template <class T> void BatchSolver::cycle_end(T *x, hipStream_t s) {
+  const int m = opt.restart;
+  T *Z = sizeof(T) == 8 ? Z64.as<T>() : Z32.as<T>();
+  const int64_t *cst = chunk_start.as<int64_t>();
+  const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>();
+  const int *jd = done.as<int>() + np;
+  const unsigned gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
+  hipLaunchKernelGGL(k_btri, dim3(gp), dim3(64), 0, s, np, (const double *)H.as<double>(),
+                     (const double *)gv.as<double>(), jd, yv.as<double>());
+  hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, m, (const T *)Z, (const double *)yv.as<double>(), kS, 1.0,
+                     x, cst, cln, cpr, jd);
+  UBT_HIP(hipGetLastError());
+}
+
+// Record what fn enqueues as a graph (on the capture stream)
+template <class F> static hipGraphExec_t capture(hipStream_t cap, F &&fn) {
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t ex = nullptr;
+  UBT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  fn();
+  UBT_HIP(hipStreamEndCapture(cap, &gr));
+  UBT_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(gr);
+  return ex;
+}
+
+// Restart cycles until every problem is done: the start, blocks of
+// kStepsPerCheck Arnoldi steps, and the end are each a graph; the host reads
+// the done flags after every block, so a cycle stops as soon as every
+// problem has converged instead of running all restart steps.
 template <class T>
 void BatchSolver::fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, hipStream_t s) {
-  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const int pi = sizeof(T) == 8 ? 0 : 1, m = opt.restart;
+  const int nblk = (m + kStepsPerCheck - 1) / kStepsPerCheck;
   UBT_HIP(hipMemcpyAsync(tol.as<double>(), tol_abs, sizeof(double) * np, hipMemcpyHostToDevice, s));
   UBT_HIP(hipStreamSynchronize(s));
-  if (use_graph && !graph[pi]) {
-    hipGraph_t gr = nullptr;
-    UBT_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    cycle<T>(b, x, cap);
-    UBT_HIP(hipStreamEndCapture(cap, &gr));
-    UBT_HIP(hipGraphInstantiate(&graph[pi], gr, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(gr);
+  auto &gx = graphs[pi];
+  if (use_graph && gx.empty()) {
+    gx.push_back(capture(cap, [&] { cycle_start<T>(b, x, cap); }));
+    for (int k = 0; k < nblk; k++)
+      gx.push_back(capture(cap, [&] { cycle_steps<T>(k * kStepsPerCheck, std::min(m, (k + 1) * kStepsPerCheck), cap); }));
+    gx.push_back(capture(cap, [&] { cycle_end<T>(x, cap); }));
   }
   std::vector<int> dn(np);
-  for (int c = 0; c < max_cycles; c++) {
-    if (use_graph)
-      UBT_HIP(hipGraphLaunch(graph[pi], cap));
-    else
-      cycle<T>(b, x, cap);
+  auto all_done = [&] {
     UBT_HIP(hipMemcpyAsync(dn.data(), done.as<int>(), sizeof(int) * np, hipMemcpyDeviceToHost, cap));
     UBT_HIP(hipStreamSynchronize(cap));
+    for (int p = 0; p < np; p++)
+      if (!dn[p]) return false;
+    return true;
+  };
+  for (int c = 0; c < max_cycles; c++) {
+    if (use_graph)
+      UBT_HIP(hipGraphLaunch(gx[0], cap));
+    else
+      cycle_start<T>(b, x, cap);
+    for (int k = 0; k < nblk; k++) {
+      if (use_graph)
+        UBT_HIP(hipGraphLaunch(gx[1 + k], cap));
+      else
+        cycle_steps<T>(k * kStepsPerCheck, std::min(m, (k + 1) * kStepsPerCheck), cap);
+      if (all_done()) break;
+    }
+    if (use_graph)
+      UBT_HIP(hipGraphLaunch(gx[1 + nblk], cap));
+    else
+      cycle_end<T>(x, cap);
     cycles++;
-    bool all = true;
-    for (int p = 0; p < np; p++) all = all && dn[p];
-    if (all) break;
+    if (all_done()) break;
   }
 }
 
