@@ -18,10 +18,10 @@ template <class T> struct BatchItem {
 // apply bodies the batch autotuner chooses from
 constexpr int kBatchBodies = 2;  // 0 tile, 1 four-angle tile
 
+// the directions of every active problem, one workgroup each
 struct BatchSweep {
-  int n_levels = 0;
-  std::vector<int64_t> level_off;
-  DevBuf prob, cells, angles, upwind;
+  int n_wg = 0;
+  DevBuf prob, dir, cells, upwind, lvl, base;
 };
 
 struct Batch {

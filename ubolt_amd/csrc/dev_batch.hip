@@ -61,14 +61,16 @@ __global__ void __launch_bounds__(kT) k_bdot(int64_t n, int k, const T *__restri
   }
 }
 
-// out[p * stride + j] = (sqrt of) the sum of problem p's chunk partials
+// out[p * stride + j] = (sqrt of) the sum of problem p's chunk partials; one
+// wavefront per (problem, column): lane-strided sums, then a butterfly
 __global__ void k_bdot_final(int k, const int *__restrict__ cfirst, const double *__restrict__ part,
                              double *__restrict__ out, int stride, int take_sqrt) {
-  const int p = blockIdx.x, j = threadIdx.x;
+  const int p = blockIdx.x, j = blockIdx.y;
   if (j >= k) return;
   double t = 0.0;
-  for (int ch = cfirst[p]; ch < cfirst[p + 1]; ch++) t += part[(int64_t)ch * kS + j];
-  out[(int64_t)p * stride + j] = take_sqrt ? sqrt(t) : t;
+  for (int ch = cfirst[p] + (int)threadIdx.x; ch < cfirst[p + 1]; ch += 64) t += part[(int64_t)ch * kS + j];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (threadIdx.x == 0) out[(int64_t)p * stride + j] = take_sqrt ? sqrt(t) : t;
 }
 
 // w += sign * sum_j coef[p][j] V[j] over each chunk of problem p, j < k (and
@@ -293,35 +295,45 @@ void Batch::build_lists() {
     if (active[p]) blk.push_back(blk.back() + (int64_t)ops[p]->disc->n_cells * ops[p]->disc->n_angles);
   bblk.upload(blk);
   n_blocks = blk.back();
-  // sweep: level l of the batch = level l of every active problem
-  int levels = 0;
-  for (size_t p = 0; p < ops.size(); p++)
-    if (active[p]) levels = std::max(levels, sweeps[p]->n_levels);
-  std::vector<int> pr, ce, an;
+  // sweep: one workgroup per direction of every active problem, the
+  // problems' per-direction schedules concatenated; the workgroups are listed
+  // longest first (most levels), so the directions that bound the sweep are
+  // dispatched before the short ones
+  struct Dir {
+    int q, a, nlev;
+    size_t p;
+  };
+  std::vector<Dir> dirs;
+  std::vector<int64_t> item0(ops.size(), 0);
+  std::vector<int> ce;
   std::vector<unsigned char> up;
-  std::vector<int64_t> loff(1, 0);
-  for (int l = 0; l < levels; l++) {
-    int q = 0;
-    for (size_t p = 0; p < ops.size(); p++) {
-      if (!active[p]) continue;
-      const Sweep &s = *sweeps[p];
-      if (l < s.n_levels)
-        for (int64_t i = s.level_off[l]; i < s.level_off[l + 1]; i++) {
-          pr.push_back(q);
-          ce.push_back(s.h_cells[i]);
-          an.push_back(s.h_angles[i]);
-          up.push_back(s.h_up[i]);
-        }
-      q++;
-    }
-    loff.push_back((int64_t)pr.size());
+  int q = 0;
+  for (size_t p = 0; p < ops.size(); p++) {
+    if (!active[p]) continue;
+    const Sweep &s = *sweeps[p];
+    item0[p] = (int64_t)ce.size();
+    ce.insert(ce.end(), s.h_cells.begin(), s.h_cells.end());
+    up.insert(up.end(), s.h_up.begin(), s.h_up.end());
+    for (int a = 0; a < ops[p]->disc->n_angles; a++) dirs.push_back({q, a, s.h_base[a + 1] - s.h_base[a] - 1, p});
+    q++;
   }
-  sweep.n_levels = levels;
-  sweep.level_off = loff;
-  sweep.prob.upload(pr);
+  std::stable_sort(dirs.begin(), dirs.end(), [](const Dir &x, const Dir &y) { return x.nlev > y.nlev; });
+  std::vector<int> wp, wd, bs(1, 0);
+  std::vector<int64_t> lv;
+  for (const Dir &d : dirs) {
+    const Sweep &s = *sweeps[d.p];
+    wp.push_back(d.q);
+    wd.push_back(d.a);
+    for (int k = s.h_base[d.a]; k < s.h_base[d.a + 1]; k++) lv.push_back(item0[d.p] + s.h_lvl[k]);
+    bs.push_back((int)lv.size());
+  }
+  sweep.n_wg = (int)wp.size();
+  sweep.prob.upload(wp);
+  sweep.dir.upload(wd);
   sweep.cells.upload(ce);
-  sweep.angles.upload(an);
   sweep.upwind.upload(up);
+  sweep.lvl.upload(lv);
+  sweep.base.upload(bs);
 }
 
 void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {
@@ -554,7 +566,7 @@ template <class T> void BatchSolver::cycle_start(const T *b, const T *x, hipStre
   batch->mult(prec, x, w, true, s);
   hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, w);
   hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
-  hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
+  hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
   hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, V, cst, cln, cpr);
   hipLaunchKernelGGL(k_bstart, dim3(gp), dim3(64), 0, s, np, m, (const double *)hn, (const double *)tol.as<double>(),
                      (const int *)live, gv.as<double>(), dn, jd);
@@ -576,15 +588,15 @@ template <class T> void BatchSolver::cycle_steps(int j0, int j1, hipStream_t s) 
     apply_pc<T>(vj, zj, s);
     batch->mult(prec, zj, w, true, s);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
     hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h1, kS, -1.0, w,
                        cst, cln, cpr, (const int *)nullptr);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
     hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0, w,
                        cst, cln, cpr, (const int *)nullptr);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
     hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, vn, cst, cln, cpr);
     hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1, (const double *)h2,
                        (const double *)hn, H.as<double>(), cs.as<double>(), sn.as<double>(), gv.as<double>(),
@@ -685,7 +697,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
   UBT_HIP(hipMemsetAsync(done.as<int>() + 2 * np, 0, sizeof(int) * np, cap));  // iteration counters
   // ||b|| per problem
   hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, b, b, cst, cln, pt);
-  hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, bnorm.as<double>(), 1, 1);
+  hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, bnorm.as<double>(), 1, 1);
   std::vector<double> bn(np), rn(np), tl(np);
   std::vector<int> live(np);
   UBT_HIP(hipMemcpyAsync(bn.data(), bnorm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
@@ -702,7 +714,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
     batch->mult(UBT_FP64, xi, w, true, cap);
     hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, w);
     hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)w, (const double *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, nrm.as<double>(), 1, 1);
+    hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, nrm.as<double>(), 1, 1);
     hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, (const double *)xi, 0.0, x);
     UBT_HIP(hipMemcpyAsync(rn.data(), nrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
     outer = 1;
@@ -714,7 +726,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
       batch->mult(UBT_FP64, x, r, true, cap);
       hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, r);
       hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)r, (const double *)r, cst, cln, pt);
-      hipLaunchKernelGGL(k_bdot_final, dim3(np), dim3(64), 0, cap, 1, cfp, (const double *)pt, onrm.as<double>(), 1, 1);
+      hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, onrm.as<double>(), 1, 1);
       UBT_HIP(hipMemcpyAsync(rn.data(), onrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
       UBT_HIP(hipStreamSynchronize(cap));
       bool all = true;

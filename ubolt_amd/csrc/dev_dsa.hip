@@ -291,10 +291,18 @@ __global__ void cg_init(MvArgs m, const double *__restrict__ b, const double *__
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
-__global__ void cg_reduce_init(const double *partial, int nblk, double *scal, double rtol2) {
-  if (threadIdx.x != 0) return;
+// the sum of the block partials by one wavefront (lane-strided, then a
+// butterfly: a fixed order, so the result is reproducible); every lane holds it
+__device__ inline double wave_total(const double *partial, int nblk) {
   double s = 0.0;
-  for (int i = 0; i < nblk; i++) s += partial[i];
+  for (int i = threadIdx.x; i < nblk; i += 64) s += partial[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+__global__ void cg_reduce_init(const double *partial, int nblk, double *scal, double rtol2) {
+  const double s = wave_total(partial, nblk);
+  if (threadIdx.x != 0) return;
   scal[0] = s;
   scal[1] = s;
   scal[4] = (s == 0.0) ? 1.0 : 0.0;
@@ -318,9 +326,9 @@ __global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ p, double *__
 }
 
 __global__ void cg_alpha(const double *partial, int nblk, double *scal) {
-  if (threadIdx.x != 0 || scal[4] != 0.0) return;
-  double s = 0.0;
-  for (int i = 0; i < nblk; i++) s += partial[i];
+  if (scal[4] != 0.0) return;
+  const double s = wave_total(partial, nblk);
+  if (threadIdx.x != 0) return;
   scal[2] = s != 0.0 ? scal[0] / s : 0.0;
   if (s == 0.0) scal[4] = 1.0;
 }
@@ -352,9 +360,9 @@ __global__ void cg_update(MvArgs m, double *x, double *r, double *z, const doubl
 }
 
 __global__ void cg_beta(const double *partial, int nblk, double *scal) {
-  if (threadIdx.x != 0 || scal[4] != 0.0) return;
-  double s = 0.0;
-  for (int i = 0; i < nblk; i++) s += partial[i];
+  if (scal[4] != 0.0) return;
+  const double s = wave_total(partial, nblk);
+  if (threadIdx.x != 0) return;
   scal[3] = scal[0] != 0.0 ? s / scal[0] : 0.0;
   scal[0] = s;
   scal[5] += 1.0;

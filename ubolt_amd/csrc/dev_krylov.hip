@@ -78,11 +78,14 @@ __global__ void __launch_bounds__(kThreads) k_mdot_partial(int64_t n, int k, con
 
 // out[j] = (accumulate ? out[j] : 0) + sum_b partial[b * k + j]; sqrt_last:
 // out[0] = sqrt(sum) (a norm)
+// one wavefront per column j (grid = k): lane-strided sums, then a butterfly
 __global__ void k_mdot_final(int nblk, int k, const double *partial, double *out, int accumulate, int take_sqrt) {
-  const int j = threadIdx.x;
+  const int j = blockIdx.x;
   if (j >= k) return;
   double t = 0.0;
-  for (int b = 0; b < nblk; b++) t += partial[(int64_t)b * k + j];
+  for (int b = threadIdx.x; b < nblk; b += 64) t += partial[(int64_t)b * k + j];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (threadIdx.x != 0) return;
   if (take_sqrt) t = sqrt(t);
   out[j] = accumulate ? out[j] + t : t;
 }
@@ -124,7 +127,7 @@ template <class T> static void mdot(const T *V, int k, const T *w, int64_t n, do
                                     double *partial, hipStream_t s) {
   const unsigned g = grid_dot(n);
   hipLaunchKernelGGL(k_mdot_partial<T>, dim3(g), dim3(kThreads), 0, s, n, k, V, w, partial);
-  hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(64), 0, s, (int)g, k, partial, out, acc ? 1 : 0, sq ? 1 : 0);
+  hipLaunchKernelGGL(k_mdot_final, dim3(k), dim3(64), 0, s, (int)g, k, partial, out, acc ? 1 : 0, sq ? 1 : 0);
 }
 
 template <class T> static double host_norm(const T *w, int64_t n, double *dev_out, double *partial, hipStream_t s) {

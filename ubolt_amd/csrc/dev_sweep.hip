@@ -6,9 +6,9 @@
 //
 // * Unstructured DG0/DG1: per angle, the cells are levelled by a topological
 //   sort of the upwind graph (host, once per discretisation). A level step
-//   processes, for every angle at once, the (cell, angle) pairs of that level:
-//   gather the upwind traces, build the cell's n_basis x n_basis block (the
-//   same block the element block inverse solves) and solve it. An edge left
+//   processes the cells of that level for one direction: gather the upwind
+//   traces, build the cell's n_basis x n_basis block (the same block the
+//   element block inverse solves) and solve it. An edge left
 //   on a cycle (possible on a general mesh, not on a Delaunay or box mesh) is
 //   lagged: its inflow is taken as zero, so the stage stays a preconditioner.
 //   Which faces read an upwind trace is the schedule's own per-item mask, not
@@ -17,8 +17,9 @@
 //   test would then read a neighbour the schedule has not ordered first.
 // * Structured FD: the levels are the diagonal hyperplanes i' + j' + k' = l of
 //   each octant, no table needed.
-// Reflective couplings are lagged the same way. Every kernel is a plain
-// launch per level on the caller's stream.
+// Reflective couplings are lagged the same way. Directions never read each
+// other's traces, so one launch sweeps all of them: a workgroup per direction
+// walks its levels with a workgroup barrier between them.
 //
 // The adjoint sweep solves L^T y = r with L^T = V P L P V^{-1} (P reverses
 // every angle, V the cell volumes): the forward schedule of angle b solves for
@@ -138,62 +139,81 @@ __device__ __forceinline__ void sweep_item(const T *__restrict__ r, T *__restric
   for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
 }
 
+// Persistent per-direction sweep: the sweep of one direction never reads
+// another direction's traces, so a workgroup owns one direction and walks all
+// of its levels in one launch, a workgroup barrier between levels (the items
+// of level l read traces the same workgroup wrote at earlier levels, through
+// the CU's own L1/L2, so the barrier is the only synchronisation needed).
+// Every direction runs concurrently on its own CU: the launch count of a
+// sweep drops from one per level to one.
 template <class T, int DIM, int NB, int NF, bool ADJ>
-__global__ void sweep_dg_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, const int *__restrict__ cell,
-                               const int *__restrict__ angle, const unsigned char *__restrict__ upwind, int64_t begin,
-                               int64_t count) {
-  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= count) return;
-  const int64_t item = begin + it;
-  sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[item], angle[item], upwind[item]);
-}
-
-// a merged level of many problems: each item names its problem
-template <class T, int DIM, int NB, int NF>
-__global__ void sweep_dg_level_batch(const T *__restrict__ r, T *__restrict__ psi, const BatchItem<T> *__restrict__ items,
-                                     const int *__restrict__ prob, const int *__restrict__ cell,
-                                     const int *__restrict__ angle, const unsigned char *__restrict__ upwind,
-                                     int64_t begin, int64_t count) {
-  const int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= count) return;
-  const int64_t item = begin + it;
-  const BatchItem<T> &bi = items[prob[item]];
-  sweep_item<T, DIM, NB, NF, false>(r + bi.off, psi + bi.off, bi.g, cell[item], angle[item], upwind[item]);
-}
-
-// Structured: thread = (angle, j', k'); the level fixes i' = l - j' - k'.
-// Every cell has the same volume, so the adjoint only swaps the angle index.
-template <class T, int DIM, bool ADJ>
-__global__ void sweep_fd_level(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g, int level) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nx = g.n[0], ny = g.n[1], nz = DIM == 3 ? g.n[2] : 1;
-  const int64_t per_angle = (int64_t)ny * nz;
-  if (t >= per_angle * g.A) return;
-  const int b = (int)(t / per_angle);
-  const int a = ADJ ? g.neg[b] : b;
-  const int64_t jk = t - (int64_t)b * per_angle;
-  const int jp = (int)(jk % ny), kp = (int)(jk / ny);
-  const int ip = level - jp - kp;
-  if (ip < 0 || ip >= nx) return;
-  const T cs[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
-  int idx[3];
-  idx[0] = cs[0] >= T(0) ? ip : nx - 1 - ip;
-  idx[1] = cs[1] >= T(0) ? jp : ny - 1 - jp;
-  idx[2] = DIM == 3 ? (cs[2] >= T(0) ? kp : nz - 1 - kp) : 0;
-  const int64_t stride[3] = {1, nx, (int64_t)nx * ny};
-  const int64_t c = idx[0] + (int64_t)nx * (idx[1] + (int64_t)ny * idx[2]);
-  const int A = g.A;
-  T num = r[c * A + a], den = g.sigt[c];
-#pragma unroll
-  for (int d = 0; d < DIM; d++) {
-    const T cd = fabs(cs[d]) * g.invh[d];
-    den += cd;
-    if (cs[d] == T(0)) continue;
-    const int up = cs[d] > T(0) ? idx[d] - 1 : idx[d] + 1;
-    const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
-    if (up >= 0 && up < nd) num += cd * psi[(c + (cs[d] > T(0) ? -stride[d] : stride[d])) * A + a];
+__global__ void __launch_bounds__(1024) sweep_dg_dir(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g,
+                                                     const int *__restrict__ cell, const unsigned char *__restrict__ up,
+                                                     const int64_t *__restrict__ lvl, const int *__restrict__ base) {
+  const int b = blockIdx.x;
+  const int l0 = base[b], l1 = base[b + 1] - 1;  // levels of direction b: lvl[l0 .. l1]
+  for (int l = l0; l < l1; l++) {
+    for (int64_t i = lvl[l] + threadIdx.x; i < lvl[l + 1]; i += blockDim.x)
+      sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[i], b, up[i]);
+    __syncthreads();
   }
-  psi[c * A + a] = num / den;
+}
+
+// the same over the directions of many problems: workgroup w owns direction
+// wdir[w] of the problem at work-list position wprob[w]
+template <class T, int DIM, int NB, int NF>
+__global__ void __launch_bounds__(256) sweep_dg_dir_batch(const T *__restrict__ r, T *__restrict__ psi,
+                                                          const BatchItem<T> *__restrict__ items,
+                                                          const int *__restrict__ wprob, const int *__restrict__ wdir,
+                                                          const int *__restrict__ cell, const unsigned char *__restrict__ up,
+                                                          const int64_t *__restrict__ lvl, const int *__restrict__ base) {
+  const int w = blockIdx.x;
+  const BatchItem<T> &bi = items[wprob[w]];
+  const int b = wdir[w];
+  const int l0 = base[w], l1 = base[w + 1] - 1;
+  for (int l = l0; l < l1; l++) {
+    for (int64_t i = lvl[l] + threadIdx.x; i < lvl[l + 1]; i += blockDim.x)
+      sweep_item<T, DIM, NB, NF, false>(r + bi.off, psi + bi.off, bi.g, cell[i], b, up[i]);
+    __syncthreads();
+  }
+}
+
+// Structured, persistent per direction: workgroup = direction, levels in one
+// launch with a workgroup barrier between them (see sweep_dg_dir)
+template <class T, int DIM, bool ADJ>
+__global__ void __launch_bounds__(1024) sweep_fd_dir(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g,
+                                                     int n_levels) {
+  const int b = blockIdx.x;
+  const int a = ADJ ? g.neg[b] : b;
+  const int nx = g.n[0], ny = g.n[1], nz = DIM == 3 ? g.n[2] : 1;
+  const int64_t per = (int64_t)ny * nz;
+  const int A = g.A;
+  const T cs[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
+  const int64_t stride[3] = {1, nx, (int64_t)nx * ny};
+  for (int level = 0; level < n_levels; level++) {
+    for (int64_t t = threadIdx.x; t < per; t += blockDim.x) {
+      const int jp = (int)(t % ny), kp = (int)(t / ny);
+      const int ip = level - jp - kp;
+      if (ip < 0 || ip >= nx) continue;
+      int idx[3];
+      idx[0] = cs[0] >= T(0) ? ip : nx - 1 - ip;
+      idx[1] = cs[1] >= T(0) ? jp : ny - 1 - jp;
+      idx[2] = DIM == 3 ? (cs[2] >= T(0) ? kp : nz - 1 - kp) : 0;
+      const int64_t c = idx[0] + (int64_t)nx * (idx[1] + (int64_t)ny * idx[2]);
+      T num = r[c * A + a], den = g.sigt[c];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) {
+        const T cd = fabs(cs[d]) * g.invh[d];
+        den += cd;
+        if (cs[d] == T(0)) continue;
+        const int upi = cs[d] > T(0) ? idx[d] - 1 : idx[d] + 1;
+        const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+        if (upi >= 0 && upi < nd) num += cd * psi[(c + (cs[d] > T(0) ? -stride[d] : stride[d])) * A + a];
+      }
+      psi[c * A + a] = num / den;
+    }
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -304,29 +324,35 @@ void Sweep::create(Operator *o, bool keep_host) {
     }
     max_level = std::max(max_level, lvl);
   }
-  // order the items by level (stable: angle, then cell within a level)
+  // per direction: the items in generation order (direction, then level),
+  // and each direction's level boundaries
   const int64_t n_items = (int64_t)item_cell.size();
-  std::vector<int64_t> count(max_level + 1, 0);
-  for (int64_t i = 0; i < n_items; i++) count[item_level[i] + 1]++;
-  for (int l = 0; l < max_level; l++) count[l + 1] += count[l];
-  level_off.assign(count.begin(), count.end());
-  std::vector<int> oc(n_items), oa(n_items);
-  std::vector<unsigned char> ol(n_items);
-  std::vector<int64_t> pos(count.begin(), count.end() - 1);
-  for (int64_t i = 0; i < n_items; i++) {
-    const int64_t p = pos[item_level[i]]++;
-    oc[p] = item_cell[i];
-    oa[p] = item_angle[i];
-    ol[p] = item_up[i];
+  std::vector<int64_t> alvl;
+  std::vector<int> abase(A + 1, 0);
+  int64_t widest = 1;
+  for (int64_t i = 0, a = 0; a < A; a++) {
+    abase[a] = (int)alvl.size();
+    int64_t start = i;
+    for (; i < n_items && item_angle[i] == a; i++)
+      if (i == start || item_level[i] != item_level[i - 1]) {
+        if (!alvl.empty() && (int)alvl.size() > abase[a]) widest = std::max(widest, i - alvl.back());
+        alvl.push_back(i);
+      }
+    if ((int)alvl.size() > abase[a]) widest = std::max(widest, i - alvl.back());
+    alvl.push_back(i);
   }
+  abase[A] = (int)alvl.size();
   n_levels = max_level;
-  cells.upload(oc);
-  angles.upload(oa);
-  upwind.upload(ol);
+  threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest + 63) / 64 * 64));
+  cells.upload(item_cell);
+  upwind.upload(item_up);
+  lvl.upload(alvl);
+  base.upload(abase);
   if (keep_host) {
-    h_cells.swap(oc);
-    h_angles.swap(oa);
-    h_up.swap(ol);
+    h_cells.swap(item_cell);
+    h_up.swap(item_up);
+    h_lvl.swap(alvl);
+    h_base.swap(abase);
   }
 }
 
@@ -337,85 +363,76 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   UBT_CHECK(r != psi, UBT_ERR_ARG_IDN, "the sweep needs distinct r and psi");
   const Disc &D = *o.disc;
   if (D.kind == DISC_FD) {
-    const int64_t per = (int64_t)D.n[1] * (D.dim == 3 ? D.n[2] : 1) * D.n_angles;
-    const unsigned grid = (unsigned)((per + 255) / 256);
-#define SWF(T, DIM)                                                                                        \
-  if (adj)                                                                                                 \
-    hipLaunchKernelGGL((sweep_fd_level<T, DIM, true>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi,  \
-                       o.args<T>(), l);                                                                    \
-  else                                                                                                     \
-    hipLaunchKernelGGL((sweep_fd_level<T, DIM, false>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
-                       o.args<T>(), l);
-    for (int l = 0; l < n_levels; l++) {
-      if (prec == UBT_FP64) {
-        if (D.dim == 2) {
-          SWF(double, 2)
-        } else {
-          SWF(double, 3)
-        }
+    const int64_t per = (int64_t)D.n[1] * (D.dim == 3 ? D.n[2] : 1);
+    const int th = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (per + 63) / 64 * 64));
+    const unsigned grid = (unsigned)D.n_angles;
+#define SWF(T, DIM)                                                                                              \
+  if (adj)                                                                                                       \
+    hipLaunchKernelGGL((sweep_fd_dir<T, DIM, true>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,          \
+                       o.args<T>(), n_levels);                                                                   \
+  else                                                                                                           \
+    hipLaunchKernelGGL((sweep_fd_dir<T, DIM, false>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,         \
+                       o.args<T>(), n_levels);
+    if (prec == UBT_FP64) {
+      if (D.dim == 2) {
+        SWF(double, 2)
       } else {
-        if (D.dim == 2) {
-          SWF(float, 2)
-        } else {
-          SWF(float, 3)
-        }
+        SWF(double, 3)
+      }
+    } else {
+      if (D.dim == 2) {
+        SWF(float, 2)
+      } else {
+        SWF(float, 3)
       }
     }
 #undef SWF
     UBT_HIP(hipGetLastError());
     return;
   }
-  const int *ci = cells.as<int>(), *ai = angles.as<int>();
-  const unsigned char *lg = upwind.as<unsigned char>();
-  for (int l = 0; l < n_levels; l++) {
-    const int64_t b = level_off[l], n = level_off[l + 1] - level_off[l];
-    if (n == 0) continue;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    bool launched = false;
-#define SW(T, DIM, NB, NF)                                                                                           \
-  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                       \
-    if (adj)                                                                                                         \
-      hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF, true>), dim3(grid), dim3(256), 0, s, (const T *)r, (T *)psi, \
-                         o.args<T>(), ci, ai, lg, b, n);                                                             \
-    else                                                                                                             \
-      hipLaunchKernelGGL((sweep_dg_level<T, DIM, NB, NF, false>), dim3(grid), dim3(256), 0, s, (const T *)r,          \
-                         (T *)psi, o.args<T>(), ci, ai, lg, b, n);                                                   \
-    launched = true;                                                                                                 \
+  const int *ci = cells.as<int>(), *bs = base.as<int>();
+  const unsigned char *up = upwind.as<unsigned char>();
+  const int64_t *lv = lvl.as<int64_t>();
+  const unsigned grid = (unsigned)D.n_angles;
+  bool launched = false;
+#define SW(T, DIM, NB, NF)                                                                                         \
+  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                     \
+    if (adj)                                                                                                       \
+      hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,      \
+                         (T *)psi, o.args<T>(), ci, up, lv, bs);                                                   \
+    else                                                                                                           \
+      hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, false>), dim3(grid), dim3(threads), 0, s, (const T *)r,     \
+                         (T *)psi, o.args<T>(), ci, up, lv, bs);                                                   \
+    launched = true;                                                                                               \
   }
 #define SW_ALL(T) SW(T, 2, 1, 3) SW(T, 2, 1, 4) SW(T, 3, 1, 4) SW(T, 3, 1, 6) SW(T, 2, 3, 3) SW(T, 2, 3, 4) SW(T, 3, 4, 4) SW(T, 3, 4, 6)
-    if (prec == UBT_FP64) {
-      SW_ALL(double)
-    } else {
-      SW_ALL(float)
-    }
+  if (prec == UBT_FP64) {
+    SW_ALL(double)
+  } else {
+    SW_ALL(float)
+  }
 #undef SW_ALL
 #undef SW
-    UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for dim %d, %d basis, %d faces", D.dim, D.nb, D.nf);
-  }
+  UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for dim %d, %d basis, %d faces", D.dim, D.nb, D.nf);
   UBT_HIP(hipGetLastError());
 }
 
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   const BatchSweep &w = b.sweep;
+  if (w.n_wg == 0) return;
   const BatchItem<T> *items = b.items[pi][0].as<BatchItem<T>>();
-  const int *pr = w.prob.as<int>(), *ci = w.cells.as<int>(), *ai = w.angles.as<int>();
-  const unsigned char *up = w.upwind.as<unsigned char>();
-  for (int l = 0; l < w.n_levels; l++) {
-    const int64_t beg = w.level_off[l], n = w.level_off[l + 1] - w.level_off[l];
-    if (n == 0) continue;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    bool launched = false;
-#define SWB(DIM, NB, NF)                                                                                           \
-  if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF) {                                                     \
-    hipLaunchKernelGGL((sweep_dg_level_batch<T, DIM, NB, NF>), dim3(grid), dim3(256), 0, s, r, psi, items, pr, ci, \
-                       ai, up, beg, n);                                                                            \
-    launched = true;                                                                                               \
+  bool launched = false;
+#define SWB(DIM, NB, NF)                                                                                            \
+  if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF) {                                                      \
+    hipLaunchKernelGGL((sweep_dg_dir_batch<T, DIM, NB, NF>), dim3(w.n_wg), dim3(256), 0, s, r, psi, items,          \
+                       w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),          \
+                       w.lvl.as<int64_t>(), w.base.as<int>());                                                      \
+    launched = true;                                                                                                \
   }
-    SWB(2, 1, 3) SWB(2, 1, 4) SWB(3, 1, 4) SWB(3, 1, 6) SWB(2, 3, 3) SWB(2, 3, 4) SWB(3, 4, 4) SWB(3, 4, 6)
+  SWB(2, 1, 3) SWB(2, 1, 4) SWB(3, 1, 4) SWB(3, 1, 6) SWB(2, 3, 3) SWB(2, 3, 4) SWB(3, 4, 4) SWB(3, 4, 6)
 #undef SWB
-    UBT_CHECK(launched, UBT_ERR_SUP, "no batched sweep for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
-  }
+  UBT_CHECK(launched, UBT_ERR_SUP, "no batched sweep for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
   UBT_HIP(hipGetLastError());
 }
 
