@@ -92,17 +92,19 @@ def cpu_baseline(prob, seconds):
                       f"{cores} OpenMP threads; restated reference semantics, not the reference binary"}
 
 
-def solve_to_1e8(prob, mixed, reps, dsa_rtol, dsa_max_it):
+def solve_to_1e8(prob, mixed, reps, dsa_rtol, dsa_max_it, use_dsa=True):
     """Wall time from a zero guess to a 1e-8 FP64 true relative residual, the
     median of `reps` solves after one warm-up; same composite preconditioner
-    (element blocks -> transport sweep -> DSA) in both precisions."""
+    (element blocks -> transport sweep [-> DSA]) in both precisions."""
     import torch
     import ubolt_amd as u
 
     op = prob.operator()
-    dsa = u.DSA(op, inner_max_it=dsa_max_it, inner_rtol=dsa_rtol)
-    dsa.set_group(0)
-    sol = u.Solver(op, dsa, rtol=1e-8, max_it=400, restart=30, mixed=mixed, inner_rtol=1e-4)
+    dsa = None
+    if use_dsa:
+        dsa = u.DSA(op, inner_max_it=dsa_max_it, inner_rtol=dsa_rtol)
+        dsa.set_group(0)
+    sol = u.Solver(op, dsa, rtol=1e-8, max_it=400, restart=30, mixed=mixed, inner_rtol=1e-4, pc_dsa=use_dsa)
     sol.refresh()
     b = op.vec()
     op.rhs(0, b)
@@ -358,15 +360,23 @@ def main():
         "wall_s": round(wall, 3),
     }
     if not args.no_solve:
-        t64, r64, phi64 = solve_to_1e8(prob, False, args.solve_reps, 1e-3, 100)
-        t32, r32, phi32 = solve_to_1e8(prob, True, args.solve_reps, 1e-3, 100)
-        out["time_to_1e-8"] = {
-            "fp64_ms": round(t64 * 1e3, 3), "mixed_ms": round(t32 * 1e3, 3), "mixed_speedup": round(t64 / t32, 3),
-            "fp64_iterations": r64["iterations"], "mixed_iterations": r32["iterations"], "mixed_outer": r32["outer"],
-            "fp64_rel_residual": r64["rel_residual"], "mixed_rel_residual": r32["rel_residual"],
-            "mixed_vs_fp64_scalar_flux_rel_l2": float(torch.linalg.norm(phi32 - phi64) / torch.linalg.norm(phi64)),
-            "preconditioner": "element blocks -> transport sweep -> DSA (DG1 MIP, FP64 CG to 1e-3)",
-        }
+        # two preconditioners: the sweep alone, and the sweep followed by DSA
+        # (DG1 MIP diffusion, FP64 CG to 1e-3 or 15 iterations); the headline
+        # fields are the one whose FP64 solve is faster, both precisions on it
+        runs = []
+        for use_dsa, name in ((False, "element blocks -> transport sweep"),
+                              (True, "element blocks -> transport sweep -> DSA (DG1 MIP, FP64 CG, 15 its or 1e-3)")):
+            t64, r64, phi64 = solve_to_1e8(prob, False, args.solve_reps, 1e-3, 15, use_dsa)
+            t32, r32, phi32 = solve_to_1e8(prob, True, args.solve_reps, 1e-3, 15, use_dsa)
+            runs.append({
+                "fp64_ms": round(t64 * 1e3, 3), "mixed_ms": round(t32 * 1e3, 3), "mixed_speedup": round(t64 / t32, 3),
+                "fp64_iterations": r64["iterations"], "mixed_iterations": r32["iterations"], "mixed_outer": r32["outer"],
+                "fp64_rel_residual": r64["rel_residual"], "mixed_rel_residual": r32["rel_residual"],
+                "mixed_vs_fp64_scalar_flux_rel_l2": float(torch.linalg.norm(phi32 - phi64) / torch.linalg.norm(phi64)),
+                "preconditioner": name,
+            })
+        best = min(runs, key=lambda r: r["fp64_ms"])
+        out["time_to_1e-8"] = dict(best, by_preconditioner=runs)
     if args.copy_ref:
         src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -23,12 +23,18 @@ def ensemble():
     return probs, ops, b
 
 
-def test_batch_mult(gpu, ensemble):
+@pytest.mark.parametrize("body", [0, 1, 2])
+def test_batch_mult(gpu, ensemble, body):
+    """every apply body (tile, four-angle tile, box tile) against the oracle"""
     probs, ops, b = ensemble
     b.set_group(0)
+    t = b.tuning()
+    assert t["us"][body] > 0 and t["us"][3 + body] > 0, f"body {body} does not apply: {t}"
     x = np.random.default_rng(1).standard_normal(b.n_rows)
     for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
-        y = b.vec(u.FP64 if dt == torch.float64 else u.FP32)
+        prec = u.FP64 if dt == torch.float64 else u.FP32
+        b.set_body(prec, body)
+        y = b.vec(prec)
         b.mult(to_dev(x, dt), y)
         torch.cuda.synchronize()
         yh = to_host(y)
@@ -36,6 +42,7 @@ def test_batch_mult(gpu, ensemble):
             sl = slice(b.offsets[p], b.offsets[p + 1])
             err = rel_max(yh[sl], oracle_mult(prob, x[sl]))
             assert err <= tol, (p, dt, err)
+        b.set_body(prec, t["body_fp64" if prec == u.FP64 else "body_fp32"])
 
 
 def test_batch_inactive_untouched(gpu, ensemble):

@@ -75,9 +75,10 @@ def built(request):
 
 
 # tile, wavefront per cell, multi-cell tile (2 and 4 cells per thread), pencil on
-# a structured box (8, 16, 32 cells per workgroup), four-angle tile; a variant
-# that does not apply to a case runs the tile kernel
-VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7]
+# a structured box (8, 16, 32 cells per workgroup), four-angle tile, box tile
+# (2 and 4 rows per thread); a variant that does not apply to a case runs the
+# tile kernel
+VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -129,7 +130,7 @@ def test_tuner_reports(gpu, built):
     op.set_variant(core.FP64, -1)
     t = op.tuning()
     print(f"{name}: tuned {t}")
-    assert 0 <= t["variant_fp64"] < 8 and 0 <= t["variant_fp32"] < 8 and t["us"][0] > 0
+    assert 0 <= t["variant_fp64"] < 10 and 0 <= t["variant_fp32"] < 10 and t["us"][0] > 0
 
 
 def test_diagonal(gpu, built):
@@ -228,3 +229,35 @@ def test_geometry_classes(gpu):
           d_op.geometry_classes(), "of", dela.disc.n_cells)
     assert box.geometry_classes() <= 27 and quad.geometry_classes() <= 9
     assert d_op.geometry_classes() == dela.disc.n_cells
+
+
+# Boxes wide and tall enough for several box tiles per axis (x halos from a
+# neighbouring tile, rows beyond a tile, a partial last tile), with vacuum,
+# inflow and reflective sides: the box-tile kernel against the oracle.
+BOX_CASES = {
+    "dg1_quad_tiles": lambda: P.dg_box_2d(23, 13, 4, order=1, bcs={1: "reflect", 3: "reflect",
+                                                                   2: {"type": "vacuum", "inflow": 1.5}}),
+    "dg1_hex_tiles": lambda: P.dg_box_3d(5, 8, order=1, bcs={3: "reflect", 6: "reflect"}),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BOX_CASES))
+def test_box_tile_apply(gpu, case):
+    prob = BOX_CASES[case]()
+    op = prob.operator()
+    us = op.tuning()["us"]
+    assert all(us[v] > 0 for v in (8, 9, 18, 19)), f"box tile not applicable: {us}"
+    x = np.random.default_rng(4).standard_normal(prob.disc.n_rows)
+    want = oracle_mult(prob, x)
+    want_l = oracle_mult(prob, x, scatter=False)
+    for v in (8, 9):
+        op.set_variant(core.FP64, v)
+        op.set_variant(core.FP32, v)
+        y, y32, yl = op.vec(), op.vec(core.FP32), op.vec()
+        op.mult(to_dev(x), y)
+        op.mult(to_dev(x, torch.float32), y32)
+        op.mult_unscattered(to_dev(x), yl)
+        torch.cuda.synchronize()
+        e64, e32, el = rel_max(to_host(y), want), rel_max(to_host(y32), want), rel_max(to_host(yl), want_l)
+        print(f"{case} variant {v}: fp64 {e64:.2e} fp32 {e32:.2e} unscattered {el:.2e}")
+        assert e64 <= FP64_TOL and el <= FP64_TOL and e32 <= FP32_TOL

@@ -13,10 +13,11 @@ namespace ubt {
 template <class T> struct BatchItem {
   OpArgs<T> g;  // the problem's current group
   int64_t off;  // first row of the problem in the batch vector
+  BoxArgs box;  // the box-tile layout (box body only)
 };
 
 // apply bodies the batch autotuner chooses from
-constexpr int kBatchBodies = 2;  // 0 tile, 1 four-angle tile
+constexpr int kBatchBodies = 3;  // 0 tile, 1 four-angle tile, 2 box tile (uniform DG1 boxes)
 
 // the directions of every active problem, one workgroup each
 struct BatchSweep {
@@ -64,8 +65,11 @@ template <class T>
 void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s);
 template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hipStream_t s);
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
-// per-problem tile size and LDS of a body; false where it does not apply
-bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *red_lanes, size_t *lds);
+// per-problem tile size, LDS and tile count of a body (the box body also
+// fills its layout); false where it does not apply
+bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *red_lanes, size_t *lds,
+                       int64_t *n_tiles, BoxArgs *box);
+constexpr int kBatchBoxRows = 4;  // rows per thread of the box body
 
 struct BatchSolver {
   Batch *batch = nullptr;

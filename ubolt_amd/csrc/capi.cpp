@@ -682,6 +682,17 @@ int ubt_batch_tuning(const ubt_batch *batch, int *body, int *persist, float *us)
     }
   });
 }
+int ubt_batch_set_body(ubt_batch *batch, int prec, int body) {
+  return guard([&] {
+    NEED(batch);
+    UBT_CHECK(prec == UBT_FP64 || prec == UBT_FP32, UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
+    UBT_CHECK(body >= 0 && body < ubt::kBatchBodies, UBT_ERR_ARG_OUTOFRANGE, "apply body must be 0 to %d",
+              ubt::kBatchBodies - 1);
+    const int pi = prec == UBT_FP64 ? 0 : 1;
+    UBT_CHECK(batch->b.body_ok[pi][body], UBT_ERR_SUP, "apply body %d does not apply to this batch", body);
+    batch->b.body[pi] = body;
+  });
+}
 int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream) {
   return guard([&] {
     NEED(batch);

@@ -259,17 +259,18 @@ static void build_items(Batch &b, int body) {
     if (!b.active[p]) continue;
     int cpb = 1, lanes = 1;
     size_t l = 0;
-    if (!batch_body_layout(*b.ops[p], pi == 0 ? UBT_FP64 : UBT_FP32, body, &cpb, &lanes, &l)) {
+    int64_t nt = 0;
+    BatchItem<T> it = {};
+    if (!batch_body_layout(*b.ops[p], pi == 0 ? UBT_FP64 : UBT_FP32, body, &cpb, &lanes, &l, &nt, &it.box)) {
       ok = false;
       break;
     }
-    BatchItem<T> it;
     it.g = b.ops[p]->args<T>();
     it.g.cpb = cpb;
     it.g.red_lanes = lanes;
     it.off = b.off[p];
     items.push_back(it);
-    tiles.push_back(tiles.back() + (b.ops[p]->disc->n_cells + cpb - 1) / cpb);
+    tiles.push_back(tiles.back() + nt);
     lds = std::max(lds, l);
   }
   b.body_ok[pi][body] = ok;

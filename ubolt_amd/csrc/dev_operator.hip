@@ -838,6 +838,309 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
 }
 
 // ---------------------------------------------------------------------------
+// Box-tile apply for DG1 on a uniform structured box (reference
+// src/termsk.kokkos.cxx streaming/removal/scatter terms, as the tile kernel).
+// Every interior cell of such a box has one geometry class, and with face
+// normals exactly on the axes the upwind side of each axis is fixed by the
+// sign of the ordinate's cosine alone. So a thread (column, angle) folds the
+// interior class once into three small matrices for its angle:
+//   M   = sum over outflow faces of s_f F_own,f  -  the gradient column,
+//   U_x = s F_up of the x inflow face, U_y (and U_z) likewise,
+// and an interior cell costs sigma psi + M psi + sum_axis U_axis psi_upwind
+// minus the scatter: about NB (1 + DIM + 1) NB multiply-adds per angle, no
+// branch on the face orientation, no class-table reads. Cells on the box
+// boundary (or of another class) take the general per-face path with the
+// class tables, including vacuum and reflective faces.
+//
+// A thread holds K rows of its column along the last axis in registers, so
+// the upwind row is a register (the first or last row of the tile a halo
+// load); the x neighbour is a neighbouring column through LDS (tile edges: a
+// halo load), the y neighbour in 3D a global load. One LDS write, two
+// barriers and one segmented sum per tile of W x K cells serve both the
+// x exchange and the scatter's angular moments; the LDS is double-buffered
+// across the tiles a workgroup walks, so no third barrier is needed.
+// ---------------------------------------------------------------------------
+template <class T, int DIM, int NB> struct BoxConst {
+  T om[DIM];
+  T M[NB][NB];
+  T U[DIM][NB][NB];
+  int in_side[DIM];  // the inflow face of each axis: 0 (the - side) when the cosine is positive
+};
+
+template <class T, int DIM, int NB>
+__device__ __forceinline__ void box_consts(const OpArgs<T> &g, const BoxArgs &bx, int a, BoxConst<T, DIM, NB> &k) {
+  constexpr int NF = 2 * DIM, FM = NB * NB;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) k.om[d] = g.omega[3 * a + d];
+#pragma unroll
+  for (int ax = 0; ax < DIM; ax++) k.in_side[ax] = k.om[ax] > T(0) ? 0 : 1;
+#pragma unroll
+  for (int r = 0; r < NB; r++)
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+      k.M[r][c] = T(0);
+#pragma unroll
+      for (int ax = 0; ax < DIM; ax++) k.U[ax][r][c] = T(0);
+    }
+  const int64_t cls = bx.cls;
+#pragma unroll
+  for (int ax = 0; ax < DIM; ax++) {
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const int f = bx.fmap[ax][side];
+      const T *fs = g.face_s + (cls * NF + f) * 3;
+      T s = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) s += k.om[d] * fs[d];
+      if (s > T(0)) {
+        const T *F = g.face_own + (cls * NF + f) * FM;
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+#pragma unroll
+          for (int c = 0; c < NB; c++) k.M[r][c] += s * F[r * NB + c];
+      } else if (side == k.in_side[ax]) {
+        const T *F = g.face_up + (cls * NF + f) * FM;
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+#pragma unroll
+          for (int c = 0; c < NB; c++) k.U[ax][r][c] = s * F[r * NB + c];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < NB; r++) {
+    T od = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) od += k.om[d] * g.grad[(cls * NB + r) * 3 + d];
+    k.M[r][0] -= od;
+  }
+}
+
+// the segmented sum of seg_sum with the angle weights ws applied
+template <class T>
+__device__ inline void seg_wsum(const T *red, const T *ws, int nseg, int len, T *out, int S) {
+  const int t = threadIdx.x, per = blockDim.x / S;
+  for (int base = 0; base < nseg; base += per) {
+    const int seg = base + t / S, l = t & (S - 1);
+    T p = T(0);
+    if (seg < nseg)
+      for (int k = l; k < len; k += S) p += ws[k] * red[seg * len + k];
+    for (int o = S >> 1; o > 0; o >>= 1) p += __shfl_xor(p, o, S);
+    if (l == 0 && seg < nseg) out[seg] = p;
+  }
+}
+
+template <class T, int DIM, int NB, int K, bool SCATTER>
+__device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
+                                         const BoxArgs &bx, const BoxConst<T, DIM, NB> &kc, int64_t t, T *buf,
+                                         T *phi, const T *ws, int lc, int a) {
+  constexpr int NF = 2 * DIM, FM = NB * NB, R = DIM - 1;
+  const int A = g.A, W = bx.W;
+  const int nx = bx.n[0], ny = bx.n[1], nR = bx.n[R];
+  const int tx = (int)(t % bx.tiles[0]);
+  const int64_t rest = t / bx.tiles[0];
+  const int j = DIM == 3 ? (int)(rest % bx.tiles[1]) : 0;
+  const int r0 = (int)(DIM == 3 ? rest / bx.tiles[1] : rest) * K;
+  const int i = tx * W + lc;
+  const bool lane = lc < W;
+  const bool col = lane && i < nx;
+  auto cell = [&](int ii, int jj, int rr) -> int64_t {
+    return DIM == 2 ? (int64_t)rr * nx + ii : ((int64_t)rr * ny + jj) * nx + ii;
+  };
+  auto row_of = [&](int64_t c) { return c * NB * A + a; };
+  T p[K][NB], hx[K][NB], hy[K][NB], hr[NB], st[K], ss[K];
+  int gc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const bool v = col && r0 + k < nR;
+    const int64_t c = v ? cell(i, j, r0 + k) : 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+    st[k] = v ? g.sigt[c] : T(0);
+    ss[k] = v ? g.sigs[c] : T(0);
+    gc[k] = v ? g.gcls[c] : -1;
+  }
+  // the upwind row beyond the tile along the register axis
+  const int rh = kc.in_side[R] == 0 ? r0 - 1 : r0 + K;
+  {
+    const bool v = col && rh >= 0 && rh < nR;
+    const int64_t c = v ? cell(i, j, rh) : 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) hr[b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+  }
+  // the x upwind column: another lane of the tile, or a halo load at its edge
+  const int ix = kc.in_side[0] == 0 ? i - 1 : i + 1;
+  const bool xin = col && ix >= 0 && ix < nx;
+  const bool xhalo = xin && (kc.in_side[0] == 0 ? lc == 0 : lc == W - 1);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const bool v = xhalo && r0 + k < nR;
+    const int64_t c = v ? cell(ix, j, r0 + k) : 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) hx[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+  }
+  // 3D: the y upwind row, a global load
+  const int jn = DIM == 3 ? (kc.in_side[DIM == 3 ? 1 : 0] == 0 ? j - 1 : j + 1) : 0;
+  const bool yin = DIM == 3 && col && jn >= 0 && jn < ny;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const bool v = yin && r0 + k < nR;
+    const int64_t c = v ? cell(i, jn, r0 + k) : 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) hy[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+  }
+  if (lane) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int b = 0; b < NB; b++) buf[((lc * K + k) * NB + b) * A + a] = p[k][b];
+  }
+  __syncthreads();
+  if (xin && !xhalo) {
+    const int l2 = kc.in_side[0] == 0 ? lc - 1 : lc + 1;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int b = 0; b < NB; b++) hx[k][b] = buf[((l2 * K + k) * NB + b) * A + a];
+  }
+  if (SCATTER) seg_wsum(buf, ws, W * K * NB, A, phi, g.red_lanes);
+  __syncthreads();
+  if (!col) return;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = r0 + k;
+    if (r >= nR) break;
+    const int64_t c = cell(i, j, r);
+    T pr[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const T lo = k > 0 ? p[k > 0 ? k - 1 : 0][b] : hr[b];
+      const T hi = k < K - 1 ? p[k < K - 1 ? k + 1 : 0][b] : hr[b];
+      pr[b] = kc.in_side[R] == 0 ? lo : hi;
+    }
+    const bool interior = gc[k] == bx.cls && i > 0 && i < nx - 1 && r > 0 && r < nR - 1 &&
+                          (DIM == 2 || (j > 0 && j < ny - 1));
+    T acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[b] = st[k] * p[k][b];
+    if (interior) {
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        T v = acc[b];
+#pragma unroll
+        for (int q = 0; q < NB; q++) v += kc.M[b][q] * p[k][q];
+#pragma unroll
+        for (int q = 0; q < NB; q++) v += kc.U[0][b][q] * hx[k][q];
+        if (DIM == 3) {
+#pragma unroll
+          for (int q = 0; q < NB; q++) v += kc.U[DIM == 3 ? 1 : 0][b][q] * hy[k][q];
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) v += kc.U[R][b][q] * pr[q];
+        acc[b] = v;
+      }
+    } else {
+      // the general path: class tables, boundary faces from face_nb
+      const int64_t gk = gc[k];
+#pragma unroll
+      for (int b = 1; b < NB; b++) {
+        T od = T(0);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) od += kc.om[d] * g.grad[(gk * NB + b) * 3 + d];
+        acc[b] -= od * p[k][0];
+      }
+#pragma unroll
+      for (int ax = 0; ax < DIM; ax++) {
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          const int f = bx.fmap[ax][side];
+          const T *fs = g.face_s + (gk * NF + f) * 3;
+          T s = T(0);
+#pragma unroll
+          for (int d = 0; d < DIM; d++) s += kc.om[d] * fs[d];
+          T pu[NB];
+          const T *F;
+          if (s > T(0)) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) pu[b] = p[k][b];
+            F = g.face_own + (gk * NF + f) * FM;
+          } else {
+            // with exactly axis-aligned normals this is the inflow side
+            const int co = ax == 0 ? i : (ax == R ? r : j);
+            const int cn = co + (side ? 1 : -1);
+            if (cn >= 0 && cn < bx.n[ax]) {
+#pragma unroll
+              for (int b = 0; b < NB; b++) pu[b] = ax == 0 ? hx[k][b] : (ax == R ? pr[b] : hy[k][b]);
+            } else {
+              const int nbk = g.face_nb[c * NF + f];
+              if (nbk == -1) continue;  // vacuum: the inflow lives in the rhs
+              // reflective: this cell, mirrored angle (the tile's LDS copy)
+              const int ra = g.reflect[3 * a + (-2 - nbk)];
+#pragma unroll
+              for (int b = 0; b < NB; b++) pu[b] = buf[((lc * K + k) * NB + b) * A + ra];
+            }
+            F = g.face_up + (gk * NF + f) * FM;
+          }
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            T v = T(0);
+#pragma unroll
+            for (int q = 0; q < NB; q++) v += F[b * NB + q] * pu[q];
+            acc[b] += s * v;
+          }
+        }
+      }
+    }
+    if (SCATTER) {
+      const T sc = ss[k] * g.inv_sum_w;
+#pragma unroll
+      for (int b = 0; b < NB; b++) acc[b] -= sc * phi[(lc * K + k) * NB + b];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; b++) y[row_of(c) + (int64_t)b * A] = acc[b];
+  }
+}
+
+// LDS of the box tile: the angle weights, then two moment and two psi buffers
+__host__ __device__ inline size_t box_lds_reals(int A, int W, int K, int nb) {
+  const size_t wpad = ((size_t)A + 1) & ~(size_t)1;
+  return wpad + 2 * (size_t)W * K * nb + 2 * (size_t)W * K * nb * A;
+}
+
+// Runs tiles [t0, t1) of one problem; every thread of the workgroup calls it.
+template <class T, int DIM, int NB, int K, bool SCATTER>
+__device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
+                                        const BoxArgs &bx, int64_t t0, int64_t t1) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int A = g.A, W = bx.W;
+  const int lc = threadIdx.x / A;
+  const int a = lc < W ? (int)threadIdx.x - lc * A : 0;
+  T *ws = reinterpret_cast<T *>(smem);
+  T *phib = ws + ((A + 1) & ~1);
+  T *bufb = phib + 2 * W * K * NB;
+  for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = g.w[e];
+  BoxConst<T, DIM, NB> kc;
+  box_consts<T, DIM, NB>(g, bx, a, kc);
+  int par = 0;
+  for (int64_t t = t0; t < t1; t++, par ^= 1)
+    box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb + par * W * K * NB * A, phib + par * W * K * NB, ws,
+                                     lc, a);
+}
+
+// One problem: a persistent grid (a multiple of 8 workgroups), each walking one
+// contiguous run of tiles, ranked XCD-major so a run's halos stay in one L2.
+template <class T, int DIM, int NB, int K, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_box(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g,
+                                                    BoxArgs bx, int64_t total) {
+  const int64_t G = gridDim.x;
+  const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  const int64_t chunk = (total + G - 1) / G;
+  const int64_t t0 = rank * chunk, t1 = t0 + chunk < total ? t0 + chunk : total;
+  if (t0 >= t1) return;
+  box_run<T, DIM, NB, K, SCATTER>(x, y, g, bx, t0, t1);
+}
+
+// ---------------------------------------------------------------------------
 // Batched apply: one persistent launch over the tiles of many problems. The
 // work list holds each active problem's operator arguments and row offset;
 // tile_off is the prefix of tiles per problem. Workgroups are ranked
@@ -861,6 +1164,30 @@ __global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T
     else
       dg_q4_body<T, DIM, NB, NF, SCATTER>(x + it.off, y + it.off, it.g, t - tile_off[p]);
     __syncthreads();  // the next tile restages the LDS
+  }
+}
+
+// Batched box tile: a workgroup's contiguous run of tiles, one problem at a
+// time (the per-angle matrices are refolded at each problem change).
+template <class T, int DIM, int NB, int K, bool SCATTER>
+__global__ void __launch_bounds__(256) dg_apply_batch_box(const T *__restrict__ x, T *__restrict__ y,
+                                                          const BatchItem<T> *__restrict__ items,
+                                                          const int64_t *__restrict__ tile_off, int64_t total) {
+  const int64_t G = gridDim.x;
+  const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  const int64_t chunk = (total + G - 1) / G;
+  int64_t t = rank * chunk;
+  const int64_t t1 = t + chunk < total ? t + chunk : total;
+  int p = 0;
+  bool first = true;
+  while (t < t1) {
+    while (t >= tile_off[p + 1]) p++;
+    const int64_t e = t1 < tile_off[p + 1] ? t1 : tile_off[p + 1];
+    if (!first) __syncthreads();  // the next problem lays the LDS out anew
+    first = false;
+    const BatchItem<T> &it = items[p];
+    box_run<T, DIM, NB, K, SCATTER>(x + it.off, y + it.off, it.g, it.box, t - tile_off[p], e - tile_off[p]);
+    t = e;
   }
 }
 
@@ -1521,6 +1848,76 @@ bool launch_apply_pencil(const Operator &op, OpArgs<T> a, const T *x, T *y, bool
 
 // The multi-cell tile kernel with CPT cells per thread; false where it does
 // not apply (FD, more than 256 angles, or a tile that overflows the LDS)
+static int device_cus(int dev) {
+  static int cache[64] = {};
+  if (dev < 0 || dev >= 64) return 256;
+  if (cache[dev] == 0) {
+    int n = 0;
+    UBT_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev] = n > 0 ? n : 256;
+  }
+  return cache[dev];
+}
+
+// The box-tile layout of an operator for K rows per thread: columns per tile
+// (as many as 256 threads hold, fewer if the LDS overflows), tiles, threads,
+// the segmented-sum lanes and the LDS; false where the kernel does not apply
+// (not a uniform DG1 box, more than 256 angles, or no column fits the LDS)
+bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads, int &lanes, size_t &lds) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles, nb = D.nb, dim = D.dim;
+  if (op.box_cls < 0 || A > 256) return false;
+  int W = 256 / A;
+  while (W > 1 && box_lds_reals(A, W, K, nb) * ts > 65536) W--;
+  lds = box_lds_reals(A, W, K, nb) * ts;
+  if (lds > 65536) return false;
+  for (int d = 0; d < 3; d++) {
+    bx.n[d] = op.box_n[d];
+    bx.fmap[d][0] = op.box_fmap[d][0];
+    bx.fmap[d][1] = op.box_fmap[d][1];
+  }
+  bx.cls = op.box_cls;
+  bx.W = W;
+  bx.tiles[0] = (op.box_n[0] + W - 1) / W;
+  bx.tiles[1] = dim == 3 ? op.box_n[1] : 1;
+  bx.tiles[2] = (op.box_n[dim - 1] + K - 1) / K;
+  threads = ((W * A + 63) / 64) * 64;
+  int S = pow2_floor(std::max(1, threads / (W * K * nb)));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  lanes = std::max(1, S);
+  return true;
+}
+
+template <class T>
+bool launch_apply_box(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int K,
+                      bool dry = false) {
+  const Disc &D = *op.disc;
+  BoxArgs bx;
+  int threads = 0, lanes = 1;
+  size_t lds = 0;
+  if (!box_layout(op, sizeof(T), K, bx, threads, lanes, lds)) return false;
+  if (dry) return true;
+  a.red_lanes = lanes;
+  const int64_t total = (int64_t)bx.tiles[0] * bx.tiles[1] * bx.tiles[2];
+  int64_t g = std::min<int64_t>(total, (int64_t)device_cus(op.device) * 8);
+  g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+  const unsigned grid = (unsigned)g;
+  bool launched = false;
+#define BOX(DIM, NB, KK)                                                                                          \
+  if (!launched && D.dim == DIM && D.nb == NB && K == KK) {                                                       \
+    if (scatter)                                                                                                  \
+      hipLaunchKernelGGL((dg_apply_box<T, DIM, NB, KK, true>), dim3(grid), dim3(threads), lds, s, x, y, a, bx,     \
+                         total);                                                                                  \
+    else                                                                                                          \
+      hipLaunchKernelGGL((dg_apply_box<T, DIM, NB, KK, false>), dim3(grid), dim3(threads), lds, s, x, y, a, bx,    \
+                         total);                                                                                  \
+    launched = true;                                                                                              \
+  }
+  BOX(2, 3, 2) BOX(2, 3, 4) BOX(3, 4, 2) BOX(3, 4, 4)
+#undef BOX
+  return launched;
+}
+
 template <class T>
 bool launch_apply_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int cpt) {
   const Disc &D = *op.disc;
@@ -1561,6 +1958,10 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
   const Disc &D = *op.disc;
   OpArgs<T> a = a0;
   const int var = op.variant[sizeof(T) == 8 ? 0 : 1];
+  if (!adj && (var == 8 || var == 9) && launch_apply_box<T>(op, a, x, y, scatter, s, var == 8 ? 2 : 4)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
   if (!adj && var == 7 && launch_apply_q4<T>(op, a, x, y, scatter, s)) {
     UBT_HIP(hipGetLastError());
     return;
@@ -1787,6 +2188,27 @@ static bool detect_box(const Disc &D, int n[3], int fmap[3][2]) {
   return true;
 }
 
+// The geometry class of a box's interior cells, or -1 where the box-tile
+// kernel does not apply: a box thinner than 3 cells along an axis, a face
+// normal with an off-axis component (the upwind side must follow from the
+// sign of one cosine, exactly, in either precision), or an ordinate with a
+// zero cosine along an axis of the box.
+static int box_interior_class(const Disc &D, const std::vector<int> &cls, const int n[3]) {
+  const int dim = D.dim, nf = D.nf;
+  for (int d = 0; d < dim; d++)
+    if (n[d] < 3) return -1;
+  for (size_t e = 0; e < (size_t)D.n_cells * nf; e++) {
+    int nonzero = 0;
+    for (int d = 0; d < 3; d++) nonzero += D.face_nA[3 * e + d] != 0.0;
+    if (nonzero != 1) return -1;
+  }
+  for (int a = 0; a < D.n_angles; a++)
+    for (int d = 0; d < dim; d++)
+      if (D.quad.omega[3 * (size_t)a + d] == 0.0) return -1;
+  const int64_t c = dim == 2 ? (int64_t)n[0] + 1 : ((int64_t)n[1] + 1) * n[0] + 1;
+  return cls[c];
+}
+
 static void upload_prec(PrecTables &t, const Disc &D, const std::vector<int> &rep, bool fp32) {
   std::vector<double> omega(D.quad.omega), w(D.quad.w);
   std::vector<double> face_s, face_own, face_up, grad, vol;
@@ -1851,6 +2273,7 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
     op->n_geo_classes = (int)op->geo_rep.size();
     op->gcls.upload(cls);
     op->box = detect_box(D, op->box_n, op->box_fmap);
+    if (op->box && D.nb > 1) op->box_cls = box_interior_class(D, cls, op->box_n);
   }
   upload_prec(op->t64, D, op->geo_rep, false);
   upload_prec(op->t32, D, op->geo_rep, true);
@@ -1933,6 +2356,9 @@ void Operator::tune() {
   auto applicable = [&](int v, int p) {
     if (v == 0) return true;
     if (v == 1) return wave_ok;
+    if (v == 8 || v == 9)
+      return p == 0 ? launch_apply_box<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, 2, true)
+                    : launch_apply_box<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, 2, true);
     if (v == 7)
       return p == 0 ? launch_apply_q4<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
                     : launch_apply_q4<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);
@@ -2177,9 +2603,17 @@ void BlockInverse::apply(int prec, const void *x, void *y, hipStream_t s, bool t
 // ---------------------------------------------------------------------------
 // Batched launchers
 // ---------------------------------------------------------------------------
-bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int *lanes_out, size_t *lds_out) {
+bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int *lanes_out, size_t *lds_out,
+                       int64_t *n_tiles, BoxArgs *box) {
   const Disc &D = *op.disc;
   if (D.kind != DISC_DG) return false;
+  if (body == 2) {
+    int threads = 0;
+    if (!box_layout(op, prec == UBT_FP64 ? 8 : 4, kBatchBoxRows, *box, threads, *lanes_out, *lds_out)) return false;
+    *cpb_out = box->W;
+    *n_tiles = (int64_t)box->tiles[0] * box->tiles[1] * box->tiles[2];
+    return true;
+  }
   const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
   const size_t ts = prec == UBT_FP64 ? 8 : 4;
   const size_t geo = (size_t)((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) * ts +
@@ -2207,6 +2641,7 @@ bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int
   *cpb_out = cpb;
   *lanes_out = lanes;
   *lds_out = lds;
+  *n_tiles = (D.n_cells + cpb - 1) / cpb;
   return true;
 }
 
@@ -2238,6 +2673,18 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   BT_ALL(1)
 #undef BT_ALL
 #undef BT
+#define BB(DIM, NB)                                                                                              \
+  if (!launched && b.dim == DIM && b.nb == NB && body == 2) {                                                    \
+    if (scatter)                                                                                                 \
+      hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, true>), dim3(grid), dim3(256), lds, s, x, \
+                         y, items, toff, total);                                                                 \
+    else                                                                                                         \
+      hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, false>), dim3(grid), dim3(256), lds, s,   \
+                         x, y, items, toff, total);                                                              \
+    launched = true;                                                                                             \
+  }
+  BB(2, 3) BB(3, 4)
+#undef BB
   UBT_CHECK(launched, UBT_ERR_SUP, "no batched apply for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
   UBT_HIP(hipGetLastError());
 }

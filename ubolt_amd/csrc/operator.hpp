@@ -45,9 +45,21 @@ struct PrecTables {
 // apply kernels the tuner chooses from: 0 tile, 1 wavefront per cell,
 // 2 and 3 the multi-cell tile with 2 and 4 cells per thread, 4-6 the pencil
 // kernel of a structured box marching 8, 16 or 32 cells per workgroup, 7 the
-// four-angle tile
-constexpr int kApplyVariants = 8;
+// four-angle tile, 8 and 9 the box tile with 2 and 4 rows per thread
+constexpr int kApplyVariants = 10;
 constexpr int kPencilMaxClasses = 64;
+
+// The box-tile kernels' view of a uniform DG1 box: a tile is W columns along
+// x (thread = column, angle) by K rows along the last axis held in
+// registers, at one y index in 3D. Tiles are numbered x fastest, then y,
+// then the register axis.
+struct BoxArgs {
+  int n[3];         // cells per axis
+  int fmap[3][2];   // local face of (axis, side)
+  int cls;          // the geometry class of the interior cells
+  int W;            // columns per tile
+  int tiles[3];     // tiles along x, y (1 in 2D) and the register axis
+};
 
 struct Operator {
   std::shared_ptr<Disc> disc;
@@ -66,6 +78,10 @@ struct Operator {
   bool box = false;
   int box_n[3] = {1, 1, 1};
   int box_fmap[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+  // a DG1 box whose interior cells share one geometry class, whose face
+  // normals are exactly axis-aligned and whose ordinates have no zero cosine
+  // (the box-tile kernel's precondition): that class
+  int box_cls = -1;
   DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
   DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
