@@ -42,6 +42,9 @@ struct Batch {
   int persist[2] = {8, 8};     // workgroups per CU of the persistent grid
   float tuned_us[2][kBatchBodies] = {};
   int n_cu = 256;
+  // box body: prefix of boundary-cell tiles over the items, per precision
+  DevBuf edge_off[2];
+  int64_t n_edge[2] = {};
   int gen = 0;                 // bumped whenever the work lists are rebuilt
   // block inverse and sweep work lists
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems

@@ -252,7 +252,7 @@ template <class T>
 static void build_items(Batch &b, int body) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   std::vector<BatchItem<T>> items;
-  std::vector<int64_t> tiles(1, 0);
+  std::vector<int64_t> tiles(1, 0), edges(1, 0);
   size_t lds = 0;
   bool ok = true;
   for (size_t p = 0; p < b.ops.size() && ok; p++) {
@@ -271,12 +271,17 @@ static void build_items(Batch &b, int body) {
     it.off = b.off[p];
     items.push_back(it);
     tiles.push_back(tiles.back() + nt);
+    edges.push_back(edges.back() + (body == 2 ? it.box.edge_tiles : 0));
     lds = std::max(lds, l);
   }
   b.body_ok[pi][body] = ok;
   if (!ok) return;
   b.items[pi][body].upload(items);
   b.tiles[pi][body].upload(tiles);
+  if (body == 2) {
+    b.edge_off[pi].upload(edges);
+    b.n_edge[pi] = edges.back();
+  }
   b.n_items[pi][body] = (int)items.size();
   b.n_tiles[pi][body] = tiles.back();
   b.lds[pi][body] = lds;

@@ -119,14 +119,18 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
 // ADJ: the adjoint A^T = V P L P V^{-1} - S^T (P reverses every angle): the
 // same kernel with Omega -> -Omega, each neighbour's trace scaled by V_c / V_n,
 // and the scatter transposed.
-template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ>
+// LIST: the tile's cells are entries tile * cpb ... of g.cells (n_list of
+// them) instead of a contiguous run of cell ids.
+template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ, bool LIST = false>
 __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
                                              int64_t tile) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int FM = NB * NB;
   const int cpb = g.cpb, A = g.A;
   const int64_t c0 = tile * cpb;
-  const int ncell = (int)((int64_t)g.n_cells - c0 < cpb ? (int64_t)g.n_cells - c0 : cpb);
+  const int64_t n_here = LIST ? (int64_t)g.n_list : (int64_t)g.n_cells;
+  const int ncell = (int)(n_here - c0 < cpb ? n_here - c0 : cpb);
+  auto cell_of = [&](int l) -> int64_t { return LIST ? (int64_t)g.cells[c0 + l] : c0 + l; };
   // LDS: F_own, F_up, face s (DIM per slot), grad (NB-1 x DIM per cell),
   // sigma_t, sigma_s, then the reduction buffers; face_nb after the reals
   T *s_own = reinterpret_cast<T *>(smem);
@@ -142,32 +146,34 @@ __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restr
   if (NB > 1) {
     for (int e = tid; e < ncell * NF * FM; e += nt) {
       const int lc = e / (NF * FM);
-      const int64_t ge = (int64_t)g.gcls[c0 + lc] * NF * FM + (e - lc * NF * FM);
+      const int64_t ge = (int64_t)g.gcls[cell_of(lc)] * NF * FM + (e - lc * NF * FM);
       s_own[e] = g.face_own[ge];
       s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int lc = e / ((NB - 1) * DIM), r = e - lc * ((NB - 1) * DIM);
       const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
-      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + lc] * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)g.gcls[cell_of(lc)] * NB + i) * 3 + d];
     }
   }
   for (int e = tid; e < ncell * NF; e += nt) {
-    s_nb[e] = g.face_nb[c0 * NF + e];
     const int lc = e / NF;
-    const int64_t gk = (int64_t)g.gcls[c0 + lc] * NF + (e - lc * NF);
+    const int64_t ce = cell_of(lc);
+    s_nb[e] = g.face_nb[ce * NF + (e - lc * NF)];
+    const int64_t gk = (int64_t)g.gcls[ce] * NF + (e - lc * NF);
 #pragma unroll
     for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
   }
   for (int e = tid; e < ncell; e += nt) {
-    s_st[e] = g.sigt[c0 + e];
-    s_ss[e] = g.sigs[c0 + e];
+    const int64_t ce = cell_of(e);
+    s_st[e] = g.sigt[ce];
+    s_ss[e] = g.sigs[ce];
   }
   __syncthreads();
 
   const int lc = tid / A, a = tid - lc * A;
-  const int64_t c = c0 + lc;
   const bool live = lc < ncell;
+  const int64_t c = live ? cell_of(lc) : 0;
   T psi[NB], acc[NB];
 #pragma unroll
   for (int j = 0; j < NB; j++) psi[j] = acc[j] = T(0);
@@ -849,16 +855,17 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
 // and an interior cell costs sigma psi + M psi + sum_axis U_axis psi_upwind
 // minus the scatter: about NB (1 + DIM + 1) NB multiply-adds per angle, no
 // branch on the face orientation, no class-table reads. Cells on the box
-// boundary (or of another class) take the general per-face path with the
-// class tables, including vacuum and reflective faces.
+// boundary (or of another class) are skipped here and computed afterwards in
+// the same launch by the tile kernel's body over their list, with vacuum and
+// reflective faces; keeping that general path out of this loop keeps the
+// register count, and so the waves per SIMD, where the interior needs it.
 //
 // A thread holds K rows of its column along the last axis in registers, so
 // the upwind row is a register (the first or last row of the tile a halo
 // load); the x neighbour is a neighbouring column through LDS (tile edges: a
 // halo load), the y neighbour in 3D a global load. One LDS write, two
 // barriers and one segmented sum per tile of W x K cells serve both the
-// x exchange and the scatter's angular moments; the LDS is double-buffered
-// across the tiles a workgroup walks, so no third barrier is needed.
+// x exchange and the scatter's angular moments.
 // ---------------------------------------------------------------------------
 template <class T, int DIM, int NB> struct BoxConst {
   T om[DIM];
@@ -934,7 +941,7 @@ template <class T, int DIM, int NB, int K, bool SCATTER>
 __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
                                          const BoxArgs &bx, const BoxConst<T, DIM, NB> &kc, int64_t t, T *buf,
                                          T *phi, const T *ws, int lc, int a) {
-  constexpr int NF = 2 * DIM, FM = NB * NB, R = DIM - 1;
+  constexpr int R = DIM - 1;
   const int A = g.A, W = bx.W;
   const int nx = bx.n[0], ny = bx.n[1], nR = bx.n[R];
   const int tx = (int)(t % bx.tiles[0]);
@@ -1003,7 +1010,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
 #pragma unroll
       for (int b = 0; b < NB; b++) hx[k][b] = buf[((l2 * K + k) * NB + b) * A + a];
   }
-  if (SCATTER) seg_wsum(buf, ws, W * K * NB, A, phi, g.red_lanes);
+  if (SCATTER) seg_wsum(buf, ws, W * K * NB, A, phi, bx.lanes);
   __syncthreads();
   if (!col) return;
 #pragma unroll
@@ -1018,78 +1025,24 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
       const T hi = k < K - 1 ? p[k < K - 1 ? k + 1 : 0][b] : hr[b];
       pr[b] = kc.in_side[R] == 0 ? lo : hi;
     }
-    const bool interior = gc[k] == bx.cls && i > 0 && i < nx - 1 && r > 0 && r < nR - 1 &&
-                          (DIM == 2 || (j > 0 && j < ny - 1));
+    // boundary cells (and any of another class) belong to the list phase
+    if (!(gc[k] == bx.cls && i > 0 && i < nx - 1 && r > 0 && r < nR - 1 && (DIM == 2 || (j > 0 && j < ny - 1))))
+      continue;
     T acc[NB];
 #pragma unroll
-    for (int b = 0; b < NB; b++) acc[b] = st[k] * p[k][b];
-    if (interior) {
+    for (int b = 0; b < NB; b++) {
+      T v = st[k] * p[k][b];
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        T v = acc[b];
+      for (int q = 0; q < NB; q++) v += kc.M[b][q] * p[k][q];
 #pragma unroll
-        for (int q = 0; q < NB; q++) v += kc.M[b][q] * p[k][q];
+      for (int q = 0; q < NB; q++) v += kc.U[0][b][q] * hx[k][q];
+      if (DIM == 3) {
 #pragma unroll
-        for (int q = 0; q < NB; q++) v += kc.U[0][b][q] * hx[k][q];
-        if (DIM == 3) {
-#pragma unroll
-          for (int q = 0; q < NB; q++) v += kc.U[DIM == 3 ? 1 : 0][b][q] * hy[k][q];
-        }
-#pragma unroll
-        for (int q = 0; q < NB; q++) v += kc.U[R][b][q] * pr[q];
-        acc[b] = v;
-      }
-    } else {
-      // the general path: class tables, boundary faces from face_nb
-      const int64_t gk = gc[k];
-#pragma unroll
-      for (int b = 1; b < NB; b++) {
-        T od = T(0);
-#pragma unroll
-        for (int d = 0; d < DIM; d++) od += kc.om[d] * g.grad[(gk * NB + b) * 3 + d];
-        acc[b] -= od * p[k][0];
+        for (int q = 0; q < NB; q++) v += kc.U[DIM == 3 ? 1 : 0][b][q] * hy[k][q];
       }
 #pragma unroll
-      for (int ax = 0; ax < DIM; ax++) {
-#pragma unroll
-        for (int side = 0; side < 2; side++) {
-          const int f = bx.fmap[ax][side];
-          const T *fs = g.face_s + (gk * NF + f) * 3;
-          T s = T(0);
-#pragma unroll
-          for (int d = 0; d < DIM; d++) s += kc.om[d] * fs[d];
-          T pu[NB];
-          const T *F;
-          if (s > T(0)) {
-#pragma unroll
-            for (int b = 0; b < NB; b++) pu[b] = p[k][b];
-            F = g.face_own + (gk * NF + f) * FM;
-          } else {
-            // with exactly axis-aligned normals this is the inflow side
-            const int co = ax == 0 ? i : (ax == R ? r : j);
-            const int cn = co + (side ? 1 : -1);
-            if (cn >= 0 && cn < bx.n[ax]) {
-#pragma unroll
-              for (int b = 0; b < NB; b++) pu[b] = ax == 0 ? hx[k][b] : (ax == R ? pr[b] : hy[k][b]);
-            } else {
-              const int nbk = g.face_nb[c * NF + f];
-              if (nbk == -1) continue;  // vacuum: the inflow lives in the rhs
-              // reflective: this cell, mirrored angle (the tile's LDS copy)
-              const int ra = g.reflect[3 * a + (-2 - nbk)];
-#pragma unroll
-              for (int b = 0; b < NB; b++) pu[b] = buf[((lc * K + k) * NB + b) * A + ra];
-            }
-            F = g.face_up + (gk * NF + f) * FM;
-          }
-#pragma unroll
-          for (int b = 0; b < NB; b++) {
-            T v = T(0);
-#pragma unroll
-            for (int q = 0; q < NB; q++) v += F[b * NB + q] * pu[q];
-            acc[b] += s * v;
-          }
-        }
-      }
+      for (int q = 0; q < NB; q++) v += kc.U[R][b][q] * pr[q];
+      acc[b] = v;
     }
     if (SCATTER) {
       const T sc = ss[k] * g.inv_sum_w;
@@ -1101,10 +1054,12 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
   }
 }
 
-// LDS of the box tile: the angle weights, then two moment and two psi buffers
+// LDS of the box tile: the angle weights, the moments, the psi tile. One
+// buffer each serves consecutive tiles: a tile's LDS reads all come before
+// its second barrier, and the next tile's writes after it.
 __host__ __device__ inline size_t box_lds_reals(int A, int W, int K, int nb) {
   const size_t wpad = ((size_t)A + 1) & ~(size_t)1;
-  return wpad + 2 * (size_t)W * K * nb + 2 * (size_t)W * K * nb * A;
+  return wpad + (size_t)W * K * nb + (size_t)W * K * nb * A;
 }
 
 // Runs tiles [t0, t1) of one problem; every thread of the workgroup calls it.
@@ -1117,18 +1072,16 @@ __device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__
   const int a = lc < W ? (int)threadIdx.x - lc * A : 0;
   T *ws = reinterpret_cast<T *>(smem);
   T *phib = ws + ((A + 1) & ~1);
-  T *bufb = phib + 2 * W * K * NB;
+  T *bufb = phib + W * K * NB;
   for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = g.w[e];
   BoxConst<T, DIM, NB> kc;
   box_consts<T, DIM, NB>(g, bx, a, kc);
-  int par = 0;
-  for (int64_t t = t0; t < t1; t++, par ^= 1)
-    box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb + par * W * K * NB * A, phib + par * W * K * NB, ws,
-                                     lc, a);
+  for (int64_t t = t0; t < t1; t++) box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb, phib, ws, lc, a);
 }
 
 // One problem: a persistent grid (a multiple of 8 workgroups), each walking one
-// contiguous run of tiles, ranked XCD-major so a run's halos stay in one L2.
+// contiguous run of tiles, ranked XCD-major so a run's halos stay in one L2,
+// then its share of the boundary-cell list through the tile kernel's body.
 template <class T, int DIM, int NB, int K, bool SCATTER>
 __global__ void __launch_bounds__(256) dg_apply_box(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g,
                                                     BoxArgs bx, int64_t total) {
@@ -1136,8 +1089,13 @@ __global__ void __launch_bounds__(256) dg_apply_box(const T *__restrict__ x, T *
   const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
   const int64_t chunk = (total + G - 1) / G;
   const int64_t t0 = rank * chunk, t1 = t0 + chunk < total ? t0 + chunk : total;
-  if (t0 >= t1) return;
-  box_run<T, DIM, NB, K, SCATTER>(x, y, g, bx, t0, t1);
+  if (t0 < t1) box_run<T, DIM, NB, K, SCATTER>(x, y, g, bx, t0, t1);
+  const int64_t E = bx.edge_tiles, ce = (E + G - 1) / G;
+  const int64_t e0 = rank * ce, e1 = e0 + ce < E ? e0 + ce : E;
+  for (int64_t e = e0; e < e1; e++) {
+    __syncthreads();  // the LDS changes layout
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true>(x, y, g, e);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,11 +1126,13 @@ __global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T
 }
 
 // Batched box tile: a workgroup's contiguous run of tiles, one problem at a
-// time (the per-angle matrices are refolded at each problem change).
+// time (the per-angle matrices are refolded at each problem change), then
+// its contiguous share of every problem's boundary-cell tiles.
 template <class T, int DIM, int NB, int K, bool SCATTER>
 __global__ void __launch_bounds__(256) dg_apply_batch_box(const T *__restrict__ x, T *__restrict__ y,
                                                           const BatchItem<T> *__restrict__ items,
-                                                          const int64_t *__restrict__ tile_off, int64_t total) {
+                                                          const int64_t *__restrict__ tile_off, int64_t total,
+                                                          const int64_t *__restrict__ edge_off, int64_t n_edge) {
   const int64_t G = gridDim.x;
   const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
   const int64_t chunk = (total + G - 1) / G;
@@ -1188,6 +1148,15 @@ __global__ void __launch_bounds__(256) dg_apply_batch_box(const T *__restrict__ 
     const BatchItem<T> &it = items[p];
     box_run<T, DIM, NB, K, SCATTER>(x + it.off, y + it.off, it.g, it.box, t - tile_off[p], e - tile_off[p]);
     t = e;
+  }
+  const int64_t ce = (n_edge + G - 1) / G;
+  const int64_t e1 = rank * ce + ce < n_edge ? rank * ce + ce : n_edge;
+  p = 0;
+  for (int64_t e = rank * ce; e < e1; e++) {
+    while (e >= edge_off[p + 1]) p++;
+    __syncthreads();  // the LDS changes layout
+    const BatchItem<T> &it = items[p];
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true>(x + it.off, y + it.off, it.g, e - edge_off[p]);
   }
 }
 
@@ -1863,14 +1832,27 @@ static int device_cus(int dev) {
 // (as many as 256 threads hold, fewer if the LDS overflows), tiles, threads,
 // the segmented-sum lanes and the LDS; false where the kernel does not apply
 // (not a uniform DG1 box, more than 256 angles, or no column fits the LDS)
-bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads, int &lanes, size_t &lds) {
+bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads, int &cpb, int &lanes,
+                size_t &lds) {
   const Disc &D = *op.disc;
-  const int A = D.n_angles, nb = D.nb, dim = D.dim;
+  const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
   if (op.box_cls < 0 || A > 256) return false;
   int W = 256 / A;
   while (W > 1 && box_lds_reals(A, W, K, nb) * ts > 65536) W--;
   lds = box_lds_reals(A, W, K, nb) * ts;
   if (lds > 65536) return false;
+  threads = ((W * A + 63) / 64) * 64;
+  // the boundary-cell phase: the tile body, cpb cells per tile
+  const size_t geo = (size_t)(2 * nf * nb * nb + nf * dim + (nb - 1) * dim + 2) * ts + sizeof(int) * (size_t)nf;
+  auto tile_lds = [&](int cp) { return (size_t)(cp * nb * A + cp * nb) * ts + (size_t)cp * geo; };
+  cpb = threads / A;
+  while (cpb > 1 && tile_lds(cpb) > 65536) cpb--;
+  if (tile_lds(cpb) > 65536) return false;
+  lds = std::max(lds, tile_lds(cpb));
+  int Se = pow2_floor(std::max(1, threads / (cpb * nb)));
+  Se = std::min(std::min(Se, 64), pow2_floor(A));
+  lanes = std::max(1, Se);
+  bx.edge_tiles = (op.n_box_edge + cpb - 1) / cpb;
   for (int d = 0; d < 3; d++) {
     bx.n[d] = op.box_n[d];
     bx.fmap[d][0] = op.box_fmap[d][0];
@@ -1881,10 +1863,9 @@ bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads,
   bx.tiles[0] = (op.box_n[0] + W - 1) / W;
   bx.tiles[1] = dim == 3 ? op.box_n[1] : 1;
   bx.tiles[2] = (op.box_n[dim - 1] + K - 1) / K;
-  threads = ((W * A + 63) / 64) * 64;
   int S = pow2_floor(std::max(1, threads / (W * K * nb)));
   S = std::min(std::min(S, 64), pow2_floor(A));
-  lanes = std::max(1, S);
+  bx.lanes = std::max(1, S);
   return true;
 }
 
@@ -1893,10 +1874,11 @@ bool launch_apply_box(const Operator &op, OpArgs<T> a, const T *x, T *y, bool sc
                       bool dry = false) {
   const Disc &D = *op.disc;
   BoxArgs bx;
-  int threads = 0, lanes = 1;
+  int threads = 0, cpb = 1, lanes = 1;
   size_t lds = 0;
-  if (!box_layout(op, sizeof(T), K, bx, threads, lanes, lds)) return false;
+  if (!box_layout(op, sizeof(T), K, bx, threads, cpb, lanes, lds)) return false;
   if (dry) return true;
+  a.cpb = cpb;
   a.red_lanes = lanes;
   const int64_t total = (int64_t)bx.tiles[0] * bx.tiles[1] * bx.tiles[2];
   int64_t g = std::min<int64_t>(total, (int64_t)device_cus(op.device) * 8);
@@ -2065,6 +2047,8 @@ template <class T> OpArgs<T> make_args(const Operator &op, const PrecTables &t) 
   a.face_own = t.face_own.as<T>();
   a.face_up = t.face_up.as<T>();
   a.grad = t.grad.as<T>();
+  a.cells = op.box_edge.as<int>();
+  a.n_list = op.n_box_edge;
   return a;
 }
 
@@ -2274,6 +2258,20 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
     op->gcls.upload(cls);
     op->box = detect_box(D, op->box_n, op->box_fmap);
     if (op->box && D.nb > 1) op->box_cls = box_interior_class(D, cls, op->box_n);
+    if (op->box_cls >= 0) {
+      // the cells the box-tile kernel leaves to its list phase: the box's
+      // boundary layer and any cell of another class
+      const int *n = op->box_n;
+      std::vector<int> edge;
+      for (int64_t c = 0; c < D.n_cells; c++) {
+        const int64_t ix = c % n[0], iy = (c / n[0]) % n[1], iz = c / ((int64_t)n[0] * n[1]);
+        bool e = cls[c] != op->box_cls || ix == 0 || ix == n[0] - 1 || iy == 0 || iy == n[1] - 1;
+        if (D.dim == 3) e = e || iz == 0 || iz == n[2] - 1;
+        if (e) edge.push_back((int)c);
+      }
+      op->box_edge.upload(edge);
+      op->n_box_edge = (int)edge.size();
+    }
   }
   upload_prec(op->t64, D, op->geo_rep, false);
   upload_prec(op->t32, D, op->geo_rep, true);
@@ -2609,8 +2607,8 @@ bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int
   if (D.kind != DISC_DG) return false;
   if (body == 2) {
     int threads = 0;
-    if (!box_layout(op, prec == UBT_FP64 ? 8 : 4, kBatchBoxRows, *box, threads, *lanes_out, *lds_out)) return false;
-    *cpb_out = box->W;
+    if (!box_layout(op, prec == UBT_FP64 ? 8 : 4, kBatchBoxRows, *box, threads, *cpb_out, *lanes_out, *lds_out))
+      return false;
     *n_tiles = (int64_t)box->tiles[0] * box->tiles[1] * box->tiles[2];
     return true;
   }
@@ -2673,14 +2671,16 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   BT_ALL(1)
 #undef BT_ALL
 #undef BT
+  const int64_t *eoff = b.edge_off[pi].as<int64_t>();
+  const int64_t n_edge = b.n_edge[pi];
 #define BB(DIM, NB)                                                                                              \
   if (!launched && b.dim == DIM && b.nb == NB && body == 2) {                                                    \
     if (scatter)                                                                                                 \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, true>), dim3(grid), dim3(256), lds, s, x, \
-                         y, items, toff, total);                                                                 \
+                         y, items, toff, total, eoff, n_edge);                                                   \
     else                                                                                                         \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, false>), dim3(grid), dim3(256), lds, s,   \
-                         x, y, items, toff, total);                                                              \
+                         x, y, items, toff, total, eoff, n_edge);                                                \
     launched = true;                                                                                             \
   }
   BB(2, 3) BB(3, 4)

@@ -32,6 +32,10 @@ template <class T> struct OpArgs {
   const T *face_up;
   const T *grad;       // classes * nb * 3
   const T *vol;        // classes: volume (the adjoint's neighbour ratio V_c / V_n)
+  // a list of cells for the kernels that run on one (the box-tile kernel's
+  // boundary cells), n_list of them
+  const int *cells;
+  int n_list;
 };
 
 // Per precision copies of the geometry and cross sections
@@ -53,12 +57,17 @@ constexpr int kPencilMaxClasses = 64;
 // x (thread = column, angle) by K rows along the last axis held in
 // registers, at one y index in 3D. Tiles are numbered x fastest, then y,
 // then the register axis.
+// The cells on the box's boundary (and any of another class) are left to a
+// second phase of the same launch that runs the tile kernel's body on their
+// list.
 struct BoxArgs {
   int n[3];         // cells per axis
   int fmap[3][2];   // local face of (axis, side)
   int cls;          // the geometry class of the interior cells
   int W;            // columns per tile
   int tiles[3];     // tiles along x, y (1 in 2D) and the register axis
+  int lanes;        // lanes per angular moment in the tile's segmented sum
+  int edge_tiles;   // tiles of the boundary-cell list (OpArgs cpb cells each)
 };
 
 struct Operator {
@@ -82,6 +91,8 @@ struct Operator {
   // normals are exactly axis-aligned and whose ordinates have no zero cosine
   // (the box-tile kernel's precondition): that class
   int box_cls = -1;
+  DevBuf box_edge;            // int: the cells the box-tile kernel leaves to its list phase
+  int n_box_edge = 0;
   DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
   DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
