@@ -60,6 +60,8 @@ constexpr int kPencilMaxClasses = 64;
 // The cells on the box's boundary (and any of another class) are left to a
 // second phase of the same launch that runs the tile kernel's body on their
 // list.
+constexpr int kBoxThreads = 256;  // box-tile workgroup: W columns x A angles fill at most this
+
 struct BoxArgs {
   int n[3];         // cells per axis
   int fmap[3][2];   // local face of (axis, side)
