@@ -23,6 +23,12 @@ constexpr int kBatchBodies = 3;  // 0 tile, 1 four-angle tile, 2 box tile (unifo
 struct BatchSweep {
   int n_wg = 0;
   DevBuf prob, dir, cells, upwind, lvl, base;
+  // the forward sweep records of every item (see Sweep), built by
+  // Batch::sweep_records for the current group
+  int slots = 2;
+  int64_t n_items = 0, widest = 0;
+  mutable DevBuf rec64, rec32, rnb;
+  mutable uint64_t rec_tag = ~0ull;
 };
 
 struct Batch {
@@ -57,11 +63,18 @@ struct Batch {
   void blocks_setup(hipStream_t s);  // singular check of every active problem's blocks
   void blocks_apply(int prec, const void *x, void *y, hipStream_t s) const;
   void sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const;
+  // (re)build the sweep records when the group or any cross sections changed
+  void sweep_records(hipStream_t s) const;
+  uint64_t xs_tag() const;
   void rhs(int prec, void *b, hipStream_t s) const;
   void set_scalar_flux(int prec, const void *psi, hipStream_t s) const;
   void tune();
   void build_lists();
 };
+
+// inflow slots of the sweep record layout for nf faces and at most nfu inflow faces
+int sweep_slots(int nf, int nfu);
+void batch_records_launch(const Batch &b, hipStream_t s);
 
 // launchers (dev_operator.hip, dev_sweep.hip)
 template <class T>

@@ -334,12 +334,46 @@ void Batch::build_lists() {
     bs.push_back((int)lv.size());
   }
   sweep.n_wg = (int)wp.size();
+  sweep.n_items = (int64_t)ce.size();
+  {
+    int nfu = 1;
+    for (size_t p = 0; p < ops.size(); p++)
+      if (active[p]) nfu = std::max(nfu, sweeps[p]->nfu);
+    sweep.slots = sweep_slots(nf, nfu);
+    sweep.widest = 0;
+    for (const Dir &d : dirs) {
+      const Sweep &sp = *sweeps[d.p];
+      sweep.widest = std::max<int64_t>(sweep.widest, sp.h_lvl[sp.h_base[d.a + 1] - 1] - sp.h_lvl[sp.h_base[d.a]]);
+    }
+  }
   sweep.prob.upload(wp);
   sweep.dir.upload(wd);
   sweep.cells.upload(ce);
   sweep.upwind.upload(up);
   sweep.lvl.upload(lv);
   sweep.base.upload(bs);
+  sweep.rec_tag = ~0ull;
+  sweep_records(nullptr);
+  UBT_HIP(hipStreamSynchronize(nullptr));
+}
+
+uint64_t Batch::xs_tag() const {
+  uint64_t t = (uint64_t)group * 0x9E3779B97F4A7C15ull;
+  for (size_t p = 0; p < ops.size(); p++) t = t * 31 + ops[p]->xs_gen;
+  return t;
+}
+
+void Batch::sweep_records(hipStream_t s) const {
+  const uint64_t tag = xs_tag();
+  if (sweep.rec_tag == tag || sweep.n_items == 0) return;
+  const size_t per = (size_t)nb * nb * (1 + sweep.slots);
+  if (sweep.rec64.bytes < sizeof(double) * per * sweep.n_items) {
+    sweep.rec64.alloc(sizeof(double) * per * sweep.n_items);
+    sweep.rec32.alloc(sizeof(float) * per * sweep.n_items);
+    sweep.rnb.alloc(sizeof(int) * (size_t)sweep.slots * sweep.n_items);
+  }
+  batch_records_launch(*this, s);
+  sweep.rec_tag = tag;
 }
 
 void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {
@@ -370,6 +404,7 @@ void Batch::blocks_apply(int prec, const void *x, void *y, hipStream_t s) const 
 }
 
 void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const {
+  sweep_records(s);
   if (prec == UBT_FP64)
     batch_sweep_launch<double>(*this, (const double *)r, (double *)psi, s);
   else

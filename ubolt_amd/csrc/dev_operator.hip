@@ -2323,6 +2323,7 @@ void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s,
     op.source.release();
   }
   op.xs_set = true;
+  op.xs_gen++;
   if (first) op.tune();
 }
 

@@ -139,6 +139,277 @@ __device__ __forceinline__ void sweep_item(const T *__restrict__ r, T *__restric
   for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
 }
 
+// ---------------------------------------------------------------------------
+// Sweep records. The block of an item (cell c, direction b) and its upwind
+// coupling do not change between sweeps, only r does, so they are folded once
+// (per group and cross sections) into
+//   psi_c = B^{-1} r_c + sum_k H_k psi_{n_k},   H_k = -B^{-1} s_k F_up,k,
+// B inverted by Gauss-Jordan with partial pivoting in FP64, the records kept
+// in both precisions. A level of the sweep then costs one round trip for the
+// upwind traces plus a few multiply-adds, instead of gathering the class
+// tables, building the block and solving it on the critical path.
+// ---------------------------------------------------------------------------
+template <int DIM, int NB, int NF, int NFU>
+__device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i, int64_t n_items, int64_t c, int b,
+                                             unsigned char up, double *__restrict__ r64, float *__restrict__ r32,
+                                             int *__restrict__ rnb) {
+  constexpr int NB2 = NB * NB;
+  const double om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : 0.0};
+  const int64_t gc = g.gcls[c];
+  double B[NB][NB], H[NFU][NB][NB];
+  int nbk[NFU];
+#pragma unroll
+  for (int p = 0; p < NB; p++)
+#pragma unroll
+    for (int q = 0; q < NB; q++) B[p][q] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NFU; k++) {
+    nbk[k] = -1;
+#pragma unroll
+    for (int p = 0; p < NB; p++)
+#pragma unroll
+      for (int q = 0; q < NB; q++) H[k][p][q] = 0.0;
+  }
+  int k = 0;
+#pragma unroll
+  for (int f = 0; f < NF; f++) {
+    const int64_t kg = gc * NF + f;
+    double sf = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) sf += om[d] * g.face_s[3 * kg + d];
+    if ((up >> f) & 1) {
+      // an inflow slot: s F_up for now, scaled by -B^{-1} below
+#pragma unroll
+      for (int kk = 0; kk < NFU; kk++)
+        if (kk == k) {
+          nbk[kk] = g.face_nb[c * NF + f];
+#pragma unroll
+          for (int p = 0; p < NB; p++)
+#pragma unroll
+            for (int q = 0; q < NB; q++) H[kk][p][q] = NB == 1 ? sf : sf * g.face_up[kg * NB2 + p * NB + q];
+        }
+      k++;
+    } else if (sf > 0.0) {
+#pragma unroll
+      for (int p = 0; p < NB; p++)
+#pragma unroll
+        for (int q = 0; q < NB; q++) B[p][q] += NB == 1 ? sf : sf * g.face_own[kg * NB2 + p * NB + q];
+    }
+  }
+  if (NB > 1) {
+#pragma unroll
+    for (int p = 1; p < NB; p++) {
+      double od = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) od += om[d] * g.grad[(gc * NB + p) * 3 + d];
+      B[p][0] -= od;
+    }
+  }
+  const double sig = g.sigt[c];
+#pragma unroll
+  for (int p = 0; p < NB; p++) B[p][p] += sig;
+  // Gauss-Jordan on [B | I]
+  double Bi[NB][NB];
+#pragma unroll
+  for (int p = 0; p < NB; p++)
+#pragma unroll
+    for (int q = 0; q < NB; q++) Bi[p][q] = p == q ? 1.0 : 0.0;
+#pragma unroll
+  for (int col = 0; col < NB; col++) {
+    int piv = col;
+#pragma unroll
+    for (int p = col + 1; p < NB; p++)
+      if (fabs(B[p][col]) > fabs(B[piv][col])) piv = p;
+#pragma unroll
+    for (int p = 0; p < NB; p++)
+      if (p == piv && p != col) {
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+          double t = B[col][q];
+          B[col][q] = B[p][q];
+          B[p][q] = t;
+          t = Bi[col][q];
+          Bi[col][q] = Bi[p][q];
+          Bi[p][q] = t;
+        }
+      }
+    const double inv = 1.0 / B[col][col];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      B[col][q] *= inv;
+      Bi[col][q] *= inv;
+    }
+#pragma unroll
+    for (int p = 0; p < NB; p++) {
+      if (p == col) continue;
+      const double fct = B[p][col];
+#pragma unroll
+      for (int q = 0; q < NB; q++) {
+        B[p][q] -= fct * B[col][q];
+        Bi[p][q] -= fct * Bi[col][q];
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NB; p++)
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      r64[(int64_t)(p * NB + q) * n_items + i] = Bi[p][q];
+      r32[(int64_t)(p * NB + q) * n_items + i] = (float)Bi[p][q];
+    }
+#pragma unroll
+  for (int kk = 0; kk < NFU; kk++) {
+    rnb[(int64_t)kk * n_items + i] = nbk[kk];
+#pragma unroll
+    for (int p = 0; p < NB; p++)
+#pragma unroll
+      for (int q = 0; q < NB; q++) {
+        double h = 0.0;
+#pragma unroll
+        for (int m = 0; m < NB; m++) h -= Bi[p][m] * H[kk][m][q];
+        const int64_t e = (int64_t)(NB2 + kk * NB2 + p * NB + q) * n_items + i;
+        r64[e] = h;
+        r32[e] = (float)h;
+      }
+  }
+}
+
+// The records of every item: blockIdx.y = schedule workgroup w (a direction,
+// of problem wprob[w] in a batch), its items lvl[base[w]] .. lvl[base[w+1]-1].
+template <int DIM, int NB, int NF, int NFU, bool BATCH>
+__global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const BatchItem<double> *__restrict__ items,
+                                                     const int *__restrict__ wprob, const int *__restrict__ wdir,
+                                                     const int *__restrict__ cell, const unsigned char *__restrict__ up,
+                                                     const int64_t *__restrict__ lvl, const int *__restrict__ base,
+                                                     int64_t n_items, double *__restrict__ r64, float *__restrict__ r32,
+                                                     int *__restrict__ rnb) {
+  const int w = blockIdx.y;
+  const OpArgs<double> &g = BATCH ? items[wprob[w]].g : g1;
+  const int b = BATCH ? wdir[w] : w;
+  const int64_t i0 = lvl[base[w]], i1 = lvl[base[w + 1] - 1];
+  for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += (int64_t)gridDim.x * blockDim.x)
+    sweep_record<DIM, NB, NF, NFU>(g, i, n_items, cell[i], b, up[i], r64, r32, rnb);
+}
+
+// One direction's levels l0 .. l1-1 from the records, a = the angle of the
+// rows. With PIPE, a thread's item of the next level (its records, r, the
+// upwind cells) is loaded while the current level's upwind traces are in
+// flight, and its cell id two levels ahead, so the barrier-to-barrier path of
+// a level is the trace load, the multiply-adds and the store.
+template <class T, int NB, int NFU, bool PIPE>
+__device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A, int a,
+                                              const int *__restrict__ cell, const T *__restrict__ rec,
+                                              const int *__restrict__ rnb, int64_t n_items,
+                                              const int64_t *__restrict__ lvl, int l0, int l1) {
+  constexpr int NB2 = NB * NB;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  struct Item {
+    int64_t c;
+    int nbk[NFU];
+    T bi[NB2], h[NFU][NB2], rc[NB];
+  };
+  auto fetch = [&](int64_t i, int64_t c, Item &it) {
+    it.c = c;
+#pragma unroll
+    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[(int64_t)k * n_items + i];
+#pragma unroll
+    for (int e = 0; e < NB2; e++) it.bi[e] = rec[(int64_t)e * n_items + i];
+#pragma unroll
+    for (int k = 0; k < NFU; k++)
+#pragma unroll
+      for (int e = 0; e < NB2; e++) it.h[k][e] = rec[(int64_t)(NB2 + k * NB2 + e) * n_items + i];
+#pragma unroll
+    for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a];
+  };
+  auto solve = [&](const Item &it) {
+    T pu[NFU][NB];
+#pragma unroll
+    for (int k = 0; k < NFU; k++)
+#pragma unroll
+      for (int j = 0; j < NB; j++) pu[k][j] = it.nbk[k] >= 0 ? psi[((int64_t)it.nbk[k] * NB + j) * A + a] : T(0);
+#pragma unroll
+    for (int p = 0; p < NB; p++) {
+      T v = T(0);
+#pragma unroll
+      for (int q = 0; q < NB; q++) v += it.bi[p * NB + q] * it.rc[q];
+#pragma unroll
+      for (int k = 0; k < NFU; k++)
+#pragma unroll
+        for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * pu[k][q];
+      psi[(it.c * NB + p) * A + a] = v;
+    }
+  };
+  if (!PIPE) {
+    for (int l = l0; l < l1; l++) {
+      for (int64_t i = lvl[l] + tid; i < lvl[l + 1]; i += nt) {
+        Item it;
+        fetch(i, cell[i], it);
+        solve(it);
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  // pipelined: cur (this level), c2 (the cell of the item two levels ahead)
+  Item cur;
+  bool has = false;
+  if (l0 < l1) {
+    const int64_t i = lvl[l0] + tid;
+    has = i < lvl[l0 + 1];
+    if (has) fetch(i, cell[i], cur);
+  }
+  int64_t c2 = 0;
+  bool has2 = false;
+  if (l0 + 1 < l1) {
+    const int64_t i = lvl[l0 + 1] + tid;
+    has2 = i < lvl[l0 + 2];
+    if (has2) c2 = cell[i];
+  }
+  for (int l = l0; l < l1; l++) {
+    Item nxt;
+    const bool hn = has2;
+    if (hn) fetch(lvl[l + 1] + tid, c2, nxt);
+    has2 = false;
+    if (l + 2 < l1) {
+      const int64_t i = lvl[l + 2] + tid;
+      has2 = i < lvl[l + 3];
+      if (has2) c2 = cell[i];
+    }
+    if (has) solve(cur);
+    // items beyond one per thread (a level wider than the workgroup)
+    for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
+      Item it;
+      fetch(i, cell[i], it);
+      solve(it);
+    }
+    __syncthreads();
+    if (hn) cur = nxt;
+    has = hn;
+  }
+}
+
+template <class T, int NB, int NFU, bool PIPE>
+__global__ void __launch_bounds__(256) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
+                                                     const int *__restrict__ cell, const T *__restrict__ rec,
+                                                     const int *__restrict__ rnb, int64_t n_items,
+                                                     const int64_t *__restrict__ lvl, const int *__restrict__ base) {
+  const int b = blockIdx.x;
+  sweep_rec_run<T, NB, NFU, PIPE>(r, psi, A, b, cell, rec, rnb, n_items, lvl, base[b], base[b + 1] - 1);
+}
+
+template <class T, int NB, int NFU, bool PIPE>
+__global__ void __launch_bounds__(256) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
+                                                          const BatchItem<T> *__restrict__ items,
+                                                          const int *__restrict__ wprob, const int *__restrict__ wdir,
+                                                          const int *__restrict__ cell, const T *__restrict__ rec,
+                                                          const int *__restrict__ rnb, int64_t n_items,
+                                                          const int64_t *__restrict__ lvl, const int *__restrict__ base) {
+  const int w = blockIdx.x;
+  const BatchItem<T> &bi = items[wprob[w]];
+  sweep_rec_run<T, NB, NFU, PIPE>(r + bi.off, psi + bi.off, bi.g.A, wdir[w], cell, rec, rnb, n_items, lvl, base[w],
+                                  base[w + 1] - 1);
+}
+
 // Persistent per-direction sweep: the sweep of one direction never reads
 // another direction's traces, so a workgroup owns one direction and walks all
 // of its levels in one launch, a workgroup barrier between levels (the items
@@ -343,6 +614,8 @@ void Sweep::create(Operator *o, bool keep_host) {
   }
   abase[A] = (int)alvl.size();
   n_levels = max_level;
+  nfu = 1;
+  for (unsigned char m : item_up) nfu = std::max(nfu, __builtin_popcount(m));
   threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest + 63) / 64 * 64));
   cells.upload(item_cell);
   upwind.upload(item_up);
@@ -354,6 +627,58 @@ void Sweep::create(Operator *o, bool keep_host) {
     h_lvl.swap(alvl);
     h_base.swap(abase);
   }
+}
+
+// inflow slots of the record layout: the smallest instantiated count that
+// holds the most inflow faces of any item
+int sweep_slots(int nf, int nfu) {
+  if (nf == 3) return 2;
+  if (nf == 4) return nfu <= 2 ? 2 : 3;
+  return nfu <= 3 ? 3 : 5;
+}
+
+// the records of every item of `g`'s (or the batch's) schedule
+static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<double> &g1,
+                           const BatchItem<double> *items, const int *wprob, const int *wdir, const int *cell,
+                           const unsigned char *up, const int64_t *lvl, const int *base, int n_wg, int64_t widest_dir,
+                           int64_t n_items, double *r64, float *r32, int *rnb, bool batch, hipStream_t s) {
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (widest_dir + 255) / 256));
+  const dim3 grid(gx, (unsigned)n_wg);
+  bool launched = false;
+#define RC(DIM, NB, NF, NFU)                                                                                         \
+  if (!launched && dim == DIM && nb == NB && nf == NF && slots == NFU) {                                             \
+    if (batch)                                                                                                       \
+      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, true>), grid, dim3(256), 0, s, g1, items, wprob, wdir,      \
+                         cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
+    else                                                                                                             \
+      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, false>), grid, dim3(256), 0, s, g1, items, wprob, wdir,     \
+                         cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
+    launched = true;                                                                                                 \
+  }
+  RC(2, 1, 3, 2) RC(2, 1, 4, 2) RC(2, 1, 4, 3) RC(3, 1, 4, 2) RC(3, 1, 4, 3) RC(3, 1, 6, 3) RC(3, 1, 6, 5)
+  RC(2, 3, 3, 2) RC(2, 3, 4, 2) RC(2, 3, 4, 3) RC(3, 4, 4, 2) RC(3, 4, 4, 3) RC(3, 4, 6, 3) RC(3, 4, 6, 5)
+#undef RC
+  UBT_CHECK(launched, UBT_ERR_SUP, "no sweep records for dim %d, %d basis, %d faces, %d slots", dim, nb, nf, slots);
+  UBT_HIP(hipGetLastError());
+}
+
+void Sweep::build_records(hipStream_t s) const {
+  const Operator &o = *op;
+  const Disc &D = *o.disc;
+  const int slots = sweep_slots(D.nf, nfu);
+  const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
+  const size_t per = (size_t)D.nb * D.nb * (1 + slots);
+  if (!rec64.ptr) {
+    rec64.alloc(sizeof(double) * per * n_items);
+    rec32.alloc(sizeof(float) * per * n_items);
+    rnb.alloc(sizeof(int) * (size_t)slots * n_items);
+  }
+  launch_records(D.dim, D.nb, D.nf, slots, o.args<double>(), nullptr, nullptr, nullptr, cells.as<int>(),
+                 upwind.as<unsigned char>(), lvl.as<int64_t>(), base.as<int>(), D.n_angles,
+                 (n_items + D.n_angles - 1) / D.n_angles, n_items, rec64.as<double>(), rec32.as<float>(),
+                 rnb.as<int>(), false, s);
+  rec_group = o.group;
+  rec_gen = o.xs_gen;
 }
 
 void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) const {
@@ -395,6 +720,37 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   const int64_t *lv = lvl.as<int64_t>();
   const unsigned grid = (unsigned)D.n_angles;
   bool launched = false;
+  if (!adj) {
+    if (rec_group != o.group || rec_gen != o.xs_gen) build_records(s);
+    const int slots = sweep_slots(D.nf, nfu);
+    const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
+    // a workgroup of at most 256 (wider levels loop); the pipelined form
+    // holds two items' records in registers, so only where they fit
+    const int th = std::min(threads, 256);
+    const int item_regs = (D.nb * D.nb * (1 + slots) + D.nb) * (prec == UBT_FP64 ? 2 : 1) + slots + 2;
+    const bool pipe = 2 * item_regs <= 200;
+#define SR(T, NB, NFU)                                                                                              \
+  if (!launched && D.nb == NB && slots == NFU) {                                                                    \
+    const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \
+    if (pipe)                                                                                                       \
+      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, true>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,       \
+                         D.n_angles, ci, rc, rnb.as<int>(), n_items, lv, bs);                                       \
+    else                                                                                                            \
+      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,      \
+                         D.n_angles, ci, rc, rnb.as<int>(), n_items, lv, bs);                                       \
+    launched = true;                                                                                                \
+  }
+    if (prec == UBT_FP64) {
+      SR(double, 1, 2) SR(double, 1, 3) SR(double, 1, 5) SR(double, 3, 2) SR(double, 3, 3) SR(double, 4, 3)
+      SR(double, 4, 5)
+    } else {
+      SR(float, 1, 2) SR(float, 1, 3) SR(float, 1, 5) SR(float, 3, 2) SR(float, 3, 3) SR(float, 4, 3) SR(float, 4, 5)
+    }
+#undef SR
+    UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for %d basis functions and %d inflow slots", D.nb, slots);
+    UBT_HIP(hipGetLastError());
+    return;
+  }
 #define SW(T, DIM, NB, NF)                                                                                         \
   if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                     \
     if (adj)                                                                                                       \
@@ -417,12 +773,34 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   UBT_HIP(hipGetLastError());
 }
 
+void batch_records_launch(const Batch &b, hipStream_t s) {
+  const BatchSweep &w = b.sweep;
+  launch_records(b.dim, b.nb, b.nf, w.slots, OpArgs<double>{}, b.items[0][0].as<BatchItem<double>>(),
+                 w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),
+                 w.lvl.as<int64_t>(), w.base.as<int>(), w.n_wg, w.widest, w.n_items, w.rec64.as<double>(),
+                 w.rec32.as<float>(), w.rnb.as<int>(), true, s);
+}
+
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   const BatchSweep &w = b.sweep;
   if (w.n_wg == 0) return;
   const BatchItem<T> *items = b.items[pi][0].as<BatchItem<T>>();
   bool launched = false;
+  const T *rec = sizeof(T) == 8 ? (const T *)w.rec64.ptr : (const T *)w.rec32.ptr;
+#define SRB(NB, NFU)                                                                                                \
+  if (!launched && b.nb == NB && w.slots == NFU) {                                                                  \
+    hipLaunchKernelGGL((sweep_dg_rec_batch<T, NB, NFU, false>), dim3(w.n_wg), dim3(256), 0, s, r, psi, items,        \
+                       w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), rec, w.rnb.as<int>(), w.n_items,        \
+                       w.lvl.as<int64_t>(), w.base.as<int>());                                                      \
+    launched = true;                                                                                                \
+  }
+  SRB(1, 2) SRB(1, 3) SRB(1, 5) SRB(3, 2) SRB(3, 3) SRB(4, 3) SRB(4, 5)
+#undef SRB
+  if (launched) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
 #define SWB(DIM, NB, NF)                                                                                            \
   if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF) {                                                      \
     hipLaunchKernelGGL((sweep_dg_dir_batch<T, DIM, NB, NF>), dim3(w.n_wg), dim3(256), 0, s, r, psi, items,          \

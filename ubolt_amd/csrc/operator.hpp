@@ -80,6 +80,7 @@ struct Operator {
   int variant[2] = {0, 0};
   float tuned_us[2][kApplyVariants] = {};
   bool xs_set = false;
+  uint64_t xs_gen = 0;        // bumped whenever the cross sections change
   PrecTables t64, t32;
   DevBuf reflect, face_nb, neg;
   DevBuf gcls;                // int per cell: geometry class (DG)

@@ -17,6 +17,17 @@ struct Sweep {
   std::vector<int> h_cells, h_base;  // host copies (kept on request, for batching)
   std::vector<unsigned char> h_up;
   std::vector<int64_t> h_lvl;
+  // DG forward sweep records, per item in schedule order (structure of
+  // arrays): the block inverse B^{-1} (nb x nb) and, per inflow slot, the
+  // coupling -B^{-1} s_f F_up,f and the upwind cell (-1: unused slot), so the
+  // sweep of an item is psi = B^{-1} r + sum_k H_k psi_up,k. Built in FP64 for
+  // the operator's current group and cross sections, stored in both
+  // precisions.
+  int nfu = 0;                      // inflow slots per item: the most inflow faces of any item
+  mutable DevBuf rec64, rec32, rnb;
+  mutable int rec_group = -1;
+  mutable uint64_t rec_gen = 0;
+  void build_records(hipStream_t s) const;
   void create(Operator *o, bool keep_host = false);
   // psi = L^{-1} r, or (adj) psi = L^{-T} r
   void apply(int prec, const void *r, void *psi, hipStream_t s, bool adj = false) const;
