@@ -19,17 +19,20 @@ template <class T> struct BatchItem {
 // apply bodies the batch autotuner chooses from
 constexpr int kBatchBodies = 3;  // 0 tile, 1 four-angle tile, 2 box tile (uniform DG1 boxes)
 
-// the directions of every active problem, one workgroup each
+// the direction groups of every active problem, one workgroup each
 struct BatchSweep {
-  int n_wg = 0;
-  DevBuf prob, dir, cells, upwind, lvl, base;
-  // the forward sweep records of every item (see Sweep), built by
-  // Batch::sweep_records for the current group
+  int n_wg = 0, threads = 256;
+  DevBuf prob, angle, cells, upwind, lvl, base;
+  // the forward sweep records of every item in this schedule's precision
+  // (see Sweep), built by Batch::sweep_records for the current group
   int slots = 2;
   int64_t n_items = 0, widest = 0;
-  mutable DevBuf rec64, rec32, rnb;
+  mutable DevBuf rec, rnb;
   mutable uint64_t rec_tag = ~0ull;
 };
+// directions of one octant per batched sweep workgroup, per precision
+// [FP64, FP32] (measured on the 64-problem ensemble: 2 and 4)
+constexpr int kBatchSweepGroup[2] = {2, 4};
 
 struct Batch {
   std::vector<Operator *> ops;
@@ -55,7 +58,7 @@ struct Batch {
   // block inverse and sweep work lists
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems
   int64_t n_blocks = 0;
-  BatchSweep sweep;
+  BatchSweep sweep[2];         // per precision: its own grouping and records
 
   void create(const std::vector<Operator *> &o);
   void set_group(int g);       // problems without group g go inactive
@@ -74,7 +77,7 @@ struct Batch {
 
 // inflow slots of the sweep record layout for nf faces and at most nfu inflow faces
 int sweep_slots(int nf, int nfu);
-void batch_records_launch(const Batch &b, hipStream_t s);
+void batch_records_launch(const Batch &b, int pi, hipStream_t s);
 
 // launchers (dev_operator.hip, dev_sweep.hip)
 template <class T>

@@ -122,9 +122,10 @@ template <class T> __global__ void k_fill(int64_t n, T v, T *__restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
 }
 
+// y = a x + b y; with b = 0, y is not read (it may hold anything, NaN included)
 template <class T> __global__ void k_axpby_b(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+    y[i] = (T)(b == 0.0 ? a * (double)x[i] : a * (double)x[i] + b * (double)y[i]);
 }
 
 // cycle start: g = beta e_0, done where beta <= tol (or the problem is idle)
@@ -301,58 +302,62 @@ void Batch::build_lists() {
     if (active[p]) blk.push_back(blk.back() + (int64_t)ops[p]->disc->n_cells * ops[p]->disc->n_angles);
   bblk.upload(blk);
   n_blocks = blk.back();
-  // sweep: one workgroup per direction of every active problem, the
-  // problems' per-direction schedules concatenated; the workgroups are listed
-  // longest first (most levels), so the directions that bound the sweep are
-  // dispatched before the short ones
-  struct Dir {
-    int q, a, nlev;
-    size_t p;
-  };
-  std::vector<Dir> dirs;
-  std::vector<int64_t> item0(ops.size(), 0);
-  std::vector<int> ce;
-  std::vector<unsigned char> up;
-  int q = 0;
-  for (size_t p = 0; p < ops.size(); p++) {
-    if (!active[p]) continue;
-    const Sweep &s = *sweeps[p];
-    item0[p] = (int64_t)ce.size();
-    ce.insert(ce.end(), s.h_cells.begin(), s.h_cells.end());
-    up.insert(up.end(), s.h_up.begin(), s.h_up.end());
-    for (int a = 0; a < ops[p]->disc->n_angles; a++) dirs.push_back({q, a, s.h_base[a + 1] - s.h_base[a] - 1, p});
-    q++;
-  }
-  std::stable_sort(dirs.begin(), dirs.end(), [](const Dir &x, const Dir &y) { return x.nlev > y.nlev; });
-  std::vector<int> wp, wd, bs(1, 0);
-  std::vector<int64_t> lv;
-  for (const Dir &d : dirs) {
-    const Sweep &s = *sweeps[d.p];
-    wp.push_back(d.q);
-    wd.push_back(d.a);
-    for (int k = s.h_base[d.a]; k < s.h_base[d.a + 1]; k++) lv.push_back(item0[d.p] + s.h_lvl[k]);
-    bs.push_back((int)lv.size());
-  }
-  sweep.n_wg = (int)wp.size();
-  sweep.n_items = (int64_t)ce.size();
-  {
-    int nfu = 1;
-    for (size_t p = 0; p < ops.size(); p++)
-      if (active[p]) nfu = std::max(nfu, sweeps[p]->nfu);
-    sweep.slots = sweep_slots(nf, nfu);
-    sweep.widest = 0;
-    for (const Dir &d : dirs) {
-      const Sweep &sp = *sweeps[d.p];
-      sweep.widest = std::max<int64_t>(sweep.widest, sp.h_lvl[sp.h_base[d.a + 1] - 1] - sp.h_lvl[sp.h_base[d.a]]);
+  // sweep, per precision: one workgroup per direction group of every active
+  // problem (kBatchSweepGroup directions of one octant), the problems'
+  // schedules concatenated; the workgroups are listed longest first (most
+  // levels), so the groups that bound the sweep are dispatched before the
+  // short ones
+  int nfu = 1;
+  for (size_t p = 0; p < ops.size(); p++)
+    if (active[p]) nfu = std::max(nfu, sweeps[p]->nfu);
+  for (int pi = 0; pi < 2; pi++) {
+    BatchSweep &w = sweep[pi];
+    struct Grp {
+      int q, g, nlev;
+      size_t p;
+    };
+    std::vector<Grp> grps;
+    std::vector<SweepLists> lists(ops.size());
+    std::vector<int64_t> item0(ops.size(), 0);
+    std::vector<int> ce, an;
+    std::vector<unsigned char> up;
+    int q = 0;
+    int64_t widest_level = 1;
+    for (size_t p = 0; p < ops.size(); p++) {
+      if (!active[p]) continue;
+      SweepLists &L = lists[p];
+      group_schedule(*sweeps[p], kBatchSweepGroup[pi], L);
+      item0[p] = (int64_t)ce.size();
+      ce.insert(ce.end(), L.cell.begin(), L.cell.end());
+      an.insert(an.end(), L.angle.begin(), L.angle.end());
+      up.insert(up.end(), L.up.begin(), L.up.end());
+      widest_level = std::max(widest_level, L.widest);
+      for (int g = 0; g + 1 < (int)L.base.size(); g++) grps.push_back({q, g, L.base[g + 1] - L.base[g] - 1, p});
+      q++;
     }
+    std::stable_sort(grps.begin(), grps.end(), [](const Grp &x, const Grp &y) { return x.nlev > y.nlev; });
+    std::vector<int> wp, bs(1, 0);
+    std::vector<int64_t> lv;
+    w.widest = 0;
+    for (const Grp &d : grps) {
+      const SweepLists &L = lists[d.p];
+      wp.push_back(d.q);
+      for (int k = L.base[d.g]; k < L.base[d.g + 1]; k++) lv.push_back(item0[d.p] + L.lvl[k]);
+      bs.push_back((int)lv.size());
+      w.widest = std::max<int64_t>(w.widest, L.lvl[L.base[d.g + 1] - 1] - L.lvl[L.base[d.g]]);
+    }
+    w.n_wg = (int)wp.size();
+    w.n_items = (int64_t)ce.size();
+    w.threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest_level + 63) / 64 * 64));
+    w.slots = sweep_slots(nf, nfu);
+    w.prob.upload(wp);
+    w.angle.upload(an);
+    w.cells.upload(ce);
+    w.upwind.upload(up);
+    w.lvl.upload(lv);
+    w.base.upload(bs);
+    w.rec_tag = ~0ull;
   }
-  sweep.prob.upload(wp);
-  sweep.dir.upload(wd);
-  sweep.cells.upload(ce);
-  sweep.upwind.upload(up);
-  sweep.lvl.upload(lv);
-  sweep.base.upload(bs);
-  sweep.rec_tag = ~0ull;
   sweep_records(nullptr);
   UBT_HIP(hipStreamSynchronize(nullptr));
 }
@@ -365,15 +370,20 @@ uint64_t Batch::xs_tag() const {
 
 void Batch::sweep_records(hipStream_t s) const {
   const uint64_t tag = xs_tag();
-  if (sweep.rec_tag == tag || sweep.n_items == 0) return;
-  const size_t per = (size_t)nb * nb * (1 + sweep.slots);
-  if (sweep.rec64.bytes < sizeof(double) * per * sweep.n_items) {
-    sweep.rec64.alloc(sizeof(double) * per * sweep.n_items);
-    sweep.rec32.alloc(sizeof(float) * per * sweep.n_items);
-    sweep.rnb.alloc(sizeof(int) * (size_t)sweep.slots * sweep.n_items);
+  for (int pi = 0; pi < 2; pi++) {
+    const BatchSweep &w = sweep[pi];
+    if (w.rec_tag == tag || w.n_items == 0) continue;
+    // 64-item blocks (see the sweep's record layout)
+    const size_t blocks64 = (size_t)((w.n_items + 63) / 64) * 64;
+    const size_t per = (size_t)nb * nb * (1 + w.slots);
+    const size_t ts = pi == 0 ? sizeof(double) : sizeof(float);
+    if (w.rec.bytes < ts * per * blocks64) {
+      w.rec.alloc(ts * per * blocks64);
+      w.rnb.alloc(sizeof(int) * (size_t)w.slots * blocks64);
+    }
+    batch_records_launch(*this, pi, s);
+    w.rec_tag = tag;
   }
-  batch_records_launch(*this, s);
-  sweep.rec_tag = tag;
 }
 
 void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {

@@ -31,9 +31,10 @@ constexpr int kMaxK = 32;  // restart + 1 at most
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 1024;
 
+// y = a x + b y; with b = 0, y is not read (it may hold anything, NaN included)
 template <class T> __global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+    y[i] = (T)(b == 0.0 ? a * (double)x[i] : a * (double)x[i] + b * (double)y[i]);
 }
 
 template <class Ti, class To> __global__ void k_convert(int64_t n, double a, const Ti *__restrict__ x, To *__restrict__ y) {

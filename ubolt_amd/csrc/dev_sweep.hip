@@ -149,6 +149,14 @@ __device__ __forceinline__ void sweep_item(const T *__restrict__ r, T *__restric
 // upwind traces plus a few multiply-adds, instead of gathering the class
 // tables, building the block and solving it on the critical path.
 // ---------------------------------------------------------------------------
+// Record layout: blocks of 64 items, each block field-major (field e of item
+// i at ((i / 64) F + e) 64 + i % 64, F fields per item), so a wavefront's
+// load of one field is 64 consecutive values and a level's records span a
+// few contiguous blocks - not F streams a whole array apart, which across
+// thousands of concurrent directions exhausts the translation caches.
+__host__ __device__ inline int64_t rix(int e, int64_t i, int F) { return ((i >> 6) * F + e) * 64 + (i & 63); }
+__host__ __device__ inline int64_t rec_len(int64_t n_items, int F) { return ((n_items + 63) >> 6) * 64 * F; }
+
 template <int DIM, int NB, int NF, int NFU>
 __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i, int64_t n_items, int64_t c, int b,
                                              unsigned char up, double *__restrict__ r64, float *__restrict__ r32,
@@ -254,12 +262,12 @@ __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i,
   for (int p = 0; p < NB; p++)
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-      r64[(int64_t)(p * NB + q) * n_items + i] = Bi[p][q];
-      r32[(int64_t)(p * NB + q) * n_items + i] = (float)Bi[p][q];
+      if (r64) r64[rix(p * NB + q, i, NB2 * (1 + NFU))] = Bi[p][q];
+      if (r32) r32[rix(p * NB + q, i, NB2 * (1 + NFU))] = (float)Bi[p][q];
     }
 #pragma unroll
   for (int kk = 0; kk < NFU; kk++) {
-    rnb[(int64_t)kk * n_items + i] = nbk[kk];
+    rnb[rix(kk, i, NFU)] = nbk[kk];
 #pragma unroll
     for (int p = 0; p < NB; p++)
 #pragma unroll
@@ -267,9 +275,9 @@ __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i,
         double h = 0.0;
 #pragma unroll
         for (int m = 0; m < NB; m++) h -= Bi[p][m] * H[kk][m][q];
-        const int64_t e = (int64_t)(NB2 + kk * NB2 + p * NB + q) * n_items + i;
-        r64[e] = h;
-        r32[e] = (float)h;
+        const int64_t e = rix(NB2 + kk * NB2 + p * NB + q, i, NB2 * (1 + NFU));
+        if (r64) r64[e] = h;
+        if (r32) r32[e] = (float)h;
       }
   }
 }
@@ -278,50 +286,53 @@ __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i,
 // of problem wprob[w] in a batch), its items lvl[base[w]] .. lvl[base[w+1]-1].
 template <int DIM, int NB, int NF, int NFU, bool BATCH>
 __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const BatchItem<double> *__restrict__ items,
-                                                     const int *__restrict__ wprob, const int *__restrict__ wdir,
+                                                     const int *__restrict__ wprob, const int *__restrict__ ang,
                                                      const int *__restrict__ cell, const unsigned char *__restrict__ up,
                                                      const int64_t *__restrict__ lvl, const int *__restrict__ base,
                                                      int64_t n_items, double *__restrict__ r64, float *__restrict__ r32,
                                                      int *__restrict__ rnb) {
   const int w = blockIdx.y;
   const OpArgs<double> &g = BATCH ? items[wprob[w]].g : g1;
-  const int b = BATCH ? wdir[w] : w;
   const int64_t i0 = lvl[base[w]], i1 = lvl[base[w + 1] - 1];
   for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += (int64_t)gridDim.x * blockDim.x)
-    sweep_record<DIM, NB, NF, NFU>(g, i, n_items, cell[i], b, up[i], r64, r32, rnb);
+    sweep_record<DIM, NB, NF, NFU>(g, i, n_items, cell[i], ang[i], up[i], r64, r32, rnb);
 }
 
-// One direction's levels l0 .. l1-1 from the records, a = the angle of the
-// rows. With PIPE, a thread's item of the next level (its records, r, the
-// upwind cells) is loaded while the current level's upwind traces are in
-// flight, and its cell id two levels ahead, so the barrier-to-barrier path of
-// a level is the trace load, the multiply-adds and the store.
+// One group's levels l0 .. l1-1 from the records (item i: cell cell[i],
+// angle ang[i]). With PIPE, a thread's item of the next level (its records,
+// r, the upwind cells) is loaded while the current level's upwind traces are
+// in flight, and its cell id two levels ahead, so the barrier-to-barrier path
+// of a level is the trace load, the multiply-adds and the store.
 template <class T, int NB, int NFU, bool PIPE>
-__device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A, int a,
-                                              const int *__restrict__ cell, const T *__restrict__ rec,
-                                              const int *__restrict__ rnb, int64_t n_items,
+__device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A,
+                                              const int *__restrict__ cell, const int *__restrict__ ang,
+                                              const T *__restrict__ rec, const int *__restrict__ rnb, int64_t n_items,
                                               const int64_t *__restrict__ lvl, int l0, int l1) {
   constexpr int NB2 = NB * NB;
   const int tid = threadIdx.x, nt = blockDim.x;
   struct Item {
     int64_t c;
+    int a;
     int nbk[NFU];
     T bi[NB2], h[NFU][NB2], rc[NB];
   };
   auto fetch = [&](int64_t i, int64_t c, Item &it) {
     it.c = c;
+    it.a = ang[i];
+    const int a = it.a;
 #pragma unroll
-    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[(int64_t)k * n_items + i];
+    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
 #pragma unroll
-    for (int e = 0; e < NB2; e++) it.bi[e] = rec[(int64_t)e * n_items + i];
+    for (int e = 0; e < NB2; e++) it.bi[e] = rec[rix(e, i, NB2 * (1 + NFU))];
 #pragma unroll
     for (int k = 0; k < NFU; k++)
 #pragma unroll
-      for (int e = 0; e < NB2; e++) it.h[k][e] = rec[(int64_t)(NB2 + k * NB2 + e) * n_items + i];
+      for (int e = 0; e < NB2; e++) it.h[k][e] = rec[rix(NB2 + k * NB2 + e, i, NB2 * (1 + NFU))];
 #pragma unroll
     for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a];
   };
   auto solve = [&](const Item &it) {
+    const int a = it.a;
     T pu[NFU][NB];
 #pragma unroll
     for (int k = 0; k < NFU; k++)
@@ -388,26 +399,29 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
   }
 }
 
+// PIPE holds two items' records in registers: at most 256 threads; the plain
+// form serves the wide levels of merged octants with up to 1024
 template <class T, int NB, int NFU, bool PIPE>
-__global__ void __launch_bounds__(256) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
-                                                     const int *__restrict__ cell, const T *__restrict__ rec,
-                                                     const int *__restrict__ rnb, int64_t n_items,
-                                                     const int64_t *__restrict__ lvl, const int *__restrict__ base) {
-  const int b = blockIdx.x;
-  sweep_rec_run<T, NB, NFU, PIPE>(r, psi, A, b, cell, rec, rnb, n_items, lvl, base[b], base[b + 1] - 1);
+__global__ void __launch_bounds__(PIPE ? 256 : 1024) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
+                                                                  const int *__restrict__ cell, const int *__restrict__ ang,
+                                                                  const T *__restrict__ rec, const int *__restrict__ rnb,
+                                                                  int64_t n_items, const int64_t *__restrict__ lvl,
+                                                                  const int *__restrict__ base) {
+  const int w = blockIdx.x;
+  sweep_rec_run<T, NB, NFU, PIPE>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1);
 }
 
-template <class T, int NB, int NFU, bool PIPE>
-__global__ void __launch_bounds__(256) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
-                                                          const BatchItem<T> *__restrict__ items,
-                                                          const int *__restrict__ wprob, const int *__restrict__ wdir,
-                                                          const int *__restrict__ cell, const T *__restrict__ rec,
-                                                          const int *__restrict__ rnb, int64_t n_items,
-                                                          const int64_t *__restrict__ lvl, const int *__restrict__ base) {
+template <class T, int NB, int NFU>
+__global__ void __launch_bounds__(1024) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
+                                                           const BatchItem<T> *__restrict__ items,
+                                                           const int *__restrict__ wprob, const int *__restrict__ cell,
+                                                           const int *__restrict__ ang, const T *__restrict__ rec,
+                                                           const int *__restrict__ rnb, int64_t n_items,
+                                                           const int64_t *__restrict__ lvl, const int *__restrict__ base) {
   const int w = blockIdx.x;
   const BatchItem<T> &bi = items[wprob[w]];
-  sweep_rec_run<T, NB, NFU, PIPE>(r + bi.off, psi + bi.off, bi.g.A, wdir[w], cell, rec, rnb, n_items, lvl, base[w],
-                                  base[w + 1] - 1);
+  sweep_rec_run<T, NB, NFU, false>(r + bi.off, psi + bi.off, bi.g.A, cell, ang, rec, rnb, n_items, lvl, base[w],
+                                   base[w + 1] - 1);
 }
 
 // Persistent per-direction sweep: the sweep of one direction never reads
@@ -419,32 +433,14 @@ __global__ void __launch_bounds__(256) sweep_dg_rec_batch(const T *__restrict__ 
 // sweep drops from one per level to one.
 template <class T, int DIM, int NB, int NF, bool ADJ>
 __global__ void __launch_bounds__(1024) sweep_dg_dir(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g,
-                                                     const int *__restrict__ cell, const unsigned char *__restrict__ up,
+                                                     const int *__restrict__ cell, const int *__restrict__ ang,
+                                                     const unsigned char *__restrict__ up,
                                                      const int64_t *__restrict__ lvl, const int *__restrict__ base) {
-  const int b = blockIdx.x;
-  const int l0 = base[b], l1 = base[b + 1] - 1;  // levels of direction b: lvl[l0 .. l1]
-  for (int l = l0; l < l1; l++) {
-    for (int64_t i = lvl[l] + threadIdx.x; i < lvl[l + 1]; i += blockDim.x)
-      sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[i], b, up[i]);
-    __syncthreads();
-  }
-}
-
-// the same over the directions of many problems: workgroup w owns direction
-// wdir[w] of the problem at work-list position wprob[w]
-template <class T, int DIM, int NB, int NF>
-__global__ void __launch_bounds__(256) sweep_dg_dir_batch(const T *__restrict__ r, T *__restrict__ psi,
-                                                          const BatchItem<T> *__restrict__ items,
-                                                          const int *__restrict__ wprob, const int *__restrict__ wdir,
-                                                          const int *__restrict__ cell, const unsigned char *__restrict__ up,
-                                                          const int64_t *__restrict__ lvl, const int *__restrict__ base) {
   const int w = blockIdx.x;
-  const BatchItem<T> &bi = items[wprob[w]];
-  const int b = wdir[w];
-  const int l0 = base[w], l1 = base[w + 1] - 1;
+  const int l0 = base[w], l1 = base[w + 1] - 1;  // levels of group w: lvl[l0 .. l1]
   for (int l = l0; l < l1; l++) {
     for (int64_t i = lvl[l] + threadIdx.x; i < lvl[l + 1]; i += blockDim.x)
-      sweep_item<T, DIM, NB, NF, false>(r + bi.off, psi + bi.off, bi.g, cell[i], b, up[i]);
+      sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[i], ang[i], up[i]);
     __syncthreads();
   }
 }
@@ -595,8 +591,8 @@ void Sweep::create(Operator *o, bool keep_host) {
     }
     max_level = std::max(max_level, lvl);
   }
-  // per direction: the items in generation order (direction, then level),
-  // and each direction's level boundaries
+  // per direction (one workgroup each): the items in generation order
+  // (direction, then level), and each direction's level boundaries
   const int64_t n_items = (int64_t)item_cell.size();
   std::vector<int64_t> alvl;
   std::vector<int> abase(A + 1, 0);
@@ -613,19 +609,73 @@ void Sweep::create(Operator *o, bool keep_host) {
     alvl.push_back(i);
   }
   abase[A] = (int)alvl.size();
+  n_groups = A;
   n_levels = max_level;
   nfu = 1;
   for (unsigned char m : item_up) nfu = std::max(nfu, __builtin_popcount(m));
   threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest + 63) / 64 * 64));
   cells.upload(item_cell);
+  angle.upload(item_angle);
   upwind.upload(item_up);
   lvl.upload(alvl);
   base.upload(abase);
   if (keep_host) {
     h_cells.swap(item_cell);
+    h_angle.swap(item_angle);
     h_up.swap(item_up);
     h_lvl.swap(alvl);
     h_base.swap(abase);
+  }
+}
+
+// Groups of up to G directions of one octant of a box, one workgroup each: a
+// group's level l is the union of its directions' levels l (a direction
+// depends only on its own earlier levels, so any grouping is a valid order).
+// On a box every direction of an octant has the same level structure, so a
+// level's items, sorted by (cell, angle), put the angles of a cell in adjacent
+// lanes and the rows they read and write are contiguous; fewer, wider
+// workgroups trade concurrency for that. Elsewhere (or G = 1) a group is one
+// direction. From the per-direction host schedule of s.
+void group_schedule(const Sweep &s, int G, SweepLists &out) {
+  const Disc &D = *s.op->disc;
+  const int A = D.n_angles;
+  std::vector<std::vector<int>> groups;
+  if (s.op->box && G > 1) {
+    std::vector<std::vector<int>> by_oct(8);
+    for (int a = 0; a < A; a++) {
+      int oc = 0;
+      for (int d = 0; d < D.dim; d++) oc |= (D.quad.omega[3 * (size_t)a + d] > 0.0 ? 1 : 0) << d;
+      by_oct[oc].push_back(a);
+    }
+    for (auto &g : by_oct)
+      for (size_t k = 0; k < g.size(); k += G)
+        groups.push_back(std::vector<int>(g.begin() + k, g.begin() + std::min(g.size(), k + (size_t)G)));
+  } else {
+    for (int a = 0; a < A; a++) groups.push_back({a});
+  }
+  out = SweepLists{};
+  out.base.push_back(0);
+  std::vector<std::pair<int64_t, int64_t>> chunk;  // (cell * A + angle, source item)
+  for (const auto &g : groups) {
+    for (int l = 0;; l++) {
+      chunk.clear();
+      for (int a : g) {
+        const int k = s.h_base[a] + l;
+        if (k + 1 >= s.h_base[a + 1]) continue;  // direction a has fewer levels
+        for (int64_t i = s.h_lvl[k]; i < s.h_lvl[k + 1]; i++) chunk.push_back({(int64_t)s.h_cells[i] * A + a, i});
+      }
+      if (chunk.empty()) break;
+      std::sort(chunk.begin(), chunk.end());
+      out.lvl.push_back((int64_t)out.cell.size());
+      for (auto &e : chunk) {
+        out.cell.push_back(s.h_cells[e.second]);
+        out.angle.push_back(s.h_angle[e.second]);
+        out.up.push_back(s.h_up[e.second]);
+      }
+      out.widest = std::max<int64_t>(out.widest, (int64_t)chunk.size());
+    }
+    out.lvl.push_back((int64_t)out.cell.size());
+    out.base.push_back((int)out.lvl.size());
   }
 }
 
@@ -639,7 +689,7 @@ int sweep_slots(int nf, int nfu) {
 
 // the records of every item of `g`'s (or the batch's) schedule
 static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<double> &g1,
-                           const BatchItem<double> *items, const int *wprob, const int *wdir, const int *cell,
+                           const BatchItem<double> *items, const int *wprob, const int *ang, const int *cell,
                            const unsigned char *up, const int64_t *lvl, const int *base, int n_wg, int64_t widest_dir,
                            int64_t n_items, double *r64, float *r32, int *rnb, bool batch, hipStream_t s) {
   const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (widest_dir + 255) / 256));
@@ -648,10 +698,10 @@ static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<doub
 #define RC(DIM, NB, NF, NFU)                                                                                         \
   if (!launched && dim == DIM && nb == NB && nf == NF && slots == NFU) {                                             \
     if (batch)                                                                                                       \
-      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, true>), grid, dim3(256), 0, s, g1, items, wprob, wdir,      \
+      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, true>), grid, dim3(256), 0, s, g1, items, wprob, ang,       \
                          cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
     else                                                                                                             \
-      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, false>), grid, dim3(256), 0, s, g1, items, wprob, wdir,     \
+      hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, false>), grid, dim3(256), 0, s, g1, items, wprob, ang,      \
                          cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
     launched = true;                                                                                                 \
   }
@@ -667,16 +717,16 @@ void Sweep::build_records(hipStream_t s) const {
   const Disc &D = *o.disc;
   const int slots = sweep_slots(D.nf, nfu);
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
-  const size_t per = (size_t)D.nb * D.nb * (1 + slots);
+  const int per = D.nb * D.nb * (1 + slots);
   if (!rec64.ptr) {
-    rec64.alloc(sizeof(double) * per * n_items);
-    rec32.alloc(sizeof(float) * per * n_items);
-    rnb.alloc(sizeof(int) * (size_t)slots * n_items);
+    rec64.alloc(sizeof(double) * rec_len(n_items, per));
+    rec32.alloc(sizeof(float) * rec_len(n_items, per));
+    rnb.alloc(sizeof(int) * rec_len(n_items, slots));
   }
-  launch_records(D.dim, D.nb, D.nf, slots, o.args<double>(), nullptr, nullptr, nullptr, cells.as<int>(),
-                 upwind.as<unsigned char>(), lvl.as<int64_t>(), base.as<int>(), D.n_angles,
-                 (n_items + D.n_angles - 1) / D.n_angles, n_items, rec64.as<double>(), rec32.as<float>(),
-                 rnb.as<int>(), false, s);
+  launch_records(D.dim, D.nb, D.nf, slots, o.args<double>(), nullptr, nullptr, angle.as<int>(), cells.as<int>(),
+                 upwind.as<unsigned char>(), lvl.as<int64_t>(), base.as<int>(), n_groups,
+                 (n_items + n_groups - 1) / n_groups, n_items, rec64.as<double>(), rec32.as<float>(), rnb.as<int>(),
+                 false, s);
   rec_group = o.group;
   rec_gen = o.xs_gen;
 }
@@ -715,29 +765,29 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
     UBT_HIP(hipGetLastError());
     return;
   }
-  const int *ci = cells.as<int>(), *bs = base.as<int>();
+  const int *ci = cells.as<int>(), *bs = base.as<int>(), *an = angle.as<int>();
   const unsigned char *up = upwind.as<unsigned char>();
   const int64_t *lv = lvl.as<int64_t>();
-  const unsigned grid = (unsigned)D.n_angles;
+  const unsigned grid = (unsigned)n_groups;
   bool launched = false;
   if (!adj) {
     if (rec_group != o.group || rec_gen != o.xs_gen) build_records(s);
     const int slots = sweep_slots(D.nf, nfu);
     const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
-    // a workgroup of at most 256 (wider levels loop); the pipelined form
-    // holds two items' records in registers, so only where they fit
-    const int th = std::min(threads, 256);
-    const int item_regs = (D.nb * D.nb * (1 + slots) + D.nb) * (prec == UBT_FP64 ? 2 : 1) + slots + 2;
-    const bool pipe = 2 * item_regs <= 200;
+    // the pipelined form holds two items' records in registers: levels of at
+    // most 256 items and records that fit; otherwise the plain form with up
+    // to 1024 threads
+    const int item_regs = (D.nb * D.nb * (1 + slots) + D.nb) * (prec == UBT_FP64 ? 2 : 1) + slots + 3;
+    const bool pipe = threads <= 256 && 2 * item_regs <= 200;
 #define SR(T, NB, NFU)                                                                                              \
   if (!launched && D.nb == NB && slots == NFU) {                                                                    \
     const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \
     if (pipe)                                                                                                       \
-      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, true>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,       \
-                         D.n_angles, ci, rc, rnb.as<int>(), n_items, lv, bs);                                       \
+      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,            \
+                         (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
     else                                                                                                            \
-      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(th), 0, s, (const T *)r, (T *)psi,      \
-                         D.n_angles, ci, rc, rnb.as<int>(), n_items, lv, bs);                                       \
+      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(threads), 0, s, (const T *)r,           \
+                         (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
     launched = true;                                                                                                \
   }
     if (prec == UBT_FP64) {
@@ -753,12 +803,8 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   }
 #define SW(T, DIM, NB, NF)                                                                                         \
   if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                     \
-    if (adj)                                                                                                       \
-      hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,      \
-                         (T *)psi, o.args<T>(), ci, up, lv, bs);                                                   \
-    else                                                                                                           \
-      hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, false>), dim3(grid), dim3(threads), 0, s, (const T *)r,     \
-                         (T *)psi, o.args<T>(), ci, up, lv, bs);                                                   \
+    hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,        \
+                       (T *)psi, o.args<T>(), ci, an, up, lv, bs);                                                 \
     launched = true;                                                                                               \
   }
 #define SW_ALL(T) SW(T, 2, 1, 3) SW(T, 2, 1, 4) SW(T, 3, 1, 4) SW(T, 3, 1, 6) SW(T, 2, 3, 3) SW(T, 2, 3, 4) SW(T, 3, 4, 4) SW(T, 3, 4, 6)
@@ -773,44 +819,32 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   UBT_HIP(hipGetLastError());
 }
 
-void batch_records_launch(const Batch &b, hipStream_t s) {
-  const BatchSweep &w = b.sweep;
+void batch_records_launch(const Batch &b, int pi, hipStream_t s) {
+  const BatchSweep &w = b.sweep[pi];
   launch_records(b.dim, b.nb, b.nf, w.slots, OpArgs<double>{}, b.items[0][0].as<BatchItem<double>>(),
-                 w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),
-                 w.lvl.as<int64_t>(), w.base.as<int>(), w.n_wg, w.widest, w.n_items, w.rec64.as<double>(),
-                 w.rec32.as<float>(), w.rnb.as<int>(), true, s);
+                 w.prob.as<int>(), w.angle.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),
+                 w.lvl.as<int64_t>(), w.base.as<int>(), w.n_wg, w.widest, w.n_items,
+                 pi == 0 ? w.rec.as<double>() : nullptr, pi == 1 ? w.rec.as<float>() : nullptr, w.rnb.as<int>(), true,
+                 s);
 }
 
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
-  const BatchSweep &w = b.sweep;
+  const BatchSweep &w = b.sweep[pi];
   if (w.n_wg == 0) return;
   const BatchItem<T> *items = b.items[pi][0].as<BatchItem<T>>();
   bool launched = false;
-  const T *rec = sizeof(T) == 8 ? (const T *)w.rec64.ptr : (const T *)w.rec32.ptr;
+  const T *rec = w.rec.as<T>();
 #define SRB(NB, NFU)                                                                                                \
   if (!launched && b.nb == NB && w.slots == NFU) {                                                                  \
-    hipLaunchKernelGGL((sweep_dg_rec_batch<T, NB, NFU, false>), dim3(w.n_wg), dim3(256), 0, s, r, psi, items,        \
-                       w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), rec, w.rnb.as<int>(), w.n_items,        \
+    hipLaunchKernelGGL((sweep_dg_rec_batch<T, NB, NFU>), dim3(w.n_wg), dim3(w.threads), 0, s, r, psi, items,         \
+                       w.prob.as<int>(), w.cells.as<int>(), w.angle.as<int>(), rec, w.rnb.as<int>(), w.n_items,      \
                        w.lvl.as<int64_t>(), w.base.as<int>());                                                      \
     launched = true;                                                                                                \
   }
   SRB(1, 2) SRB(1, 3) SRB(1, 5) SRB(3, 2) SRB(3, 3) SRB(4, 3) SRB(4, 5)
 #undef SRB
-  if (launched) {
-    UBT_HIP(hipGetLastError());
-    return;
-  }
-#define SWB(DIM, NB, NF)                                                                                            \
-  if (!launched && b.dim == DIM && b.nb == NB && b.nf == NF) {                                                      \
-    hipLaunchKernelGGL((sweep_dg_dir_batch<T, DIM, NB, NF>), dim3(w.n_wg), dim3(256), 0, s, r, psi, items,          \
-                       w.prob.as<int>(), w.dir.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),          \
-                       w.lvl.as<int64_t>(), w.base.as<int>());                                                      \
-    launched = true;                                                                                                \
-  }
-  SWB(2, 1, 3) SWB(2, 1, 4) SWB(3, 1, 4) SWB(3, 1, 6) SWB(2, 3, 3) SWB(2, 3, 4) SWB(3, 4, 4) SWB(3, 4, 6)
-#undef SWB
-  UBT_CHECK(launched, UBT_ERR_SUP, "no batched sweep for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
+  UBT_CHECK(launched, UBT_ERR_SUP, "no batched sweep for %d basis functions and %d inflow slots", b.nb, w.slots);
   UBT_HIP(hipGetLastError());
 }
 

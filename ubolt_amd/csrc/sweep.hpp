@@ -6,15 +6,31 @@
 
 namespace ubt {
 
+struct Sweep;
+
+// a sweep schedule as host lists: items (cell, angle, upwind mask) in level
+// order, level boundaries lvl[base[g] .. base[g + 1]) of workgroup g
+struct SweepLists {
+  std::vector<int> cell, angle, base;
+  std::vector<unsigned char> up;
+  std::vector<int64_t> lvl;
+  int64_t widest = 1;  // the widest level
+};
+// the schedule of s (kept on the host) regrouped into workgroups of up to G
+// directions of one octant (box meshes; otherwise one direction each)
+void group_schedule(const Sweep &s, int G, SweepLists &out);
+
 struct Sweep {
   Operator *op = nullptr;
   int n_levels = 0;                 // the most levels of any direction
   int64_t n_lagged = 0;             // upwind edges lagged to break cycles
   int threads = 64;                 // DG: workgroup size (the widest level, at most 1024)
-  // DG, per direction: items (cell, upwind face mask) in level order, level
-  // boundaries lvl[base[a] .. base[a + 1]) of direction a
-  DevBuf cells, upwind, lvl, base;
-  std::vector<int> h_cells, h_base;  // host copies (kept on request, for batching)
+  // DG, per direction (one workgroup each): items (cell, angle, upwind face
+  // mask) in level order, level boundaries lvl[base[g] .. base[g + 1]) of
+  // direction g
+  int n_groups = 0;
+  DevBuf cells, angle, upwind, lvl, base;
+  std::vector<int> h_cells, h_angle, h_base;  // host copies (kept on request, for batching)
   std::vector<unsigned char> h_up;
   std::vector<int64_t> h_lvl;
   // DG forward sweep records, per item in schedule order (structure of
