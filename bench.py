@@ -249,6 +249,25 @@ def batch_bench(args):
                                 "separate_launches_us": round(per_sep * 1e6, 1)}
         del x, y, xs, ys
     torch.cuda.empty_cache()
+    # the sweep alone, both precisions (group 0)
+    r = b.vec()
+    b.rhs(r)
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        rr = r.to(dt)
+        ps = torch.empty_like(rr)
+        b.sweep_apply(rr, ps)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(5):
+            b.sweep_apply(rr, ps)
+        e1.record()
+        torch.cuda.synchronize()
+        out[f"sweep_{name}_ms"] = round(e0.elapsed_time(e1) / 5, 3)
+        del rr, ps
+    del r
+    if args.no_solve:
+        print(json.dumps(out))
+        return
     for mode in ("fp64", "mixed"):
         sol = u.BatchSolver(b, rtol=1e-8, max_it=600, restart=20, mixed=(mode == "mixed"), inner_rtol=1e-4)
         total, first_total, its, worst, per_group = 0.0, 0.0, [], 0.0, []

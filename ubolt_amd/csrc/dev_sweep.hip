@@ -488,6 +488,14 @@ __global__ void __launch_bounds__(1024) sweep_fd_dir(const T *__restrict__ r, T 
 // ---------------------------------------------------------------------------
 // Host: the DG level schedule
 // ---------------------------------------------------------------------------
+// inflow slots of the record layout: the smallest instantiated count that
+// holds the most inflow faces of any item
+int sweep_slots(int nf, int nfu) {
+  if (nf == 3) return 2;
+  if (nf == 4) return nfu <= 2 ? 2 : 3;
+  return nfu <= 3 ? 3 : 5;
+}
+
 void Sweep::create(Operator *o, bool keep_host) {
   op = o;
   const Disc &D = *o->disc;
@@ -614,6 +622,8 @@ void Sweep::create(Operator *o, bool keep_host) {
   nfu = 1;
   for (unsigned char m : item_up) nfu = std::max(nfu, __builtin_popcount(m));
   threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest + 63) / 64 * 64));
+  rec_group = -1;
+  if (!keep_host) alloc_records();
   cells.upload(item_cell);
   angle.upload(item_angle);
   upwind.upload(item_up);
@@ -679,13 +689,6 @@ void group_schedule(const Sweep &s, int G, SweepLists &out) {
   }
 }
 
-// inflow slots of the record layout: the smallest instantiated count that
-// holds the most inflow faces of any item
-int sweep_slots(int nf, int nfu) {
-  if (nf == 3) return 2;
-  if (nf == 4) return nfu <= 2 ? 2 : 3;
-  return nfu <= 3 ? 3 : 5;
-}
 
 // the records of every item of `g`'s (or the batch's) schedule
 static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<double> &g1,
@@ -712,17 +715,21 @@ static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<doub
   UBT_HIP(hipGetLastError());
 }
 
+void Sweep::alloc_records() const {
+  const Disc &D = *op->disc;
+  const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
+  const int slots = sweep_slots(D.nf, nfu), per = D.nb * D.nb * (1 + slots);
+  rec64.alloc(sizeof(double) * rec_len(n_items, per));
+  rec32.alloc(sizeof(float) * rec_len(n_items, per));
+  rnb.alloc(sizeof(int) * rec_len(n_items, slots));
+}
+
 void Sweep::build_records(hipStream_t s) const {
   const Operator &o = *op;
   const Disc &D = *o.disc;
   const int slots = sweep_slots(D.nf, nfu);
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
-  const int per = D.nb * D.nb * (1 + slots);
-  if (!rec64.ptr) {
-    rec64.alloc(sizeof(double) * rec_len(n_items, per));
-    rec32.alloc(sizeof(float) * rec_len(n_items, per));
-    rnb.alloc(sizeof(int) * rec_len(n_items, slots));
-  }
+  if (!rec64.ptr) alloc_records();
   launch_records(D.dim, D.nb, D.nf, slots, o.args<double>(), nullptr, nullptr, angle.as<int>(), cells.as<int>(),
                  upwind.as<unsigned char>(), lvl.as<int64_t>(), base.as<int>(), n_groups,
                  (n_items + n_groups - 1) / n_groups, n_items, rec64.as<double>(), rec32.as<float>(), rnb.as<int>(),

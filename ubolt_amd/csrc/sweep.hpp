@@ -44,6 +44,10 @@ struct Sweep {
   mutable int rec_group = -1;
   mutable uint64_t rec_gen = 0;
   void build_records(hipStream_t s) const;
+  void alloc_records() const;
+  // keep_host: the schedule stays on the host for a batch, which builds its
+  // own records (this sweep's are then allocated by its first forward sweep);
+  // otherwise the record storage is allocated here
   void create(Operator *o, bool keep_host = false);
   // psi = L^{-1} r, or (adj) psi = L^{-T} r
   void apply(int prec, const void *r, void *psi, hipStream_t s, bool adj = false) const;
