@@ -99,6 +99,9 @@ typedef struct {
   double inner_rtol;  /* inner tolerance in mixed mode, e.g. 1e-4 */
   int max_outer;      /* refinement steps in mixed mode */
   int pc_blocks, pc_sweep, pc_dsa, use_amat;
+  int gs_passes;      /* Gram-Schmidt passes per Arnoldi step: 1 classical (PETSc's
+                       * default, KSPGMRESClassicalGramSchmidtOrthogonalization with
+                       * refinement never), 2 classical twice; 0 means 2 */
 } ubt_solver_options;
 
 /* ------------------------------------------------------------------ errors */

@@ -89,11 +89,12 @@ def test_batch_blocks_and_sweep(gpu, ensemble):
         assert rel_max(to_host(ys)[sl], ref) <= 1e-11
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_batch_solve(gpu, ensemble, mixed):
+@pytest.mark.parametrize("mixed,gs_passes", [(False, 2), (True, 2), (False, 1), (True, 1)])
+def test_batch_solve(gpu, ensemble, mixed, gs_passes):
+    """FP64 and mixed, classical Gram-Schmidt twice (default) and once"""
     probs, ops, b = ensemble
     b.set_group(0)
-    sol = u.BatchSolver(b, rtol=1e-8 if mixed else 1e-11, max_it=600, restart=20, mixed=mixed)
+    sol = u.BatchSolver(b, rtol=1e-8 if mixed else 1e-11, max_it=600, restart=20, mixed=mixed, gs_passes=gs_passes)
     sol.refresh()
     rhs = b.vec()
     b.rhs(rhs)

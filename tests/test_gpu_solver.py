@@ -102,11 +102,13 @@ def phi_of(prob, psi):
     return oracle.moments(psi, prob.quad.w)
 
 
-def test_solve_fp64_parity(gpu, built):
+@pytest.mark.parametrize("gs_passes", [2, 1])
+def test_solve_fp64_parity(gpu, built, gs_passes):
+    """classical Gram-Schmidt twice (default) and once (PETSc's default)"""
     name, prob, op = built
     dsa = u.DSA(op)
     dsa.set_group(0)
-    sol = u.Solver(op, dsa, rtol=1e-12, max_it=400)
+    sol = u.Solver(op, dsa, rtol=1e-12, max_it=400, gs_passes=gs_passes)
     sol.refresh()
     b = op.vec()
     op.rhs(0, b)
@@ -195,4 +197,7 @@ def test_errors(gpu):
     assert e.value.status == "ERR_SUP"
     with pytest.raises(u.UbtError) as e:
         u.Solver(op, dsa, restart=40)
+    assert e.value.status == "ERR_ARG_OUTOFRANGE"
+    with pytest.raises(u.UbtError) as e:
+        u.Solver(op, dsa, gs_passes=3)
     assert e.value.status == "ERR_ARG_OUTOFRANGE"

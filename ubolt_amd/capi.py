@@ -30,7 +30,7 @@ class UbtError(RuntimeError):
 class SolverOptions(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_it", C.c_int), ("restart", C.c_int), ("mixed", C.c_int),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("pc_blocks", C.c_int), ("pc_sweep", C.c_int),
-                ("pc_dsa", C.c_int), ("use_amat", C.c_int)]
+                ("pc_dsa", C.c_int), ("use_amat", C.c_int), ("gs_passes", C.c_int)]
 
 
 class Bc(C.Structure):

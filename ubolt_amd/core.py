@@ -319,11 +319,11 @@ class Solver(_Handle):
     _destroy = "ubt_solver_destroy"
 
     def __init__(self, op, dsa=None, rtol=1e-8, max_it=500, restart=30, mixed=False, inner_rtol=1e-4, max_outer=20,
-                 pc_blocks=True, pc_sweep=True, pc_dsa=True, use_amat=True):
+                 pc_blocks=True, pc_sweep=True, pc_dsa=True, use_amat=True, gs_passes=2):
         super().__init__()
         self.op, self.dsa = op, dsa
         o = capi.SolverOptions(rtol, max_it, restart, int(mixed), inner_rtol, max_outer, int(pc_blocks), int(pc_sweep),
-                               int(pc_dsa), int(use_amat))
+                               int(pc_dsa), int(use_amat), int(gs_passes))
         call("ubt_solver_create", op.h, dsa.h if dsa is not None else None, C.byref(o), C.byref(self.h))
 
     def refresh(self, stream=None):
@@ -417,11 +417,11 @@ class BatchSolver(_Handle):
     _destroy = "ubt_batch_solver_destroy"
 
     def __init__(self, batch, rtol=1e-8, max_it=500, restart=30, mixed=False, inner_rtol=1e-4, max_outer=20,
-                 pc_blocks=True, pc_sweep=True):
+                 pc_blocks=True, pc_sweep=True, gs_passes=2):
         super().__init__()
         self.batch = batch
         o = capi.SolverOptions(rtol, max_it, restart, int(mixed), inner_rtol, max_outer, int(pc_blocks), int(pc_sweep),
-                               0, 0)
+                               0, 0, int(gs_passes))
         call("ubt_batch_solver_create", batch.h, C.byref(o), C.byref(self.h))
 
     def refresh(self, stream=None):

@@ -567,7 +567,10 @@ int ubt_solver_create(ubt_op *op, ubt_dsa *dsa, const ubt_solver_options *option
       o.pc_sweep = options->pc_sweep;
       o.pc_dsa = options->pc_dsa;
       o.use_amat = options->use_amat;
+      o.gs_passes = options->gs_passes > 0 ? options->gs_passes : 2;
     }
+    UBT_CHECK(o.gs_passes == 1 || o.gs_passes == 2, UBT_ERR_ARG_OUTOFRANGE, "gs_passes must be 1 or 2, got %d",
+              o.gs_passes);
     UBT_CHECK(!o.mixed || (o.inner_rtol > 0.0 && o.inner_rtol < 1.0 && o.max_outer >= 1), UBT_ERR_ARG_OUTOFRANGE,
               "mixed mode needs 0 < inner_rtol < 1 and max_outer >= 1");
     auto h = std::make_unique<ubt_solver_s>();
@@ -745,7 +748,10 @@ int ubt_batch_solver_create(ubt_batch *batch, const ubt_solver_options *options,
       o.pc_sweep = options->pc_sweep;
       o.pc_dsa = 0;
       o.use_amat = options->use_amat;
+      o.gs_passes = options->gs_passes > 0 ? options->gs_passes : 2;
     }
+    UBT_CHECK(o.gs_passes == 1 || o.gs_passes == 2, UBT_ERR_ARG_OUTOFRANGE, "gs_passes must be 1 or 2, got %d",
+              o.gs_passes);
     UBT_CHECK(!o.mixed || (o.inner_rtol > 0.0 && o.inner_rtol < 1.0 && o.max_outer >= 1), UBT_ERR_ARG_OUTOFRANGE,
               "mixed mode needs 0 < inner_rtol < 1 and max_outer >= 1");
     auto h = std::make_unique<ubt_batch_solver_s>();

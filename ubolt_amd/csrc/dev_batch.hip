@@ -151,7 +151,7 @@ __global__ void k_bgivens(int np, int j, const double *__restrict__ h1, const do
   if (p >= np || done[p]) return;
   double *Hp = H + (int64_t)p * kS * kS, *c = cs + (int64_t)p * kS, *s = sn + (int64_t)p * kS,
          *gp = g + (int64_t)p * kS;
-  for (int i = 0; i <= j; i++) Hp[i * kS + j] = h1[(int64_t)p * kS + i] + h2[(int64_t)p * kS + i];
+  for (int i = 0; i <= j; i++) Hp[i * kS + j] = h1[(int64_t)p * kS + i] + (h2 ? h2[(int64_t)p * kS + i] : 0.0);
   const double hnext = hn[p];
   for (int i = 0; i < j; i++) {
     const double a = Hp[i * kS + j], b = Hp[(i + 1) * kS + j];
@@ -642,14 +642,17 @@ template <class T> void BatchSolver::cycle_steps(int j0, int j1, hipStream_t s) 
     hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
     hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h1, kS, -1.0, w,
                        cst, cln, cpr, (const int *)nullptr);
-    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
-    hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0, w,
-                       cst, cln, cpr, (const int *)nullptr);
+    if (opt.gs_passes > 1) {  // classical Gram-Schmidt twice
+      hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
+      hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
+      hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0,
+                         w, cst, cln, cpr, (const int *)nullptr);
+    }
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
     hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, vn, cst, cln, cpr);
-    hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1, (const double *)h2,
+    hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1,
+                       (const double *)(opt.gs_passes > 1 ? h2 : nullptr),
                        (const double *)hn, H.as<double>(), cs.as<double>(), sn.as<double>(), gv.as<double>(),
                        res.as<double>(), (const double *)tol.as<double>(), dn, jd, its);
   }

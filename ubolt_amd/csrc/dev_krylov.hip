@@ -231,17 +231,21 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
       T *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *vn = V + (int64_t)(j + 1) * n;
       apply_pc<T>(vj, zj, s);
       apply_op<T>(zj, w, true, s);
-      // CGS2: h = V^T w, w -= V h, twice; the second pass accumulates into h
+      // classical Gram-Schmidt: h = V^T w, w -= V h, once or twice (the
+      // second pass's coefficients added to the first's)
       mdot<T>(V, j + 1, w, n, hc, false, false, part, s);
       hipLaunchKernelGGL(k_maxpy<T>, dim3(gv), dim3(kThreads), 0, s, n, j + 1, (const T *)V, (const double *)hc, -1.0, w);
-      mdot<T>(V, j + 1, w, n, hc + kMaxK, false, false, part, s);
-      hipLaunchKernelGGL(k_maxpy<T>, dim3(gv), dim3(kThreads), 0, s, n, j + 1, (const T *)V,
-                         (const double *)(hc + kMaxK), -1.0, w);
+      std::vector<double> h2(j + 1, 0.0);
+      if (opt.gs_passes > 1) {
+        mdot<T>(V, j + 1, w, n, hc + kMaxK, false, false, part, s);
+        hipLaunchKernelGGL(k_maxpy<T>, dim3(gv), dim3(kThreads), 0, s, n, j + 1, (const T *)V,
+                           (const double *)(hc + kMaxK), -1.0, w);
+      }
       mdot<T>(w, 1, w, n, hc + j + 1, false, true, part, s);
       hipLaunchKernelGGL(k_scale_dev<T>, dim3(gv), dim3(kThreads), 0, s, n, (const T *)w, (const double *)(hc + j + 1), vn);
-      std::vector<double> h2(j + 1);
       UBT_HIP(hipMemcpyAsync(h_host.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
-      UBT_HIP(hipMemcpyAsync(h2.data(), hc + kMaxK, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+      if (opt.gs_passes > 1)
+        UBT_HIP(hipMemcpyAsync(h2.data(), hc + kMaxK, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
       UBT_HIP(hipStreamSynchronize(s));
       for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h_host[i] + h2[i];
       H[(size_t)(j + 1) * m + j] = h_host[j + 1];

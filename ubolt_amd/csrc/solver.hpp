@@ -19,6 +19,7 @@ struct SolverOptions {
   int pc_sweep = 1;         // stage 1: transport sweep (the streaming stage)
   int pc_dsa = 1;           // stage 2: DSA (if a DSA object is given)
   int use_amat = 1;         // residual updates between stages with A (else L) when DSA is on
+  int gs_passes = 2;        // classical Gram-Schmidt passes per Arnoldi step (1 or 2)
 };
 
 struct Solver {
