@@ -98,6 +98,41 @@ def test_sweep_inverts_L(gpu, built):
     assert e64 <= 1e-11 and e32 <= 1e-4
 
 
+def test_sweep_records_follow_cross_sections(gpu):
+    """the sweep's per-item records (block inverses, upwind couplings) are
+    refolded when the cross sections or the group change: sweep, change
+    sigma_t, sweep again, switch group, sweep again - each against the oracle"""
+    prob = P.dg1_delaunay(n_side=8, sn=4, seed=3)
+    op = prob.operator()
+    sw = u.Sweep(op)
+    r = np.random.default_rng(5).standard_normal(prob.disc.n_rows)
+    nc = prob.disc.n_cells
+    rng = np.random.default_rng(6)
+    for step in range(3):
+        if step == 1:
+            st = rng.uniform(0.2, 4.0, nc)
+            op.set_xs(st, 0.3 * st, np.ones(nc))
+            prob = P.Problem(prob.name, prob.quad, prob.disc, st, 0.3 * st, np.ones(nc))
+        want = spl.spsolve(oracle_rows(prob).to_scipy().tocsc(), r)
+        psi = op.vec()
+        sw.apply(to_dev(r), psi)
+        torch.cuda.synchronize()
+        err = rel_max(to_host(psi), want)
+        print(f"step {step}: sweep vs oracle {err:.2e}")
+        assert err <= 1e-11
+    mg = P.dg_box_2d(5, 4, 4, order=1, G=2)
+    op2 = mg.operator()
+    sw2 = u.Sweep(op2)
+    r2 = np.random.default_rng(7).standard_normal(mg.disc.n_rows)
+    for g in (0, 1, 0):
+        op2.set_group(g)
+        want = spl.spsolve(oracle_rows(mg, g=g).to_scipy().tocsc(), r2)
+        psi = op2.vec()
+        sw2.apply(to_dev(r2), psi)
+        torch.cuda.synchronize()
+        assert rel_max(to_host(psi), want) <= 1e-11, g
+
+
 def phi_of(prob, psi):
     return oracle.moments(psi, prob.quad.w)
 
