@@ -400,9 +400,13 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 }
 
 // PIPE holds two items' records in registers: at most 256 threads; the plain
-// form serves the wide levels of merged octants with up to 1024
+// form serves wide levels with up to sweep_max_threads(NB) (one item's
+// records of an NB = 4 hex or tet are 64-96 values: 512 threads keep them in
+// registers, 1024 would spill)
+__host__ __device__ constexpr int sweep_max_threads(int nb) { return nb >= 4 ? 512 : 1024; }
+
 template <class T, int NB, int NFU, bool PIPE>
-__global__ void __launch_bounds__(PIPE ? 256 : 1024) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
+__global__ void __launch_bounds__(PIPE ? 256 : sweep_max_threads(NB)) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
                                                                   const int *__restrict__ cell, const int *__restrict__ ang,
                                                                   const T *__restrict__ rec, const int *__restrict__ rnb,
                                                                   int64_t n_items, const int64_t *__restrict__ lvl,
@@ -412,7 +416,7 @@ __global__ void __launch_bounds__(PIPE ? 256 : 1024) sweep_dg_rec(const T *__res
 }
 
 template <class T, int NB, int NFU>
-__global__ void __launch_bounds__(1024) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
+__global__ void __launch_bounds__(sweep_max_threads(NB)) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
                                                            const BatchItem<T> *__restrict__ items,
                                                            const int *__restrict__ wprob, const int *__restrict__ cell,
                                                            const int *__restrict__ ang, const T *__restrict__ rec,
@@ -786,6 +790,7 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
     // to 1024 threads
     const int item_regs = (D.nb * D.nb * (1 + slots) + D.nb) * (prec == UBT_FP64 ? 2 : 1) + slots + 3;
     const bool pipe = threads <= 256 && 2 * item_regs <= 200;
+    const int th = std::min(threads, sweep_max_threads(D.nb));
 #define SR(T, NB, NFU)                                                                                              \
   if (!launched && D.nb == NB && slots == NFU) {                                                                    \
     const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \
@@ -793,7 +798,7 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
       hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,            \
                          (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
     else                                                                                                            \
-      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(threads), 0, s, (const T *)r,           \
+      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(th), 0, s, (const T *)r,                \
                          (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
     launched = true;                                                                                                \
   }
@@ -844,7 +849,8 @@ template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, h
   const T *rec = w.rec.as<T>();
 #define SRB(NB, NFU)                                                                                                \
   if (!launched && b.nb == NB && w.slots == NFU) {                                                                  \
-    hipLaunchKernelGGL((sweep_dg_rec_batch<T, NB, NFU>), dim3(w.n_wg), dim3(w.threads), 0, s, r, psi, items,         \
+    hipLaunchKernelGGL((sweep_dg_rec_batch<T, NB, NFU>), dim3(w.n_wg),                                                \
+                       dim3(std::min(w.threads, sweep_max_threads(NB))), 0, s, r, psi, items,                       \
                        w.prob.as<int>(), w.cells.as<int>(), w.angle.as<int>(), rec, w.rnb.as<int>(), w.n_items,      \
                        w.lvl.as<int64_t>(), w.base.as<int>());                                                      \
     launched = true;                                                                                                \
