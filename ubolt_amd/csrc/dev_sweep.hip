@@ -5,10 +5,11 @@
 // wavefronts of cells whose upwind neighbours are all done.
 //
 // * Unstructured DG0/DG1: per angle, the cells are levelled by a topological
-//   sort of the upwind graph (host, once per discretisation). A level step
-//   processes the cells of that level for one direction: gather the upwind
-//   traces, build the cell's n_basis x n_basis block (the same block the
-//   element block inverse solves) and solve it. An edge left
+//   sort of the upwind graph (host, once per discretisation). Each (cell,
+//   direction) item's n_basis x n_basis block (the same block the element
+//   block inverse solves) is inverted once per group into its sweep record,
+//   with the upwind couplings; a level step then gathers the upwind traces
+//   and applies the record. An edge left
 //   on a cycle (possible on a general mesh, not on a Delaunay or box mesh) is
 //   lagged: its inflow is taken as zero, so the stage stays a preconditioner.
 //   Which faces read an upwind trace is the schedule's own per-item mask, not
@@ -23,8 +24,9 @@
 //
 // The adjoint sweep solves L^T y = r with L^T = V P L P V^{-1} (P reverses
 // every angle, V the cell volumes): the forward schedule of angle b solves for
-// u = L^{-1} P V^{-1} r, whose angle b lives in y's angle -b, so the ADJ
-// kernels read and write the reversed angle's data and scale by the volumes.
+// u = L^{-1} P V^{-1} r, whose angle b lives in y's angle -b, so the adjoint
+// sweep runs the forward records, reading and writing the reversed angle's
+// data and scaling by the volumes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,109 +37,6 @@
 namespace ubt {
 
 namespace {
-
-// one (cell, direction) item of a level: gather, build the block, solve
-template <class T, int DIM, int NB, int NF, bool ADJ>
-__device__ __forceinline__ void sweep_item(const T *__restrict__ r, T *__restrict__ psi, const OpArgs<T> &g,
-                                           int64_t c, int b, unsigned char up) {
-  const int a = ADJ ? g.neg[b] : b;        // b: the direction swept; a: the angle whose data it carries
-  const int A = g.A;
-  T om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : T(0)};
-  T B[NB][NB], v[NB];
-  const int64_t base = c * NB * A + a;
-  const int64_t gc = g.gcls[c];
-  const T inv_vc = ADJ ? T(1) / g.vol[gc] : T(1);
-#pragma unroll
-  for (int i = 0; i < NB; i++) {
-    v[i] = inv_vc * r[base + (int64_t)i * A];
-#pragma unroll
-    for (int j = 0; j < NB; j++) B[i][j] = T(0);
-  }
-#pragma unroll
-  for (int f = 0; f < NF; f++) {
-    const int64_t k = c * NF + f, kg = gc * NF + f;
-    T s = T(0);
-#pragma unroll
-    for (int d = 0; d < DIM; d++) s += om[d] * g.face_s[3 * kg + d];
-    if ((up >> f) & 1) {
-      const int n = g.face_nb[k];
-      const int64_t ub = (int64_t)n * NB * A + a;
-      const T inv_vn = ADJ ? T(1) / g.vol[g.gcls[n]] : T(1);
-      if (NB == 1) {
-        v[0] -= s * inv_vn * psi[ub];
-      } else {
-        T pu[NB];
-#pragma unroll
-        for (int j = 0; j < NB; j++) pu[j] = inv_vn * psi[ub + (int64_t)j * A];
-        const T *F = g.face_up + kg * NB * NB;
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-          T t = T(0);
-#pragma unroll
-          for (int j = 0; j < NB; j++) t += F[i * NB + j] * pu[j];
-          v[i] -= s * t;
-        }
-      }
-    } else if (s > T(0)) {  // outflow; an inflow face without the bit is vacuum / reflective / lagged
-      if (NB == 1) {
-        B[0][0] += s;
-      } else {
-        const T *F = g.face_own + kg * NB * NB;
-#pragma unroll
-        for (int i = 0; i < NB; i++)
-#pragma unroll
-          for (int j = 0; j < NB; j++) B[i][j] += s * F[i * NB + j];
-      }
-    }
-  }
-  if (NB > 1) {
-#pragma unroll
-    for (int i = 1; i < NB; i++) {
-      const T *gr = g.grad + (gc * NB + i) * 3;
-      T od = T(0);
-#pragma unroll
-      for (int d = 0; d < DIM; d++) od += om[d] * gr[d];
-      B[i][0] -= od;
-    }
-  }
-  const T sig = g.sigt[c];
-#pragma unroll
-  for (int i = 0; i < NB; i++) B[i][i] += sig;
-  // Gauss-Jordan with partial pivoting (the blocks are non-singular by setup)
-#pragma unroll
-  for (int k = 0; k < NB; k++) {
-    int piv = k;
-#pragma unroll
-    for (int i = k + 1; i < NB; i++)
-      if (fabs(B[i][k]) > fabs(B[piv][k])) piv = i;
-    if (piv != k) {
-#pragma unroll
-      for (int j = 0; j < NB; j++) {
-        T t = B[k][j];
-        B[k][j] = B[piv][j];
-        B[piv][j] = t;
-      }
-      T t = v[k];
-      v[k] = v[piv];
-      v[piv] = t;
-    }
-    const T inv = T(1) / B[k][k];
-#pragma unroll
-    for (int j = 0; j < NB; j++) B[k][j] *= inv;
-    v[k] *= inv;
-#pragma unroll
-    for (int i = 0; i < NB; i++) {
-      if (i == k) continue;
-      const T f = B[i][k];
-#pragma unroll
-      for (int j = 0; j < NB; j++) B[i][j] -= f * B[k][j];
-      v[i] -= f * v[k];
-    }
-  }
-  const T vc = ADJ ? g.vol[gc] : T(1);
-#pragma unroll
-  for (int i = 0; i < NB; i++) psi[base + (int64_t)i * A] = vc * v[i];
-}
 
 // ---------------------------------------------------------------------------
 // Sweep records. The block of an item (cell c, direction b) and its upwind
@@ -303,11 +202,18 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 // r, the upwind cells) is loaded while the current level's upwind traces are
 // in flight, and its cell id two levels ahead, so the barrier-to-barrier path
 // of a level is the trace load, the multiply-adds and the store.
-template <class T, int NB, int NFU, bool PIPE>
+// ADJ: the adjoint sweep L^T = V P L P V^{-1} from the same records: item
+// (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
+// 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
+// note at the top of this file).
+template <class T, int NB, int NFU, bool PIPE, bool ADJ = false>
 __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A,
                                               const int *__restrict__ cell, const int *__restrict__ ang,
                                               const T *__restrict__ rec, const int *__restrict__ rnb, int64_t n_items,
-                                              const int64_t *__restrict__ lvl, int l0, int l1) {
+                                              const int64_t *__restrict__ lvl, int l0, int l1,
+                                              const int *__restrict__ neg = nullptr,
+                                              const int *__restrict__ gcls = nullptr,
+                                              const T *__restrict__ vol = nullptr) {
   constexpr int NB2 = NB * NB;
   const int tid = threadIdx.x, nt = blockDim.x;
   struct Item {
@@ -315,13 +221,19 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     int a;
     int nbk[NFU];
     T bi[NB2], h[NFU][NB2], rc[NB];
+    T vc, ivn[NFU];  // ADJ: V_c, 1 / V_n per slot
   };
   auto fetch = [&](int64_t i, int64_t c, Item &it) {
     it.c = c;
-    it.a = ang[i];
+    it.a = ADJ ? neg[ang[i]] : ang[i];
     const int a = it.a;
 #pragma unroll
     for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
+    if (ADJ) {
+      it.vc = vol[gcls[c]];
+#pragma unroll
+      for (int k = 0; k < NFU; k++) it.ivn[k] = it.nbk[k] >= 0 ? T(1) / vol[gcls[it.nbk[k]]] : T(0);
+    }
 #pragma unroll
     for (int e = 0; e < NB2; e++) it.bi[e] = rec[rix(e, i, NB2 * (1 + NFU))];
 #pragma unroll
@@ -329,7 +241,7 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 #pragma unroll
       for (int e = 0; e < NB2; e++) it.h[k][e] = rec[rix(NB2 + k * NB2 + e, i, NB2 * (1 + NFU))];
 #pragma unroll
-    for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a];
+    for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a] * (ADJ ? T(1) / it.vc : T(1));
   };
   auto solve = [&](const Item &it) {
     const int a = it.a;
@@ -337,7 +249,8 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 #pragma unroll
     for (int k = 0; k < NFU; k++)
 #pragma unroll
-      for (int j = 0; j < NB; j++) pu[k][j] = it.nbk[k] >= 0 ? psi[((int64_t)it.nbk[k] * NB + j) * A + a] : T(0);
+      for (int j = 0; j < NB; j++)
+        pu[k][j] = it.nbk[k] >= 0 ? psi[((int64_t)it.nbk[k] * NB + j) * A + a] * (ADJ ? it.ivn[k] : T(1)) : T(0);
 #pragma unroll
     for (int p = 0; p < NB; p++) {
       T v = T(0);
@@ -347,7 +260,7 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
       for (int k = 0; k < NFU; k++)
 #pragma unroll
         for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * pu[k][q];
-      psi[(it.c * NB + p) * A + a] = v;
+      psi[(it.c * NB + p) * A + a] = ADJ ? it.vc * v : v;
     }
   };
   if (!PIPE) {
@@ -405,14 +318,16 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 // registers, 1024 would spill)
 __host__ __device__ constexpr int sweep_max_threads(int nb) { return nb >= 4 ? 512 : 1024; }
 
-template <class T, int NB, int NFU, bool PIPE>
+template <class T, int NB, int NFU, bool PIPE, bool ADJ>
 __global__ void __launch_bounds__(PIPE ? 256 : sweep_max_threads(NB)) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
                                                                   const int *__restrict__ cell, const int *__restrict__ ang,
                                                                   const T *__restrict__ rec, const int *__restrict__ rnb,
                                                                   int64_t n_items, const int64_t *__restrict__ lvl,
-                                                                  const int *__restrict__ base) {
+                                                                  const int *__restrict__ base, const int *__restrict__ neg,
+                                                                  const int *__restrict__ gcls, const T *__restrict__ vol) {
   const int w = blockIdx.x;
-  sweep_rec_run<T, NB, NFU, PIPE>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1);
+  sweep_rec_run<T, NB, NFU, PIPE, ADJ>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1, neg, gcls,
+                                       vol);
 }
 
 template <class T, int NB, int NFU>
@@ -426,27 +341,6 @@ __global__ void __launch_bounds__(sweep_max_threads(NB)) sweep_dg_rec_batch(cons
   const BatchItem<T> &bi = items[wprob[w]];
   sweep_rec_run<T, NB, NFU, false>(r + bi.off, psi + bi.off, bi.g.A, cell, ang, rec, rnb, n_items, lvl, base[w],
                                    base[w + 1] - 1);
-}
-
-// Persistent per-direction sweep: the sweep of one direction never reads
-// another direction's traces, so a workgroup owns one direction and walks all
-// of its levels in one launch, a workgroup barrier between levels (the items
-// of level l read traces the same workgroup wrote at earlier levels, through
-// the CU's own L1/L2, so the barrier is the only synchronisation needed).
-// Every direction runs concurrently on its own CU: the launch count of a
-// sweep drops from one per level to one.
-template <class T, int DIM, int NB, int NF, bool ADJ>
-__global__ void __launch_bounds__(1024) sweep_dg_dir(const T *__restrict__ r, T *__restrict__ psi, OpArgs<T> g,
-                                                     const int *__restrict__ cell, const int *__restrict__ ang,
-                                                     const unsigned char *__restrict__ up,
-                                                     const int64_t *__restrict__ lvl, const int *__restrict__ base) {
-  const int w = blockIdx.x;
-  const int l0 = base[w], l1 = base[w + 1] - 1;  // levels of group w: lvl[l0 .. l1]
-  for (int l = l0; l < l1; l++) {
-    for (int64_t i = lvl[l] + threadIdx.x; i < lvl[l + 1]; i += blockDim.x)
-      sweep_item<T, DIM, NB, NF, ADJ>(r, psi, g, cell[i], ang[i], up[i]);
-    __syncthreads();
-  }
 }
 
 // Structured, persistent per direction: workgroup = direction, levels in one
@@ -777,57 +671,48 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
     return;
   }
   const int *ci = cells.as<int>(), *bs = base.as<int>(), *an = angle.as<int>();
-  const unsigned char *up = upwind.as<unsigned char>();
   const int64_t *lv = lvl.as<int64_t>();
   const unsigned grid = (unsigned)n_groups;
   bool launched = false;
-  if (!adj) {
-    if (rec_group != o.group || rec_gen != o.xs_gen) build_records(s);
-    const int slots = sweep_slots(D.nf, nfu);
-    const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
-    // the pipelined form holds two items' records in registers: levels of at
-    // most 256 items and records that fit; otherwise the plain form with up
-    // to 1024 threads
-    const int item_regs = (D.nb * D.nb * (1 + slots) + D.nb) * (prec == UBT_FP64 ? 2 : 1) + slots + 3;
-    const bool pipe = threads <= 256 && 2 * item_regs <= 200;
-    const int th = std::min(threads, sweep_max_threads(D.nb));
+  // forward and adjoint sweeps both run from the forward records (the
+  // adjoint reverses the angle and scales by the volumes in the kernel)
+  if (rec_group != o.group || rec_gen != o.xs_gen) build_records(s);
+  const int slots = sweep_slots(D.nf, nfu);
+  const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
+  // the pipelined form holds two items' records in registers: levels of at
+  // most 256 items and records that fit; otherwise the plain form with up
+  // to sweep_max_threads(nb) threads
+  const int item_regs =
+      (D.nb * D.nb * (1 + slots) + D.nb + (adj ? 1 + slots : 0)) * (prec == UBT_FP64 ? 2 : 1) + slots + 3;
+  const bool pipe = threads <= 256 && 2 * item_regs <= 200;
+  const int th = std::min(threads, sweep_max_threads(D.nb));
+#define SR_K(T, NB, NFU, PIPE, ADJ, TH)                                                                            \
+  hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ>), dim3(grid), dim3(TH), 0, s, (const T *)r, (T *)psi,     \
+                     D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs, o.neg.as<int>(), o.gcls.as<int>(), vl);
 #define SR(T, NB, NFU)                                                                                              \
   if (!launched && D.nb == NB && slots == NFU) {                                                                    \
     const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \
-    if (pipe)                                                                                                       \
-      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,            \
-                         (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
-    else                                                                                                            \
-      hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, false>), dim3(grid), dim3(th), 0, s, (const T *)r,                \
-                         (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs);                         \
+    const T *vl = (prec == UBT_FP64 ? o.t64.vol : o.t32.vol).as<T>();                                               \
+    if (pipe && adj) {                                                                                              \
+      SR_K(T, NB, NFU, true, true, threads)                                                                         \
+    } else if (pipe) {                                                                                              \
+      SR_K(T, NB, NFU, true, false, threads)                                                                        \
+    } else if (adj) {                                                                                               \
+      SR_K(T, NB, NFU, false, true, th)                                                                             \
+    } else {                                                                                                        \
+      SR_K(T, NB, NFU, false, false, th)                                                                            \
+    }                                                                                                               \
     launched = true;                                                                                                \
   }
-    if (prec == UBT_FP64) {
-      SR(double, 1, 2) SR(double, 1, 3) SR(double, 1, 5) SR(double, 3, 2) SR(double, 3, 3) SR(double, 4, 3)
-      SR(double, 4, 5)
-    } else {
-      SR(float, 1, 2) SR(float, 1, 3) SR(float, 1, 5) SR(float, 3, 2) SR(float, 3, 3) SR(float, 4, 3) SR(float, 4, 5)
-    }
-#undef SR
-    UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for %d basis functions and %d inflow slots", D.nb, slots);
-    UBT_HIP(hipGetLastError());
-    return;
-  }
-#define SW(T, DIM, NB, NF)                                                                                         \
-  if (!launched && D.dim == DIM && D.nb == NB && D.nf == NF) {                                                     \
-    hipLaunchKernelGGL((sweep_dg_dir<T, DIM, NB, NF, true>), dim3(grid), dim3(threads), 0, s, (const T *)r,        \
-                       (T *)psi, o.args<T>(), ci, an, up, lv, bs);                                                 \
-    launched = true;                                                                                               \
-  }
-#define SW_ALL(T) SW(T, 2, 1, 3) SW(T, 2, 1, 4) SW(T, 3, 1, 4) SW(T, 3, 1, 6) SW(T, 2, 3, 3) SW(T, 2, 3, 4) SW(T, 3, 4, 4) SW(T, 3, 4, 6)
   if (prec == UBT_FP64) {
-    SW_ALL(double)
+    SR(double, 1, 2) SR(double, 1, 3) SR(double, 1, 5) SR(double, 3, 2) SR(double, 3, 3) SR(double, 4, 3)
+    SR(double, 4, 5)
   } else {
-    SW_ALL(float)
+    SR(float, 1, 2) SR(float, 1, 3) SR(float, 1, 5) SR(float, 3, 2) SR(float, 3, 3) SR(float, 4, 3) SR(float, 4, 5)
   }
-#undef SW_ALL
-#undef SW
-  UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for dim %d, %d basis, %d faces", D.dim, D.nb, D.nf);
+#undef SR
+#undef SR_K
+  UBT_CHECK(launched, UBT_ERR_SUP, "no sweep kernel for %d basis functions and %d inflow slots", D.nb, slots);
   UBT_HIP(hipGetLastError());
 }
 
