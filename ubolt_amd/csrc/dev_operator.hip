@@ -42,6 +42,12 @@ __device__ inline void seg_sum(const T *red, int nseg, int len, T *out, int S) {
   }
 }
 
+// the geometry class of cell c: the cell itself when every cell is its own
+// class (one dependent load fewer on a mesh with no repeated geometry)
+template <class T> __device__ __forceinline__ int64_t cls_of(const OpArgs<T> &g, int64_t c) {
+  return g.gcls_identity ? c : (int64_t)g.gcls[c];
+}
+
 // ---------------------------------------------------------------------------
 // Workgroup -> XCD-contiguous tile. The dispatcher deals consecutive
 // workgroup ids round-robin over the 8 XCDs; remapping id b to tile
@@ -146,21 +152,21 @@ __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restr
   if (NB > 1) {
     for (int e = tid; e < ncell * NF * FM; e += nt) {
       const int lc = e / (NF * FM);
-      const int64_t ge = (int64_t)g.gcls[cell_of(lc)] * NF * FM + (e - lc * NF * FM);
+      const int64_t ge = (int64_t)cls_of(g, cell_of(lc)) * NF * FM + (e - lc * NF * FM);
       s_own[e] = g.face_own[ge];
       s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int lc = e / ((NB - 1) * DIM), r = e - lc * ((NB - 1) * DIM);
       const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
-      s_gr[e] = g.grad[((int64_t)g.gcls[cell_of(lc)] * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)cls_of(g, cell_of(lc)) * NB + i) * 3 + d];
     }
   }
   for (int e = tid; e < ncell * NF; e += nt) {
     const int lc = e / NF;
     const int64_t ce = cell_of(lc);
     s_nb[e] = g.face_nb[ce * NF + (e - lc * NF)];
-    const int64_t gk = (int64_t)g.gcls[ce] * NF + (e - lc * NF);
+    const int64_t gk = (int64_t)cls_of(g, ce) * NF + (e - lc * NF);
 #pragma unroll
     for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
   }
@@ -230,7 +236,7 @@ __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restr
           ua = g.reflect[3 * a + (-2 - nbk)];
         }
         T pu[NB];
-        const T vr = (ADJ && nbk >= 0) ? g.vol[g.gcls[c]] / g.vol[g.gcls[nbk]] : T(1);
+        const T vr = (ADJ && nbk >= 0) ? g.vol[cls_of(g, c)] / g.vol[cls_of(g, nbk)] : T(1);
 #pragma unroll
         for (int j = 0; j < NB; j++) pu[j] = vr * x[ub + (int64_t)j * A + ua];
         if (NB == 1) {
@@ -320,7 +326,7 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
     const int64_t base = c * NB * A + ac;
 #pragma unroll
     for (int j = 0; j < NB; j++) psi[k][j] = live ? x[base + (int64_t)j * A] : T(0);
-    const int64_t gc = g.gcls[c];
+    const int64_t gc = cls_of(g, c);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       const int nbk = g.face_nb[c * NF + f];
@@ -339,14 +345,14 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
   if (NB > 1) {
     for (int e = tid; e < ncell * NF * FM; e += nt) {
       const int lc = e / (NF * FM);
-      const int64_t ge = (int64_t)g.gcls[c0 + lc] * NF * FM + (e - lc * NF * FM);
+      const int64_t ge = (int64_t)cls_of(g, c0 + lc) * NF * FM + (e - lc * NF * FM);
       s_own[e] = g.face_own[ge];
       s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int lc = e / ((NB - 1) * DIM), rr = e - lc * ((NB - 1) * DIM);
       const int i = 1 + rr / DIM, d = rr - (i - 1) * DIM;
-      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + lc] * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)cls_of(g, c0 + lc) * NB + i) * 3 + d];
     }
   }
   if (SCATTER) {
@@ -477,20 +483,20 @@ __device__ __forceinline__ void dg_q4_body(const T *__restrict__ x, T *__restric
   if (NB > 1) {
     for (int e = tid; e < ncell * NF * FM; e += nt) {
       const int l = e / (NF * FM);
-      const int64_t ge = (int64_t)g.gcls[c0 + l] * NF * FM + (e - l * NF * FM);
+      const int64_t ge = (int64_t)cls_of(g, c0 + l) * NF * FM + (e - l * NF * FM);
       s_own[e] = g.face_own[ge];
       s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int l = e / ((NB - 1) * DIM), r = e - l * ((NB - 1) * DIM);
       const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
-      s_gr[e] = g.grad[((int64_t)g.gcls[c0 + l] * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)cls_of(g, c0 + l) * NB + i) * 3 + d];
     }
   }
   for (int e = tid; e < ncell * NF; e += nt) {
     s_nb[e] = g.face_nb[c0 * NF + e];
     const int l = e / NF;
-    const int64_t gk = (int64_t)g.gcls[c0 + l] * NF + (e - l * NF);
+    const int64_t gk = (int64_t)cls_of(g, c0 + l) * NF + (e - l * NF);
 #pragma unroll
     for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
   }
@@ -729,7 +735,7 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
   load_row(m0 + 1, pp);
   load_y(m0, hy);
   load_hx(m0, hx);
-  int gc = col && m0 < nm ? g.gcls[cell_at(m0)] : 0;
+  int gc = col && m0 < nm ? cls_of(g, cell_at(m0)) : 0;
   T st = col && m0 < nm ? g.sigt[cell_at(m0)] : T(0), ssg = col && m0 < nm ? g.sigs[cell_at(m0)] : T(0);
   __syncthreads();  // class tables staged
   for (int m = m0, par = 0; m < m1; m++, par ^= 1) {
@@ -738,7 +744,7 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
     load_y(m + 1, hyn);
     load_hx(m + 1, hxn);
     const bool nxt = col && m + 1 < nm;
-    const int gcn = nxt ? g.gcls[cell_at(m + 1)] : 0;
+    const int gcn = nxt ? cls_of(g, cell_at(m + 1)) : 0;
     const T stn = nxt ? g.sigt[cell_at(m + 1)] : T(0), ssn = nxt ? g.sigs[cell_at(m + 1)] : T(0);
     const int64_t c = cell_at(m);
     int fnb[NF];
@@ -965,7 +971,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
     st[k] = v ? g.sigt[c] : T(0);
     ss[k] = v ? g.sigs[c] : T(0);
-    gc[k] = v ? g.gcls[c] : -1;
+    gc[k] = v ? cls_of(g, c) : -1;
   }
   // the upwind row beyond the tile along the register axis
   const int rh = kc.in_side[R] == 0 ? r0 - 1 : r0 + K;
@@ -1189,7 +1195,7 @@ __global__ void __launch_bounds__(256) dg_apply_wave(const T *__restrict__ x, T 
   T psi[APL][NB], acc[APL][NB], om[APL][3];
   bool on[APL];
   const T sig = g.sigt[c];
-  const int gc = g.gcls[c];
+  const int gc = cls_of(g, c);
 #pragma unroll
   for (int q = 0; q < APL; q++) {
     const int a = lane + 64 * q;
@@ -1361,7 +1367,7 @@ __global__ void diag_kernel(T *__restrict__ d, OpArgs<T> g, int is_fd) {
     for (int dd = 0; dd < g.dim; dd++) v += fabs(g.omega[3 * a + dd]) * g.invh[dd];
   } else {
     for (int f = 0; f < g.nf; f++) {
-      const int64_t kg = (int64_t)g.gcls[c] * g.nf + f;
+      const int64_t kg = (int64_t)cls_of(g, c) * g.nf + f;
       T s = T(0);
       for (int dd = 0; dd < g.dim; dd++) s += g.omega[3 * a + dd] * g.face_s[3 * kg + dd];
       if (s > T(0)) v += g.nb == 1 ? s : s * g.face_own[(kg * g.nb + i) * g.nb + i];
@@ -1534,7 +1540,7 @@ __device__ inline bool build_block(const OpArgs<T> &g, int64_t c, int a, int is_
     return true;
   }
   const T ox = dir * g.omega[3 * a], oy = dir * g.omega[3 * a + 1], oz = dir * g.omega[3 * a + 2];
-  const int64_t gc = g.gcls[c];
+  const int64_t gc = cls_of(g, c);
   for (int f = 0; f < g.nf; f++) {
     const int64_t k = gc * g.nf + f;
     const T s = ox * g.face_s[3 * k] + oy * g.face_s[3 * k + 1] + oz * g.face_s[3 * k + 2];
@@ -2047,6 +2053,7 @@ template <class T> OpArgs<T> make_args(const Operator &op, const PrecTables &t) 
   a.face_own = t.face_own.as<T>();
   a.face_up = t.face_up.as<T>();
   a.grad = t.grad.as<T>();
+  a.gcls_identity = op.gcls_identity ? 1 : 0;
   a.cells = op.box_edge.as<int>();
   a.n_list = op.n_box_edge;
   return a;
@@ -2255,6 +2262,8 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
     std::vector<int> cls;
     geometry_classes(D, cls, op->geo_rep);
     op->n_geo_classes = (int)op->geo_rep.size();
+    op->gcls_identity = op->n_geo_classes == D.n_cells;
+    for (int c = 0; c < D.n_cells && op->gcls_identity; c++) op->gcls_identity = cls[c] == c;
     op->gcls.upload(cls);
     op->box = detect_box(D, op->box_n, op->box_fmap);
     if (op->box && D.nb > 1) op->box_cls = box_interior_class(D, cls, op->box_n);

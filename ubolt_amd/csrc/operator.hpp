@@ -27,6 +27,7 @@ template <class T> struct OpArgs {
   // a few KB of geometry instead of a record per cell. Neighbours stay per cell.
   const int *face_nb;  // cells * nf (per cell)
   const int *gcls;     // cells: geometry class
+  int gcls_identity;   // every cell is its own class, numbered as the cell: no lookup
   const T *face_s;     // classes * nf * 3: nA (DG1) or nA / V (DG0)
   const T *face_own;   // classes * nf * nb * nb
   const T *face_up;
@@ -85,6 +86,7 @@ struct Operator {
   DevBuf reflect, face_nb, neg;
   DevBuf gcls;                // int per cell: geometry class (DG)
   int n_geo_classes = 0;
+  bool gcls_identity = false;  // class c is cell c for every cell (a mesh with no repeated geometry)
   std::vector<int> geo_rep;   // representative cell of each class
   // a DG mesh that is a lexicographic box (the pencil kernel's precondition)
   bool box = false;
