@@ -81,6 +81,8 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
   const int64_t c = tile * g.cpb + lc;
   const bool live = lc < g.cpb && c < g.n_cells;
   T psi = T(0), acc = T(0);
+  // the scatter's cross section too is loaded before the barriers
+  const T ssg = SCATTER && live ? g.sigs[c] : T(0);
   if (live) {
     psi = x[c * A + a];
     acc = g.sigt[c] * psi;
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
     __syncthreads();
     seg_sum(red, g.cpb, A, phi, g.red_lanes);
     __syncthreads();
-    if (live) acc -= g.sigs[c] * g.inv_sum_w * (ADJ ? g.w[a] : T(1)) * phi[lc];
+    if (live) acc -= ssg * g.inv_sum_w * (ADJ ? g.w[a] : T(1)) * phi[lc];
   }
   if (live) y[c * A + a] = acc;
 }
@@ -316,8 +318,8 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
 #pragma unroll
   for (int d = 0; d < DIM; d++) om[d] = g.omega[3 * ac + d];
 
-  // 1. global loads: own psi, inflow test, upwind traces
-  T psi[CPT][NB], pu[CPT][NF][NB], sf[CPT][NF];
+  // 1. global loads: own psi, cross sections, inflow test, upwind traces
+  T psi[CPT][NB], pu[CPT][NF][NB], sf[CPT][NF], sig[CPT], ssg[CPT];
 #pragma unroll
   for (int k = 0; k < CPT; k++) {
     const int lc = r + k * R;
@@ -326,6 +328,8 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
     const int64_t base = c * NB * A + ac;
 #pragma unroll
     for (int j = 0; j < NB; j++) psi[k][j] = live ? x[base + (int64_t)j * A] : T(0);
+    sig[k] = g.sigt[c];
+    ssg[k] = SCATTER ? g.sigs[c] : T(0);
     const int64_t gc = cls_of(g, c);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
@@ -378,9 +382,8 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
     if (!lane || lc >= ncell) continue;
     const int64_t c = c0 + lc;
     T acc[NB];
-    const T sig = g.sigt[c];
 #pragma unroll
-    for (int i = 0; i < NB; i++) acc[i] = sig * psi[k][i];
+    for (int i = 0; i < NB; i++) acc[i] = sig[k] * psi[k][i];
     if (NB > 1) {
 #pragma unroll
       for (int i = 1; i < NB; i++) {
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(256) dg_apply_mc(const T *__restrict__ x, T *_
       }
     }
     if (SCATTER) {
-      const T ss = g.sigs[c] * g.inv_sum_w;
+      const T ss = ssg[k] * g.inv_sum_w;
 #pragma unroll
       for (int i = 0; i < NB; i++) acc[i] -= ss * phi[lc * NB + i];
     }
