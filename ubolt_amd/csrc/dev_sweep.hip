@@ -200,8 +200,11 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 // One group's levels l0 .. l1-1 from the records (item i: cell cell[i],
 // angle ang[i]). With PIPE, a thread's item of the next level (its records,
 // r, the upwind cells) is loaded while the current level's upwind traces are
-// in flight, and its cell id two levels ahead, so the barrier-to-barrier path
-// of a level is the trace load, the multiply-adds and the store.
+// in flight, and its cell and angle two levels ahead. Vector memory loads
+// complete in issue order, so the trace loads are issued before that
+// prefetch: waiting for them then leaves the prefetch in flight, and the
+// barrier-to-barrier path of a level is the trace load, the multiply-adds
+// and the store.
 // ADJ: the adjoint sweep L^T = V P L P V^{-1} from the same records: item
 // (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
 // 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
@@ -223,9 +226,10 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     T bi[NB2], h[NFU][NB2], rc[NB];
     T vc, ivn[NFU];  // ADJ: V_c, 1 / V_n per slot
   };
-  auto fetch = [&](int64_t i, int64_t c, Item &it) {
+  // item i = (cell c, angle b) of the schedule
+  auto fetch = [&](int64_t i, int64_t c, int b, Item &it) {
     it.c = c;
-    it.a = ADJ ? neg[ang[i]] : ang[i];
+    it.a = ADJ ? neg[b] : b;
     const int a = it.a;
 #pragma unroll
     for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
@@ -243,14 +247,21 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 #pragma unroll
     for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a] * (ADJ ? T(1) / it.vc : T(1));
   };
-  auto solve = [&](const Item &it) {
-    const int a = it.a;
-    T pu[NFU][NB];
+  // the upwind traces of an item (issued first: the level's critical loads);
+  // an unused slot loads cell 0's row and discards it, so the number of loads
+  // in flight is the same on every path and waits can count past them
+  auto traces = [&](const Item &it, T (&pu)[NFU][NB]) {
 #pragma unroll
-    for (int k = 0; k < NFU; k++)
+    for (int k = 0; k < NFU; k++) {
+      const int64_t n = it.nbk[k] >= 0 ? it.nbk[k] : 0;
 #pragma unroll
-      for (int j = 0; j < NB; j++)
-        pu[k][j] = it.nbk[k] >= 0 ? psi[((int64_t)it.nbk[k] * NB + j) * A + a] * (ADJ ? it.ivn[k] : T(1)) : T(0);
+      for (int j = 0; j < NB; j++) {
+        const T v = psi[(n * NB + j) * A + it.a];
+        pu[k][j] = it.nbk[k] >= 0 ? v : T(0);
+      }
+    }
+  };
+  auto finish = [&](const Item &it, const T (&pu)[NFU][NB]) {
 #pragma unroll
     for (int p = 0; p < NB; p++) {
       T v = T(0);
@@ -259,56 +270,87 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 #pragma unroll
       for (int k = 0; k < NFU; k++)
 #pragma unroll
-        for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * pu[k][q];
-      psi[(it.c * NB + p) * A + a] = ADJ ? it.vc * v : v;
+        for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * (ADJ ? pu[k][q] * it.ivn[k] : pu[k][q]);
+      psi[(it.c * NB + p) * A + it.a] = ADJ ? it.vc * v : v;
     }
+  };
+  auto solve = [&](const Item &it) {
+    T pu[NFU][NB];
+    traces(it, pu);
+    finish(it, pu);
   };
   if (!PIPE) {
     for (int l = l0; l < l1; l++) {
       for (int64_t i = lvl[l] + tid; i < lvl[l + 1]; i += nt) {
         Item it;
-        fetch(i, cell[i], it);
+        fetch(i, cell[i], ang[i], it);
         solve(it);
       }
       __syncthreads();
     }
     return;
   }
-  // pipelined: cur (this level), c2 (the cell of the item two levels ahead)
-  Item cur;
-  bool has = false;
-  if (l0 < l1) {
-    const int64_t i = lvl[l0] + tid;
-    has = i < lvl[l0 + 1];
-    if (has) fetch(i, cell[i], cur);
+  if (l0 >= l1) return;
+  // pipelined: the items of two consecutive levels alternate between x and
+  // y (a copy between them would wait for the whole prefetch); c2, b2: the
+  // cell and angle of the item two levels ahead. Every load of the loop is
+  // unconditional: a thread without an item at some level loads the
+  // group's first item instead and discards it.
+  const int64_t i_any = lvl[l0];
+  Item x, y;
+  bool hx, hy = false;
+  {
+    int64_t i = lvl[l0] + tid;
+    hx = i < lvl[l0 + 1];
+    i = hx ? i : i_any;
+    fetch(i, cell[i], ang[i], x);
   }
-  int64_t c2 = 0;
+  int c2, b2;
   bool has2 = false;
-  if (l0 + 1 < l1) {
-    const int64_t i = lvl[l0 + 1] + tid;
-    has2 = i < lvl[l0 + 2];
-    if (has2) c2 = cell[i];
-  }
-  for (int l = l0; l < l1; l++) {
-    Item nxt;
-    const bool hn = has2;
-    if (hn) fetch(lvl[l + 1] + tid, c2, nxt);
-    has2 = false;
-    if (l + 2 < l1) {
-      const int64_t i = lvl[l + 2] + tid;
-      has2 = i < lvl[l + 3];
-      if (has2) c2 = cell[i];
+  {
+    int64_t i = i_any;
+    if (l0 + 1 < l1) {
+      i = lvl[l0 + 1] + tid;
+      has2 = i < lvl[l0 + 2];
+      i = has2 ? i : i_any;
     }
-    if (has) solve(cur);
+    c2 = cell[i];
+    b2 = ang[i];
+  }
+  // level l from cur, the next level's item loaded into nxt
+  auto level = [&](int l, const Item &cur, bool hc, Item &nxt, bool &hn) {
+    T pu[NFU][NB];
+    traces(cur, pu);
+    hn = has2;
+    {
+      // c2 and b2 are first used here: without the empty asm the compiler
+      // extends them right after their load, and that wait (loads complete
+      // in order) would cover the whole prefetch issued before them
+      int cn = c2, bn = b2;
+      asm volatile("" : "+v"(cn), "+v"(bn));
+      fetch(hn ? lvl[l + 1] + tid : i_any, cn, bn, nxt);
+    }
+    has2 = false;
+    int64_t i2 = i_any;
+    if (l + 2 < l1) {
+      i2 = lvl[l + 2] + tid;
+      has2 = i2 < lvl[l + 3];
+      i2 = has2 ? i2 : i_any;
+    }
+    c2 = cell[i2];
+    b2 = ang[i2];
+    if (hc) finish(cur, pu);
     // items beyond one per thread (a level wider than the workgroup)
     for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
       Item it;
-      fetch(i, cell[i], it);
+      fetch(i, cell[i], ang[i], it);
       solve(it);
     }
     __syncthreads();
-    if (hn) cur = nxt;
-    has = hn;
+  };
+  for (int l = l0; l < l1; l += 2) {
+    level(l, x, hx, y, hy);
+    if (l + 1 < l1) level(l + 1, y, hy, x, hx);
   }
 }
 
