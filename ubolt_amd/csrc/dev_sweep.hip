@@ -55,11 +55,53 @@ namespace {
 // thousands of concurrent directions exhausts the translation caches.
 __host__ __device__ inline int64_t rix(int e, int64_t i, int F) { return ((i >> 6) * F + e) * 64 + (i & 63); }
 __host__ __device__ inline int64_t rec_len(int64_t n_items, int F) { return ((n_items + 63) >> 6) * 64 * F; }
+// The records of a single problem's pipelined sweep group the fields by 16
+// bytes instead (W = 4 floats, 2 doubles): in a block of 64 items, 16-byte
+// group q of item i at ((i / 64) ceil(F / W) + q) 64 + i % 64 (in 16-byte
+// units). A thread reads its record in 16-byte loads, a few instructions
+// rather than one per field, so more levels of records fit the 63 vector
+// memory instructions a wavefront keeps in flight, and a wavefront's load is
+// still 1 KB contiguous. (The plain form over wide levels keeps one field per
+// load: with many wavefronts per level its loads are issue-bound, not
+// latency-bound, and there the grouped loads measured slower.)
+template <class T> __host__ __device__ constexpr int rec_w() { return 16 / (int)sizeof(T); }
+template <class T> __host__ __device__ constexpr int rec_pad(int F) { return (F + rec_w<T>() - 1) / rec_w<T>() * rec_w<T>(); }
+template <class T> __host__ __device__ inline int64_t vix(int e, int64_t i, int F) {
+  constexpr int W = rec_w<T>();
+  return (((i >> 6) * (rec_pad<T>(F) / W) + e / W) * 64 + (i & 63)) * W + e % W;
+}
+template <class T> inline int64_t vrec_len(int64_t n_items, int F) { return ((n_items + 63) >> 6) * 64 * rec_pad<T>(F); }
+// field e of item i (F fields) in either layout
+template <class T> __host__ __device__ inline int64_t rat(bool grouped, int e, int64_t i, int F) {
+  return grouped ? vix<T>(e, i, F) : rix(e, i, F);
+}
+// 16 bytes from p (16-byte aligned) into out[0 .. 16 / sizeof(T))
+__device__ __forceinline__ void load16(const float *p, float *out) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+  out[3] = v.w;
+}
+__device__ __forceinline__ void load16(const double *p, double *out) {
+  const double2 v = *reinterpret_cast<const double2 *>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+}
+__device__ __forceinline__ void load16(const int *p, int *out) {
+  const int4 v = *reinterpret_cast<const int4 *>(p);
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+  out[3] = v.w;
+}
 
+// g64 / g32: r64 / r32 in the 16-byte grouped layout (else one field per
+// load); rnb always one field per load
 template <int DIM, int NB, int NF, int NFU>
 __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i, int64_t n_items, int64_t c, int b,
                                              unsigned char up, double *__restrict__ r64, float *__restrict__ r32,
-                                             int *__restrict__ rnb) {
+                                             int *__restrict__ rnb, bool g64, bool g32) {
   constexpr int NB2 = NB * NB;
   const double om[3] = {g.omega[3 * b], g.omega[3 * b + 1], DIM == 3 ? g.omega[3 * b + 2] : 0.0};
   const int64_t gc = g.gcls[c];
@@ -161,8 +203,8 @@ __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i,
   for (int p = 0; p < NB; p++)
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-      if (r64) r64[rix(p * NB + q, i, NB2 * (1 + NFU))] = Bi[p][q];
-      if (r32) r32[rix(p * NB + q, i, NB2 * (1 + NFU))] = (float)Bi[p][q];
+      if (r64) r64[rat<double>(g64, p * NB + q, i, NB2 * (1 + NFU))] = Bi[p][q];
+      if (r32) r32[rat<float>(g32, p * NB + q, i, NB2 * (1 + NFU))] = (float)Bi[p][q];
     }
 #pragma unroll
   for (int kk = 0; kk < NFU; kk++) {
@@ -174,11 +216,17 @@ __device__ __forceinline__ void sweep_record(const OpArgs<double> &g, int64_t i,
         double h = 0.0;
 #pragma unroll
         for (int m = 0; m < NB; m++) h -= Bi[p][m] * H[kk][m][q];
-        const int64_t e = rix(NB2 + kk * NB2 + p * NB + q, i, NB2 * (1 + NFU));
-        if (r64) r64[e] = h;
-        if (r32) r32[e] = (float)h;
+        const int e = NB2 + kk * NB2 + p * NB + q;
+        if (r64) r64[rat<double>(g64, e, i, NB2 * (1 + NFU))] = h;
+        if (r32) r32[rat<float>(g32, e, i, NB2 * (1 + NFU))] = (float)h;
       }
   }
+  // the padding of the grouped layout, read and discarded by its loads
+  constexpr int F = NB2 * (1 + NFU);
+  for (int e = F; e < rec_pad<double>(F); e++)
+    if (r64 && g64) r64[vix<double>(e, i, F)] = 0.0;
+  for (int e = F; e < rec_pad<float>(F); e++)
+    if (r32 && g32) r32[vix<float>(e, i, F)] = 0.0f;
 }
 
 // The records of every item: blockIdx.y = schedule workgroup w (a direction,
@@ -189,12 +237,12 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
                                                      const int *__restrict__ cell, const unsigned char *__restrict__ up,
                                                      const int64_t *__restrict__ lvl, const int *__restrict__ base,
                                                      int64_t n_items, double *__restrict__ r64, float *__restrict__ r32,
-                                                     int *__restrict__ rnb) {
+                                                     int *__restrict__ rnb, bool g64, bool g32) {
   const int w = blockIdx.y;
   const OpArgs<double> &g = BATCH ? items[wprob[w]].g : g1;
   const int64_t i0 = lvl[base[w]], i1 = lvl[base[w + 1] - 1];
   for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += (int64_t)gridDim.x * blockDim.x)
-    sweep_record<DIM, NB, NF, NFU>(g, i, n_items, cell[i], ang[i], up[i], r64, r32, rnb);
+    sweep_record<DIM, NB, NF, NFU>(g, i, n_items, cell[i], ang[i], up[i], r64, r32, rnb, g64, g32);
 }
 
 // One group's levels l0 .. l1-1 from the records (item i: cell cell[i],
@@ -209,7 +257,7 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 // (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
 // 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
 // note at the top of this file).
-template <class T, int NB, int NFU, bool PIPE, bool ADJ = false>
+template <class T, int NB, int NFU, bool PIPE, bool ADJ, bool GROUPED>
 __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A,
                                               const int *__restrict__ cell, const int *__restrict__ ang,
                                               const T *__restrict__ rec, const int *__restrict__ rnb, int64_t n_items,
@@ -238,12 +286,25 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 #pragma unroll
       for (int k = 0; k < NFU; k++) it.ivn[k] = it.nbk[k] >= 0 ? T(1) / vol[gcls[it.nbk[k]]] : T(0);
     }
+    if (GROUPED) {
+      constexpr int F = NB2 * (1 + NFU), W = rec_w<T>();
+      T v[rec_pad<T>(F)];
 #pragma unroll
-    for (int e = 0; e < NB2; e++) it.bi[e] = rec[rix(e, i, NB2 * (1 + NFU))];
+      for (int q = 0; q < rec_pad<T>(F); q += W) load16(rec + vix<T>(q, i, F), v + q);
 #pragma unroll
-    for (int k = 0; k < NFU; k++)
+      for (int e = 0; e < NB2; e++) it.bi[e] = v[e];
 #pragma unroll
-      for (int e = 0; e < NB2; e++) it.h[k][e] = rec[rix(NB2 + k * NB2 + e, i, NB2 * (1 + NFU))];
+      for (int k = 0; k < NFU; k++)
+#pragma unroll
+        for (int e = 0; e < NB2; e++) it.h[k][e] = v[NB2 + k * NB2 + e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < NB2; e++) it.bi[e] = rec[rix(e, i, NB2 * (1 + NFU))];
+#pragma unroll
+      for (int k = 0; k < NFU; k++)
+#pragma unroll
+        for (int e = 0; e < NB2; e++) it.h[k][e] = rec[rix(NB2 + k * NB2 + e, i, NB2 * (1 + NFU))];
+    }
 #pragma unroll
     for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a] * (ADJ ? T(1) / it.vc : T(1));
   };
@@ -291,54 +352,55 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     return;
   }
   if (l0 >= l1) return;
-  // pipelined: the items of two consecutive levels alternate between x and
-  // y (a copy between them would wait for the whole prefetch); c2, b2: the
-  // cell and angle of the item two levels ahead. Every load of the loop is
-  // unconditional: a thread without an item at some level loads the
-  // group's first item instead and discards it.
+  // pipelined, three levels deep: level l's item is solved while level
+  // l + 1's records are in flight and level l + 2's are requested, and the
+  // cell and angle of level l + 3's item are loaded. The items rotate
+  // between x, y and z (a copy would wait for its loads). Loads complete in
+  // issue order, so each level issues its trace loads first and the cell and
+  // angle before the records they index: waiting for either then leaves the
+  // younger record loads in flight. Every load is unconditional: a thread
+  // without an item at some level loads the group's first item and
+  // discards it, so the count of loads in flight is the same on every path.
   const int64_t i_any = lvl[l0];
-  Item x, y;
-  bool hx, hy = false;
+  // (cell, angle) of this thread's item at level l, or of item i_any
+  auto slot_of = [&](int l, bool &has) {
+    int64_t i = i_any;
+    has = false;
+    if (l < l1) {
+      i = lvl[l] + tid;
+      has = i < lvl[l + 1];
+      i = has ? i : i_any;
+    }
+    return i;
+  };
+  Item x, y, z;
+  bool hx, hy, hz = false;
   {
-    int64_t i = lvl[l0] + tid;
-    hx = i < lvl[l0 + 1];
-    i = hx ? i : i_any;
+    const int64_t i = slot_of(l0, hx);
     fetch(i, cell[i], ang[i], x);
   }
-  int c2, b2;
-  bool has2 = false;
   {
-    int64_t i = i_any;
-    if (l0 + 1 < l1) {
-      i = lvl[l0 + 1] + tid;
-      has2 = i < lvl[l0 + 2];
-      i = has2 ? i : i_any;
-    }
-    c2 = cell[i];
-    b2 = ang[i];
+    const int64_t i = slot_of(l0 + 1, hy);
+    fetch(i, cell[i], ang[i], y);
   }
-  // level l from cur, the next level's item loaded into nxt
+  bool h3;
+  int64_t i3 = slot_of(l0 + 2, h3);
+  int c3 = cell[i3], b3 = ang[i3];
+  // level l from cur; level l + 2's item loaded into nxt
   auto level = [&](int l, const Item &cur, bool hc, Item &nxt, bool &hn) {
     T pu[NFU][NB];
     traces(cur, pu);
-    hn = has2;
-    {
-      // c2 and b2 are first used here: without the empty asm the compiler
-      // extends them right after their load, and that wait (loads complete
-      // in order) would cover the whole prefetch issued before them
-      int cn = c2, bn = b2;
-      asm volatile("" : "+v"(cn), "+v"(bn));
-      fetch(hn ? lvl[l + 1] + tid : i_any, cn, bn, nxt);
-    }
-    has2 = false;
-    int64_t i2 = i_any;
-    if (l + 2 < l1) {
-      i2 = lvl[l + 2] + tid;
-      has2 = i2 < lvl[l + 3];
-      i2 = has2 ? i2 : i_any;
-    }
-    c2 = cell[i2];
-    b2 = ang[i2];
+    int cn = c3, bn = b3;
+    const int64_t in = i3;
+    hn = h3;
+    i3 = slot_of(l + 3, h3);
+    c3 = cell[i3];
+    b3 = ang[i3];
+    // cn and bn are first used here: without the empty asm the compiler
+    // extends them right after their load, and that wait would cover the
+    // record loads issued after them
+    asm volatile("" : "+v"(cn), "+v"(bn));
+    fetch(in, cn, bn, nxt);
     if (hc) finish(cur, pu);
     // items beyond one per thread (a level wider than the workgroup)
     for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
@@ -348,13 +410,14 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     }
     __syncthreads();
   };
-  for (int l = l0; l < l1; l += 2) {
-    level(l, x, hx, y, hy);
+  for (int l = l0; l < l1; l += 3) {
+    level(l, x, hx, z, hz);
     if (l + 1 < l1) level(l + 1, y, hy, x, hx);
+    if (l + 2 < l1) level(l + 2, z, hz, y, hy);
   }
 }
 
-// PIPE holds two items' records in registers: at most 256 threads; the plain
+// PIPE holds three items' records in registers: at most 256 threads; the plain
 // form serves wide levels with up to sweep_max_threads(NB) (one item's
 // records of an NB = 4 hex or tet are 64-96 values: 512 threads keep them in
 // registers, 1024 would spill)
@@ -368,8 +431,8 @@ __global__ void __launch_bounds__(PIPE ? 256 : sweep_max_threads(NB)) sweep_dg_r
                                                                   const int *__restrict__ base, const int *__restrict__ neg,
                                                                   const int *__restrict__ gcls, const T *__restrict__ vol) {
   const int w = blockIdx.x;
-  sweep_rec_run<T, NB, NFU, PIPE, ADJ>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1, neg, gcls,
-                                       vol);
+  sweep_rec_run<T, NB, NFU, PIPE, ADJ, PIPE>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1,
+                                             neg, gcls, vol);
 }
 
 template <class T, int NB, int NFU>
@@ -381,8 +444,8 @@ __global__ void __launch_bounds__(sweep_max_threads(NB)) sweep_dg_rec_batch(cons
                                                            const int64_t *__restrict__ lvl, const int *__restrict__ base) {
   const int w = blockIdx.x;
   const BatchItem<T> &bi = items[wprob[w]];
-  sweep_rec_run<T, NB, NFU, false>(r + bi.off, psi + bi.off, bi.g.A, cell, ang, rec, rnb, n_items, lvl, base[w],
-                                   base[w + 1] - 1);
+  sweep_rec_run<T, NB, NFU, false, false, false>(r + bi.off, psi + bi.off, bi.g.A, cell, ang, rec, rnb, n_items, lvl,
+                                                 base[w], base[w + 1] - 1);
 }
 
 // Structured, persistent per direction: workgroup = direction, levels in one
@@ -562,6 +625,15 @@ void Sweep::create(Operator *o, bool keep_host) {
   nfu = 1;
   for (unsigned char m : item_up) nfu = std::max(nfu, __builtin_popcount(m));
   threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (widest + 63) / 64 * 64));
+  // the pipelined form holds three items' records in registers (the
+  // adjoint's, the larger): levels of at most 256 items and records that
+  // fit; otherwise the plain form with up to sweep_max_threads(nb) threads
+  {
+    const int slots = sweep_slots(nf, nfu), nb = D.nb;
+    auto fits = [&](int words) { return threads <= 256 && 3 * ((nb * nb * (1 + slots) + nb + 1 + slots) * words + slots + 3) <= 300; };
+    pipe64 = fits(2);
+    pipe32 = fits(1);
+  }
   rec_group = -1;
   if (!keep_host) alloc_records();
   cells.upload(item_cell);
@@ -634,7 +706,8 @@ void group_schedule(const Sweep &s, int G, SweepLists &out) {
 static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<double> &g1,
                            const BatchItem<double> *items, const int *wprob, const int *ang, const int *cell,
                            const unsigned char *up, const int64_t *lvl, const int *base, int n_wg, int64_t widest_dir,
-                           int64_t n_items, double *r64, float *r32, int *rnb, bool batch, hipStream_t s) {
+                           int64_t n_items, double *r64, float *r32, int *rnb, bool g64, bool g32, bool batch,
+                           hipStream_t s) {
   const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (widest_dir + 255) / 256));
   const dim3 grid(gx, (unsigned)n_wg);
   bool launched = false;
@@ -642,10 +715,10 @@ static void launch_records(int dim, int nb, int nf, int slots, const OpArgs<doub
   if (!launched && dim == DIM && nb == NB && nf == NF && slots == NFU) {                                             \
     if (batch)                                                                                                       \
       hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, true>), grid, dim3(256), 0, s, g1, items, wprob, ang,       \
-                         cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
+                         cell, up, lvl, base, n_items, r64, r32, rnb, g64, g32);                                     \
     else                                                                                                             \
       hipLaunchKernelGGL((sweep_records<DIM, NB, NF, NFU, false>), grid, dim3(256), 0, s, g1, items, wprob, ang,      \
-                         cell, up, lvl, base, n_items, r64, r32, rnb);                                               \
+                         cell, up, lvl, base, n_items, r64, r32, rnb, g64, g32);                                     \
     launched = true;                                                                                                 \
   }
   RC(2, 1, 3, 2) RC(2, 1, 4, 2) RC(2, 1, 4, 3) RC(3, 1, 4, 2) RC(3, 1, 4, 3) RC(3, 1, 6, 3) RC(3, 1, 6, 5)
@@ -659,8 +732,8 @@ void Sweep::alloc_records() const {
   const Disc &D = *op->disc;
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
   const int slots = sweep_slots(D.nf, nfu), per = D.nb * D.nb * (1 + slots);
-  rec64.alloc(sizeof(double) * rec_len(n_items, per));
-  rec32.alloc(sizeof(float) * rec_len(n_items, per));
+  rec64.alloc(sizeof(double) * (pipe64 ? vrec_len<double>(n_items, per) : rec_len(n_items, per)));
+  rec32.alloc(sizeof(float) * (pipe32 ? vrec_len<float>(n_items, per) : rec_len(n_items, per)));
   rnb.alloc(sizeof(int) * rec_len(n_items, slots));
 }
 
@@ -673,7 +746,7 @@ void Sweep::build_records(hipStream_t s) const {
   launch_records(D.dim, D.nb, D.nf, slots, o.args<double>(), nullptr, nullptr, angle.as<int>(), cells.as<int>(),
                  upwind.as<unsigned char>(), lvl.as<int64_t>(), base.as<int>(), n_groups,
                  (n_items + n_groups - 1) / n_groups, n_items, rec64.as<double>(), rec32.as<float>(), rnb.as<int>(),
-                 false, s);
+                 pipe64, pipe32, false, s);
   rec_group = o.group;
   rec_gen = o.xs_gen;
 }
@@ -721,12 +794,7 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   if (rec_group != o.group || rec_gen != o.xs_gen) build_records(s);
   const int slots = sweep_slots(D.nf, nfu);
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
-  // the pipelined form holds two items' records in registers: levels of at
-  // most 256 items and records that fit; otherwise the plain form with up
-  // to sweep_max_threads(nb) threads
-  const int item_regs =
-      (D.nb * D.nb * (1 + slots) + D.nb + (adj ? 1 + slots : 0)) * (prec == UBT_FP64 ? 2 : 1) + slots + 3;
-  const bool pipe = threads <= 256 && 2 * item_regs <= 200;
+  const bool pipe = prec == UBT_FP64 ? pipe64 : pipe32;
   const int th = std::min(threads, sweep_max_threads(D.nb));
 #define SR_K(T, NB, NFU, PIPE, ADJ, TH)                                                                            \
   hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ>), dim3(grid), dim3(TH), 0, s, (const T *)r, (T *)psi,     \
@@ -763,8 +831,8 @@ void batch_records_launch(const Batch &b, int pi, hipStream_t s) {
   launch_records(b.dim, b.nb, b.nf, w.slots, OpArgs<double>{}, b.items[0][0].as<BatchItem<double>>(),
                  w.prob.as<int>(), w.angle.as<int>(), w.cells.as<int>(), w.upwind.as<unsigned char>(),
                  w.lvl.as<int64_t>(), w.base.as<int>(), w.n_wg, w.widest, w.n_items,
-                 pi == 0 ? w.rec.as<double>() : nullptr, pi == 1 ? w.rec.as<float>() : nullptr, w.rnb.as<int>(), true,
-                 s);
+                 pi == 0 ? w.rec.as<double>() : nullptr, pi == 1 ? w.rec.as<float>() : nullptr, w.rnb.as<int>(), false,
+                 false, true, s);
 }
 
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {

@@ -40,6 +40,9 @@ struct Sweep {
   // the operator's current group and cross sections, stored in both
   // precisions.
   int nfu = 0;                      // inflow slots per item: the most inflow faces of any item
+  // per precision: the pipelined sweep form (narrow levels, records that fit
+  // in registers three deep), whose records use the 16-byte grouped layout
+  bool pipe64 = false, pipe32 = false;
   mutable DevBuf rec64, rec32, rnb;
   mutable int rec_group = -1;
   mutable uint64_t rec_gen = 0;
