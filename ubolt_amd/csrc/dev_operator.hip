@@ -42,10 +42,14 @@ __device__ inline void seg_sum(const T *red, int nseg, int len, T *out, int S) {
   }
 }
 
-// the geometry class of cell c: the cell itself when every cell is its own
-// class (one dependent load fewer on a mesh with no repeated geometry)
-template <class T> __device__ __forceinline__ int64_t cls_of(const OpArgs<T> &g, int64_t c) {
-  return g.gcls_identity ? c : (int64_t)g.gcls[c];
+// the geometry class of cell c: with IDC, the cell itself when every cell is
+// its own class (one dependent load fewer on a mesh with no repeated
+// geometry). The test is a branch around the load, and a load under a branch
+// makes the compiler's waits for the other loads in flight conservative, so
+// kernels for box meshes (whose geometry always repeats) leave IDC off: in the
+// batched box tile the branch cost 30%.
+template <bool IDC = true, class T> __device__ __forceinline__ int64_t cls_of(const OpArgs<T> &g, int64_t c) {
+  return IDC && g.gcls_identity ? c : (int64_t)g.gcls[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -128,8 +132,8 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
 // same kernel with Omega -> -Omega, each neighbour's trace scaled by V_c / V_n,
 // and the scatter transposed.
 // LIST: the tile's cells are entries tile * cpb ... of g.cells (n_list of
-// them) instead of a contiguous run of cell ids.
-template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ, bool LIST = false>
+// them) instead of a contiguous run of cell ids. IDC: see cls_of.
+template <class T, int DIM, int NB, int NF, bool SCATTER, bool ADJ, bool LIST = false, bool IDC = true>
 __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
                                              int64_t tile) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -154,21 +158,21 @@ __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restr
   if (NB > 1) {
     for (int e = tid; e < ncell * NF * FM; e += nt) {
       const int lc = e / (NF * FM);
-      const int64_t ge = (int64_t)cls_of(g, cell_of(lc)) * NF * FM + (e - lc * NF * FM);
+      const int64_t ge = (int64_t)cls_of<IDC>(g, cell_of(lc)) * NF * FM + (e - lc * NF * FM);
       s_own[e] = g.face_own[ge];
       s_up[e] = g.face_up[ge];
     }
     for (int e = tid; e < ncell * (NB - 1) * DIM; e += nt) {
       const int lc = e / ((NB - 1) * DIM), r = e - lc * ((NB - 1) * DIM);
       const int i = 1 + r / DIM, d = r - (i - 1) * DIM;
-      s_gr[e] = g.grad[((int64_t)cls_of(g, cell_of(lc)) * NB + i) * 3 + d];
+      s_gr[e] = g.grad[((int64_t)cls_of<IDC>(g, cell_of(lc)) * NB + i) * 3 + d];
     }
   }
   for (int e = tid; e < ncell * NF; e += nt) {
     const int lc = e / NF;
     const int64_t ce = cell_of(lc);
     s_nb[e] = g.face_nb[ce * NF + (e - lc * NF)];
-    const int64_t gk = (int64_t)cls_of(g, ce) * NF + (e - lc * NF);
+    const int64_t gk = (int64_t)cls_of<IDC>(g, ce) * NF + (e - lc * NF);
 #pragma unroll
     for (int d = 0; d < DIM; d++) s_fs[e * DIM + d] = g.face_s[gk * 3 + d];
   }
@@ -238,7 +242,7 @@ __device__ __forceinline__ void dg_tile_body(const T *__restrict__ x, T *__restr
           ua = g.reflect[3 * a + (-2 - nbk)];
         }
         T pu[NB];
-        const T vr = (ADJ && nbk >= 0) ? g.vol[cls_of(g, c)] / g.vol[cls_of(g, nbk)] : T(1);
+        const T vr = (ADJ && nbk >= 0) ? g.vol[cls_of<IDC>(g, c)] / g.vol[cls_of<IDC>(g, nbk)] : T(1);
 #pragma unroll
         for (int j = 0; j < NB; j++) pu[j] = vr * x[ub + (int64_t)j * A + ua];
         if (NB == 1) {
@@ -738,7 +742,7 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
   load_row(m0 + 1, pp);
   load_y(m0, hy);
   load_hx(m0, hx);
-  int gc = col && m0 < nm ? cls_of(g, cell_at(m0)) : 0;
+  int gc = col && m0 < nm ? cls_of<false>(g, cell_at(m0)) : 0;
   T st = col && m0 < nm ? g.sigt[cell_at(m0)] : T(0), ssg = col && m0 < nm ? g.sigs[cell_at(m0)] : T(0);
   __syncthreads();  // class tables staged
   for (int m = m0, par = 0; m < m1; m++, par ^= 1) {
@@ -747,7 +751,7 @@ __global__ void __launch_bounds__(256) dg_apply_pencil(const T *__restrict__ x, 
     load_y(m + 1, hyn);
     load_hx(m + 1, hxn);
     const bool nxt = col && m + 1 < nm;
-    const int gcn = nxt ? cls_of(g, cell_at(m + 1)) : 0;
+    const int gcn = nxt ? cls_of<false>(g, cell_at(m + 1)) : 0;
     const T stn = nxt ? g.sigt[cell_at(m + 1)] : T(0), ssn = nxt ? g.sigs[cell_at(m + 1)] : T(0);
     const int64_t c = cell_at(m);
     int fnb[NF];
@@ -974,7 +978,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
     st[k] = v ? g.sigt[c] : T(0);
     ss[k] = v ? g.sigs[c] : T(0);
-    gc[k] = v ? cls_of(g, c) : -1;
+    gc[k] = v ? cls_of<false>(g, c) : -1;
   }
   // the upwind row beyond the tile along the register axis
   const int rh = kc.in_side[R] == 0 ? r0 - 1 : r0 + K;
@@ -1103,7 +1107,7 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_box(const T *__restrict_
   const int64_t e0 = rank * ce, e1 = e0 + ce < E ? e0 + ce : E;
   for (int64_t e = e0; e < e1; e++) {
     __syncthreads();  // the LDS changes layout
-    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true>(x, y, g, e);
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x, y, g, e);
   }
 }
 
@@ -1127,7 +1131,7 @@ __global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T
     while (t >= tile_off[p + 1]) p++;
     const BatchItem<T> &it = items[p];
     if (BODY == 0)
-      dg_tile_body<T, DIM, NB, NF, SCATTER, false>(x + it.off, y + it.off, it.g, t - tile_off[p]);
+      dg_tile_body<T, DIM, NB, NF, SCATTER, false, false, false>(x + it.off, y + it.off, it.g, t - tile_off[p]);
     else
       dg_q4_body<T, DIM, NB, NF, SCATTER>(x + it.off, y + it.off, it.g, t - tile_off[p]);
     __syncthreads();  // the next tile restages the LDS
@@ -1165,7 +1169,7 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_box(const T *__res
     while (e >= edge_off[p + 1]) p++;
     __syncthreads();  // the LDS changes layout
     const BatchItem<T> &it = items[p];
-    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true>(x + it.off, y + it.off, it.g, e - edge_off[p]);
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x + it.off, y + it.off, it.g, e - edge_off[p]);
   }
 }
 
