@@ -275,12 +275,11 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     T vc, ivn[NFU];  // ADJ: V_c, 1 / V_n per slot
   };
   // item i = (cell c, angle b) of the schedule
-  auto fetch = [&](int64_t i, int64_t c, int b, Item &it) {
-    it.c = c;
-    it.a = ADJ ? neg[b] : b;
+  // the rest of item i's loads once its cell, angle and upwind cells are
+  // in it
+  auto fetch_rest = [&](int64_t i, Item &it) {
+    const int64_t c = it.c;
     const int a = it.a;
-#pragma unroll
-    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
     if (ADJ) {
       it.vc = vol[gcls[c]];
 #pragma unroll
@@ -307,6 +306,13 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     }
 #pragma unroll
     for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a] * (ADJ ? T(1) / it.vc : T(1));
+  };
+  auto fetch = [&](int64_t i, int64_t c, int b, Item &it) {
+    it.c = c;
+    it.a = ADJ ? neg[b] : b;
+#pragma unroll
+    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
+    fetch_rest(i, it);
   };
   // the upwind traces of an item (issued first: the level's critical loads);
   // an unused slot loads cell 0's row and discards it, so the number of loads
@@ -340,18 +346,56 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     traces(it, pu);
     finish(it, pu);
   };
+  if (l0 >= l1) return;
   if (!PIPE) {
+    // plain form (wide levels): the cell, angle and upwind cells of a
+    // thread's first item of level l + 1 are loaded during level l, after
+    // level l's loads (loads complete in issue order), so that item issues
+    // its trace loads at once, ahead of its records. A thread without such
+    // an item loads the group's first one and discards it.
+    const int64_t i_any = lvl[l0];
+    struct Head {
+      int64_t i;
+      int c, a, nbk[NFU];
+      bool h;
+    } hd;
+    auto head_of = [&](int l) {
+      int64_t i = i_any;
+      hd.h = false;
+      if (l < l1) {
+        i = lvl[l] + tid;
+        hd.h = i < lvl[l + 1];
+        i = hd.h ? i : i_any;
+      }
+      hd.i = i;
+      hd.c = cell[i];
+      hd.a = ADJ ? neg[ang[i]] : ang[i];
+#pragma unroll
+      for (int k = 0; k < NFU; k++) hd.nbk[k] = rnb[rix(k, i, NFU)];
+    };
+    head_of(l0);
     for (int l = l0; l < l1; l++) {
-      for (int64_t i = lvl[l] + tid; i < lvl[l + 1]; i += nt) {
-        Item it;
-        fetch(i, cell[i], ang[i], it);
-        solve(it);
+      Item it;
+      it.c = hd.c;
+      it.a = hd.a;
+#pragma unroll
+      for (int k = 0; k < NFU; k++) it.nbk[k] = hd.nbk[k];
+      const bool h = hd.h;
+      const int64_t i0 = hd.i;
+      T pu[NFU][NB];
+      traces(it, pu);
+      fetch_rest(i0, it);
+      head_of(l + 1);
+      if (h) finish(it, pu);
+      for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
+        Item jt;
+        fetch(i, cell[i], ang[i], jt);
+        solve(jt);
       }
       __syncthreads();
     }
     return;
   }
-  if (l0 >= l1) return;
   // pipelined, three levels deep: level l's item is solved while level
   // l + 1's records are in flight and level l + 2's are requested, and the
   // cell and angle of level l + 3's item are loaded. The items rotate
