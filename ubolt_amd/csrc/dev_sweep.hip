@@ -257,20 +257,28 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 // (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
 // 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
 // note at the top of this file).
-template <class T, int NB, int NFU, bool PIPE, bool ADJ, bool GROUPED>
+// LDSR: upwind traces read from the LDS slots (own_slot / up_slot) rather
+// than from psi in HBM; every item whose trace is read later also writes it
+// to its slot
+template <class T, int NB, int NFU, bool PIPE, bool ADJ, bool GROUPED, bool LDSR = false>
 __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__restrict__ psi, int A,
                                               const int *__restrict__ cell, const int *__restrict__ ang,
                                               const T *__restrict__ rec, const int *__restrict__ rnb, int64_t n_items,
                                               const int64_t *__restrict__ lvl, int l0, int l1,
                                               const int *__restrict__ neg = nullptr,
                                               const int *__restrict__ gcls = nullptr,
-                                              const T *__restrict__ vol = nullptr) {
+                                              const T *__restrict__ vol = nullptr,
+                                              const int *__restrict__ oslot = nullptr,
+                                              const int *__restrict__ uslot = nullptr) {
   constexpr int NB2 = NB * NB;
   const int tid = threadIdx.x, nt = blockDim.x;
+  extern __shared__ __align__(16) unsigned char sweep_lds[];
+  T *ring = reinterpret_cast<T *>(sweep_lds);
   struct Item {
     int64_t c;
     int a;
     int nbk[NFU];
+    int os, us[NFU];  // LDSR: own and upwind trace slots
     T bi[NB2], h[NFU][NB2], rc[NB];
     T vc, ivn[NFU];  // ADJ: V_c, 1 / V_n per slot
   };
@@ -310,8 +318,15 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
   auto fetch = [&](int64_t i, int64_t c, int b, Item &it) {
     it.c = c;
     it.a = ADJ ? neg[b] : b;
+    if (LDSR) {
+      it.os = oslot[i];
 #pragma unroll
-    for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
+      for (int k = 0; k < NFU; k++) it.us[k] = uslot[rix(k, i, NFU)];
+    }
+    if (!LDSR || ADJ) {
+#pragma unroll
+      for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
+    }
     fetch_rest(i, it);
   };
   // the upwind traces of an item (issued first: the level's critical loads);
@@ -320,11 +335,20 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
   auto traces = [&](const Item &it, T (&pu)[NFU][NB]) {
 #pragma unroll
     for (int k = 0; k < NFU; k++) {
-      const int64_t n = it.nbk[k] >= 0 ? it.nbk[k] : 0;
+      if (LDSR) {
+        const int sl = it.us[k] >= 0 ? it.us[k] : 0;
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        const T v = psi[(n * NB + j) * A + it.a];
-        pu[k][j] = it.nbk[k] >= 0 ? v : T(0);
+        for (int j = 0; j < NB; j++) {
+          const T v = ring[sl * NB + j];
+          pu[k][j] = it.us[k] >= 0 ? v : T(0);
+        }
+      } else {
+        const int64_t n = it.nbk[k] >= 0 ? it.nbk[k] : 0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          const T v = psi[(n * NB + j) * A + it.a];
+          pu[k][j] = it.nbk[k] >= 0 ? v : T(0);
+        }
       }
     }
   };
@@ -338,7 +362,9 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
       for (int k = 0; k < NFU; k++)
 #pragma unroll
         for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * (ADJ ? pu[k][q] * it.ivn[k] : pu[k][q]);
-      psi[(it.c * NB + p) * A + it.a] = ADJ ? it.vc * v : v;
+      const T o = ADJ ? it.vc * v : v;
+      psi[(it.c * NB + p) * A + it.a] = o;
+      if (LDSR && it.os >= 0) ring[it.os * NB + p] = o;
     }
   };
   auto solve = [&](const Item &it) {
@@ -467,16 +493,18 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
 // registers, 1024 would spill)
 __host__ __device__ constexpr int sweep_max_threads(int nb) { return nb >= 4 ? 512 : 1024; }
 
-template <class T, int NB, int NFU, bool PIPE, bool ADJ>
+template <class T, int NB, int NFU, bool PIPE, bool ADJ, bool LDSR>
 __global__ void __launch_bounds__(PIPE ? 256 : sweep_max_threads(NB)) sweep_dg_rec(const T *__restrict__ r, T *__restrict__ psi, int A,
                                                                   const int *__restrict__ cell, const int *__restrict__ ang,
                                                                   const T *__restrict__ rec, const int *__restrict__ rnb,
                                                                   int64_t n_items, const int64_t *__restrict__ lvl,
                                                                   const int *__restrict__ base, const int *__restrict__ neg,
-                                                                  const int *__restrict__ gcls, const T *__restrict__ vol) {
+                                                                  const int *__restrict__ gcls, const T *__restrict__ vol,
+                                                                  const int *__restrict__ oslot,
+                                                                  const int *__restrict__ uslot) {
   const int w = blockIdx.x;
-  sweep_rec_run<T, NB, NFU, PIPE, ADJ, PIPE>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w], base[w + 1] - 1,
-                                             neg, gcls, vol);
+  sweep_rec_run<T, NB, NFU, PIPE, ADJ, PIPE, LDSR>(r, psi, A, cell, ang, rec, rnb, n_items, lvl, base[w],
+                                                   base[w + 1] - 1, neg, gcls, vol, oslot, uslot);
 }
 
 template <class T, int NB, int NFU>
@@ -678,6 +706,67 @@ void Sweep::create(Operator *o, bool keep_host) {
     pipe64 = fits(2);
     pipe32 = fits(1);
   }
+  // LDS trace slots for the pipelined form. Per direction, an item whose
+  // trace a later item reads holds an LDS slot from its level up to the
+  // last level that reads it (register allocation over those live ranges:
+  // a slot freed by a trace last read at level L - 1 is reused from level L,
+  // after the barrier that ends L - 1). Its readers get that slot per inflow
+  // slot. The sweep then reads every upwind trace from LDS: no trace load
+  // waits behind the record loads in flight (vector memory loads complete in
+  // issue order). Unused when the slots would not fit kLdsTraceBytes.
+  trace_slots = 0;
+  if (!keep_host && (pipe64 || pipe32)) {
+    const int slots = sweep_slots(nf, nfu);
+    std::vector<int> own((size_t)n_items, -1), ups((size_t)rec_len(n_items, slots), -1);
+    std::vector<int> pos((size_t)nc, -1), last;
+    std::vector<std::vector<int>> release;
+    std::vector<int> free_slots;
+    int most = 0;
+    for (int64_t s0 = 0; s0 < n_items;) {
+      int64_t s1 = s0;
+      while (s1 < n_items && item_angle[s1] == item_angle[s0]) s1++;
+      for (int64_t i = s0; i < s1; i++) pos[item_cell[i]] = (int)(i - s0);
+      last.assign((size_t)(s1 - s0), -1);
+      auto upwind_of = [&](int64_t i, auto &&fn) {
+        const int c = item_cell[i];
+        for (int f = 0, k = 0; f < nf; f++)
+          if ((item_up[i] >> f) & 1) fn(k++, pos[D.face_nb[(int64_t)c * nf + f]]);
+      };
+      for (int64_t i = s0; i < s1; i++)
+        upwind_of(i, [&](int, int j) { last[j] = std::max<int>(last[j], (int)item_level[i]); });
+      const int n_lv = (int)item_level[s1 - 1] + 2;
+      release.assign((size_t)n_lv + 1, {});
+      free_slots.clear();
+      int used = 0, cur = -1;
+      for (int64_t i = s0; i < s1; i++) {
+        const int L = (int)item_level[i];
+        if (L != cur) {
+          for (int l = cur + 1; l <= L; l++)
+            for (int sl : release[l]) free_slots.push_back(sl);
+          cur = L;
+        }
+        const int lu = last[i - s0];
+        if (lu < 0) continue;
+        int sl;
+        if (free_slots.empty()) {
+          sl = used++;
+        } else {
+          sl = free_slots.back();
+          free_slots.pop_back();
+        }
+        own[i] = sl;
+        release[lu + 1].push_back(sl);
+      }
+      for (int64_t i = s0; i < s1; i++) upwind_of(i, [&](int k, int j) { ups[rix(k, i, slots)] = own[s0 + j]; });
+      most = std::max(most, used);
+      s0 = s1;
+    }
+    if ((int64_t)most * D.nb * (int64_t)sizeof(double) <= kLdsTraceBytes) {
+      trace_slots = std::max(most, 1);
+      own_slot.upload(own);
+      up_slot.upload(ups);
+    }
+  }
   rec_group = -1;
   if (!keep_host) alloc_records();
   cells.upload(item_cell);
@@ -839,10 +928,18 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   const int slots = sweep_slots(D.nf, nfu);
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
   const bool pipe = prec == UBT_FP64 ? pipe64 : pipe32;
+  const bool lds = trace_slots > 0;
   const int th = std::min(threads, sweep_max_threads(D.nb));
 #define SR_K(T, NB, NFU, PIPE, ADJ, TH)                                                                            \
-  hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ>), dim3(grid), dim3(TH), 0, s, (const T *)r, (T *)psi,     \
-                     D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs, o.neg.as<int>(), o.gcls.as<int>(), vl);
+  if (lds && PIPE)                                                                                                  \
+    hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ, PIPE>), dim3(grid), dim3(TH),                            \
+                       (size_t)trace_slots * NB * sizeof(T), s, (const T *)r, (T *)psi, D.n_angles, ci, an, rc,       \
+                       rnb.as<int>(), n_items, lv, bs, o.neg.as<int>(), o.gcls.as<int>(), vl, own_slot.as<int>(),     \
+                       up_slot.as<int>());                                                                           \
+  else                                                                                                              \
+    hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ, false>), dim3(grid), dim3(TH), 0, s, (const T *)r,      \
+                       (T *)psi, D.n_angles, ci, an, rc, rnb.as<int>(), n_items, lv, bs, o.neg.as<int>(),            \
+                       o.gcls.as<int>(), vl, nullptr, nullptr);
 #define SR(T, NB, NFU)                                                                                              \
   if (!launched && D.nb == NB && slots == NFU) {                                                                    \
     const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \

@@ -43,6 +43,13 @@ struct Sweep {
   // per precision: the pipelined sweep form (narrow levels, records that fit
   // in registers three deep), whose records use the 16-byte grouped layout
   bool pipe64 = false, pipe32 = false;
+  // the pipelined form's LDS trace slots: per item its own slot (-1: no
+  // later item reads it) and, per inflow slot, the upwind item's slot (in
+  // the one-field-per-load record layout); trace_slots = the most any
+  // direction holds at once (0: traces read from HBM)
+  static constexpr int64_t kLdsTraceBytes = 64 * 1024;
+  int trace_slots = 0;
+  DevBuf own_slot, up_slot;
   mutable DevBuf rec64, rec32, rnb;
   mutable int rec_group = -1;
   mutable uint64_t rec_gen = 0;
