@@ -422,50 +422,73 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     }
     return;
   }
-  // pipelined, three levels deep: level l's item is solved while level
-  // l + 1's records are in flight and level l + 2's are requested, and the
-  // cell and angle of level l + 3's item are loaded. The items rotate
-  // between x, y and z (a copy would wait for its loads). Loads complete in
-  // issue order, so each level issues its trace loads first and the cell and
-  // angle before the records they index: waiting for either then leaves the
-  // younger record loads in flight. Every load is unconditional: a thread
-  // without an item at some level loads the group's first item and
-  // discards it, so the count of loads in flight is the same on every path.
+  // pipelined DEPTH levels deep: level l's item is solved while the records
+  // of levels l + 1 .. l + DEPTH - 2 are in flight and level l + DEPTH - 1's
+  // are requested. The items rotate through DEPTH register sets (a copy would
+  // wait for its loads). Loads complete in issue order, so the cell and
+  // angle that index a fetch are loaded before the records of an earlier
+  // level: waiting for them then leaves the younger record loads in flight
+  // (and without LDS slots, each level issues its trace loads first). Every
+  // load is unconditional: a thread without an item at some level loads the
+  // group's first item and discards it, so the count of loads in flight is
+  // the same on every path. DEPTH is 4 where three items' loads fit the 63 a
+  // wavefront can have in flight (FP32 records of at most 36 values: 9
+  // grouped loads), else 3. With DEPTH 4 the cell and angle are loaded two
+  // levels before their fetch (alternating between two sets), so every
+  // record load has three levels to arrive; with DEPTH 3, one level before,
+  // and two levels to arrive.
+  constexpr int DEPTH = sizeof(T) == 4 && NB2 * (1 + NFU) <= 36 ? 4 : 3;
   const int64_t i_any = lvl[l0];
-  // (cell, angle) of this thread's item at level l, or of item i_any
-  auto slot_of = [&](int l, bool &has) {
-    int64_t i = i_any;
-    has = false;
-    if (l < l1) {
-      i = lvl[l] + tid;
-      has = i < lvl[l + 1];
-      i = has ? i : i_any;
-    }
-    return i;
+  // this thread's item at level l (or item i_any) and its cell and angle
+  struct Slot {
+    int64_t i;
+    int c, b;
+    bool h;
   };
-  Item x, y, z;
-  bool hx, hy, hz = false;
-  {
-    const int64_t i = slot_of(l0, hx);
-    fetch(i, cell[i], ang[i], x);
-  }
-  {
-    const int64_t i = slot_of(l0 + 1, hy);
-    fetch(i, cell[i], ang[i], y);
-  }
-  bool h3;
-  int64_t i3 = slot_of(l0 + 2, h3);
-  int c3 = cell[i3], b3 = ang[i3];
-  // level l from cur; level l + 2's item loaded into nxt
-  auto level = [&](int l, const Item &cur, bool hc, Item &nxt, bool &hn) {
+  auto slot_of = [&](int l, Slot &sl) {
+    int64_t i = i_any;
+    sl.h = false;
+    if (l >= l0 && l < l1) {
+      i = lvl[l] + tid;
+      sl.h = i < lvl[l + 1];
+      i = sl.h ? i : i_any;
+    }
+    sl.i = i;
+    sl.c = cell[i];
+    sl.b = ang[i];
+  };
+  // The pipeline fills in DEPTH - 1 levels before l0 that solve nothing
+  // (items with no upwind slot), inside the loop: fetches issued before it
+  // would reach the loop's first level with a different count of loads in
+  // flight than its back edge brings, and the compiler would then wait for
+  // all of them there on every trip.
+  Item x, y, z, u;
+  auto none = [](Item &it) {
+    it.c = 0;
+    it.a = 0;
+    it.os = -1;
+#pragma unroll
+    for (int k = 0; k < NFU; k++) it.nbk[k] = it.us[k] = -1;
+  };
+  none(x);
+  none(y);
+  none(z);
+  none(u);
+  bool hx = false, hy = false, hz = false, hu = false;
+  Slot sa, sb;
+  slot_of(l0, sa);
+  if (DEPTH == 4) slot_of(l0 + 1, sb);
+  __builtin_amdgcn_s_waitcnt(0);
+  // level l from cur; level l + DEPTH - 1's item, whose slot is sl, loaded
+  // into nxt; sl then reloaded for the fetch of level l + 2 (DEPTH 4, sl
+  // alternates) or l + 1 (DEPTH 3)
+  auto level = [&](int l, const Item &cur, bool hc, Item &nxt, bool &hn, Slot &sl) {
     T pu[NFU][NB];
     traces(cur, pu);
-    int cn = c3, bn = b3;
-    const int64_t in = i3;
-    hn = h3;
-    i3 = slot_of(l + 3, h3);
-    c3 = cell[i3];
-    b3 = ang[i3];
+    int cn = sl.c, bn = sl.b;
+    const int64_t in = sl.i;
+    hn = sl.h;
+    slot_of(l + (DEPTH == 4 ? DEPTH + 1 : DEPTH), sl);
     // cn and bn are first used here: without the empty asm the compiler
     // extends them right after their load, and that wait would cover the
     // record loads issued after them
@@ -473,21 +496,31 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     fetch(in, cn, bn, nxt);
     if (hc) finish(cur, pu);
     // items beyond one per thread (a level wider than the workgroup)
-    for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
-      Item it;
-      fetch(i, cell[i], ang[i], it);
-      solve(it);
-    }
+    if (l >= l0)
+      for (int64_t i = lvl[l] + tid + nt; i < lvl[l + 1]; i += nt) {
+        Item it;
+        fetch(i, cell[i], ang[i], it);
+        solve(it);
+      }
     __syncthreads();
   };
-  for (int l = l0; l < l1; l += 3) {
-    level(l, x, hx, z, hz);
-    if (l + 1 < l1) level(l + 1, y, hy, x, hx);
-    if (l + 2 < l1) level(l + 2, z, hz, y, hy);
+  if (DEPTH == 4) {
+    for (int l = l0 - 3; l < l1; l += 4) {
+      level(l, x, hx, u, hu, sa);
+      if (l + 1 < l1) level(l + 1, y, hy, x, hx, sb);
+      if (l + 2 < l1) level(l + 2, z, hz, y, hy, sa);
+      if (l + 3 < l1) level(l + 3, u, hu, z, hz, sb);
+    }
+  } else {
+    for (int l = l0 - 2; l < l1; l += 3) {
+      level(l, x, hx, z, hz, sa);
+      if (l + 1 < l1) level(l + 1, y, hy, x, hx, sa);
+      if (l + 2 < l1) level(l + 2, z, hz, y, hy, sa);
+    }
   }
 }
 
-// PIPE holds three items' records in registers: at most 256 threads; the plain
+// PIPE holds three or four items' records in registers: at most 256 threads; the plain
 // form serves wide levels with up to sweep_max_threads(NB) (one item's
 // records of an NB = 4 hex or tet are 64-96 values: 512 threads keep them in
 // registers, 1024 would spill)
