@@ -382,7 +382,7 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     const int64_t i_any = lvl[l0];
     struct Head {
       int64_t i;
-      int c, a, nbk[NFU];
+      int c, a, nbk[NFU], os, us[NFU];
       bool h;
     } hd;
     auto head_of = [&](int l) {
@@ -396,16 +396,27 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
       hd.i = i;
       hd.c = cell[i];
       hd.a = ADJ ? neg[ang[i]] : ang[i];
+      if (LDSR) {
+        hd.os = oslot[i];
 #pragma unroll
-      for (int k = 0; k < NFU; k++) hd.nbk[k] = rnb[rix(k, i, NFU)];
+        for (int k = 0; k < NFU; k++) hd.us[k] = uslot[rix(k, i, NFU)];
+      }
+      if (!LDSR || ADJ) {
+#pragma unroll
+        for (int k = 0; k < NFU; k++) hd.nbk[k] = rnb[rix(k, i, NFU)];
+      }
     };
     head_of(l0);
     for (int l = l0; l < l1; l++) {
       Item it;
       it.c = hd.c;
       it.a = hd.a;
+      it.os = hd.os;
 #pragma unroll
-      for (int k = 0; k < NFU; k++) it.nbk[k] = hd.nbk[k];
+      for (int k = 0; k < NFU; k++) {
+        it.nbk[k] = hd.nbk[k];
+        it.us[k] = hd.us[k];
+      }
       const bool h = hd.h;
       const int64_t i0 = hd.i;
       T pu[NFU][NB];
@@ -540,6 +551,8 @@ __global__ void __launch_bounds__(PIPE ? 256 : sweep_max_threads(NB)) sweep_dg_r
                                                    base[w + 1] - 1, neg, gcls, vol, oslot, uslot);
 }
 
+// (traces from HBM: LDS trace slots measured slower here, the LDS per
+// workgroup costing the occupancy this traffic-bound sweep needs)
 template <class T, int NB, int NFU>
 __global__ void __launch_bounds__(sweep_max_threads(NB)) sweep_dg_rec_batch(const T *__restrict__ r, T *__restrict__ psi,
                                                            const BatchItem<T> *__restrict__ items,
@@ -602,6 +615,61 @@ int sweep_slots(int nf, int nfu) {
   if (nf == 3) return 2;
   if (nf == 4) return nfu <= 2 ? 2 : 3;
   return nfu <= 3 ? 3 : 5;
+}
+
+// LDS trace slots of the workgroups of one problem's schedule (items cell /
+// angle / up in level order, levels lvl[base[g] .. base[g + 1]] of workgroup
+// g, item indices local to the list; the list's first item is i0 in own /
+// ups). Within a workgroup an item whose trace a later item reads holds a
+// slot from its level up to the last level that reads it (register
+// allocation over those live ranges: a slot freed by a trace last read at
+// level L - 1 is reused from level L, after the barrier that ends L - 1);
+// ups[rix(k, i0 + i, slots)] = the slot of item i's k-th upwind item. A
+// direction depends only on itself, so every upwind item is in the reader's
+// workgroup. Returns the most slots any workgroup holds at once.
+int assign_trace_slots(const Disc &D, int slots, const int *cell, const int *angle, const unsigned char *up,
+                       const int64_t *lvl, const int *base, int n_wg, int64_t i0, int *own, int *ups) {
+  const int nf = D.nf, A = D.n_angles;
+  std::vector<int> pos((size_t)D.n_cells * A, -1), last, free_slots;
+  std::vector<std::vector<int>> release;
+  auto upwind_of = [&](int64_t i, auto &&fn) {
+    const int c = cell[i];
+    for (int f = 0, k = 0; f < nf; f++)
+      if ((up[i] >> f) & 1) fn(k++, pos[(size_t)D.face_nb[(int64_t)c * nf + f] * A + angle[i]]);
+  };
+  int most = 0;
+  for (int g = 0; g < n_wg; g++) {
+    const int n_lv = base[g + 1] - 1 - base[g];
+    const int64_t a0 = lvl[base[g]], a1 = lvl[base[g + 1] - 1];
+    for (int64_t i = a0; i < a1; i++) pos[(size_t)cell[i] * A + angle[i]] = (int)i;
+    last.assign((size_t)(a1 - a0), -1);
+    for (int l = 0; l < n_lv; l++)
+      for (int64_t i = lvl[base[g] + l]; i < lvl[base[g] + l + 1]; i++)
+        upwind_of(i, [&](int, int j) { last[j - a0] = std::max(last[j - a0], l); });
+    release.assign((size_t)n_lv + 2, {});
+    free_slots.clear();
+    int used = 0;
+    for (int l = 0; l < n_lv; l++) {
+      for (int sl : release[l]) free_slots.push_back(sl);
+      for (int64_t i = lvl[base[g] + l]; i < lvl[base[g] + l + 1]; i++) {
+        const int lu = last[i - a0];
+        if (lu < 0) continue;
+        int sl;
+        if (free_slots.empty()) {
+          sl = used++;
+        } else {
+          sl = free_slots.back();
+          free_slots.pop_back();
+        }
+        own[i0 + i] = sl;
+        release[lu + 1].push_back(sl);
+      }
+    }
+    for (int64_t i = a0; i < a1; i++)
+      upwind_of(i, [&](int k, int j) { ups[rix(k, i0 + i, slots)] = own[i0 + j]; });
+    most = std::max(most, used);
+  }
+  return most;
 }
 
 void Sweep::create(Operator *o, bool keep_host) {
@@ -739,62 +807,17 @@ void Sweep::create(Operator *o, bool keep_host) {
     pipe64 = fits(2);
     pipe32 = fits(1);
   }
-  // LDS trace slots for the pipelined form. Per direction, an item whose
-  // trace a later item reads holds an LDS slot from its level up to the
-  // last level that reads it (register allocation over those live ranges:
-  // a slot freed by a trace last read at level L - 1 is reused from level L,
-  // after the barrier that ends L - 1). Its readers get that slot per inflow
-  // slot. The sweep then reads every upwind trace from LDS: no trace load
-  // waits behind the record loads in flight (vector memory loads complete in
-  // issue order). Unused when the slots would not fit kLdsTraceBytes.
+  // LDS trace slots (see assign_trace_slots): the sweep then reads every
+  // upwind trace from LDS, and no trace read waits behind the record loads
+  // in flight (vector memory loads complete in issue order). A precision
+  // whose slots would not fit kLdsTraceBytes reads the traces from HBM.
   trace_slots = 0;
-  if (!keep_host && (pipe64 || pipe32)) {
+  if (!keep_host) {
     const int slots = sweep_slots(nf, nfu);
     std::vector<int> own((size_t)n_items, -1), ups((size_t)rec_len(n_items, slots), -1);
-    std::vector<int> pos((size_t)nc, -1), last;
-    std::vector<std::vector<int>> release;
-    std::vector<int> free_slots;
-    int most = 0;
-    for (int64_t s0 = 0; s0 < n_items;) {
-      int64_t s1 = s0;
-      while (s1 < n_items && item_angle[s1] == item_angle[s0]) s1++;
-      for (int64_t i = s0; i < s1; i++) pos[item_cell[i]] = (int)(i - s0);
-      last.assign((size_t)(s1 - s0), -1);
-      auto upwind_of = [&](int64_t i, auto &&fn) {
-        const int c = item_cell[i];
-        for (int f = 0, k = 0; f < nf; f++)
-          if ((item_up[i] >> f) & 1) fn(k++, pos[D.face_nb[(int64_t)c * nf + f]]);
-      };
-      for (int64_t i = s0; i < s1; i++)
-        upwind_of(i, [&](int, int j) { last[j] = std::max<int>(last[j], (int)item_level[i]); });
-      const int n_lv = (int)item_level[s1 - 1] + 2;
-      release.assign((size_t)n_lv + 1, {});
-      free_slots.clear();
-      int used = 0, cur = -1;
-      for (int64_t i = s0; i < s1; i++) {
-        const int L = (int)item_level[i];
-        if (L != cur) {
-          for (int l = cur + 1; l <= L; l++)
-            for (int sl : release[l]) free_slots.push_back(sl);
-          cur = L;
-        }
-        const int lu = last[i - s0];
-        if (lu < 0) continue;
-        int sl;
-        if (free_slots.empty()) {
-          sl = used++;
-        } else {
-          sl = free_slots.back();
-          free_slots.pop_back();
-        }
-        own[i] = sl;
-        release[lu + 1].push_back(sl);
-      }
-      for (int64_t i = s0; i < s1; i++) upwind_of(i, [&](int k, int j) { ups[rix(k, i, slots)] = own[s0 + j]; });
-      most = std::max(most, used);
-      s0 = s1;
-    }
-    if ((int64_t)most * D.nb * (int64_t)sizeof(double) <= kLdsTraceBytes) {
+    const int most = assign_trace_slots(D, slots, item_cell.data(), item_angle.data(), item_up.data(), alvl.data(),
+                                        abase.data(), A, 0, own.data(), ups.data());
+    if ((int64_t)most * D.nb * (int64_t)sizeof(float) <= kLdsTraceBytes) {
       trace_slots = std::max(most, 1);
       own_slot.upload(own);
       up_slot.upload(ups);
@@ -961,11 +984,12 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
   const int slots = sweep_slots(D.nf, nfu);
   const int64_t n_items = (int64_t)D.n_cells * D.n_angles;
   const bool pipe = prec == UBT_FP64 ? pipe64 : pipe32;
-  const bool lds = trace_slots > 0;
+  const bool lds =
+      trace_slots > 0 && (int64_t)trace_slots * D.nb * (prec == UBT_FP64 ? 8 : 4) <= kLdsTraceBytes;
   const int th = std::min(threads, sweep_max_threads(D.nb));
 #define SR_K(T, NB, NFU, PIPE, ADJ, TH)                                                                            \
-  if (lds && PIPE)                                                                                                  \
-    hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ, PIPE>), dim3(grid), dim3(TH),                            \
+  if (lds)                                                                                                          \
+    hipLaunchKernelGGL((sweep_dg_rec<T, NB, NFU, PIPE, ADJ, true>), dim3(grid), dim3(TH),                            \
                        (size_t)trace_slots * NB * sizeof(T), s, (const T *)r, (T *)psi, D.n_angles, ci, an, rc,       \
                        rnb.as<int>(), n_items, lv, bs, o.neg.as<int>(), o.gcls.as<int>(), vl, own_slot.as<int>(),     \
                        up_slot.as<int>());                                                                           \

@@ -19,6 +19,11 @@ struct SweepLists {
 // the schedule of s (kept on the host) regrouped into workgroups of up to G
 // directions of one octant (box meshes; otherwise one direction each)
 void group_schedule(const Sweep &s, int G, SweepLists &out);
+// LDS trace slots of a schedule's workgroups (one problem's disc D; items
+// and level boundaries local to the list, written at i0 in own / ups); the
+// most slots any workgroup holds at once
+int assign_trace_slots(const Disc &D, int slots, const int *cell, const int *angle, const unsigned char *up,
+                       const int64_t *lvl, const int *base, int n_wg, int64_t i0, int *own, int *ups);
 
 struct Sweep {
   Operator *op = nullptr;
@@ -43,10 +48,11 @@ struct Sweep {
   // per precision: the pipelined sweep form (narrow levels, records that fit
   // in registers three deep), whose records use the 16-byte grouped layout
   bool pipe64 = false, pipe32 = false;
-  // the pipelined form's LDS trace slots: per item its own slot (-1: no
-  // later item reads it) and, per inflow slot, the upwind item's slot (in
-  // the one-field-per-load record layout); trace_slots = the most any
-  // direction holds at once (0: traces read from HBM)
+  // LDS trace slots: per item its own slot (-1: no later item reads it)
+  // and, per inflow slot, the upwind item's slot (in the one-field-per-load
+  // record layout); trace_slots = the most any direction holds at once (0:
+  // traces read from HBM; a precision uses them where they fit
+  // kLdsTraceBytes)
   static constexpr int64_t kLdsTraceBytes = 64 * 1024;
   int trace_slots = 0;
   DevBuf own_slot, up_slot;
