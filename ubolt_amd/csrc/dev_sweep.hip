@@ -256,7 +256,9 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 // ADJ: the adjoint sweep L^T = V P L P V^{-1} from the same records: item
 // (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
 // 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
-// note at the top of this file).
+// note at the top of this file). For ADJ, ang holds each item's reversed
+// angle and vol each cell's volume, so no load of the fetch depends on
+// another (each such dependence is a wait for every load issued before).
 // LDSR: upwind traces read from the LDS slots (own_slot / up_slot) rather
 // than from psi in HBM; every item whose trace is read later also writes it
 // to its slot
@@ -289,9 +291,11 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     const int64_t c = it.c;
     const int a = it.a;
     if (ADJ) {
-      it.vc = vol[gcls[c]];
+      it.vc = vol[c];
+      if (!LDSR) {
 #pragma unroll
-      for (int k = 0; k < NFU; k++) it.ivn[k] = it.nbk[k] >= 0 ? T(1) / vol[gcls[it.nbk[k]]] : T(0);
+        for (int k = 0; k < NFU; k++) it.ivn[k] = it.nbk[k] >= 0 ? T(1) / vol[it.nbk[k]] : T(0);
+      }
     }
     if (GROUPED) {
       constexpr int F = NB2 * (1 + NFU), W = rec_w<T>();
@@ -313,17 +317,17 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
         for (int e = 0; e < NB2; e++) it.h[k][e] = rec[rix(NB2 + k * NB2 + e, i, NB2 * (1 + NFU))];
     }
 #pragma unroll
-    for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a] * (ADJ ? T(1) / it.vc : T(1));
+    for (int j = 0; j < NB; j++) it.rc[j] = r[(c * NB + j) * A + a];
   };
   auto fetch = [&](int64_t i, int64_t c, int b, Item &it) {
     it.c = c;
-    it.a = ADJ ? neg[b] : b;
+    it.a = b;
     if (LDSR) {
       it.os = oslot[i];
 #pragma unroll
       for (int k = 0; k < NFU; k++) it.us[k] = uslot[rix(k, i, NFU)];
     }
-    if (!LDSR || ADJ) {
+    if (!LDSR) {
 #pragma unroll
       for (int k = 0; k < NFU; k++) it.nbk[k] = rnb[rix(k, i, NFU)];
     }
@@ -357,14 +361,16 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
     for (int p = 0; p < NB; p++) {
       T v = T(0);
 #pragma unroll
-      for (int q = 0; q < NB; q++) v += it.bi[p * NB + q] * it.rc[q];
+      for (int q = 0; q < NB; q++) v += it.bi[p * NB + q] * (ADJ ? it.rc[q] * (T(1) / it.vc) : it.rc[q]);
 #pragma unroll
       for (int k = 0; k < NFU; k++)
 #pragma unroll
-        for (int q = 0; q < NB; q++) v += it.h[k][p * NB + q] * (ADJ ? pu[k][q] * it.ivn[k] : pu[k][q]);
-      const T o = ADJ ? it.vc * v : v;
-      psi[(it.c * NB + p) * A + it.a] = o;
-      if (LDSR && it.os >= 0) ring[it.os * NB + p] = o;
+        for (int q = 0; q < NB; q++)
+          v += it.h[k][p * NB + q] * (ADJ && !LDSR ? pu[k][q] * it.ivn[k] : pu[k][q]);
+      psi[(it.c * NB + p) * A + it.a] = ADJ ? it.vc * v : v;
+      // the adjoint's slot holds the trace before the volume scaling: the
+      // reader's 1 / V_n is then already applied
+      if (LDSR && it.os >= 0) ring[it.os * NB + p] = v;
     }
   };
   auto solve = [&](const Item &it) {
@@ -395,13 +401,13 @@ __device__ __forceinline__ void sweep_rec_run(const T *__restrict__ r, T *__rest
       }
       hd.i = i;
       hd.c = cell[i];
-      hd.a = ADJ ? neg[ang[i]] : ang[i];
+      hd.a = ang[i];
       if (LDSR) {
         hd.os = oslot[i];
 #pragma unroll
         for (int k = 0; k < NFU; k++) hd.us[k] = uslot[rix(k, i, NFU)];
       }
-      if (!LDSR || ADJ) {
+      if (!LDSR) {
 #pragma unroll
         for (int k = 0; k < NFU; k++) hd.nbk[k] = rnb[rix(k, i, NFU)];
       }
@@ -564,6 +570,13 @@ __global__ void __launch_bounds__(sweep_max_threads(NB)) sweep_dg_rec_batch(cons
   const BatchItem<T> &bi = items[wprob[w]];
   sweep_rec_run<T, NB, NFU, false, false, false>(r + bi.off, psi + bi.off, bi.g.A, cell, ang, rec, rnb, n_items, lvl,
                                                  base[w], base[w + 1] - 1);
+}
+
+// out[c] = vol[gcls[c]]: the class volumes per cell
+template <class T>
+__global__ void k_cell_volumes(int n, const int *__restrict__ gcls, const T *__restrict__ vol, T *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) out[c] = vol[gcls[c]];
 }
 
 // Structured, persistent per direction: workgroup = direction, levels in one
@@ -825,6 +838,26 @@ void Sweep::create(Operator *o, bool keep_host) {
   }
   rec_group = -1;
   if (!keep_host) alloc_records();
+  {
+    // the adjoint sweep's per-item reversed angle and per-cell volumes
+    std::vector<int> neg(A), adj((size_t)n_items);
+    for (int a = 0; a < A; a++) {
+      int b = a;
+      for (int d = 0; d < D.dim; d++) b = D.quad.reflect[d][b];
+      neg[a] = b;
+    }
+    for (int64_t i = 0; i < n_items; i++) adj[i] = neg[item_angle[i]];
+    angle_adj.upload(adj);
+    vcell64.alloc(sizeof(double) * nc);
+    vcell32.alloc(sizeof(float) * nc);
+    const unsigned g = (unsigned)((nc + 255) / 256);
+    hipLaunchKernelGGL(k_cell_volumes<double>, dim3(g), dim3(256), 0, nullptr, nc, o->gcls.as<int>(),
+                       o->t64.vol.as<double>(), vcell64.as<double>());
+    hipLaunchKernelGGL(k_cell_volumes<float>, dim3(g), dim3(256), 0, nullptr, nc, o->gcls.as<int>(),
+                       o->t32.vol.as<float>(), vcell32.as<float>());
+    UBT_HIP(hipGetLastError());
+    UBT_HIP(hipStreamSynchronize(nullptr));
+  }
   cells.upload(item_cell);
   angle.upload(item_angle);
   upwind.upload(item_up);
@@ -974,7 +1007,7 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
     UBT_HIP(hipGetLastError());
     return;
   }
-  const int *ci = cells.as<int>(), *bs = base.as<int>(), *an = angle.as<int>();
+  const int *ci = cells.as<int>(), *bs = base.as<int>(), *an = (adj ? angle_adj : angle).as<int>();
   const int64_t *lv = lvl.as<int64_t>();
   const unsigned grid = (unsigned)n_groups;
   bool launched = false;
@@ -1000,7 +1033,7 @@ void Sweep::apply(int prec, const void *r, void *psi, hipStream_t s, bool adj) c
 #define SR(T, NB, NFU)                                                                                              \
   if (!launched && D.nb == NB && slots == NFU) {                                                                    \
     const T *rc = prec == UBT_FP64 ? rec64.as<T>() : rec32.as<T>();                                                 \
-    const T *vl = (prec == UBT_FP64 ? o.t64.vol : o.t32.vol).as<T>();                                               \
+    const T *vl = (prec == UBT_FP64 ? vcell64 : vcell32).as<T>();                                                   \
     if (pipe && adj) {                                                                                              \
       SR_K(T, NB, NFU, true, true, threads)                                                                         \
     } else if (pipe) {                                                                                              \

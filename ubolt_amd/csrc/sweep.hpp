@@ -56,6 +56,8 @@ struct Sweep {
   static constexpr int64_t kLdsTraceBytes = 64 * 1024;
   int trace_slots = 0;
   DevBuf own_slot, up_slot;
+  // the adjoint sweep's per-item reversed angle and per-cell volumes
+  DevBuf angle_adj, vcell64, vcell32;
   mutable DevBuf rec64, rec32, rnb;
   mutable int rec_group = -1;
   mutable uint64_t rec_gen = 0;
