@@ -168,6 +168,13 @@ void Solver::create(Operator *o, DSA *d, const SolverOptions &opts) {
   }
   hcol.alloc(sizeof(double) * 2 * kMaxK);
   partial.alloc(sizeof(double) * kMaxBlocks * kMaxK);
+  if (!h_pin) UBT_HIP(hipHostMalloc((void **)&h_pin, sizeof(double) * 2 * kMaxK, hipHostMallocDefault));
+  if (!h_ev) UBT_HIP(hipEventCreateWithFlags(&h_ev, hipEventDisableTiming));
+}
+
+Solver::~Solver() {
+  if (h_pin) (void)hipHostFree(h_pin);
+  if (h_ev) (void)hipEventDestroy(h_ev);
 }
 
 void Solver::refresh(hipStream_t s) {
@@ -214,7 +221,6 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
   double *hc = hcol.as<double>(), *part = partial.as<double>();
   const unsigned gv = grid_for(n);
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
-  h_host.resize(m + 2);
   double resid = 0.0;
   while (true) {
     apply_op<T>(x, w, true, s);
@@ -243,12 +249,15 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
       }
       mdot<T>(w, 1, w, n, hc + j + 1, false, true, part, s);
       hipLaunchKernelGGL(k_scale_dev<T>, dim3(gv), dim3(kThreads), 0, s, n, (const T *)w, (const double *)(hc + j + 1), vn);
-      UBT_HIP(hipMemcpyAsync(h_host.data(), hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
+      UBT_HIP(hipMemcpyAsync(h_pin, hc, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s));
       if (opt.gs_passes > 1)
-        UBT_HIP(hipMemcpyAsync(h2.data(), hc + kMaxK, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
-      UBT_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h_host[i] + h2[i];
-      H[(size_t)(j + 1) * m + j] = h_host[j + 1];
+        UBT_HIP(hipMemcpyAsync(h_pin + kMaxK, hc + kMaxK, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+      UBT_HIP(hipEventRecord(h_ev, s));
+      UBT_HIP(hipEventSynchronize(h_ev));
+      if (opt.gs_passes > 1)
+        for (int i = 0; i <= j; i++) h2[i] = h_pin[kMaxK + i];
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h_pin[i] + h2[i];
+      H[(size_t)(j + 1) * m + j] = h_pin[j + 1];
       for (int i = 0; i < j; i++) {
         const double a = H[(size_t)i * m + j], bb = H[(size_t)(i + 1) * m + j];
         H[(size_t)i * m + j] = cs[i] * a + sn[i] * bb;
@@ -266,7 +275,7 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
       (*its)++;
       j++;
       history.push_back(resid);
-      if (h_host[j] == 0.0) breakdown = true;
+      if (h_pin[j] == 0.0) breakdown = true;
       if (!(resid > tol_abs) || breakdown) break;
     }
     // x += Z y, H y = g on the leading j x j triangle

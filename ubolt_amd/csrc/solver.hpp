@@ -33,7 +33,15 @@ struct Solver {
   DevBuf V64, Z64, V32, Z32;
   DevBuf w64, t64, u64, r64, d32, w32, t32, u32, b32;  // work vectors
   DevBuf hcol, partial;   // device GS coefficients (restart+2), multi-dot partials
-  std::vector<double> h_host;
+  // pinned host copy of the Hessenberg column (both passes; an asynchronous
+  // copy, where pageable memory would be staged) and the event recorded
+  // after it
+  double *h_pin = nullptr;
+  hipEvent_t h_ev = nullptr;
+  Solver() = default;
+  Solver(const Solver &) = delete;
+  Solver &operator=(const Solver &) = delete;
+  ~Solver();
   // results
   int iterations = 0, outer = 0, reason = 0;
   double rel_residual = 0.0;
