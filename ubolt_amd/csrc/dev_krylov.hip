@@ -131,12 +131,14 @@ template <class T> static void mdot(const T *V, int k, const T *w, int64_t n, do
   hipLaunchKernelGGL(k_mdot_final, dim3(k), dim3(64), 0, s, (int)g, k, partial, out, acc ? 1 : 0, sq ? 1 : 0);
 }
 
-template <class T> static double host_norm(const T *w, int64_t n, double *dev_out, double *partial, hipStream_t s) {
+// pin: pinned host memory for the result (an asynchronous copy; pageable
+// memory would be staged)
+template <class T>
+static double host_norm(const T *w, int64_t n, double *dev_out, double *partial, hipStream_t s, double *pin) {
   mdot<T>(w, 1, w, n, dev_out, false, true, partial, s);
-  double h = 0.0;
-  UBT_HIP(hipMemcpyAsync(&h, dev_out, sizeof(double), hipMemcpyDeviceToHost, s));
+  UBT_HIP(hipMemcpyAsync(pin, dev_out, sizeof(double), hipMemcpyDeviceToHost, s));
   UBT_HIP(hipStreamSynchronize(s));
-  return h;
+  return *pin;
 }
 
 void Solver::create(Operator *o, DSA *d, const SolverOptions &opts) {
@@ -168,7 +170,7 @@ void Solver::create(Operator *o, DSA *d, const SolverOptions &opts) {
   }
   hcol.alloc(sizeof(double) * 2 * kMaxK);
   partial.alloc(sizeof(double) * kMaxBlocks * kMaxK);
-  if (!h_pin) UBT_HIP(hipHostMalloc((void **)&h_pin, sizeof(double) * 2 * kMaxK, hipHostMallocDefault));
+  if (!h_pin) UBT_HIP(hipHostMalloc((void **)&h_pin, sizeof(double) * (2 * kMaxK + 1), hipHostMallocDefault));
   if (!h_ev) UBT_HIP(hipEventCreateWithFlags(&h_ev, hipEventDisableTiming));
 }
 
@@ -225,7 +227,7 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
   while (true) {
     apply_op<T>(x, w, true, s);
     hipLaunchKernelGGL(k_axpby<T>, dim3(gv), dim3(kThreads), 0, s, n, 1.0, b, -1.0, w);
-    const double beta = host_norm<T>(w, n, hc, part, s);
+    const double beta = host_norm<T>(w, n, hc, part, s, h_pin + 2 * kMaxK);
     resid = beta;
     if (!(beta > tol_abs) || *its >= max_it) return beta;
     hipLaunchKernelGGL((k_convert<T, T>), dim3(gv), dim3(kThreads), 0, s, n, 1.0 / beta, (const T *)w, V);
@@ -284,15 +286,16 @@ template <class T> double Solver::fgmres(const T *b, T *x, double tol_abs, int m
       for (int k = i + 1; k < j; k++) v -= H[(size_t)i * m + k] * y[k];
       y[i] = H[(size_t)i * m + i] != 0.0 ? v / H[(size_t)i * m + i] : 0.0;
     }
-    UBT_HIP(hipMemcpyAsync(hc, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s));
+    std::copy(y.begin(), y.begin() + j, h_pin);  // pinned: an asynchronous copy
+    UBT_HIP(hipMemcpyAsync(hc, h_pin, sizeof(double) * j, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_maxpy<T>, dim3(gv), dim3(kThreads), 0, s, n, j, (const T *)Z, (const double *)hc, 1.0, x);
     UBT_HIP(hipGetLastError());
-    UBT_HIP(hipStreamSynchronize(s));  // y on the host goes out of scope
+    UBT_HIP(hipStreamSynchronize(s));  // before h_pin is reused
     if (*its >= max_it && resid > tol_abs) {
       // one last true residual
       apply_op<T>(x, w, true, s);
       hipLaunchKernelGGL(k_axpby<T>, dim3(gv), dim3(kThreads), 0, s, n, 1.0, b, -1.0, w);
-      return host_norm<T>(w, n, hc, part, s);
+      return host_norm<T>(w, n, hc, part, s, h_pin + 2 * kMaxK);
     }
   }
 }
@@ -312,7 +315,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s, bool adj) {
   const unsigned gv = grid_for(n);
   double *part = partial.as<double>(), *hc = hcol.as<double>();
   hipLaunchKernelGGL((k_convert<double, double>), dim3(gv), dim3(kThreads), 0, s, n, 0.0, b, x);  // x = 0
-  const double bnorm = host_norm<double>(b, n, hc, part, s);
+  const double bnorm = host_norm<double>(b, n, hc, part, s, h_pin + 2 * kMaxK);
   if (bnorm == 0.0) {
     rel_residual = 0.0;
     reason = 1;
@@ -335,7 +338,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s, bool adj) {
   for (outer = 0; outer <= opt.max_outer; outer++) {
     apply_op<double>(x, r, true, s);
     hipLaunchKernelGGL(k_axpby<double>, dim3(gv), dim3(kThreads), 0, s, n, 1.0, b, -1.0, r);
-    const double rn = host_norm<double>(r, n, hc, part, s);
+    const double rn = host_norm<double>(r, n, hc, part, s, h_pin + 2 * kMaxK);
     rel_residual = rn / bnorm;
     hist.push_back(rel_residual);
     if (rn <= tol) {
