@@ -246,13 +246,11 @@ __global__ void __launch_bounds__(256) sweep_records(OpArgs<double> g1, const Ba
 }
 
 // One group's levels l0 .. l1-1 from the records (item i: cell cell[i],
-// angle ang[i]). With PIPE, a thread's item of the next level (its records,
-// r, the upwind cells) is loaded while the current level's upwind traces are
-// in flight, and its cell and angle two levels ahead. Vector memory loads
-// complete in issue order, so the trace loads are issued before that
-// prefetch: waiting for them then leaves the prefetch in flight, and the
-// barrier-to-barrier path of a level is the trace load, the multiply-adds
-// and the store.
+// angle ang[i]). With PIPE (narrow levels), the records of a thread's items
+// two or three levels ahead are in flight while a level is solved (see the
+// pipelined loop below); otherwise (wide levels) a thread's first item of
+// the next level has its cell, angle and upwind slots loaded one level
+// ahead. Vector memory loads complete in issue order, which shapes both.
 // ADJ: the adjoint sweep L^T = V P L P V^{-1} from the same records: item
 // (c, b) reads r and writes psi at the reversed angle neg[b], r scaled by
 // 1/V_c, the upwind traces by 1/V_n, the result by V_c (see the ADJ kernels'
