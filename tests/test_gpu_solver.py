@@ -98,6 +98,45 @@ def test_sweep_inverts_L(gpu, built):
     assert e64 <= 1e-11 and e32 <= 1e-4
 
 
+@pytest.mark.parametrize("case", ["dg1_delaunay", "dg1_hex"])
+def test_sweep_lds_and_hbm_traces_agree(gpu, monkeypatch, case):
+    """the sweep reads its upwind traces from LDS slots where they fit and
+    from psi in HBM otherwise (forced here by UBT_SWEEP_LDS_TRACES=0 at
+    creation): both, forward and adjoint, in both precisions, against the
+    oracle's direct solves, and against each other - the forward bit for bit,
+    the adjoint (whose LDS slot holds the trace before the volume scaling)
+    to rounding. The hex case runs the FP64 sweep in the plain (wide-level)
+    form and the FP32 sweep pipelined."""
+    prob = P.dg1_delaunay(n_side=10, sn=4, seed=4) if case == "dg1_delaunay" else P.dg_box_3d(5, 4)
+    op = prob.operator()
+    r = np.random.default_rng(8).standard_normal(prob.disc.n_rows)
+    L = oracle_rows(prob).to_scipy()
+    want = spl.spsolve(L.tocsc(), r)
+    want_t = spl.spsolve(L.T.tocsc(), r)
+    out = {}
+    for lds in ("1", "0"):
+        monkeypatch.setenv("UBT_SWEEP_LDS_TRACES", lds)
+        sw = u.Sweep(op)
+        res = {}
+        for prec, dt in ((u.FP64, torch.float64), (u.FP32, torch.float32)):
+            for adj in (False, True):
+                y = op.vec(prec)
+                (sw.apply_transpose if adj else sw.apply)(to_dev(r, dt), y)
+                res[(prec, adj)] = y
+        torch.cuda.synchronize()
+        out[lds] = {k: to_host(v) for k, v in res.items()}
+    for lds, res in out.items():
+        e = [rel_max(res[(u.FP64, False)], want), rel_max(res[(u.FP32, False)], want),
+             rel_max(res[(u.FP64, True)], want_t), rel_max(res[(u.FP32, True)], want_t)]
+        print(f"{case} lds={lds}: fwd fp64 {e[0]:.2e} fp32 {e[1]:.2e}, adj fp64 {e[2]:.2e} fp32 {e[3]:.2e}")
+        assert e[0] <= 1e-11 and e[2] <= 1e-11 and e[1] <= 1e-4 and e[3] <= 1e-4
+    a, b = out["1"], out["0"]
+    assert np.array_equal(a[(u.FP64, False)], b[(u.FP64, False)])
+    assert np.array_equal(a[(u.FP32, False)], b[(u.FP32, False)])
+    assert rel_max(a[(u.FP64, True)], b[(u.FP64, True)]) <= 1e-13
+    assert rel_max(a[(u.FP32, True)], b[(u.FP32, True)]) <= 1e-5
+
+
 def test_sweep_records_follow_cross_sections(gpu):
     """the sweep's per-item records (block inverses, upwind couplings) are
     refolded when the cross sections or the group change: sweep, change

@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "batch.hpp"
 #include "sweep.hpp"
@@ -821,9 +823,11 @@ void Sweep::create(Operator *o, bool keep_host) {
   // LDS trace slots (see assign_trace_slots): the sweep then reads every
   // upwind trace from LDS, and no trace read waits behind the record loads
   // in flight (vector memory loads complete in issue order). A precision
-  // whose slots would not fit kLdsTraceBytes reads the traces from HBM.
+  // whose slots would not fit kLdsTraceBytes reads the traces from HBM, as
+  // does every sweep created with UBT_SWEEP_LDS_TRACES=0 in the environment.
   trace_slots = 0;
-  if (!keep_host) {
+  const char *lds_env = std::getenv("UBT_SWEEP_LDS_TRACES");
+  if (!keep_host && !(lds_env && std::strcmp(lds_env, "0") == 0)) {
     const int slots = sweep_slots(nf, nfu);
     std::vector<int> own((size_t)n_items, -1), ups((size_t)rec_len(n_items, slots), -1);
     const int most = assign_trace_slots(D, slots, item_cell.data(), item_angle.data(), item_up.data(), alvl.data(),
