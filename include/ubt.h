@@ -333,7 +333,13 @@ int ubt_dsa_destroy(ubt_dsa *dsa);
  * the reference's PCAIR stage (src/transport_solverk.kokkos.cxx:238), which
  * approximates the same inverse. Level schedule built once. */
 int ubt_sweep_create(ubt_op *op, ubt_sweep **sw);
+/* As ubt_sweep_create; lds_traces = 0 keeps every upwind trace read in HBM
+ * (a diagnostic switch, no counterpart in the reference; by default a DG
+ * sweep reads upwind traces from LDS slots wherever they fit 64 KB). */
+int ubt_sweep_create_opts(ubt_op *op, int lds_traces, ubt_sweep **sw);
 int ubt_sweep_info(const ubt_sweep *sw, int *n_levels, int64_t *n_lagged);
+/* LDS trace slots per direction workgroup (0: the traces are read from HBM). */
+int ubt_sweep_trace_slots(const ubt_sweep *sw, int *slots);
 int ubt_sweep_apply(ubt_sweep *sw, int prec, const void *r, void *psi, void *stream);
 /* psi = L^{-T} r: the adjoint sweep, the same level schedule run on the
  * reversed angles' data with the volume scaling of L^T = V P L P V^{-1}. */

@@ -143,3 +143,46 @@ def test_batch_multigroup(gpu, ensemble):
             phi = oracle.moments(xh[sl], prob.quad.w)
             assert np.linalg.norm(phi - caches[p][-1]) / np.linalg.norm(caches[p][-1]) <= 1e-8, (p, g)
     b.set_group(0)
+
+
+def test_batch_follows_set_xs(gpu):
+    """new cross sections on a member after the batch and its solver were
+    built (and the solver's cycles captured): the batched apply, sweep and the
+    graph-replayed solve use them, and agree with the solve without graphs"""
+    probs = P.batch_quads(n_problems=3, seed=7, side=(3, 6), sns=(2, 4), groups=(1, 1))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    b.set_group(0)
+    sol = u.BatchSolver(b, rtol=1e-11, max_it=600, restart=20)
+    sol.refresh()
+    rhs = b.vec()
+    b.rhs(rhs)
+    x = b.vec()
+    sol.solve(rhs, x)  # captures the cycles on the old cross sections
+    rng = np.random.default_rng(21)
+    for k in (0, 2):
+        pr = probs[k]
+        pr.sigma_t = np.asarray(pr.sigma_t) * rng.uniform(1.5, 3.0, np.asarray(pr.sigma_t).shape)
+        pr.sigma_s = np.asarray(pr.sigma_s) * rng.uniform(0.2, 0.9, np.asarray(pr.sigma_s).shape)
+        ops[k].set_xs(pr.sigma_t, pr.sigma_s, pr.source)
+    xr = rng.standard_normal(b.n_rows)
+    y, ys = b.vec(), b.vec()
+    b.mult(to_dev(xr), y)
+    b.sweep_apply(to_dev(xr), ys)
+    rhs2 = b.vec()
+    b.rhs(rhs2)
+    xg, xn = b.vec(), b.vec()
+    res_g = sol.solve(rhs2, xg)
+    sol.set_graph(False)
+    res_n = sol.solve(rhs2, xn)
+    torch.cuda.synchronize()
+    yh, ysh, bh = to_host(y), to_host(ys), to_host(rhs2)
+    print(f"[case] set_xs on batch members 0, 2: graph {res_g['iterations']} plain {res_n['iterations']}", flush=True)
+    for p, prob in enumerate(probs):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        assert rel_max(yh[sl], oracle_mult(prob, xr[sl])) <= 1e-12, p
+        assert rel_max(ysh[sl], spl.spsolve(oracle_rows(prob).to_scipy().tocsc(), xr[sl])) <= 1e-11, p
+        want = oracle_full(prob) @ to_host(xg)[sl]
+        assert np.linalg.norm(bh[sl] - want) / np.linalg.norm(bh[sl]) <= 1e-10, p
+        assert res_g["iterations"][p] == res_n["iterations"][p], (p, res_g, res_n)
+        assert rel_max(to_host(xg)[sl], to_host(xn)[sl]) <= 1e-12, p

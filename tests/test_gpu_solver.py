@@ -99,9 +99,9 @@ def test_sweep_inverts_L(gpu, built):
 
 
 @pytest.mark.parametrize("case", ["dg1_delaunay", "dg1_hex"])
-def test_sweep_lds_and_hbm_traces_agree(gpu, monkeypatch, case):
+def test_sweep_lds_and_hbm_traces_agree(gpu, case):
     """the sweep reads its upwind traces from LDS slots where they fit and
-    from psi in HBM otherwise (forced here by UBT_SWEEP_LDS_TRACES=0 at
+    from psi in HBM otherwise (forced here by lds_traces=False at
     creation): both, forward and adjoint, in both precisions, against the
     oracle's direct solves, and against each other - the forward bit for bit,
     the adjoint (whose LDS slot holds the trace before the volume scaling)
@@ -115,8 +115,9 @@ def test_sweep_lds_and_hbm_traces_agree(gpu, monkeypatch, case):
     want_t = spl.spsolve(L.T.tocsc(), r)
     out = {}
     for lds in ("1", "0"):
-        monkeypatch.setenv("UBT_SWEEP_LDS_TRACES", lds)
-        sw = u.Sweep(op)
+        sw = u.Sweep(op, lds_traces=(lds == "1"))
+        # the path under test is the one that ran
+        assert (sw.trace_slots > 0) == (lds == "1"), sw.trace_slots
         res = {}
         for prec, dt in ((u.FP64, torch.float64), (u.FP32, torch.float32)):
             for adj in (False, True):

@@ -98,7 +98,9 @@ SIGNATURES = {
     "ubt_dsa_info": [P, PI, PI],
     "ubt_dsa_destroy": [P],
     "ubt_sweep_create": [P, PP],
+    "ubt_sweep_create_opts": [P, I, PP],
     "ubt_sweep_info": [P, PI, PI64],
+    "ubt_sweep_trace_slots": [P, PI],
     "ubt_sweep_apply": [P, I, P, P, P],
     "ubt_sweep_apply_transpose": [P, I, P, P, P],
     "ubt_sweep_destroy": [P],

@@ -296,13 +296,14 @@ class Sweep(_Handle):
     """Transport sweep psi = L^{-1} r (the streaming stage)."""
     _destroy = "ubt_sweep_destroy"
 
-    def __init__(self, op):
+    def __init__(self, op, lds_traces=True):
         super().__init__()
         self.op = op
-        call("ubt_sweep_create", op.h, C.byref(self.h))
-        nl, nlag = C.c_int(), C.c_int64()
+        call("ubt_sweep_create_opts", op.h, int(lds_traces), C.byref(self.h))
+        nl, nlag, ts = C.c_int(), C.c_int64(), C.c_int()
         call("ubt_sweep_info", self.h, C.byref(nl), C.byref(nlag))
-        self.n_levels, self.n_lagged = nl.value, nlag.value
+        call("ubt_sweep_trace_slots", self.h, C.byref(ts))
+        self.n_levels, self.n_lagged, self.trace_slots = nl.value, nlag.value, ts.value
 
     def apply(self, r, psi, stream=None):
         call("ubt_sweep_apply", self.h, Operator._prec(r), dptr(r), dptr(psi), stream_ptr(stream))

@@ -104,6 +104,7 @@ struct BatchSolver {
   DevBuf onrm;                // per problem: the outer residual norm (refinement scale)
   DevBuf done;                // int per problem x 4: done, steps this cycle, iterations, live
   int gen = -1;               // the batch work lists the graphs were captured on
+  uint64_t xs_tag = 0;        // the members' cross-section generations the records and graphs follow
   std::vector<int> iterations, reason;
   std::vector<double> rel_residual;
   int outer = 0, cycles = 0;

@@ -526,6 +526,22 @@ int ubt_sweep_create(ubt_op *op, ubt_sweep **sw) {
     *sw = h.release();
   });
 }
+int ubt_sweep_create_opts(ubt_op *op, int lds_traces, ubt_sweep **sw) {
+  return guard([&] {
+    NEED(op);
+    NEED(sw);
+    auto h = std::make_unique<ubt_sweep_s>();
+    h->s.create(op->op.get(), false, lds_traces != 0);
+    *sw = h.release();
+  });
+}
+int ubt_sweep_trace_slots(const ubt_sweep *sw, int *slots) {
+  return guard([&] {
+    NEED(sw);
+    NEED(slots);
+    *slots = sw->s.trace_slots;
+  });
+}
 int ubt_sweep_info(const ubt_sweep *sw, int *n_levels, int64_t *n_lagged) {
   return guard([&] {
     NEED(sw);

@@ -573,8 +573,10 @@ void BatchSolver::refresh(hipStream_t s) {
   cfirst.upload(first);
   UBT_HIP(hipMemcpyAsync(done.as<int>() + 3 * np, live.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
   UBT_HIP(hipStreamSynchronize(s));
+  batch->sweep_records(s);
   drop_graphs();
   gen = batch->gen;
+  xs_tag = batch->xs_tag();
 }
 
 // y = M^{-1} r: element blocks, then the sweep on the L-residual
@@ -744,6 +746,16 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
   double *pt = part.as<double>();
   // the caller's stream, then the capture stream
   UBT_HIP(hipStreamSynchronize(s));
+  // a member's cross sections changed since the refresh (ubt_op_set_xs): the
+  // captured cycles would replay the old sweep records, so check the blocks,
+  // rebuild the records outside any capture and capture the cycles again
+  const uint64_t tag = batch->xs_tag();
+  if (tag != xs_tag) {
+    drop_graphs();
+    batch->blocks_setup(cap);
+    batch->sweep_records(cap);
+    xs_tag = tag;
+  }
   cycles = outer = 0;
   iterations.assign(np, 0);
   reason.assign(np, 0);

@@ -685,7 +685,7 @@ int assign_trace_slots(const Disc &D, int slots, const int *cell, const int *ang
   return most;
 }
 
-void Sweep::create(Operator *o, bool keep_host) {
+void Sweep::create(Operator *o, bool keep_host, bool lds_traces) {
   op = o;
   const Disc &D = *o->disc;
   if (D.kind == DISC_FD) {
@@ -824,10 +824,9 @@ void Sweep::create(Operator *o, bool keep_host) {
   // upwind trace from LDS, and no trace read waits behind the record loads
   // in flight (vector memory loads complete in issue order). A precision
   // whose slots would not fit kLdsTraceBytes reads the traces from HBM, as
-  // does every sweep created with UBT_SWEEP_LDS_TRACES=0 in the environment.
+  // does every sweep created with lds_traces = false.
   trace_slots = 0;
-  const char *lds_env = std::getenv("UBT_SWEEP_LDS_TRACES");
-  if (!keep_host && !(lds_env && std::strcmp(lds_env, "0") == 0)) {
+  if (!keep_host && lds_traces) {
     const int slots = sweep_slots(nf, nfu);
     std::vector<int> own((size_t)n_items, -1), ups((size_t)rec_len(n_items, slots), -1);
     const int most = assign_trace_slots(D, slots, item_cell.data(), item_angle.data(), item_up.data(), alvl.data(),

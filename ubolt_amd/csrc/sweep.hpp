@@ -66,7 +66,9 @@ struct Sweep {
   // keep_host: the schedule stays on the host for a batch, which builds its
   // own records (this sweep's are then allocated by its first forward sweep);
   // otherwise the record storage is allocated here
-  void create(Operator *o, bool keep_host = false);
+  // lds_traces = false keeps every upwind trace read in HBM (a diagnostic
+  // switch: the LDS path is the default wherever the slots fit)
+  void create(Operator *o, bool keep_host = false, bool lds_traces = true);
   // psi = L^{-1} r, or (adj) psi = L^{-T} r
   void apply(int prec, const void *r, void *psi, hipStream_t s, bool adj = false) const;
 };

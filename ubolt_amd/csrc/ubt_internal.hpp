@@ -71,8 +71,11 @@ struct DevBuf {
     bytes = 0;
   }
   template <class T> T *as() const { return static_cast<T *>(ptr); }
+  // A table of the size already held is copied into the existing allocation,
+  // so device pointers handed out earlier (a batch's work items hold an
+  // operator's cross-section pointers) stay valid across re-uploads.
   template <class T> void upload(const std::vector<T> &h) {
-    alloc(h.size() * sizeof(T));
+    if (bytes != h.size() * sizeof(T)) alloc(h.size() * sizeof(T));
     if (!h.empty()) UBT_HIP(hipMemcpy(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   }
   // Upload a double table converted to T (the FP32 copies of the geometry)
