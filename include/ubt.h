@@ -240,14 +240,16 @@ int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
  * four consecutive angles of a cell: 16-byte loads, per-cell work shared;
  * DG with A % 4 == 0), 8 and 9 the box-tile kernel (a uniform DG1 box: the
  * interior class folded per angle into three small matrices, 2 or 4 rows
- * per thread in registers); -1 re-runs the autotuner, which times every applicable
+ * per thread in registers), 10 the strip march (a uniform DG1 box: a
+ * workgroup walks a strip of cells along the last axis with the neighbouring
+ * rows in registers); -1 re-runs the autotuner, which times every applicable
  * kernel on scratch vectors when the cross sections are first set and keeps
  * the fastest. A variant that does not apply falls back to 0. No reference
  * counterpart (the reference has one Kokkos SpMV); exposed so the tests can
  * check every kernel. */
 int ubt_op_set_variant(ubt_op *op, int prec, int variant);
-/* variant[2] (FP64, FP32) in use and us[20] the tuned times (FP64 variants
- * 0-9, then FP32 variants 0-9; 0 where not applicable). */
+/* variant[2] (FP64, FP32) in use and us[22] the tuned times (FP64 variants
+ * 0-10, then FP32 variants 0-10; 0 where not applicable). */
 int ubt_op_tuning(const ubt_op *op, int *variant, float *us);
 int ubt_op_destroy(ubt_op *op);
 
@@ -389,12 +391,12 @@ int ubt_batch_offsets(const ubt_batch *batch, int64_t *off);
 int ubt_batch_set_group(ubt_batch *batch, int g);
 /* y = A x for every active problem in one persistent launch. */
 int ubt_batch_mult(ubt_batch *batch, int prec, const void *x, void *y, void *stream);
-/* body[2] (FP64, FP32: 0 tile, 1 four-angle tile, 2 box tile when every
- * problem is a uniform DG1 box) and persist[2] (workgroups per CU) chosen by
- * the batch autotuner, us[6] the best time per body (FP64 bodies 0-2, then
- * FP32 bodies 0-2; 0 if it does not apply). */
+/* body[2] (FP64, FP32: 0 tile, 1 four-angle tile, 2 box tile, 3 strip march,
+ * the last two when every problem is a uniform DG1 box) and persist[2]
+ * (workgroups per CU) chosen by the batch autotuner, us[8] the best time per
+ * body (FP64 bodies 0-3, then FP32 bodies 0-3; 0 if it does not apply). */
 int ubt_batch_tuning(const ubt_batch *batch, int *body, int *persist, float *us);
-/* Force the apply body of one precision (0, 1, 2 as above; an error where it
+/* Force the apply body of one precision (0 to 3 as above; an error where it
  * does not apply to every active problem). No reference counterpart; exposed
  * so the tests can check every body. */
 int ubt_batch_set_body(ubt_batch *batch, int prec, int body);

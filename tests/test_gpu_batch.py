@@ -23,13 +23,13 @@ def ensemble():
     return probs, ops, b
 
 
-@pytest.mark.parametrize("body", [0, 1, 2])
+@pytest.mark.parametrize("body", [0, 1, 2, 3])
 def test_batch_mult(gpu, ensemble, body):
-    """every apply body (tile, four-angle tile, box tile) against the oracle"""
+    """every apply body (tile, four-angle tile, box tile, strip march) against the oracle"""
     probs, ops, b = ensemble
     b.set_group(0)
     t = b.tuning()
-    assert t["us"][body] > 0 and t["us"][3 + body] > 0, f"body {body} does not apply: {t}"
+    assert t["us"][body] > 0 and t["us"][4 + body] > 0, f"body {body} does not apply: {t}"
     x = np.random.default_rng(1).standard_normal(b.n_rows)
     for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
         prec = u.FP64 if dt == torch.float64 else u.FP32
@@ -186,3 +186,27 @@ def test_batch_follows_set_xs(gpu):
         assert np.linalg.norm(bh[sl] - want) / np.linalg.norm(bh[sl]) <= 1e-10, p
         assert res_g["iterations"][p] == res_n["iterations"][p], (p, res_g, res_n)
         assert rel_max(to_host(xg)[sl], to_host(xn)[sl]) <= 1e-12, p
+
+
+def test_batch_strip_bands(gpu):
+    """the strip march and the box tile over problems of several strips and
+    bands (sides 20..140, S4-S12, all vacuum: no list phase), both precisions,
+    problem by problem against the oracle"""
+    probs = P.batch_quads(n_problems=4, seed=9, side=(20, 140), sns=(4, 8, 12), groups=(1, 1))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    b.set_group(0)
+    x = np.random.default_rng(3).standard_normal(b.n_rows)
+    for body in (2, 3):
+        for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+            prec = u.FP64 if dt == torch.float64 else u.FP32
+            b.set_body(prec, body)
+            y = b.vec(prec)
+            b.mult(to_dev(x, dt), y)
+            torch.cuda.synchronize()
+            yh = to_host(y)
+            for p, prob in enumerate(probs):
+                sl = slice(b.offsets[p], b.offsets[p + 1])
+                err = rel_max(yh[sl], oracle_mult(prob, x[sl]))
+                print(f"[case] body {body} {dt} problem {p} ({prob.name}): {err:.2e}", flush=True)
+                assert err <= tol, (body, p, dt, err)

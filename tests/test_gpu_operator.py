@@ -78,7 +78,7 @@ def built(request):
 # a structured box (8, 16, 32 cells per workgroup), four-angle tile, box tile
 # (2 and 4 rows per thread); a variant that does not apply to a case runs the
 # tile kernel
-VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9]
+VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -130,7 +130,7 @@ def test_tuner_reports(gpu, built):
     op.set_variant(core.FP64, -1)
     t = op.tuning()
     print(f"{name}: tuned {t}")
-    assert 0 <= t["variant_fp64"] < 10 and 0 <= t["variant_fp32"] < 10 and t["us"][0] > 0
+    assert 0 <= t["variant_fp64"] < 11 and 0 <= t["variant_fp32"] < 11 and t["us"][0] > 0
 
 
 def test_diagonal(gpu, built):
@@ -238,6 +238,11 @@ BOX_CASES = {
     "dg1_quad_tiles": lambda: P.dg_box_2d(23, 13, 4, order=1, bcs={1: "reflect", 3: "reflect",
                                                                    2: {"type": "vacuum", "inflow": 1.5}}),
     "dg1_hex_tiles": lambda: P.dg_box_3d(5, 8, order=1, bcs={3: "reflect", 6: "reflect"}),
+    # every side vacuum: no list phase; three bands of the strip march, a
+    # partial last strip and band
+    "dg1_quad_bands_vacuum": lambda: P.dg_box_2d(23, 150, 6, order=1),
+    "dg1_quad_bands_s12": lambda: P.dg_box_2d(9, 131, 12, order=1, bcs={4: "reflect"}),
+    "dg1_hex_bands": lambda: P.dg_box_3d(4, 2, order=1),
 }
 
 
@@ -246,11 +251,14 @@ def test_box_tile_apply(gpu, case):
     prob = BOX_CASES[case]()
     op = prob.operator()
     us = op.tuning()["us"]
-    assert all(us[v] > 0 for v in (8, 9, 18, 19)), f"box tile not applicable: {us}"
+    assert all(us[v] > 0 for v in (8, 9, 19, 20)), f"box tile not applicable: {us}"
+    # the strip march holds one halo element (nb x angles of a cell) per thread
+    strip = prob.disc.nb * prob.disc.n_angles <= 256
+    assert (us[10] > 0 and us[21] > 0) == strip, f"strip march applicability: {us}"
     x = np.random.default_rng(4).standard_normal(prob.disc.n_rows)
     want = oracle_mult(prob, x)
     want_l = oracle_mult(prob, x, scatter=False)
-    for v in (8, 9):
+    for v in (8, 9, 10) if strip else (8, 9):
         op.set_variant(core.FP64, v)
         op.set_variant(core.FP32, v)
         y, y32, yl = op.vec(), op.vec(core.FP32), op.vec()

@@ -170,12 +170,12 @@ class Operator(_Handle):
     def set_variant(self, prec, variant):
         """apply kernel: 0 tile, 1 wavefront per cell, 2/3 multi-cell tile (2/4 cells per thread),
         4/5/6 pencil on a structured box (8/16/32 cells per workgroup), 7 four-angle tile, 8/9 box tile on a
-        uniform DG1 box (2/4 rows per thread), -1 re-tune"""
+        uniform DG1 box (2/4 rows per thread), 10 strip march on a uniform DG1 box, -1 re-tune"""
         call("ubt_op_set_variant", self.h, prec, variant)
 
     def tuning(self):
         v = (C.c_int * 2)()
-        us = (C.c_float * 20)()
+        us = (C.c_float * 22)()
         call("ubt_op_tuning", self.h, v, us)
         return {"variant_fp64": v[0], "variant_fp32": v[1], "us": [round(x, 2) for x in us]}
 
@@ -400,12 +400,12 @@ class Batch(_Handle):
         call("ubt_batch_set_scalar_flux", self.h, Operator._prec(psi), dptr(psi), stream_ptr(stream))
 
     def set_body(self, prec, body):
-        """apply body of one precision: 0 tile, 1 four-angle tile, 2 box tile (uniform DG1 boxes)"""
+        """apply body of one precision: 0 tile, 1 four-angle tile, 2 box tile, 3 strip march (uniform DG1 boxes)"""
         call("ubt_batch_set_body", self.h, prec, body)
 
     def tuning(self):
         body, persist = (C.c_int * 2)(), (C.c_int * 2)()
-        us = (C.c_float * 6)()
+        us = (C.c_float * 8)()
         call("ubt_batch_tuning", self.h, body, persist, us)
         return {"body_fp64": body[0], "body_fp32": body[1], "persist_fp64": persist[0], "persist_fp32": persist[1],
                 "us": [round(x, 2) for x in us]}

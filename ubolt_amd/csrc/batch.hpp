@@ -17,7 +17,7 @@ template <class T> struct BatchItem {
 };
 
 // apply bodies the batch autotuner chooses from
-constexpr int kBatchBodies = 3;  // 0 tile, 1 four-angle tile, 2 box tile (uniform DG1 boxes)
+constexpr int kBatchBodies = 4;  // 0 tile, 1 four-angle tile, 2 box tile, 3 strip march (uniform DG1 boxes)
 
 // the direction groups of every active problem, one workgroup each
 struct BatchSweep {
@@ -52,8 +52,8 @@ struct Batch {
   float tuned_us[2][kBatchBodies] = {};
   int n_cu = 256;
   // box body: prefix of boundary-cell tiles over the items, per precision
-  DevBuf edge_off[2];
-  int64_t n_edge[2] = {};
+  DevBuf edge_off[2][kBatchBodies];  // box bodies: prefix of boundary-cell tiles per problem
+  int64_t n_edge[2][kBatchBodies] = {};
   int gen = 0;                 // bumped whenever the work lists are rebuilt
   // block inverse and sweep work lists
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems

@@ -272,16 +272,16 @@ static void build_items(Batch &b, int body) {
     it.off = b.off[p];
     items.push_back(it);
     tiles.push_back(tiles.back() + nt);
-    edges.push_back(edges.back() + (body == 2 ? it.box.edge_tiles : 0));
+    edges.push_back(edges.back() + (body >= 2 ? it.box.edge_tiles : 0));
     lds = std::max(lds, l);
   }
   b.body_ok[pi][body] = ok;
   if (!ok) return;
   b.items[pi][body].upload(items);
   b.tiles[pi][body].upload(tiles);
-  if (body == 2) {
-    b.edge_off[pi].upload(edges);
-    b.n_edge[pi] = edges.back();
+  if (body >= 2) {
+    b.edge_off[pi][body].upload(edges);
+    b.n_edge[pi][body] = edges.back();
   }
   b.n_items[pi][body] = (int)items.size();
   b.n_tiles[pi][body] = tiles.back();

@@ -14,6 +14,7 @@
 // contiguous run for the angles that see it upwind.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <unordered_map>
 
 #include "batch.hpp"
@@ -40,6 +41,16 @@ __device__ inline void seg_sum(const T *red, int nseg, int len, T *out, int S) {
     for (int o = S >> 1; o > 0; o >>= 1) p += __shfl_xor(p, o, S);
     if (l == 0 && seg < nseg) out[seg] = p;
   }
+}
+
+// A load through a pointer the compiler cannot place (one read out of device
+// memory, as a batch's work items hold them) is a flat load, which also
+// counts against the LDS counter: every barrier's wait for the LDS then waits
+// for it too, and no such load stays in flight across a barrier. The tables
+// behind the operator's pointers are global memory; ldg loads through the
+// global address space.
+template <class T> __device__ __forceinline__ T ldg(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
 }
 
 // the geometry class of cell c: with IDC, the cell itself when every cell is
@@ -969,16 +980,16 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
   };
   auto row_of = [&](int64_t c) { return c * NB * A + a; };
   T p[K][NB], hx[K][NB], hy[K][NB], hr[NB], st[K], ss[K];
-  int gc[K];
+  bool okc[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const bool v = col && r0 + k < nR;
     const int64_t c = v ? cell(i, j, r0 + k) : 0;
 #pragma unroll
     for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
-    st[k] = v ? g.sigt[c] : T(0);
-    ss[k] = v ? g.sigs[c] : T(0);
-    gc[k] = v ? cls_of<false>(g, c) : -1;
+    st[k] = v ? ldg(g.sigt + c) : T(0);
+    ss[k] = v ? ldg(g.sigs + c) : T(0);
+    okc[k] = v && (bx.ok == nullptr || ldg(bx.ok + c));
   }
   // the upwind row beyond the tile along the register axis
   const int rh = kc.in_side[R] == 0 ? r0 - 1 : r0 + K;
@@ -1038,9 +1049,9 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
       const T hi = k < K - 1 ? p[k < K - 1 ? k + 1 : 0][b] : hr[b];
       pr[b] = kc.in_side[R] == 0 ? lo : hi;
     }
-    // boundary cells (and any of another class) belong to the list phase
-    if (!(gc[k] == bx.cls && i > 0 && i < nx - 1 && r > 0 && r < nR - 1 && (DIM == 2 || (j > 0 && j < ny - 1))))
-      continue;
+    // cells the box formula does not cover belong to the list phase (a
+    // vacuum boundary face reads the zero halo loaded for it above)
+    if (!okc[k]) continue;
     T acc[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
@@ -1086,7 +1097,7 @@ __device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__
   T *ws = reinterpret_cast<T *>(smem);
   T *phib = ws + ((A + 1) & ~1);
   T *bufb = phib + W * K * NB;
-  for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = g.w[e];
+  for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = ldg(g.w + e);
   BoxConst<T, DIM, NB> kc;
   box_consts<T, DIM, NB>(g, bx, a, kc);
   for (int64_t t = t0; t < t1; t++) box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb, phib, ws, lc, a);
@@ -1103,6 +1114,212 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_box(const T *__restrict_
   const int64_t chunk = (total + G - 1) / G;
   const int64_t t0 = rank * chunk, t1 = t0 + chunk < total ? t0 + chunk : total;
   if (t0 < t1) box_run<T, DIM, NB, K, SCATTER>(x, y, g, bx, t0, t1);
+  const int64_t E = bx.edge_tiles, ce = (E + G - 1) / G;
+  const int64_t e0 = rank * ce, e1 = e0 + ce < E ? e0 + ce : E;
+  for (int64_t e = e0; e < e1; e++) {
+    __syncthreads();  // the LDS changes layout
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x, y, g, e);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Strip march over a uniform DG1 box. A unit is W columns along x (thread =
+// column, angle; at one y index in 3D) by a band of R cells along the last
+// axis, which the workgroup walks one row at a time. The rows either side of
+// the current one stay in registers, so the upwind trace along the march axis
+// is never loaded twice, and the next rows are requested two steps ahead. Per
+// row one LDS write of the row, one barrier (the LDS is double-buffered by
+// row parity): the x upwind trace comes from the neighbouring lane's slot
+// (the strip's edge lane loads it: the neighbouring strip, marching at the
+// same time on the same XCD, has it in L2), the scatter's moments are a
+// segmented sum over the row, and a row's result is stored one step later,
+// once its moments have passed the next barrier. Boundary cells whose box
+// faces are vacuum run the same formula with a zero trace; cells the formula
+// does not cover are left to the list phase.
+// ---------------------------------------------------------------------------
+// rows requested ahead by the strip march, per precision
+template <class T> constexpr int kStripAhead = sizeof(T) == 8 ? 3 : 4;
+
+__host__ __device__ inline size_t strip_lds_reals(int A, int W, int nb) {
+  const size_t wpad = ((size_t)A + 1) & ~(size_t)1;
+  return wpad + 2 * (size_t)W * nb + 2 * (size_t)(W + 2) * nb * A;
+}
+
+// Calls f(integral_constant<K>) for K = 0 .. N - 1 while f returns true: a
+// loop body unrolled with a compile-time step index.
+template <int K, int N, class F> __device__ __forceinline__ bool unroll_while(F &&f) {
+  if constexpr (K < N) {
+    if (!f(std::integral_constant<int, K>{})) return false;
+    return unroll_while<K + 1, N>(f);
+  }
+  return true;
+}
+
+// PF: rows requested ahead of the current one. The rows live in a ring of
+// L = PF + 3 register slots (rows r - 1 .. r + PF + 1), the row loop is
+// unrolled by L so every slot is a fixed register, and every load is
+// unconditional (clamped to a valid cell, the value selected at its use):
+// copying a register whose load is still in flight, or a load under a
+// branch, makes the compiler wait for every load in flight.
+template <class T, int DIM, int NB, int PF, bool SCATTER>
+__device__ __forceinline__ void strip_unit(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
+                                           const BoxArgs &bx, const BoxConst<T, DIM, NB> &kc, int64_t u, T *buf,
+                                           T *phib, const T *ws, int lc, int a) {
+  constexpr int R = DIM - 1, L = PF + 3;
+  const int A = g.A, W = bx.W, WN = W * NB, NA = NB * A;
+  const int nx = bx.n[0], ny = bx.n[1], nR = bx.n[R];
+  const int sx = (int)(u % bx.units[0]);
+  const int64_t rest = u / bx.units[0];
+  const int j = DIM == 3 ? (int)(rest % bx.units[1]) : 0;
+  const int r0 = (int)(DIM == 3 ? rest / bx.units[1] : rest) * bx.R;
+  const int r1 = r0 + bx.R < nR ? r0 + bx.R : nR;
+  const int i0 = sx * W;
+  const int i = i0 + lc;
+  const bool lane = lc < W;
+  const bool col = lane && i < nx;
+  const int ic = col ? i : i0;
+  auto cell = [&](int ii, int jj, int rr) -> int64_t {
+    return DIM == 2 ? (int64_t)rr * nx + ii : ((int64_t)rr * ny + jj) * nx + ii;
+  };
+  auto clampr = [&](int rr) { return rr < 0 ? 0 : (rr < nR ? rr : nR - 1); };
+  // the march axis' upwind row is r - 1 where the cosine is positive
+  const bool lo = kc.in_side[R] == 0;
+  // x: the neighbour column in the LDS row (columns 0 and W + 1 hold the
+  // strip's halo cells); the halo element e = threadIdx.x of the row, taken
+  // from the left or the right halo cell by its angle's inflow side
+  const int xcol = kc.in_side[0] == 0 ? lc : lc + 2;
+  const int e = threadIdx.x;
+  const bool he = e < NA;
+  const int ea = he ? e % A : 0;
+  const bool hleft = g.omega[3 * ea] > T(0);
+  const int ih = hleft ? i0 - 1 : i0 + W;
+  const bool hv = he && ih >= 0 && ih < nx;
+  const int ihc = hv ? ih : i0;
+  const int hslot = (hleft ? 0 : W + 1) * NA + e;
+  // y (3D): the neighbour row j -+ 1 by the angle's inflow side
+  const int jn = DIM == 3 ? (kc.in_side[DIM == 3 ? 1 : 0] == 0 ? j - 1 : j + 1) : 0;
+  const bool yin = DIM == 3 && col && jn >= 0 && jn < ny;
+  const int jnc = yin ? jn : j;
+  T q[L][NB], hy[DIM == 3 ? L : 1][NB], hx[L], st[L], ss[L], acc[NB];
+  unsigned char ok[L];
+  // the loads of row rr into ring slot s
+  auto fetch = [&](int s, int rr) {
+    const int rc = clampr(rr);
+    const int64_t c = cell(ic, j, rc);
+#pragma unroll
+    for (int b = 0; b < NB; b++) q[s][b] = ldg(x + c * NA + b * A + a);
+    st[s] = ldg(g.sigt + c);
+    ss[s] = ldg(g.sigs + c);
+    ok[s] = ldg(bx.ok + c);
+    hx[s] = ldg(x + cell(ihc, j, rc) * NA + (he ? e : 0));
+    if (DIM == 3) {
+      const int64_t cy = cell(ic, jnc, rc);
+#pragma unroll
+      for (int b = 0; b < NB; b++) hy[DIM == 3 ? s : 0][b] = ldg(x + cy * NA + b * A + a);
+    }
+  };
+  // rows r0 - 1 .. r0 + PF into slots 0 .. PF + 1
+#pragma unroll
+  for (int d = -1; d <= PF; d++) fetch(d + 1, r0 + d);
+  T ssp = T(0);
+  bool okp = false;
+  int64_t cp = 0;
+  int t = 0;
+  auto step = [&](auto kk) -> bool {
+    constexpr int k = decltype(kk)::value;
+    constexpr int s0 = (k + 1) % L, sm = k % L, sp = (k + 2) % L, sa = (k + PF + 2) % L;
+    const int r = r0 + t + k;
+    if (r >= r1) return false;
+    fetch(sa, r + PF + 1);
+    const int64_t c = cell(ic, j, r);
+    T *bb = buf + (size_t)((t + k) & 1) * (W + 2) * NA;
+    T *ph = phib + ((t + k) & 1) * WN;
+    if (lane) {
+#pragma unroll
+      for (int b = 0; b < NB; b++) bb[(lc + 1) * NA + b * A + a] = col ? q[s0][b] : T(0);
+    }
+    if (he) bb[hslot] = hv ? hx[s0] : T(0);
+    __syncthreads();
+    T hn[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) hn[b] = lane ? bb[xcol * NA + b * A + a] : T(0);
+    if (SCATTER) {
+      seg_wsum(bb + NA, ws, WN, A, ph, bx.lanes);
+      if (okp) {  // the previous row: its moments passed this step's barrier
+        const T sc = ssp * g.inv_sum_w;
+        const T *pp = phib + ((t + k + 1) & 1) * WN + lc * NB;
+#pragma unroll
+        for (int b = 0; b < NB; b++) y[cp * NA + b * A + a] = acc[b] - sc * pp[b];
+      }
+    }
+    const bool upv = lo ? r > 0 : r + 1 < nR;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      T v = st[s0] * q[s0][b];
+#pragma unroll
+      for (int qq = 0; qq < NB; qq++) v += kc.M[b][qq] * q[s0][qq];
+#pragma unroll
+      for (int qq = 0; qq < NB; qq++) v += kc.U[0][b][qq] * hn[qq];
+      if (DIM == 3) {
+#pragma unroll
+        for (int qq = 0; qq < NB; qq++) v += kc.U[DIM == 3 ? 1 : 0][b][qq] * (yin ? hy[DIM == 3 ? s0 : 0][qq] : T(0));
+      }
+#pragma unroll
+      for (int qq = 0; qq < NB; qq++) v += kc.U[R][b][qq] * (upv ? (lo ? q[sm][qq] : q[sp][qq]) : T(0));
+      acc[b] = v;
+    }
+    const bool okc = col && ok[s0];
+    if (!SCATTER && okc) {
+#pragma unroll
+      for (int b = 0; b < NB; b++) y[c * NA + b * A + a] = acc[b];
+    }
+    okp = okc;
+    cp = c;
+    ssp = ss[s0];
+    return true;
+  };
+  while (unroll_while<0, L>(step)) t += L;
+  if (SCATTER) {
+    __syncthreads();
+    if (okp) {
+      const T sc = ssp * g.inv_sum_w;
+      const T *pp = phib + ((r1 - 1 - r0) & 1) * WN + lc * NB;
+#pragma unroll
+      for (int b = 0; b < NB; b++) y[cp * NA + b * A + a] = acc[b] - sc * pp[b];
+    }
+  }
+}
+
+// LDS carve of the strip march: angle weights, two rows of moments, two rows
+// of W + 2 columns (the strip and its two halo cells)
+template <class T, int NB>
+__device__ __forceinline__ void strip_carve(int A, int W, T *&ws, T *&phib, T *&buf) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  ws = reinterpret_cast<T *>(smem);
+  phib = ws + ((A + 1) & ~1);
+  buf = phib + 2 * W * NB;
+}
+
+// One problem: a persistent grid, unit u = rank + k * grid, ranked XCD-major
+// so the units one XCD runs at a time are neighbouring strips; then the
+// boundary-cell list through the tile kernel's body.
+template <class T, int DIM, int NB, int PF, bool SCATTER>
+__global__ void __launch_bounds__(kBoxThreads) dg_apply_strip(const T *__restrict__ x, T *__restrict__ y,
+                                                      OpArgs<T> g, BoxArgs bx, int64_t total) {
+  const int64_t G = gridDim.x;
+  const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  const int A = g.A, W = bx.W;
+  const int lc = threadIdx.x / A;
+  const int a = lc < W ? (int)threadIdx.x - lc * A : 0;
+  T *ws, *phib, *buf;
+  strip_carve<T, NB>(A, W, ws, phib, buf);
+  for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = ldg(g.w + e);
+  BoxConst<T, DIM, NB> kc;
+  box_consts<T, DIM, NB>(g, bx, a, kc);
+  for (int64_t u = rank; u < total; u += G) {
+    __syncthreads();  // the previous unit's last reads of the row buffers
+    strip_unit<T, DIM, NB, PF, SCATTER>(x, y, g, bx, kc, u, buf, phib, ws, lc, a);
+  }
   const int64_t E = bx.edge_tiles, ce = (E + G - 1) / G;
   const int64_t e0 = rank * ce, e1 = e0 + ce < E ? e0 + ce : E;
   for (int64_t e = e0; e < e1; e++) {
@@ -1130,10 +1347,11 @@ __global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T
   for (int64_t t = t0; t < t1; t++) {
     while (t >= tile_off[p + 1]) p++;
     const BatchItem<T> &it = items[p];
+    const OpArgs<T> &g = it.g;
     if (BODY == 0)
-      dg_tile_body<T, DIM, NB, NF, SCATTER, false, false, false>(x + it.off, y + it.off, it.g, t - tile_off[p]);
+      dg_tile_body<T, DIM, NB, NF, SCATTER, false, false, false>(x + it.off, y + it.off, g, t - tile_off[p]);
     else
-      dg_q4_body<T, DIM, NB, NF, SCATTER>(x + it.off, y + it.off, it.g, t - tile_off[p]);
+      dg_q4_body<T, DIM, NB, NF, SCATTER>(x + it.off, y + it.off, g, t - tile_off[p]);
     __syncthreads();  // the next tile restages the LDS
   }
 }
@@ -1159,7 +1377,8 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_box(const T *__res
     if (!first) __syncthreads();  // the next problem lays the LDS out anew
     first = false;
     const BatchItem<T> &it = items[p];
-    box_run<T, DIM, NB, K, SCATTER>(x + it.off, y + it.off, it.g, it.box, t - tile_off[p], e - tile_off[p]);
+    box_run<T, DIM, NB, K, SCATTER>(x + it.off, y + it.off, it.g, it.box, t - tile_off[p],
+                                    e - tile_off[p]);
     t = e;
   }
   const int64_t ce = (n_edge + G - 1) / G;
@@ -1169,7 +1388,59 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_box(const T *__res
     while (e >= edge_off[p + 1]) p++;
     __syncthreads();  // the LDS changes layout
     const BatchItem<T> &it = items[p];
-    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x + it.off, y + it.off, it.g, e - edge_off[p]);
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x + it.off, y + it.off, it.g,
+                                                                    e - edge_off[p]);
+  }
+}
+
+// Batched strip march: unit u = rank + k * grid over every problem's units
+// (the problem found by bisection of the unit prefix), so the units one XCD
+// runs at a time are neighbouring strips of one problem; the per-angle
+// matrices are refolded at each problem change. Then the workgroup's
+// contiguous share of every problem's boundary-cell tiles.
+template <class T, int DIM, int NB, int PF, bool SCATTER>
+__global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_strip(const T *__restrict__ x, T *__restrict__ y,
+                                                            const BatchItem<T> *__restrict__ items,
+                                                            const int64_t *__restrict__ unit_off, int n_items,
+                                                            int64_t total, const int64_t *__restrict__ edge_off,
+                                                            int64_t n_edge) {
+  const int64_t G = gridDim.x;
+  const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  int p = -1;
+  int lc = 0, a = 0;
+  T *ws = nullptr, *phib = nullptr, *buf = nullptr;
+  BoxConst<T, DIM, NB> kc;
+  for (int64_t u = rank; u < total; u += G) {
+    int lo = 0, hi = n_items;  // unit_off[lo] <= u < unit_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (u >= unit_off[mid]) lo = mid; else hi = mid;
+    }
+    __syncthreads();  // the previous unit's last reads of the LDS
+    const BatchItem<T> &it = items[lo];
+    const OpArgs<T> &g = it.g;
+    const BoxArgs &bx = it.box;
+    if (lo != p) {
+      p = lo;
+      const int A = g.A, W = bx.W;
+      lc = threadIdx.x / A;
+      a = lc < W ? (int)threadIdx.x - lc * A : 0;
+      strip_carve<T, NB>(A, W, ws, phib, buf);
+      for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = ldg(g.w + e);
+      box_consts<T, DIM, NB>(g, bx, a, kc);
+      __syncthreads();  // the weights
+    }
+    strip_unit<T, DIM, NB, PF, SCATTER>(x + it.off, y + it.off, g, bx, kc, u - unit_off[lo], buf, phib, ws, lc, a);
+  }
+  const int64_t ce = (n_edge + G - 1) / G;
+  const int64_t e1 = rank * ce + ce < n_edge ? rank * ce + ce : n_edge;
+  p = 0;
+  for (int64_t e = rank * ce; e < e1; e++) {
+    while (e >= edge_off[p + 1]) p++;
+    __syncthreads();  // the LDS changes layout
+    const BatchItem<T> &it = items[p];
+    dg_tile_body<T, DIM, NB, 2 * DIM, SCATTER, false, true, false>(x + it.off, y + it.off, it.g,
+                                                                    e - edge_off[p]);
   }
 }
 
@@ -1646,12 +1917,13 @@ __global__ void blockinv_batch(const T *__restrict__ x, T *__restrict__ y, const
     else hi = mid;
   }
   const BatchItem<T> &it = items[lo];
-  const int A = it.g.A;
+  const OpArgs<T> &g = it.g;
+  const int A = g.A;
   const int64_t bb = b - blk_off[lo];
   const int64_t c = bb / A;
   const int a = (int)(bb - c * A);
   T B[NB][NB], v[NB];
-  build_block<T, NB>(it.g, c, a, 0, B);
+  build_block<T, NB>(g, c, a, 0, B);
   const int64_t base = it.off + c * NB * A + a;
 #pragma unroll
   for (int j = 0; j < NB; j++) v[j] = x[base + (int64_t)j * A];
@@ -1879,6 +2151,9 @@ bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads,
   int S = pow2_floor(std::max(1, threads / (W * K * nb)));
   S = std::min(std::min(S, 64), pow2_floor(A));
   bx.lanes = std::max(1, S);
+  bx.R = 0;
+  bx.units[0] = bx.units[1] = bx.units[2] = 0;
+  bx.ok = op.n_box_edge > 0 ? op.box_ok.as<unsigned char>() : nullptr;
   return true;
 }
 
@@ -1911,6 +2186,78 @@ bool launch_apply_box(const Operator &op, OpArgs<T> a, const T *x, T *y, bool sc
   BOX(2, 3, 2) BOX(2, 3, 4) BOX(3, 4, 2) BOX(3, 4, 4)
 #undef BOX
   return launched;
+}
+
+// The strip-march layout of an operator: columns per strip (as many as 256
+// threads hold), units, threads, the list phase's tile and the LDS; false
+// where the kernel does not apply (as box_layout)
+bool strip_layout(const Operator &op, size_t ts, BoxArgs &bx, int &threads, int &cpb, int &lanes, size_t &lds) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
+  if (op.box_cls < 0 || A > 256) return false;
+  const int W = kBoxThreads / A;
+  threads = ((W * A + 63) / 64) * 64;
+  if (nb * A > threads) return false;  // one halo element per thread
+  lds = strip_lds_reals(A, W, nb) * ts;
+  // the boundary-cell phase: the tile body, cpb cells per tile
+  const size_t geo = (size_t)(2 * nf * nb * nb + nf * dim + (nb - 1) * dim + 2) * ts + sizeof(int) * (size_t)nf;
+  auto tile_lds = [&](int cp) { return (size_t)(cp * nb * A + cp * nb) * ts + (size_t)cp * geo; };
+  cpb = threads / A;
+  while (cpb > 1 && tile_lds(cpb) > 65536) cpb--;
+  if (tile_lds(cpb) > 65536 || lds > 65536) return false;
+  lds = std::max(lds, tile_lds(cpb));
+  int Se = pow2_floor(std::max(1, threads / (cpb * nb)));
+  Se = std::min(std::min(Se, 64), pow2_floor(A));
+  lanes = std::max(1, Se);
+  bx.edge_tiles = (op.n_box_edge + cpb - 1) / cpb;
+  for (int d = 0; d < 3; d++) {
+    bx.n[d] = op.box_n[d];
+    bx.fmap[d][0] = op.box_fmap[d][0];
+    bx.fmap[d][1] = op.box_fmap[d][1];
+  }
+  bx.cls = op.box_cls;
+  bx.W = W;
+  bx.tiles[0] = bx.tiles[1] = bx.tiles[2] = 0;
+  bx.R = kStripRows;
+  bx.units[0] = (op.box_n[0] + W - 1) / W;
+  bx.units[1] = dim == 3 ? op.box_n[1] : 1;
+  bx.units[2] = (op.box_n[dim - 1] + bx.R - 1) / bx.R;
+  int S = pow2_floor(std::max(1, threads / (W * nb)));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  bx.lanes = std::max(1, S);
+  bx.ok = op.box_ok.as<unsigned char>();  // read per cell by the strip march
+  return true;
+}
+
+template <class T>
+bool launch_apply_strip(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s,
+                        bool dry = false) {
+  const Disc &D = *op.disc;
+  BoxArgs bx;
+  int threads = 0, cpb = 1, lanes = 1;
+  size_t lds = 0;
+  if (!strip_layout(op, sizeof(T), bx, threads, cpb, lanes, lds)) return false;
+  if (D.nb != (D.dim == 2 ? 3 : 4)) return false;
+  if (dry) return true;
+  a.cpb = cpb;
+  a.red_lanes = lanes;
+  const int64_t total = (int64_t)bx.units[0] * bx.units[1] * bx.units[2];
+  int64_t g = std::min<int64_t>(total, (int64_t)device_cus(op.device) * 8);
+  g = std::max<int64_t>(8, (g + 7) / 8 * 8);
+  const unsigned grid = (unsigned)g;
+#define STRIP(DIM, NB)                                                                                      \
+  if (D.dim == DIM) {                                                                                       \
+    if (scatter)                                                                                            \
+      hipLaunchKernelGGL((dg_apply_strip<T, DIM, NB, kStripAhead<T>, true>), dim3(grid), dim3(threads), lds, s, x, y, a, bx, \
+                         total);                                                                            \
+    else                                                                                                    \
+      hipLaunchKernelGGL((dg_apply_strip<T, DIM, NB, kStripAhead<T>, false>), dim3(grid), dim3(threads), lds, s, x, y, a,    \
+                         bx, total);                                                                        \
+    return true;                                                                                            \
+  }
+  STRIP(2, 3) STRIP(3, 4)
+#undef STRIP
+  return false;
 }
 
 template <class T>
@@ -1954,6 +2301,10 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
   OpArgs<T> a = a0;
   const int var = op.variant[sizeof(T) == 8 ? 0 : 1];
   if (!adj && (var == 8 || var == 9) && launch_apply_box<T>(op, a, x, y, scatter, s, var == 8 ? 2 : 4)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  if (!adj && var == 10 && launch_apply_strip<T>(op, a, x, y, scatter, s)) {
     UBT_HIP(hipGetLastError());
     return;
   }
@@ -2207,6 +2558,44 @@ static int box_interior_class(const Disc &D, const std::vector<int> &cls, const 
   return cls[c];
 }
 
+// Per cell whether the box kernels' formula holds: a cell of the interior
+// class, or a cell whose faces on the box boundary are all vacuum and whose
+// geometry record matches the interior class's (within the class tolerance,
+// 1e-13 of each quantity's scale) except for the upwind face matrices of
+// those faces, which the ghost-flux treatment leaves zero: the kernels read a
+// zero trace there, so the product is the same.
+static void box_formula_cells(const Disc &D, const std::vector<int> &cls, int rep, int box_cls,
+                              std::vector<unsigned char> &ok) {
+  const int nc = D.n_cells, nf = D.nf, nb = D.nb, FM = nb * nb;
+  double scale[4] = {0, 0, 0, 0};  // nA, face matrices, gradients, volume
+  for (double v : D.face_nA) scale[0] = std::max(scale[0], std::fabs(v));
+  for (double v : D.face_own) scale[1] = std::max(scale[1], std::fabs(v));
+  for (double v : D.face_up) scale[1] = std::max(scale[1], std::fabs(v));
+  for (double v : D.basis_grad) scale[2] = std::max(scale[2], std::fabs(v));
+  for (double v : D.volume) scale[3] = std::max(scale[3], std::fabs(v));
+  for (double &v : scale) v = (v > 0.0 ? v : 1.0) * 1e-13;
+  auto same = [&](double a, double b, int k) { return std::fabs(a - b) <= scale[k]; };
+  ok.assign(nc, 0);
+  for (int c = 0; c < nc; c++) {
+    if (cls[c] == box_cls) {
+      ok[c] = 1;
+      continue;
+    }
+    bool m = same(D.volume[c], D.volume[rep], 3);
+    for (int e = 0; e < 3 * nb && m; e++) m = same(D.basis_grad[(size_t)c * 3 * nb + e], D.basis_grad[(size_t)rep * 3 * nb + e], 2);
+    for (int f = 0; f < nf && m; f++) {
+      const int nbr = D.face_nb[(size_t)c * nf + f];
+      if (nbr < -1) m = false;  // reflective: the mirrored angle of the same cell
+      for (int e = 0; e < 3 && m; e++) m = same(D.face_nA[((size_t)c * nf + f) * 3 + e], D.face_nA[((size_t)rep * nf + f) * 3 + e], 0);
+      for (int e = 0; e < FM && m; e++) {
+        const size_t kc = ((size_t)c * nf + f) * FM + e, kr = ((size_t)rep * nf + f) * FM + e;
+        m = same(D.face_own[kc], D.face_own[kr], 1) && (nbr == -1 || same(D.face_up[kc], D.face_up[kr], 1));
+      }
+    }
+    ok[c] = m ? 1 : 0;
+  }
+}
+
 static void upload_prec(PrecTables &t, const Disc &D, const std::vector<int> &rep, bool fp32) {
   std::vector<double> omega(D.quad.omega), w(D.quad.w);
   std::vector<double> face_s, face_own, face_up, grad, vol;
@@ -2275,18 +2664,16 @@ std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups
     op->box = detect_box(D, op->box_n, op->box_fmap);
     if (op->box && D.nb > 1) op->box_cls = box_interior_class(D, cls, op->box_n);
     if (op->box_cls >= 0) {
-      // the cells the box-tile kernel leaves to its list phase: the box's
-      // boundary layer and any cell of another class
-      const int *n = op->box_n;
+      // the cells the box kernels leave to their list phase: those the box
+      // formula does not cover (a reflective face, another geometry)
+      std::vector<unsigned char> ok;
+      box_formula_cells(D, cls, op->geo_rep[op->box_cls], op->box_cls, ok);
       std::vector<int> edge;
-      for (int64_t c = 0; c < D.n_cells; c++) {
-        const int64_t ix = c % n[0], iy = (c / n[0]) % n[1], iz = c / ((int64_t)n[0] * n[1]);
-        bool e = cls[c] != op->box_cls || ix == 0 || ix == n[0] - 1 || iy == 0 || iy == n[1] - 1;
-        if (D.dim == 3) e = e || iz == 0 || iz == n[2] - 1;
-        if (e) edge.push_back((int)c);
-      }
+      for (int64_t c = 0; c < D.n_cells; c++)
+        if (!ok[c]) edge.push_back((int)c);
       op->box_edge.upload(edge);
       op->n_box_edge = (int)edge.size();
+      op->box_ok.upload(ok);
     }
   }
   upload_prec(op->t64, D, op->geo_rep, false);
@@ -2374,6 +2761,9 @@ void Operator::tune() {
     if (v == 8 || v == 9)
       return p == 0 ? launch_apply_box<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, 2, true)
                     : launch_apply_box<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, 2, true);
+    if (v == 10)
+      return p == 0 ? launch_apply_strip<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
+                    : launch_apply_strip<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);
     if (v == 7)
       return p == 0 ? launch_apply_q4<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
                     : launch_apply_q4<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);
@@ -2629,6 +3019,14 @@ bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int
     *n_tiles = (int64_t)box->tiles[0] * box->tiles[1] * box->tiles[2];
     return true;
   }
+  if (body == 3) {
+    int threads = 0;
+    if (D.nb != (D.dim == 2 ? 3 : 4) ||
+        !strip_layout(op, prec == UBT_FP64 ? 8 : 4, *box, threads, *cpb_out, *lanes_out, *lds_out))
+      return false;
+    *n_tiles = (int64_t)box->units[0] * box->units[1] * box->units[2];
+    return true;
+  }
   const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
   const size_t ts = prec == UBT_FP64 ? 8 : 4;
   const size_t geo = (size_t)((nb > 1 ? 2 * nf * nb * nb : 0) + nf * dim + (nb > 1 ? (nb - 1) * dim : 0) + 2) * ts +
@@ -2688,8 +3086,8 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   BT_ALL(1)
 #undef BT_ALL
 #undef BT
-  const int64_t *eoff = b.edge_off[pi].as<int64_t>();
-  const int64_t n_edge = b.n_edge[pi];
+  const int64_t *eoff = body >= 2 ? b.edge_off[pi][body].as<int64_t>() : nullptr;
+  const int64_t n_edge = body >= 2 ? b.n_edge[pi][body] : 0;
 #define BB(DIM, NB)                                                                                    \
   if (!launched && b.dim == DIM && b.nb == NB && body == 2) {                                          \
     if (scatter)                                                                                       \
@@ -2702,6 +3100,19 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   }
   BB(2, 3) BB(3, 4)
 #undef BB
+  const int nit = b.n_items[pi][body];
+#define BS(DIM, NB)                                                                                    \
+  if (!launched && b.dim == DIM && b.nb == NB && body == 3) {                                          \
+    if (scatter)                                                                                       \
+      hipLaunchKernelGGL((dg_apply_batch_strip<T, DIM, NB, kStripAhead<T>, true>), dim3(grid), dim3(kBoxThreads), lds, \
+                         s, x, y, items, toff, nit, total, eoff, n_edge);                              \
+    else                                                                                               \
+      hipLaunchKernelGGL((dg_apply_batch_strip<T, DIM, NB, kStripAhead<T>, false>), dim3(grid), dim3(kBoxThreads), lds, \
+                         s, x, y, items, toff, nit, total, eoff, n_edge);                              \
+    launched = true;                                                                                   \
+  }
+  BS(2, 3) BS(3, 4)
+#undef BS
   UBT_CHECK(launched, UBT_ERR_SUP, "no batched apply for dim %d, %d basis, %d faces", b.dim, b.nb, b.nf);
   UBT_HIP(hipGetLastError());
 }

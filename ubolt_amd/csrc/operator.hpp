@@ -50,8 +50,9 @@ struct PrecTables {
 // apply kernels the tuner chooses from: 0 tile, 1 wavefront per cell,
 // 2 and 3 the multi-cell tile with 2 and 4 cells per thread, 4-6 the pencil
 // kernel of a structured box marching 8, 16 or 32 cells per workgroup, 7 the
-// four-angle tile, 8 and 9 the box tile with 2 and 4 rows per thread
-constexpr int kApplyVariants = 10;
+// four-angle tile, 8 and 9 the box tile with 2 and 4 rows per thread, 10 the
+// strip march on a uniform DG1 box
+constexpr int kApplyVariants = 11;
 constexpr int kPencilMaxClasses = 64;
 
 // The box-tile kernels' view of a uniform DG1 box: a tile is W columns along
@@ -71,7 +72,18 @@ struct BoxArgs {
   int tiles[3];     // tiles along x, y (1 in 2D) and the register axis
   int lanes;        // lanes per angular moment in the tile's segmented sum
   int edge_tiles;   // tiles of the boundary-cell list (OpArgs cpb cells each)
+  // the strip march: a unit is W columns (at one y index in 3D) by a band of
+  // R cells along the last axis; units[] counts strips along x, y indices
+  // (1 in 2D) and bands
+  int R;
+  int units[3];
+  // per cell 1 where the box formula holds (interior class, or a boundary
+  // cell whose box-boundary faces are all vacuum); null when every cell does
+  const unsigned char *ok;
 };
+
+// the strip-march unit's band length along the last axis
+constexpr int kStripRows = 64;
 
 struct Operator {
   std::shared_ptr<Disc> disc;
@@ -98,6 +110,7 @@ struct Operator {
   int box_cls = -1;
   DevBuf box_edge;            // int: the cells the box-tile kernel leaves to its list phase
   int n_box_edge = 0;
+  DevBuf box_ok;              // uint8 per cell: the box formula holds (see BoxArgs::ok)
   DevBuf face_nA_raw;         // double, 3 per slot (rhs inflow, DSA normals)
   DevBuf volume_d, face_area_d, face_dist_d;  // double: cells, slots, 2*slots (DSA assembly)
   DevBuf inv_volume;          // double, cells (rhs inflow)
