@@ -324,6 +324,17 @@ int ubt_dsa_apply_transpose(ubt_dsa *dsa, int prec, const void *x, void *y, void
 int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream);
 /* x = D_diff^{-1} b by the inner solve (x starts at zero). */
 int ubt_dsa_solve(ubt_dsa *dsa, const double *b, double *x, void *stream);
+/* x = D_diff^{-1} b exactly as ubt_dsa_solve, allocating nothing: the inner
+ * CG's workspace (four node vectors, block partial sums, scalars) is a
+ * file-scope device array of 2^21 nodes (64 MiB per GPU, held until the
+ * process ends), so the call makes no hipMalloc/hipFree or other allocating
+ * call. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture - set the inner check interval to 0
+ * so no host read is made -, tight serving loops, harnesses that forbid
+ * allocation). An operator of more than 2^21 diffusion nodes is refused with
+ * UBT_ERR_ARG_SIZ. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams, two threads) race. */
+int ubt_dsa_solve_noalloc(ubt_dsa *dsa, const double *b, double *x, void *stream);
 /* Inner iterations the last solve took. */
 int ubt_dsa_info(ubt_dsa *dsa, int *inner_iterations, int *n_nodes);
 int ubt_dsa_destroy(ubt_dsa *dsa);

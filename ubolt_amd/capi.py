@@ -95,6 +95,7 @@ SIGNATURES = {
     "ubt_dsa_apply_transpose": [P, I, P, P, P],
     "ubt_dsa_matvec": [P, P, P, P],
     "ubt_dsa_solve": [P, P, P, P],
+    "ubt_dsa_solve_noalloc": [P, P, P, P],
     "ubt_dsa_info": [P, PI, PI],
     "ubt_dsa_destroy": [P],
     "ubt_sweep_create": [P, PP],

@@ -500,6 +500,14 @@ int ubt_dsa_solve(ubt_dsa *dsa, const double *b, double *x, void *stream) {
     dsa->d.solve(b, x, (hipStream_t)stream);
   });
 }
+int ubt_dsa_solve_noalloc(ubt_dsa *dsa, const double *b, double *x, void *stream) {
+  return guard([&] {
+    NEED(dsa);
+    NEED(b);
+    NEED(x);
+    dsa->d.solve_noalloc(b, x, (hipStream_t)stream);
+  });
+}
 int ubt_dsa_info(ubt_dsa *dsa, int *inner_iterations, int *n_nodes) {
   return guard([&] {
     NEED(dsa);

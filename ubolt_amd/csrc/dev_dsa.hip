@@ -16,6 +16,9 @@
 // blend (D^p + (m h)^p)^(1/p), p = 1.5, on FD and DG0 faces.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include <cmath>
 
 #include "dsa.hpp"
@@ -581,13 +584,25 @@ void DSA::matvec(const double *x, double *y, hipStream_t s) const {
   UBT_HIP(hipGetLastError());
 }
 
-template <int NB> static void cg_run(DSA &d, const double *b, const double *wgt, double *x, hipStream_t s) {
+// The inner CG's workspace: four node vectors, the per-block partial sums
+// and the CG scalars (every entry is written before it is read)
+struct CgWork {
+  double *r, *z, *p, *q, *part, *sc;
+};
+
+static CgWork own_work(DSA &d) {
+  return {d.r.as<double>(), d.z.as<double>(), d.p.as<double>(), d.q.as<double>(), d.partial.as<double>(),
+          d.scal.as<double>()};
+}
+
+template <int NB>
+static void cg_run(DSA &d, const double *b, const double *wgt, double *x, hipStream_t s, const CgWork &w) {
   const MvArgs m = mv_args(d);
   const int64_t N = d.n_nodes;
   const unsigned gnode = (unsigned)((N + kCgThreads - 1) / kCgThreads);
   const unsigned gcell = (unsigned)((d.n_cells + kCgThreads - 1) / kCgThreads);
-  double *sc = d.scal.as<double>(), *part = d.partial.as<double>();
-  double *r = d.r.as<double>(), *z = d.z.as<double>(), *p = d.p.as<double>(), *q = d.q.as<double>();
+  double *sc = w.sc, *part = w.part;
+  double *r = w.r, *z = w.z, *p = w.p, *q = w.q;
   hipLaunchKernelGGL(cg_init<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, b, wgt, x, r, z, p, part);
   hipLaunchKernelGGL(cg_reduce_init, dim3(1), dim3(64), 0, s, part, (int)gcell, sc, d.opt.rtol * d.opt.rtol);
   for (int it = 0; it < d.opt.max_it; it++) {
@@ -605,15 +620,64 @@ template <int NB> static void cg_run(DSA &d, const double *b, const double *wgt,
   UBT_HIP(hipGetLastError());
 }
 
+static void cg_dispatch(DSA &d, const double *b, const double *wgt, double *x, hipStream_t s, const CgWork &w) {
+  if (d.nb == 1)
+    cg_run<1>(d, b, wgt, x, s, w);
+  else if (d.nb == 3)
+    cg_run<3>(d, b, wgt, x, s, w);
+  else
+    cg_run<4>(d, b, wgt, x, s, w);
+}
+
 void DSA::solve(const double *b, double *x, hipStream_t s) {
   UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
             "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
-  if (nb == 1)
-    cg_run<1>(*this, b, nullptr, x, s);
-  else if (nb == 3)
-    cg_run<3>(*this, b, nullptr, x, s);
-  else
-    cg_run<4>(*this, b, nullptr, x, s);
+  cg_dispatch(*this, b, nullptr, x, s, own_work(*this));
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free inner solve. Its workspace is the file-scope array
+// below: one copy per GPU, which the HIP runtime places when it loads this
+// code object on that GPU and which lives until the process ends. In host
+// code the array's name is only a stand-in, so its device address is looked
+// up once per GPU (hipGetSymbolAddress, with that GPU current) and that
+// address is what the kernels get. Every call on one GPU shares the array:
+// calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocBlocks = (kNoAllocNodes + kCgThreads - 1) / kCgThreads;
+constexpr int64_t kNoAllocDoubles = 4 * kNoAllocNodes + kNoAllocBlocks + 8;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static std::mutex scratch_mu;
+static std::vector<double *> scratch_ptr;  // per device: the array's address there
+
+static double *scratch_on(int dev) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if ((int)scratch_ptr.size() <= dev) scratch_ptr.resize(dev + 1, nullptr);
+  if (!scratch_ptr[dev]) {
+    int cur = 0;
+    UBT_HIP(hipGetDevice(&cur));
+    if (cur != dev) UBT_HIP(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) UBT_HIP(hipSetDevice(cur));
+    UBT_HIP(e);
+    UBT_CHECK(ptr != nullptr, UBT_ERR_GPU, "no device address for the inner-solve workspace on device %d", dev);
+    scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return scratch_ptr[dev];
+}
+
+void DSA::solve_noalloc(const double *b, double *x, hipStream_t s) {
+  UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
+            "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
+  UBT_CHECK(n_nodes <= kNoAllocNodes, UBT_ERR_ARG_SIZ,
+            "the allocation-free inner solve takes at most %lld diffusion nodes, this operator has %lld",
+            (long long)kNoAllocNodes, (long long)n_nodes);
+  double *base = scratch_on(op->device);
+  const int64_t N = n_nodes, nblk = (N + kCgThreads - 1) / kCgThreads;
+  const CgWork w = {base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 4 * N + nblk};
+  cg_dispatch(*this, b, nullptr, x, s, w);
 }
 
 void DSA::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose) {
@@ -626,12 +690,7 @@ void DSA::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose)
   op->moments_f64(prec, x, rhs.as<double>(), s, transpose);
   double *xs = sol.as<double>();
   const double *w = transpose ? nullptr : weight.as<double>();
-  if (nb == 1)
-    cg_run<1>(*this, rhs.as<double>(), w, xs, s);
-  else if (nb == 3)
-    cg_run<3>(*this, rhs.as<double>(), w, xs, s);
-  else
-    cg_run<4>(*this, rhs.as<double>(), w, xs, s);
+  cg_dispatch(*this, rhs.as<double>(), w, xs, s, own_work(*this));
   const int64_t rows = op->disc->n_rows();
   const unsigned grid = (unsigned)((rows + 255) / 256);
   const double inv_w = 1.0 / op->disc->quad.sum_weights;

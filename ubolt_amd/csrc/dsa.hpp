@@ -8,6 +8,10 @@
 
 namespace ubt {
 
+// the most diffusion nodes the allocation-free inner solve takes (2^21: a
+// 128^3 structured box; its workspace is 4 node vectors, 64 MiB per GPU)
+constexpr int64_t kNoAllocNodes = int64_t(1) << 21;
+
 struct DSAOptions {
   int consistent_d = 1;      // blend the upwind numerical diffusion m h into face D
   double consistent_power = 1.5;
@@ -47,6 +51,9 @@ struct DSA {
   void matvec(const double *x, double *y, hipStream_t s) const;
   // sol = D^{-1} rhs by the inner CG (x0 = 0)
   void solve(const double *b, double *x, hipStream_t s);
+  // the same solve with its workspace in a file-scope device array (at most
+  // kNoAllocNodes nodes): allocates nothing
+  void solve_noalloc(const double *b, double *x, hipStream_t s);
 };
 
 }  // namespace ubt
