@@ -233,8 +233,9 @@ int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
 /* The apply kernel variant per precision: 0 the LDS-staged tile kernel (whole
  * cells, every angle, per workgroup), 1 the wavefront-per-cell kernel
  * (per-cell data through the scalar cache, no barrier), 2 and 3 the
- * multi-cell tile kernel (2 or 4 cells per thread, all loads before the first
- * barrier; DG only), 4, 5 and 6 the pencil kernel (DG on a structured box:
+ * multi-cell tile kernel (DG: 2 or 4 cells per thread, all loads before the
+ * first barrier; structured FD: a run of 4 or 8 cells along x per lane, the x
+ * upwind trace from registers), 4, 5 and 6 the pencil kernel (DG on a structured box:
  * workgroups march 8, 16 or 32 cells along the last axis with the upwind
  * traces in registers and LDS), 7 the four-angle tile kernel (a thread owns
  * four consecutive angles of a cell: 16-byte loads, per-cell work shared;

@@ -84,6 +84,52 @@ __device__ inline int64_t xcd_tile(int64_t n_tiles) {
 // ADJ: the adjoint A^T - direction reversed (the upwind neighbour becomes the
 // downwind one), the scatter transposed (unweighted moment, w_a on output).
 // On a grid every cell has the same volume, so no volume ratio appears.
+// The grid indices of cell c (a grid holds fewer than 2^31 cells, so 32-bit
+// division)
+template <int DIM>
+__device__ __forceinline__ void fd_index(const int n[3], int64_t c, int idx[3]) {
+  const unsigned cu = (unsigned)c, n0 = (unsigned)n[0], n1 = (unsigned)n[1];
+  const unsigned q = cu / n0;
+  idx[0] = (int)(cu - q * n0);
+  idx[1] = DIM == 3 ? (int)(q % n1) : (int)q;
+  idx[2] = DIM == 3 ? (int)(q / n1) : 0;
+}
+
+// Streaming + removal of row (c, a): every load is issued before any is
+// used - the upwind neighbour's index is clamped to the cell itself where
+// there is none and its weight is then zero - so the own trace and the DIM
+// upwind traces are in flight together (a load under a branch, or used right
+// after it, would serialise them). A reflective boundary face, whose trace is
+// the cell's own reflected angle, takes one more load on a branch only
+// boundary lanes enter.
+template <class T, int DIM, bool ADJ>
+__device__ __forceinline__ T fd_row(const T *__restrict__ x, const OpArgs<T> &g, int64_t c, const int idx[3], int a,
+                                    T &psi) {
+  const int A = g.A;
+  const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
+  T cd[DIM], nbv[DIM];
+  bool refl[DIM];
+  psi = x[c * A + a];
+  const T sig = g.sigt[c];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    const T cs = ADJ ? -g.omega[3 * a + d] : g.omega[3 * a + d];
+    const int up = cs > T(0) ? idx[d] - 1 : idx[d] + 1;
+    const bool in = cs != T(0) && up >= 0 && up < g.n[d];
+    refl[d] = cs != T(0) && !in && g.face_type[d][cs > T(0) ? 0 : 1] == kReflect;
+    cd[d] = fabs(cs) * g.invh[d];
+    nbv[d] = x[(in ? c + (cs > T(0) ? -stride[d] : stride[d]) : c) * A + a];
+    if (!in) nbv[d] = T(0);
+  }
+  T acc = sig * psi;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) acc += cd[d] * psi - cd[d] * nbv[d];
+#pragma unroll
+  for (int d = 0; d < DIM; d++)
+    if (refl[d]) acc -= cd[d] * x[c * A + g.reflect[3 * a + d]];
+  return acc;
+}
+
 template <class T, int DIM, bool SCATTER, bool ADJ = false>
 __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -95,33 +141,14 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
   const int lc = threadIdx.x / A, a = threadIdx.x - lc * A;
   const int64_t c = tile * g.cpb + lc;
   const bool live = lc < g.cpb && c < g.n_cells;
-  T psi = T(0), acc = T(0);
+  // every thread computes on a valid cell; the dead ones store nothing
+  const int64_t cl = live ? c : tile * g.cpb;
+  int idx[3];
+  fd_index<DIM>(g.n, cl, idx);
   // the scatter's cross section too is loaded before the barriers
-  const T ssg = SCATTER && live ? g.sigs[c] : T(0);
-  if (live) {
-    psi = x[c * A + a];
-    acc = g.sigt[c] * psi;
-    int idx[3];
-    int64_t cc = c;
-    idx[0] = (int)(cc % g.n[0]);
-    cc /= g.n[0];
-    idx[1] = (int)(cc % g.n[1]);
-    idx[2] = (int)(cc / g.n[1]);
-    const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      const T cs = ADJ ? -g.omega[3 * a + d] : g.omega[3 * a + d];
-      const T cd = fabs(cs) * g.invh[d];
-      acc += cd * psi;
-      if (cs != T(0)) {
-        const int up = cs > T(0) ? idx[d] - 1 : idx[d] + 1;
-        if (up >= 0 && up < g.n[d])
-          acc -= cd * x[(c + (cs > T(0) ? -stride[d] : stride[d])) * A + a];
-        else if (g.face_type[d][cs > T(0) ? 0 : 1] == kReflect)
-          acc -= cd * x[c * A + g.reflect[3 * a + d]];
-      }
-    }
-  }
+  const T ssg = SCATTER ? g.sigs[cl] : T(0);
+  T psi;
+  T acc = fd_row<T, DIM, ADJ>(x, g, cl, idx, a, psi);
   if (SCATTER) {
     if (lc < g.cpb) red[lc * A + a] = live ? (ADJ ? psi : g.w[a] * psi) : T(0);
     __syncthreads();
@@ -130,6 +157,108 @@ __global__ void __launch_bounds__(1024) fd_apply(const T *__restrict__ x, T *__r
     if (live) acc -= ssg * g.inv_sum_w * (ADJ ? g.w[a] : T(1)) * phi[lc];
   }
   if (live) y[c * A + a] = acc;
+}
+
+// Multi-cell structured apply: a lane column owns CPT consecutive cells (one
+// angle per thread), so each workgroup moves CPT times the data of the tile
+// kernel per barrier pair, and every load - the CPT own traces, their y (and
+// z) upwind traces, the one x halo beyond the run - is in flight together.
+// The x upwind trace of every cell but the run's first (last, for a negative
+// cosine) is the neighbouring cell's own trace, already in registers.
+template <class T, int DIM, int CPT, bool SCATTER>
+__global__ void __launch_bounds__(256) fd_apply_mc(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int A = g.A, R = g.cpb;
+  const int64_t per = (int64_t)R * CPT;
+  T *red = reinterpret_cast<T *>(smem);
+  T *phi = red + per * A;
+  const int64_t tile = xcd_tile(((int64_t)g.n_cells + per - 1) / per);
+  if (tile * per >= g.n_cells) return;
+  const int lc = threadIdx.x / A, a = threadIdx.x - lc * A;
+  const bool lane = lc < R;
+  const int64_t c0 = tile * per + (int64_t)(lane ? lc : 0) * CPT;
+  const int64_t last = g.n_cells - 1;
+  const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
+  T cs[DIM], cd[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) {
+    cs[d] = g.omega[3 * a + d];
+    cd[d] = fabs(cs[d]) * g.invh[d];
+  }
+  int idx0[3];
+  fd_index<DIM>(g.n, c0 < last ? c0 : last, idx0);
+  // the loads
+  T p[CPT], st[CPT], ss[CPT], nb[DIM > 1 ? DIM - 1 : 1][CPT];
+  bool live[CPT];
+  int ix[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    const int64_t c = c0 + k < last ? c0 + k : last;
+    live[k] = lane && c0 + k <= last;
+    int idx[3];
+    fd_index<DIM>(g.n, c, idx);
+    ix[k] = idx[0];
+    p[k] = x[c * A + a];
+    st[k] = g.sigt[c];
+    ss[k] = SCATTER ? g.sigs[c] : T(0);
+#pragma unroll
+    for (int d = 1; d < DIM; d++) {
+      const int up = cs[d] > T(0) ? idx[d] - 1 : idx[d] + 1;
+      const bool in = cs[d] != T(0) && up >= 0 && up < g.n[d];
+      const T v = x[(in ? c + (cs[d] > T(0) ? -stride[d] : stride[d]) : c) * A + a];
+      nb[d - 1][k] = in ? v : T(0);
+    }
+  }
+  // the x halo: the cell before the run (positive cosine) or after it
+  const bool pos = cs[0] > T(0);
+  const int64_t ch = pos ? c0 - 1 : c0 + CPT;
+  const T hx = x[(ch < 0 ? 0 : (ch > last ? last : ch)) * A + a];
+  T acc[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    T v = st[k] * p[k];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) v += cd[d] * p[k];
+#pragma unroll
+    for (int d = 1; d < DIM; d++) v -= cd[d] * nb[d - 1][k];
+    // x upwind: inside the box unless the cell is on the inflow x face
+    const bool xin = cs[0] != T(0) && (pos ? ix[k] > 0 : ix[k] < g.n[0] - 1);
+    const T ux = pos ? (k > 0 ? p[k > 0 ? k - 1 : 0] : hx) : (k < CPT - 1 ? p[k < CPT - 1 ? k + 1 : 0] : hx);
+    if (xin) v -= cd[0] * ux;
+    acc[k] = v;
+  }
+  // reflective faces: the cell's own reflected angle, on a branch only
+  // boundary lanes enter
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    if (!live[k]) continue;
+    int idx[3];
+    fd_index<DIM>(g.n, c0 + k, idx);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      if (cs[d] == T(0)) continue;
+      const int up = cs[d] > T(0) ? idx[d] - 1 : idx[d] + 1;
+      if ((up < 0 || up >= g.n[d]) && g.face_type[d][cs[d] > T(0) ? 0 : 1] == kReflect)
+        acc[k] -= cd[d] * x[(c0 + k) * A + g.reflect[3 * a + d]];
+    }
+  }
+  if (SCATTER) {
+    if (lane) {
+      const T wa = g.w[a];
+#pragma unroll
+      for (int k = 0; k < CPT; k++) red[(lc * CPT + k) * A + a] = live[k] ? wa * p[k] : T(0);
+    }
+    __syncthreads();
+    seg_sum(red, (int)per, A, phi, g.red_lanes);
+    __syncthreads();
+    if (lane) {
+#pragma unroll
+      for (int k = 0; k < CPT; k++) acc[k] -= ss[k] * g.inv_sum_w * phi[lc * CPT + k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CPT; k++)
+    if (live[k]) y[(c0 + k) * A + a] = acc[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1578,37 +1707,13 @@ __global__ void __launch_bounds__(256) fd_apply_wave(const T *__restrict__ x, T 
   if (tile >= n_tiles || c >= g.n_cells) return;
   const int A = g.A;
   int idx[3];
-  {
-    int64_t cc = c;
-    idx[0] = (int)(cc % g.n[0]);
-    cc /= g.n[0];
-    idx[1] = (int)(cc % g.n[1]);
-    idx[2] = (int)(cc / g.n[1]);
-  }
-  const int64_t stride[3] = {1, g.n[0], (int64_t)g.n[0] * g.n[1]};
-  const T sig = g.sigt[c];
+  fd_index<DIM>(g.n, c, idx);
   T psi[APL], acc[APL];
 #pragma unroll
   for (int q = 0; q < APL; q++) {
-    const int a = lane + 64 * q;
-    psi[q] = T(0);
-    acc[q] = T(0);
-    if (a >= A) continue;
-    psi[q] = x[c * A + a];
-    T v = sig * psi[q];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      const T cs = g.omega[3 * a + d];
-      const T cd = fabs(cs) * g.invh[d];
-      v += cd * psi[q];
-      if (cs == T(0)) continue;
-      const int up = cs > T(0) ? idx[d] - 1 : idx[d] + 1;
-      if (up >= 0 && up < g.n[d])
-        v -= cd * x[(c + (cs > T(0) ? -stride[d] : stride[d])) * A + a];
-      else if (g.face_type[d][cs > T(0) ? 0 : 1] == kReflect)
-        v -= cd * x[c * A + g.reflect[3 * a + d]];
-    }
-    acc[q] = v;
+    // lanes past the last angle compute on the last one and store nothing
+    const int a = lane + 64 * q < A ? lane + 64 * q : A - 1;
+    acc[q] = fd_row<T, DIM, false>(x, g, c, idx, a, psi[q]);
   }
   if (SCATTER) {
     T part = T(0);
@@ -2260,11 +2365,44 @@ bool launch_apply_strip(const Operator &op, OpArgs<T> a, const T *x, T *y, bool 
   return false;
 }
 
+// The structured multi-cell apply, CPT cells per lane column
+template <class T>
+bool launch_apply_fd_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int cpt,
+                        bool dry = false) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles;
+  if (D.kind != DISC_FD || A > 256) return false;
+  const int R = 256 / A;
+  const int64_t per = (int64_t)R * cpt;
+  const size_t lds = (size_t)(per * A + per) * sizeof(T);
+  if (lds > 65536) return false;
+  if (dry) return true;
+  const int threads = ((R * A + 63) / 64) * 64;
+  int S = pow2_floor(std::max<int64_t>(1, threads / per));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  a.cpb = R;
+  a.red_lanes = std::max(1, S);
+  const int64_t n_tiles = (D.n_cells + per - 1) / per;
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+#define FDMC(DIM, CPT)                                                                                        \
+  if (D.dim == DIM && cpt == CPT) {                                                                           \
+    if (scatter)                                                                                              \
+      hipLaunchKernelGGL((fd_apply_mc<T, DIM, CPT, true>), dim3(grid), dim3(threads), lds, s, x, y, a);       \
+    else                                                                                                      \
+      hipLaunchKernelGGL((fd_apply_mc<T, DIM, CPT, false>), dim3(grid), dim3(threads), lds, s, x, y, a);      \
+    return true;                                                                                              \
+  }
+  FDMC(2, 4) FDMC(2, 8) FDMC(3, 4) FDMC(3, 8)
+#undef FDMC
+  return false;
+}
+
 template <class T>
 bool launch_apply_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int cpt) {
   const Disc &D = *op.disc;
   const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
-  if (D.kind != DISC_DG || A > 256) return false;
+  if (D.kind == DISC_FD) return launch_apply_fd_mc<T>(op, a, x, y, scatter, s, cpt == 2 ? 4 : 8);
+  if (A > 256) return false;
   const int R = 256 / A, tcells = R * cpt;
   const size_t reals = (size_t)tcells * ((nb > 1 ? 2 * nf * nb * nb + (nb - 1) * dim : 0) + nb * A + nb);
   const size_t lds = reals * sizeof(T);
@@ -2770,7 +2908,12 @@ void Operator::tune() {
     if (v >= 4)
       return p == 0 ? launch_apply_pencil<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, 8, true)
                     : launch_apply_pencil<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, 8, true);
-    if (D.kind != DISC_DG || D.n_angles > 256) return false;
+    if (D.kind == DISC_FD)
+      return p == 0 ? launch_apply_fd_mc<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, v == 2 ? 4 : 8,
+                                                 true)
+                    : launch_apply_fd_mc<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, v == 2 ? 4 : 8,
+                                                true);
+    if (D.n_angles > 256) return false;
     const int R = 256 / D.n_angles, tcells = R * (v == 2 ? 2 : 4);
     const size_t reals = (size_t)tcells * ((D.nb > 1 ? 2 * D.nf * D.nb * D.nb + (D.nb - 1) * D.dim : 0) +
                                            D.nb * D.n_angles + D.nb);
