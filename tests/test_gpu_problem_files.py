@@ -65,3 +65,32 @@ def test_problem_file_upscatter(gpu, name):
         rel = np.linalg.norm(r) / np.linalg.norm(b)
         print(f"[case] {name} group {g}: multigroup residual {rel:.2e} after {outers} outer passes", flush=True)
         assert rel <= 1e-9, (g, rel)
+
+
+@pytest.mark.parametrize("name,max_dsa", [("box_diffusive", 7), ("cube_diffusive", 7), ("plex_box_diffusive_dg1", 9),
+                                          ("plex_cube_diffusive_dg1", 9)])
+def test_dsa_in_the_diffusive_regime(gpu, name, max_dsa):
+    """the reference's diffusive problem files (sigma_t 100 over 0.1-wide cells,
+    scattering ratio 0.99): the DSA stage after the sweep cuts the Krylov
+    iterations the way the reference's DSA pins do (docs/dev/iteration_counts.md:
+    box_diffusive 29 -> 6, plex_box_diffusive_dg1 34 -> 7, plex_cube_diffusive_dg1
+    8 with DSA; those counts are with PCAIR, this build's stage is the sweep)"""
+    import ubolt_amd as u
+    pf = pfm.load(os.path.join(PROBLEMS, name + ".json"))
+    op = pf.problem.operator()
+    its = {}
+    for use in (False, True):
+        d = None
+        if use:
+            d = u.DSA(op, inner_max_it=200, inner_rtol=1e-6)
+            d.set_group(0)
+        sol = u.Solver(op, d, rtol=1e-8, max_it=500, pc_dsa=use)
+        sol.refresh()
+        b = op.vec()
+        op.rhs(0, b)
+        x = op.vec()
+        res = sol.solve(b, x)
+        assert res["reason"] == 1
+        its[use] = res["iterations"]
+    print(f"[case] {name}: FGMRES iterations without DSA {its[False]}, with DSA {its[True]}", flush=True)
+    assert its[True] <= max_dsa and 3 * its[True] <= its[False]

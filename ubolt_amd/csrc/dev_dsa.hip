@@ -242,7 +242,8 @@ __global__ void dsa_matvec(MvArgs m, const double *__restrict__ x, double *__res
 
 // ---------------------------------------------------------------------------
 // CG with device scalars. scal: 0 rz, 1 rz0, 2 alpha, 3 beta, 4 done, 5 its,
-// 6 rtol^2. Partial sums per block, reduced in fixed order by one block.
+// 6 rtol^2, 7 the reductions' ticket (an int). Partial sums per block,
+// reduced in fixed order by the block that finishes last.
 // ---------------------------------------------------------------------------
 constexpr int kCgThreads = 256;
 
@@ -303,9 +304,10 @@ __device__ inline double wave_total(const double *partial, int nblk) {
   return s;
 }
 
-__global__ void cg_reduce_init(const double *partial, int nblk, double *scal, double rtol2) {
+__global__ void cg_reduce_init(const double *partial, int nblk, double *scal, double rtol2, int *ticket) {
   const double s = wave_total(partial, nblk);
   if (threadIdx.x != 0) return;
+  *ticket = 0;
   scal[0] = s;
   scal[1] = s;
   scal[4] = (s == 0.0) ? 1.0 : 0.0;
@@ -313,9 +315,35 @@ __global__ void cg_reduce_init(const double *partial, int nblk, double *scal, do
   scal[6] = rtol2;
 }
 
+// The block that finishes last (an atomic ticket taken after its partial is
+// published) sums every block's partial in index order - one wavefront's
+// lane-strided sum and butterfly, the fixed order the separate reduction
+// kernel used - so the reduction needs no launch of its own; the ticket is
+// reset for the next one. True on that block's thread 0 only, with the total.
+__device__ inline bool last_block_total(double v, double *partial, int *ticket, double &total) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partial[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last || threadIdx.x >= 64) return false;
+  __threadfence();
+  double s = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += 64)
+    s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x != 0) return false;
+  __hip_atomic_store(ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  total = s;
+  return true;
+}
+
+// q = D p and p.q; the last block forms alpha
 template <int NB>
 __global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ p, double *__restrict__ q, double *partial,
-                              const double *scal) {
+                              double *scal, int *ticket) {
   if (scal[4] != 0.0) return;
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
@@ -325,20 +353,18 @@ __global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ p, double *__
     v = p[n] * qq;
   }
   v = block_sum(v);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+  double s;
+  if (last_block_total(v, partial, ticket, s)) {
+    scal[2] = s != 0.0 ? scal[0] / s : 0.0;
+    if (s == 0.0) scal[4] = 1.0;
+  }
 }
 
-__global__ void cg_alpha(const double *partial, int nblk, double *scal) {
-  if (scal[4] != 0.0) return;
-  const double s = wave_total(partial, nblk);
-  if (threadIdx.x != 0) return;
-  scal[2] = s != 0.0 ? scal[0] / s : 0.0;
-  if (s == 0.0) scal[4] = 1.0;
-}
-
+// x += alpha p, r -= alpha q, z = M r and r.z; the last block forms beta and
+// the convergence flag
 template <int NB>
 __global__ void cg_update(MvArgs m, double *x, double *r, double *z, const double *p, const double *q, double *partial,
-                          const double *scal) {
+                          double *scal, int *ticket) {
   if (scal[4] != 0.0) return;
   const double alpha = scal[2];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,23 +385,38 @@ __global__ void cg_update(MvArgs m, double *x, double *r, double *z, const doubl
     }
   }
   v = block_sum(v);
-  if (threadIdx.x == 0) partial[blockIdx.x] = v;
-}
-
-__global__ void cg_beta(const double *partial, int nblk, double *scal) {
-  if (scal[4] != 0.0) return;
-  const double s = wave_total(partial, nblk);
-  if (threadIdx.x != 0) return;
-  scal[3] = scal[0] != 0.0 ? s / scal[0] : 0.0;
-  scal[0] = s;
-  scal[5] += 1.0;
-  if (s <= scal[6] * scal[1]) scal[4] = 1.0;
+  double s;
+  if (last_block_total(v, partial, ticket, s)) {
+    scal[3] = scal[0] != 0.0 ? s / scal[0] : 0.0;
+    scal[0] = s;
+    scal[5] += 1.0;
+    if (s <= scal[6] * scal[1]) scal[4] = 1.0;
+  }
 }
 
 __global__ void cg_direction(int64_t N, const double *z, double *p, const double *scal) {
   if (scal[4] != 0.0) return;
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) p[n] = z[n] + scal[3] * p[n];
+}
+
+// the void census of one group: void cells (sigma_t <= 0), the first of
+// them, absorbing cells (sigma_t > sigma_s(g -> g))
+__global__ void census_init(int *flags) {
+  flags[0] = 0;
+  flags[1] = 0;
+  flags[2] = 0x7fffffff;
+  flags[3] = 0;
+}
+
+__global__ void void_census(const double *__restrict__ st, const double *__restrict__ ss, int64_t n, int *flags) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  if (!(st[c] > 0.0)) {
+    atomicAdd(&flags[1], 1);
+    atomicMin(&flags[2], (int)c);
+  }
+  if (st[c] - ss[c] > 0.0) atomicAdd(&flags[3], 1);
 }
 
 template <class T>
@@ -486,20 +527,14 @@ void DSA::set_group(int g, hipStream_t s) {
   // the void census: this operator has no void masking or bridging, so a
   // non-positive sigma_t is refused, and a pure-Neumann absorption-free group
   const double *st = o.t64.sigt.as<double>() + (int64_t)g * n_cells;
-  {
-    std::vector<double> h(n_cells), hs(n_cells);
-    UBT_HIP(hipMemcpy(h.data(), st, sizeof(double) * n_cells, hipMemcpyDeviceToHost));
-    UBT_HIP(hipMemcpy(hs.data(), o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells,
-                      sizeof(double) * n_cells, hipMemcpyDeviceToHost));
-    double max_sa = 0.0;
-    for (int64_t c = 0; c < n_cells; c++) {
-      UBT_CHECK(h[c] > 0.0, UBT_ERR_SUP, "cell %lld is a void in group %d (sigma_t %g): DSA void treatment is not "
-                "supported", (long long)c, g, h[c]);
-      max_sa = std::max(max_sa, h[c] - hs[c]);
-    }
-    UBT_CHECK(any_vacuum || max_sa > 0.0, UBT_ERR_ARG_WRONGSTATE,
-              "the DSA diffusion operator is singular: every face is reflective and sigma_a is zero everywhere");
-  }
+  const double *ssg = o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells;
+  // flags (one read back below, on the caller's stream): 0 singular diagonal
+  // blocks, 1 void cells (sigma_t <= 0), 2 the first void cell, 3 absorbing
+  // cells (sigma_a > 0)
+  int *flags = o.flag.as<int>();
+  const unsigned grid = (unsigned)((n_cells + 255) / 256);
+  hipLaunchKernelGGL(census_init, dim3(1), dim3(1), 0, s, flags);
+  hipLaunchKernelGGL(void_census, dim3(grid), dim3(256), 0, s, st, ssg, n_cells, flags);
   AsmArgs a{};
   a.kind = D.kind;
   a.dim = D.dim;
@@ -521,7 +556,6 @@ void DSA::set_group(int g, hipStream_t s) {
   }
   a.own = own.as<double>();
   a.off = off.as<double>();
-  const unsigned grid = (unsigned)((n_cells + 255) / 256);
   if (D.kind == DISC_FD) {
     hipLaunchKernelGGL(asm_structured, dim3(grid), dim3(256), 0, s, a);
   } else {
@@ -543,8 +577,7 @@ void DSA::set_group(int g, hipStream_t s) {
       hipLaunchKernelGGL(asm_dg1<4>, dim3(grid), dim3(256), 0, s, a);
   }
   UBT_HIP(hipGetLastError());
-  int *bad = o.flag.as<int>();
-  UBT_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  int *bad = flags;
   if (nb == 1)
     hipLaunchKernelGGL(jacobi_invert<1>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
   else if (nb == 3)
@@ -552,10 +585,16 @@ void DSA::set_group(int g, hipStream_t s) {
   else
     hipLaunchKernelGGL(jacobi_invert<4>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
   UBT_HIP(hipGetLastError());
-  int hbad = 0;
-  UBT_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  int h[4] = {0, 0, 0, 0};
+  UBT_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
   UBT_HIP(hipStreamSynchronize(s));
-  UBT_CHECK(hbad == 0, UBT_ERR_MAT_LU_ZRPVT, "%d diffusion diagonal blocks are singular", hbad);
+  // this operator has no void masking or bridging, so a non-positive sigma_t
+  // is refused, and so is an absorption-free group with no vacuum face
+  UBT_CHECK(h[1] == 0, UBT_ERR_SUP, "cell %d is a void in group %d (%d void cells): DSA void treatment is not "
+            "supported", h[2], g, h[1]);
+  UBT_CHECK(any_vacuum || h[3] > 0, UBT_ERR_ARG_WRONGSTATE,
+            "the DSA diffusion operator is singular: every face is reflective and sigma_a is zero everywhere");
+  UBT_CHECK(h[0] == 0, UBT_ERR_MAT_LU_ZRPVT, "%d diffusion diagonal blocks are singular", h[0]);
   group = g;
 }
 
@@ -588,11 +627,12 @@ void DSA::matvec(const double *x, double *y, hipStream_t s) const {
 // and the CG scalars (every entry is written before it is read)
 struct CgWork {
   double *r, *z, *p, *q, *part, *sc;
+  int *ticket;  // the last-block reductions' counter
 };
 
 static CgWork own_work(DSA &d) {
   return {d.r.as<double>(), d.z.as<double>(), d.p.as<double>(), d.q.as<double>(), d.partial.as<double>(),
-          d.scal.as<double>()};
+          d.scal.as<double>(), reinterpret_cast<int *>(d.scal.as<double>() + 7)};
 }
 
 template <int NB>
@@ -604,12 +644,14 @@ static void cg_run(DSA &d, const double *b, const double *wgt, double *x, hipStr
   double *sc = w.sc, *part = w.part;
   double *r = w.r, *z = w.z, *p = w.p, *q = w.q;
   hipLaunchKernelGGL(cg_init<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, b, wgt, x, r, z, p, part);
-  hipLaunchKernelGGL(cg_reduce_init, dim3(1), dim3(64), 0, s, part, (int)gcell, sc, d.opt.rtol * d.opt.rtol);
+  hipLaunchKernelGGL(cg_reduce_init, dim3(1), dim3(64), 0, s, part, (int)gcell, sc, d.opt.rtol * d.opt.rtol,
+                     w.ticket);
+  // three launches per iteration: the matvec and dot with alpha, the update
+  // and preconditioner with beta (each reduced by its last block), the new
+  // direction
   for (int it = 0; it < d.opt.max_it; it++) {
-    hipLaunchKernelGGL(cg_matvec_dot<NB>, dim3(gnode), dim3(kCgThreads), 0, s, m, p, q, part, sc);
-    hipLaunchKernelGGL(cg_alpha, dim3(1), dim3(64), 0, s, part, (int)gnode, sc);
-    hipLaunchKernelGGL(cg_update<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, x, r, z, p, q, part, sc);
-    hipLaunchKernelGGL(cg_beta, dim3(1), dim3(64), 0, s, part, (int)gcell, sc);
+    hipLaunchKernelGGL(cg_matvec_dot<NB>, dim3(gnode), dim3(kCgThreads), 0, s, m, p, q, part, sc, w.ticket);
+    hipLaunchKernelGGL(cg_update<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, x, r, z, p, q, part, sc, w.ticket);
     hipLaunchKernelGGL(cg_direction, dim3(gnode), dim3(kCgThreads), 0, s, N, z, p, sc);
     if (d.opt.check_every > 0 && (it + 1) % d.opt.check_every == 0 && it + 1 < d.opt.max_it) {
       UBT_HIP(hipMemcpyAsync(d.flag_host, sc + 4, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -676,7 +718,8 @@ void DSA::solve_noalloc(const double *b, double *x, hipStream_t s) {
             (long long)kNoAllocNodes, (long long)n_nodes);
   double *base = scratch_on(op->device);
   const int64_t N = n_nodes, nblk = (N + kCgThreads - 1) / kCgThreads;
-  const CgWork w = {base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 4 * N + nblk};
+  double *sc = base + 4 * N + nblk;
+  const CgWork w = {base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, sc, reinterpret_cast<int *>(sc + 7)};
   cg_dispatch(*this, b, nullptr, x, s, w);
 }
 
