@@ -412,6 +412,15 @@ int ubt_batch_tuning(const ubt_batch *batch, int *body, int *persist, float *us)
  * does not apply to every active problem). No reference counterpart; exposed
  * so the tests can check every body. */
 int ubt_batch_set_body(ubt_batch *batch, int prec, int body);
+/* The batched sweep's form: box != 0 (the default) sweeps a batch whose
+ * active problems are all uniform all-vacuum 2D DG1 boxes (at most 256 cells
+ * along x) without per-item records - the blocks formed on the fly from
+ * sigma_t and per-angle matrices, a workgroup per problem, quadrant and angle
+ * group marching anti-diagonals -; 0 uses the record sweep on every batch.
+ * No reference counterpart; a solver refreshes before its next solve. */
+int ubt_batch_set_sweep(ubt_batch *batch, int box);
+/* box[2] (FP64, FP32): 1 where the record-free box sweep runs */
+int ubt_batch_sweep_info(const ubt_batch *batch, int *box);
 /* the right-hand side of every active problem (ubt_rhs), zero elsewhere */
 int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream);
 int ubt_batch_set_scalar_flux(ubt_batch *batch, int prec, const void *psi, void *stream);

@@ -210,3 +210,33 @@ def test_batch_strip_bands(gpu):
                 err = rel_max(yh[sl], oracle_mult(prob, x[sl]))
                 print(f"[case] body {body} {dt} problem {p} ({prob.name}): {err:.2e}", flush=True)
                 assert err <= tol, (body, p, dt, err)
+
+
+def test_batch_box_sweep(gpu):
+    """the record-free box sweep (every problem a uniform all-vacuum DG1 quad
+    box) against the oracle's direct solve of L, problem by problem, and
+    against the record sweep of the same batch, both precisions"""
+    probs = P.batch_quads(n_problems=5, seed=13, side=(5, 90), sns=(2, 4, 8, 12), groups=(1, 1))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    b.set_group(0)
+    assert b.sweep_info() == {"box_fp64": True, "box_fp32": True}
+    x = np.random.default_rng(17).standard_normal(b.n_rows)
+    out = {}
+    for box in (True, False):
+        b.set_sweep(box)
+        assert b.sweep_info()["box_fp64"] == box
+        for dt in (torch.float64, torch.float32):
+            y = b.vec(u.FP64 if dt == torch.float64 else u.FP32)
+            b.sweep_apply(to_dev(x, dt), y)
+            torch.cuda.synchronize()
+            out[(box, dt)] = to_host(y)
+    b.set_sweep(True)
+    for p, prob in enumerate(probs):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        want = spl.spsolve(oracle_rows(prob).to_scipy().tocsc(), x[sl])
+        e64 = rel_max(out[(True, torch.float64)][sl], want)
+        e32 = rel_max(out[(True, torch.float32)][sl], want)
+        print(f"[case] box sweep problem {p} ({prob.name}): fp64 {e64:.2e} fp32 {e32:.2e}", flush=True)
+        assert e64 <= 1e-11 and e32 <= 1e-4, (p, e64, e32)
+        assert rel_max(out[(True, torch.float64)][sl], out[(False, torch.float64)][sl]) <= 1e-12

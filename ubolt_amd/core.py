@@ -403,6 +403,15 @@ class Batch(_Handle):
         """apply body of one precision: 0 tile, 1 four-angle tile, 2 box tile, 3 strip march (uniform DG1 boxes)"""
         call("ubt_batch_set_body", self.h, prec, body)
 
+    def set_sweep(self, box):
+        """the record-free box sweep where the batch allows it (True), or the record sweep"""
+        call("ubt_batch_set_sweep", self.h, int(box))
+
+    def sweep_info(self):
+        box = (C.c_int * 2)()
+        call("ubt_batch_sweep_info", self.h, box)
+        return {"box_fp64": bool(box[0]), "box_fp32": bool(box[1])}
+
     def tuning(self):
         body, persist = (C.c_int * 2)(), (C.c_int * 2)()
         us = (C.c_float * 8)()

@@ -30,6 +30,29 @@ struct BatchSweep {
   mutable DevBuf rec, rnb;
   mutable uint64_t rec_tag = ~0ull;
 };
+// The batched sweep of uniform all-vacuum 2D DG1 boxes (no records): one
+// workgroup per problem, quadrant and group of kBoxSweepAngles of its angles
+constexpr int kBoxSweepThreads = 256;
+template <class T> constexpr int kBoxSweepAngles = sizeof(T) == 8 ? 4 : 8;
+// diagonals whose r and sigma_t are in flight while one is solved
+template <class T> constexpr int kBoxSweepDepth = sizeof(T) == 8 ? 2 : 1;
+struct BoxSweepGroup {
+  int item;      // the problem's entry in the strip body's work list
+  int q, k0;     // the quadrant, the group's first position among its angles
+  int angle[8];  // the quadrant's angles (-1: none)
+};
+// r and psi pass through a quadrant-major copy, row (q, cell, basis) of the
+// quadrant's A / 4 angles contiguous, so the sweep's loads and stores of one
+// quadrant are whole lines
+struct BoxSweepPlan {
+  bool ok = false;   // every active problem is such a box (else the record sweep runs)
+  int n_wg = 0, nx_max = 0;
+  DevBuf groups;
+  DevBuf qpos;       // int per (problem, angle): q * A / 4 + position, problems concatenated
+  DevBuf qoff;       // int64 per problem + 1: prefix of the problems' angle counts
+  mutable DevBuf rq, pq;  // the quadrant-major copies of r and psi
+};
+
 // directions of one octant per batched sweep workgroup, per precision
 // [FP64, FP32] (measured on the 64-problem ensemble: 2 and 4)
 constexpr int kBatchSweepGroup[2] = {2, 4};
@@ -59,6 +82,7 @@ struct Batch {
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems
   int64_t n_blocks = 0;
   BatchSweep sweep[2];         // per precision: its own grouping and records
+  BoxSweepPlan box_sweep[2];   // per precision: the record-free sweep of box batches
 
   void create(const std::vector<Operator *> &o);
   void set_group(int g);       // problems without group g go inactive
@@ -73,6 +97,8 @@ struct Batch {
   void set_scalar_flux(int prec, const void *psi, hipStream_t s) const;
   void tune();
   void build_lists();
+  void build_box_sweep();
+  bool box_sweep_enabled = true;  // false: the record sweep on every batch
 };
 
 // inflow slots of the sweep record layout for nf faces and at most nfu inflow faces
@@ -84,6 +110,7 @@ template <class T>
 void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s);
 template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hipStream_t s);
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
+template <class T> void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
 // per-problem tile size, LDS and tile count of a body (the box body also
 // fills its layout); false where it does not apply
 bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *red_lanes, size_t *lds,

@@ -720,6 +720,22 @@ int ubt_batch_set_body(ubt_batch *batch, int prec, int body) {
     batch->b.body[pi] = body;
   });
 }
+int ubt_batch_set_sweep(ubt_batch *batch, int box) {
+  return guard([&] {
+    NEED(batch);
+    batch->b.box_sweep_enabled = box != 0;
+    batch->b.build_box_sweep();
+    batch->b.gen++;  // a solver refreshes (and recaptures) before its next solve
+  });
+}
+int ubt_batch_sweep_info(const ubt_batch *batch, int *box) {
+  return guard([&] {
+    NEED(batch);
+    NEED(box);
+    box[0] = batch->b.box_sweep[0].ok ? 1 : 0;
+    box[1] = batch->b.box_sweep[1].ok ? 1 : 0;
+  });
+}
 int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream) {
   return guard([&] {
     NEED(batch);

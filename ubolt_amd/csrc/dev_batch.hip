@@ -358,8 +358,76 @@ void Batch::build_lists() {
     w.base.upload(bs);
     w.rec_tag = ~0ull;
   }
+  build_box_sweep();
   sweep_records(nullptr);
   UBT_HIP(hipStreamSynchronize(nullptr));
+}
+
+// The record-free box sweep's work list, per precision: every active problem
+// a uniform all-vacuum 2D DG1 box (and the strip body's layout, whose work
+// items the sweep reads, in place), at most 256 cells along x (the LDS
+// diagonal); one workgroup per problem, quadrant and group of the quadrant's
+// angles, the problems with the most diagonals first
+void Batch::build_box_sweep() {
+  for (int pi = 0; pi < 2; pi++) {
+    BoxSweepPlan &w = box_sweep[pi];
+    const int AG = pi == 0 ? kBoxSweepAngles<double> : kBoxSweepAngles<float>;
+    w.ok = dim == 2 && nb == 3 && body_ok[pi][3] && box_sweep_enabled;
+    w.nx_max = 0;
+    std::vector<std::pair<int, BoxSweepGroup>> grp;  // (diagonals, group)
+    std::vector<int> qpos;
+    std::vector<int64_t> qoff(1, 0);
+    int item = 0;
+    for (size_t p = 0; p < ops.size() && w.ok; p++) {
+      if (!active[p]) continue;
+      const Operator &o = *ops[p];
+      const Quadrature &q = o.disc->quad;
+      w.ok = o.box_cls >= 0 && o.n_box_edge == 0 && o.box_n[0] <= 256 && q.n_angles % 4 == 0;
+      if (!w.ok) break;
+      w.nx_max = std::max(w.nx_max, o.box_n[0]);
+      const int Aq = q.n_angles / 4;
+      std::vector<int> pos(q.n_angles, -1);
+      for (int sx = 0; sx < 2; sx++)
+        for (int sy = 0; sy < 2; sy++) {
+          const int quad = 2 * sx + sy;
+          std::vector<int> an;
+          for (int a = 0; a < q.n_angles; a++)
+            if ((q.omega[3 * a] > 0.0) == (sx == 0) && (q.omega[3 * a + 1] > 0.0) == (sy == 0)) an.push_back(a);
+          if ((int)an.size() != Aq) {
+            w.ok = false;
+            break;
+          }
+          for (int j = 0; j < Aq; j++) pos[an[j]] = quad * Aq + j;
+          for (size_t k0 = 0; k0 < an.size(); k0 += AG) {
+            BoxSweepGroup gw;
+            gw.item = item;
+            gw.q = quad;
+            gw.k0 = (int)k0;
+            for (int k = 0; k < 8; k++) gw.angle[k] = k < AG && k0 + k < an.size() ? an[k0 + k] : -1;
+            grp.push_back({o.box_n[0] + o.box_n[1] - 1, gw});
+          }
+        }
+      qpos.insert(qpos.end(), pos.begin(), pos.end());
+      qoff.push_back(qoff.back() + q.n_angles);
+      item++;
+    }
+    if (!w.ok) {
+      w.n_wg = 0;
+      continue;
+    }
+    std::stable_sort(grp.begin(), grp.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::vector<BoxSweepGroup> g;
+    for (const auto &e : grp) g.push_back(e.second);
+    w.n_wg = (int)g.size();
+    w.groups.upload(g);
+    w.qpos.upload(qpos);
+    w.qoff.upload(qoff);
+    const size_t ts = pi == 0 ? sizeof(double) : sizeof(float);
+    if (w.rq.bytes < ts * (size_t)n_rows) {
+      w.rq.alloc(ts * (size_t)n_rows);
+      w.pq.alloc(ts * (size_t)n_rows);
+    }
+  }
 }
 
 uint64_t Batch::xs_tag() const {
@@ -372,6 +440,7 @@ void Batch::sweep_records(hipStream_t s) const {
   const uint64_t tag = xs_tag();
   for (int pi = 0; pi < 2; pi++) {
     const BatchSweep &w = sweep[pi];
+    if (box_sweep[pi].ok) continue;  // the box sweep forms its blocks on the fly
     if (w.rec_tag == tag || w.n_items == 0) continue;
     // 64-item blocks (see the sweep's record layout)
     const size_t blocks64 = (size_t)((w.n_items + 63) / 64) * 64;
@@ -414,6 +483,14 @@ void Batch::blocks_apply(int prec, const void *x, void *y, hipStream_t s) const 
 }
 
 void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const {
+  const int pi = prec == UBT_FP64 ? 0 : 1;
+  if (box_sweep[pi].ok) {
+    if (prec == UBT_FP64)
+      batch_box_sweep_launch<double>(*this, (const double *)r, (double *)psi, s);
+    else
+      batch_box_sweep_launch<float>(*this, (const float *)r, (float *)psi, s);
+    return;
+  }
   sweep_records(s);
   if (prec == UBT_FP64)
     batch_sweep_launch<double>(*this, (const double *)r, (double *)psi, s);

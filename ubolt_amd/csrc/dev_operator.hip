@@ -2698,10 +2698,14 @@ static int box_interior_class(const Disc &D, const std::vector<int> &cls, const 
 
 // Per cell whether the box kernels' formula holds: a cell of the interior
 // class, or a cell whose faces on the box boundary are all vacuum and whose
-// geometry record matches the interior class's (within the class tolerance,
-// 1e-13 of each quantity's scale) except for the upwind face matrices of
-// those faces, which the ghost-flux treatment leaves zero: the kernels read a
-// zero trace there, so the product is the same.
+// geometry record matches the interior class's within 2e-13 of each
+// quantity's scale except for the upwind face matrices of those faces, which
+// the ghost-flux treatment leaves zero: the kernels read a zero trace there,
+// so the product is the same. (Twice the class tolerance: a box's cell widths
+// differ in the last bits of its coordinates, by up to 1.0e-13 of the face
+// matrices' scale on a 253 x 252 box, and a cell that misses by that much
+// would cost a list phase; the apply then differs from the cell's own
+// geometry by at most that fraction, well inside the 1e-12 parity bar.)
 static void box_formula_cells(const Disc &D, const std::vector<int> &cls, int rep, int box_cls,
                               std::vector<unsigned char> &ok) {
   const int nc = D.n_cells, nf = D.nf, nb = D.nb, FM = nb * nb;
@@ -2711,7 +2715,7 @@ static void box_formula_cells(const Disc &D, const std::vector<int> &cls, int re
   for (double v : D.face_up) scale[1] = std::max(scale[1], std::fabs(v));
   for (double v : D.basis_grad) scale[2] = std::max(scale[2], std::fabs(v));
   for (double v : D.volume) scale[3] = std::max(scale[3], std::fabs(v));
-  for (double &v : scale) v = (v > 0.0 ? v : 1.0) * 1e-13;
+  for (double &v : scale) v = (v > 0.0 ? v : 1.0) * 2e-13;
   auto same = [&](double a, double b, int k) { return std::fabs(a - b) <= scale[k]; };
   ok.assign(nc, 0);
   for (int c = 0; c < nc; c++) {
@@ -3275,6 +3279,218 @@ template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hi
   }
   UBT_HIP(hipGetLastError());
 }
+
+namespace {
+// ---------------------------------------------------------------------------
+// Batched sweep of uniform all-vacuum 2D DG1 boxes. On such a box every
+// item's block is sigma_t(c) I + M(a) and its upwind couplings U_x(a),
+// U_y(a) - the box apply's folded per-angle matrices - so no per-item record
+// is stored or read: a workgroup owns one problem, one quadrant and up to AG
+// of its angles, keeps those matrices in registers, and marches the
+// quadrant's anti-diagonals i' + j' = d (i', j' counted from the quadrant's
+// inflow corner). Cells of a diagonal are independent; their upwind traces
+// are the previous diagonal's results, held in LDS by i' (x upwind at
+// i' - 1, y upwind at i'), so the only global traffic is r in, psi out and
+// sigma_t. A thread solves up to AG cells of a diagonal (AG angle lanes per
+// cell); the next diagonal's r and sigma_t are requested before the current
+// one is solved, into the other of two register sets (the loop is unrolled
+// by two so neither set is copied), and every load is unconditional on a
+// clamped cell.
+// ---------------------------------------------------------------------------
+template <class T, int AG, int DEPTH>
+__global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__restrict__ r, T *__restrict__ psi,
+                                                            const BatchItem<T> *__restrict__ items,
+                                                            const BoxSweepGroup *__restrict__ groups,
+                                                            int nx_max) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NB = 3, SL = kBoxSweepThreads / AG, IPT = 256 / SL;
+  T *ring = reinterpret_cast<T *>(smem);
+  const BoxSweepGroup &gw = groups[blockIdx.x];
+  const BatchItem<T> &it = items[gw.item];
+  const OpArgs<T> &g = it.g;
+  const BoxArgs &bx = it.box;
+  const int k = threadIdx.x % AG, slot = threadIdx.x / AG;
+  const int a0 = gw.angle[k];
+  const bool va = a0 >= 0;
+  const int a = va ? a0 : gw.angle[0];
+  const int A = g.A, nx = bx.n[0], ny = bx.n[1];
+  // the quadrant-major copies: row (q, cell, basis) of the A / 4 quadrant
+  // angles, this lane's at position k0 + k
+  const int Aq = A / 4, kq = gw.k0 + (va ? k : 0);
+  const T *rp = r + it.off + (int64_t)gw.q * g.n_cells * NB * Aq + kq;
+  T *pp = psi + it.off + (int64_t)gw.q * g.n_cells * NB * Aq + kq;
+  BoxConst<T, 2, NB> kc;
+  box_consts<T, 2, NB>(g, bx, a, kc);
+  const bool px = kc.in_side[0] == 0, py = kc.in_side[1] == 0;
+  const int NA = NB * Aq;
+  // the block sigma_t I + M solved in closed form (Cayley-Hamilton for 3 x 3):
+  // adj(s I + M) = s^2 I + s (tr M I - M) + adj M,
+  // det(s I + M) = s^3 + tr M s^2 + c2 s + det M (c2: the principal 2 x 2
+  // minors of M), so per item a few multiply-adds, one division, no pivoting
+  const T (&M)[NB][NB] = kc.M;
+  const T trM = M[0][0] + M[1][1] + M[2][2];
+  T adjM[NB][NB];
+  adjM[0][0] = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  adjM[0][1] = M[0][2] * M[2][1] - M[0][1] * M[2][2];
+  adjM[0][2] = M[0][1] * M[1][2] - M[0][2] * M[1][1];
+  adjM[1][0] = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  adjM[1][1] = M[0][0] * M[2][2] - M[0][2] * M[2][0];
+  adjM[1][2] = M[0][2] * M[1][0] - M[0][0] * M[1][2];
+  adjM[2][0] = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  adjM[2][1] = M[0][1] * M[2][0] - M[0][0] * M[2][1];
+  adjM[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+  const T c2 = adjM[0][0] + adjM[1][1] + adjM[2][2];
+  const T detM = M[0][0] * adjM[0][0] + M[0][1] * adjM[1][0] + M[0][2] * adjM[2][0];
+  auto cell_of = [&](int ip, int d) -> int64_t {
+    const int jp = d - ip;
+    const int i = px ? ip : nx - 1 - ip, j = py ? jp : ny - 1 - jp;
+    return (int64_t)j * nx + i;
+  };
+  // the cells of diagonal d this thread solves: ip = lo + slot + m SL
+  struct Set {
+    T r[IPT][NB], st[IPT];
+  };
+  auto fetch = [&](int d, Set &s) {
+    const int lo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, hi = d < nx - 1 ? d : nx - 1;
+#pragma unroll
+    for (int m = 0; m < IPT; m++) {
+      int ip = lo + slot + m * SL;
+      ip = ip <= hi ? ip : lo;  // clamped: a valid cell, its values discarded
+      const int64_t c = cell_of(ip, d);
+#pragma unroll
+      for (int b = 0; b < NB; b++) s.r[m][b] = ldg(rp + c * NA + b * Aq);
+      s.st[m] = ldg(g.sigt + c);
+    }
+  };
+  const int n_diag = nx + ny - 1;
+  auto solve = [&](int d, const Set &s) {
+    const int lo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, hi = d < nx - 1 ? d : nx - 1;
+    const T *prev = ring + (size_t)((d + 1) & 1) * nx_max * AG * NB;
+    T *cur = ring + (size_t)(d & 1) * nx_max * AG * NB;
+#pragma unroll
+    for (int m = 0; m < IPT; m++) {
+      const int ip = lo + slot + m * SL;
+      if (ip > hi) break;
+      const int jp = d - ip;
+      T hx[NB], hy[NB], v[NB];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        hx[b] = ip > 0 ? prev[((ip - 1) * AG + k) * NB + b] : T(0);
+        hy[b] = jp > 0 ? prev[(ip * AG + k) * NB + b] : T(0);
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        T w = s.r[m][b];
+#pragma unroll
+        for (int q = 0; q < NB; q++) w -= kc.U[0][b][q] * hx[q] + kc.U[1][b][q] * hy[q];
+        v[b] = w;
+      }
+      const T sg = s.st[m];
+      const T inv_det = T(1) / (((sg + trM) * sg + c2) * sg + detM);
+      T x[NB];
+#pragma unroll
+      for (int p = 0; p < NB; p++) {
+        T w = T(0);
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+          const T adj = sg * ((p == q ? sg + trM : T(0)) - M[p][q]) + adjM[p][q];
+          w += adj * v[q];
+        }
+        x[p] = w * inv_det;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; b++) v[b] = x[b];
+#pragma unroll
+      for (int b = 0; b < NB; b++) cur[(ip * AG + k) * NB + b] = v[b];
+      if (va) {
+        const int64_t c = cell_of(ip, d);
+#pragma unroll
+        for (int b = 0; b < NB; b++) pp[c * NA + b * Aq] = v[b];
+      }
+    }
+    __syncthreads();
+  };
+  // diagonals d + 1 .. d + DEPTH requested while d is solved: DEPTH + 1
+  // register sets in rotation, the loop unrolled by DEPTH + 1
+  if (DEPTH == 1) {
+    Set s0, s1;
+    fetch(0, s0);
+    for (int d = 0; d < n_diag; d += 2) {
+      if (d + 1 < n_diag) fetch(d + 1, s1);
+      solve(d, s0);
+      if (d + 1 >= n_diag) break;
+      if (d + 2 < n_diag) fetch(d + 2, s0);
+      solve(d + 1, s1);
+    }
+  } else {
+    Set s0, s1, s2;
+    fetch(0, s0);
+    if (1 < n_diag) fetch(1, s1);
+    for (int d = 0; d < n_diag; d += 3) {
+      if (d + 2 < n_diag) fetch(d + 2, s2);
+      solve(d, s0);
+      if (d + 1 >= n_diag) break;
+      if (d + 3 < n_diag) fetch(d + 3, s0);
+      solve(d + 1, s1);
+      if (d + 2 >= n_diag) break;
+      if (d + 4 < n_diag) fetch(d + 4, s1);
+      solve(d + 2, s2);
+    }
+  }
+}
+
+}  // namespace
+
+// Reference order <-> quadrant-major order of every active problem's rows:
+// (cell, basis, angle a) <-> (q, cell, basis, position), q * A / 4 +
+// position = qpos[a]. A grid-stride loop per problem; TO_Q reads the
+// reference order in sequence, the other direction writes it in sequence.
+template <class T, bool TO_Q>
+__global__ void __launch_bounds__(256) quad_reorder(const T *__restrict__ src, T *__restrict__ dst,
+                                                   const BatchItem<T> *__restrict__ items, int n_items,
+                                                   const int *__restrict__ qpos, const int64_t *__restrict__ qoff) {
+  constexpr int NB = 3;
+  for (int p = 0; p < n_items; p++) {
+    const BatchItem<T> &it = items[p];
+    const int A = it.g.A, Aq = A / 4;
+    const unsigned nc = (unsigned)it.g.n_cells;
+    const unsigned rows = nc * NB * (unsigned)A;
+    const int *qp = qpos + qoff[p];
+    for (unsigned lr = blockIdx.x * blockDim.x + threadIdx.x; lr < rows; lr += gridDim.x * blockDim.x) {
+      const unsigned node = lr / (unsigned)A, a = lr - node * (unsigned)A;
+      const unsigned c = node / NB, b = node - c * NB;
+      const unsigned qk = (unsigned)qp[a], q = qk / (unsigned)Aq, kq = qk - q * (unsigned)Aq;
+      const int64_t qi = (((int64_t)q * nc + c) * NB + b) * Aq + kq;
+      if (TO_Q)
+        dst[it.off + qi] = src[it.off + lr];
+      else
+        dst[it.off + lr] = src[it.off + qi];
+    }
+  }
+}
+
+template <class T>
+void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const BoxSweepPlan &w = b.box_sweep[pi];
+  constexpr int AG = kBoxSweepAngles<T>;
+  const size_t lds = (size_t)2 * w.nx_max * AG * 3 * sizeof(T);
+  const BatchItem<T> *items = b.items[pi][3].as<BatchItem<T>>();
+  const int n_items = b.n_items[pi][3];
+  T *rq = w.rq.as<T>(), *pq = w.pq.as<T>();
+  const unsigned grid = (unsigned)(b.n_cu * 8);
+  hipLaunchKernelGGL((quad_reorder<T, true>), dim3(grid), dim3(256), 0, s, r, rq, items, n_items, w.qpos.as<int>(),
+                     w.qoff.as<int64_t>());
+  hipLaunchKernelGGL((sweep_box_batch<T, AG, kBoxSweepDepth<T>>), dim3(w.n_wg), dim3(kBoxSweepThreads), lds, s,
+                     (const T *)rq, pq,
+                     items, w.groups.as<BoxSweepGroup>(), w.nx_max);
+  hipLaunchKernelGGL((quad_reorder<T, false>), dim3(grid), dim3(256), 0, s, (const T *)pq, psi, items, n_items,
+                     w.qpos.as<int>(), w.qoff.as<int64_t>());
+  UBT_HIP(hipGetLastError());
+}
+
+template void batch_box_sweep_launch<double>(const Batch &, const double *, double *, hipStream_t);
+template void batch_box_sweep_launch<float>(const Batch &, const float *, float *, hipStream_t);
 
 template void batch_apply_launch<double>(const Batch &, int, const double *, double *, bool, hipStream_t);
 template void batch_apply_launch<float>(const Batch &, int, const float *, float *, bool, hipStream_t);
