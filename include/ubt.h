@@ -90,7 +90,11 @@ typedef struct ubt_batch_solver_s ubt_batch_solver;
  * multiplicative composite: stage 0 the element block inverse (pc_blocks),
  * stage 1 the streaming stage (pc_sweep: the transport sweep), stage 2 DSA
  * (pc_dsa, when a DSA handle is given); later stages correct the residual with
- * the full operator when DSA is on and use_amat is set, else with L. */
+ * the full operator when DSA is on and use_amat is set, else with L. Where the
+ * stages correct with L and the sweep is L's exact inverse S (a DG sweep with
+ * no lagged edge and no reflective face), stages 0 and 1 together are the
+ * sweep alone - y + S (r - L y) = S r for any stage-0 output y - and stage 0
+ * is not run. */
 typedef struct {
   double rtol;        /* relative true residual ||b - A x|| / ||b||, e.g. 1e-8 */
   int max_it;         /* Krylov iterations in total */
@@ -419,8 +423,12 @@ int ubt_batch_set_body(ubt_batch *batch, int prec, int body);
  * group marching anti-diagonals -; 0 uses the record sweep on every batch.
  * No reference counterpart; a solver refreshes before its next solve. */
 int ubt_batch_set_sweep(ubt_batch *batch, int box);
-/* box[2] (FP64, FP32): 1 where the record-free box sweep runs */
-int ubt_batch_sweep_info(const ubt_batch *batch, int *box);
+/* box[2] (FP64, FP32): 1 where the record-free box sweep runs; exact[2]
+ * (optional): 1 where the sweep inverts every active problem's L exactly (no
+ * lagged edge, no reflective face). There the batch solver's element-block
+ * stage is left out: blocks then the sweep on the L-residual equal the sweep
+ * alone, y + S (r - L y) = S r. */
+int ubt_batch_sweep_info(const ubt_batch *batch, int *box, int *exact);
 /* the right-hand side of every active problem (ubt_rhs), zero elsewhere */
 int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream);
 int ubt_batch_set_scalar_flux(ubt_batch *batch, int prec, const void *psi, void *stream);

@@ -220,7 +220,7 @@ def test_batch_box_sweep(gpu):
     ops = [p.operator() for p in probs]
     b = u.Batch(ops)
     b.set_group(0)
-    assert b.sweep_info() == {"box_fp64": True, "box_fp32": True}
+    assert b.sweep_info() == {"box_fp64": True, "box_fp32": True, "exact_fp64": True, "exact_fp32": True}
     x = np.random.default_rng(17).standard_normal(b.n_rows)
     out = {}
     for box in (True, False):
@@ -240,3 +240,40 @@ def test_batch_box_sweep(gpu):
         print(f"[case] box sweep problem {p} ({prob.name}): fp64 {e64:.2e} fp32 {e32:.2e}", flush=True)
         assert e64 <= 1e-11 and e32 <= 1e-4, (p, e64, e32)
         assert rel_max(out[(True, torch.float64)][sl], out[(False, torch.float64)][sl]) <= 1e-12
+
+
+def test_batch_pc_folds_to_exact_sweep(gpu):
+    """On a batch whose sweep inverts every problem's L exactly the batch
+    solver's preconditioner is the sweep alone: the composite it stands for -
+    the batched element blocks, then the sweep on the L-residual - formed here
+    from the batch's own stages, with the box sweep and with the record sweep,
+    against the sweep of r; a batch holding a problem with reflective faces
+    is reported inexact (it keeps both stages)"""
+    probs = P.batch_quads(n_problems=4, seed=23, side=(4, 40), sns=(2, 4, 8), groups=(1, 1))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    b.set_group(0)
+    x = np.random.default_rng(29).standard_normal(b.n_rows)
+    for box in (True, False):
+        b.set_sweep(box)
+        info = b.sweep_info()
+        assert info["exact_fp64"] and info["exact_fp32"] and info["box_fp64"] == box
+        for dt, tol in ((torch.float64, 1e-11), (torch.float32, 1e-4)):
+            prec = u.FP64 if dt == torch.float64 else u.FP32
+            r = to_dev(x, dt)
+            y = b.blocks_apply(r, b.vec(prec))
+            t = b.vec(prec)
+            for p, op in enumerate(ops):
+                op.mult_unscattered(b.part(y, p), b.part(t, p))
+            comp = y + b.sweep_apply(r - t, b.vec(prec))
+            sr = b.sweep_apply(r, b.vec(prec))
+            torch.cuda.synchronize()
+            err = rel_max(to_host(sr), to_host(comp))
+            print(f"[case] box sweep {box} {dt}: S r vs composite {err:.2e}", flush=True)
+            assert err <= tol, (box, dt, err)
+    b.set_sweep(True)
+    refl = P.dg_box_2d(5, 6, 4, order=1, bcs={1: "reflect", 2: "vacuum", 3: "vacuum", 4: "vacuum"})
+    b2 = u.Batch(ops[:2] + [refl.operator()])
+    b2.set_group(0)
+    info = b2.sweep_info()
+    assert not info["exact_fp64"] and not info["exact_fp32"] and not info["box_fp64"]

@@ -98,6 +98,54 @@ def test_sweep_inverts_L(gpu, built):
     assert e64 <= 1e-11 and e32 <= 1e-4
 
 
+FOLD_CASES = {
+    "dg1_quad": lambda: P.dg_box_2d(6, 6, 6, order=1, c_range=(0.5, 0.99)),
+    "dg1_delaunay": lambda: P.dg1_delaunay(n_side=10, sn=8, seed=3),
+    "dg1_hex": lambda: P.dg_box_3d(3, 4, order=1),
+    "dg1_quad_reflect": lambda: P.dg_box_2d(6, 6, 6, order=1, bcs={1: "reflect", 4: "reflect", 2: "vacuum",
+                                                                    3: "vacuum"}),
+    "fd2": lambda: P.fd_box_2d_central_source(n=12, sn=4),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FOLD_CASES))
+def test_pc_stages_fold_to_exact_sweep(gpu, case):
+    """The composite preconditioner's first two stages - element blocks, then
+    the sweep S on the L-residual - against the same composite formed here
+    from the stage objects. Where S is L's exact inverse (a DG sweep with no
+    lagged edge and no reflective face) the solver runs S alone, so this pins
+    y + S (r - L y) = S r on the GPU; the reflective case, whose sweep leaves
+    the reflected inflow out, must run both stages (its S r differs)."""
+    prob = FOLD_CASES[case]()
+    op = prob.operator()
+    solvers = {u.FP64: u.Solver(op, pc_dsa=False), u.FP32: u.Solver(op, pc_dsa=False, mixed=True)}
+    for sv in solvers.values():
+        sv.refresh()
+    bi = u.BlockInverse(op)
+    bi.setup()
+    sw = u.Sweep(op)
+    exact = case.startswith("dg") and "reflect" not in case
+    assert (sw.n_lagged == 0) or not exact
+    r = np.random.default_rng(8).standard_normal(prob.disc.n_rows)
+    for dt, tol in ((torch.float64, 1e-11), (torch.float32, 1e-4)):
+        prec = u.FP64 if dt == torch.float64 else u.FP32
+        rr = to_dev(r, dt)
+        y = bi.apply(rr, op.vec(prec))
+        t = op.mult_unscattered(y, op.vec(prec))
+        comp = y + sw.apply(rr - t, op.vec(prec))
+        pc = solvers[prec].apply_pc(rr, op.vec(prec))
+        sr = sw.apply(rr, op.vec(prec))
+        torch.cuda.synchronize()
+        err = rel_max(to_host(pc), to_host(comp))
+        off = rel_max(to_host(sr), to_host(comp))
+        print(f"[case] {case} {dt}: pc vs composite {err:.2e}, S r vs composite {off:.2e}", flush=True)
+        assert err <= tol, (case, dt, err)
+        if exact:
+            assert torch.equal(pc, sr)  # the solver ran the sweep alone
+        if "reflect" in case:
+            assert off > 1e-3  # the sweep alone is not the composite here
+
+
 @pytest.mark.parametrize("case", ["dg1_delaunay", "dg1_hex"])
 def test_sweep_lds_and_hbm_traces_agree(gpu, case):
     """the sweep reads its upwind traces from LDS slots where they fit and

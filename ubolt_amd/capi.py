@@ -119,7 +119,7 @@ SIGNATURES = {
     "ubt_batch_mult": [P, I, P, P, P],
     "ubt_batch_tuning": [P, PI, PI, C.POINTER(C.c_float)],
     "ubt_batch_set_sweep": [P, I],
-    "ubt_batch_sweep_info": [P, PI],
+    "ubt_batch_sweep_info": [P, PI, PI],
     "ubt_batch_set_body": [P, I, I],
     "ubt_batch_rhs": [P, I, P, P],
     "ubt_batch_set_scalar_flux": [P, I, P, P],

@@ -408,9 +408,10 @@ class Batch(_Handle):
         call("ubt_batch_set_sweep", self.h, int(box))
 
     def sweep_info(self):
-        box = (C.c_int * 2)()
-        call("ubt_batch_sweep_info", self.h, box)
-        return {"box_fp64": bool(box[0]), "box_fp32": bool(box[1])}
+        box, exact = (C.c_int * 2)(), (C.c_int * 2)()
+        call("ubt_batch_sweep_info", self.h, box, exact)
+        return {"box_fp64": bool(box[0]), "box_fp32": bool(box[1]), "exact_fp64": bool(exact[0]),
+                "exact_fp32": bool(exact[1])}
 
     def tuning(self):
         body, persist = (C.c_int * 2)(), (C.c_int * 2)()

@@ -90,6 +90,9 @@ struct Batch {
   void blocks_setup(hipStream_t s);  // singular check of every active problem's blocks
   void blocks_apply(int prec, const void *x, void *y, hipStream_t s) const;
   void sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const;
+  // the sweep inverts every active problem's L exactly (the box sweep, or
+  // record sweeps with no lagged or reflective coupling)
+  bool sweep_exact(int prec) const;
   // (re)build the sweep records when the group or any cross sections changed
   void sweep_records(hipStream_t s) const;
   uint64_t xs_tag() const;
@@ -128,6 +131,11 @@ struct BatchSolver {
   DevBuf V64, Z64, V32, Z32, w64, t64, u64, w32, t32, u32, d32, b32, r64;
   DevBuf cfirst;              // per problem: its first chunk (prefix, np + 1)
   DevBuf part, H, cs, sn, gv, hc, yv, nrm, tol, res, bnorm;
+  // per problem x kS: the scale s_i of each stored basis vector. The basis is
+  // kept as W_i with V_i = s_i W_i (W_0 the residual, W_{j+1} the
+  // orthogonalised vector before its normalisation, s = 1 / its norm), so no
+  // pass over the vector only to normalise it
+  DevBuf vsc;
   DevBuf onrm;                // per problem: the outer residual norm (refinement scale)
   DevBuf done;                // int per problem x 4: done, steps this cycle, iterations, live
   int gen = -1;               // the batch work lists the graphs were captured on

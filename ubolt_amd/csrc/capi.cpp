@@ -728,12 +728,16 @@ int ubt_batch_set_sweep(ubt_batch *batch, int box) {
     batch->b.gen++;  // a solver refreshes (and recaptures) before its next solve
   });
 }
-int ubt_batch_sweep_info(const ubt_batch *batch, int *box) {
+int ubt_batch_sweep_info(const ubt_batch *batch, int *box, int *exact) {
   return guard([&] {
     NEED(batch);
     NEED(box);
     box[0] = batch->b.box_sweep[0].ok ? 1 : 0;
     box[1] = batch->b.box_sweep[1].ok ? 1 : 0;
+    if (exact) {
+      exact[0] = batch->b.sweep_exact(UBT_FP64) ? 1 : 0;
+      exact[1] = batch->b.sweep_exact(UBT_FP32) ? 1 : 0;
+    }
   });
 }
 int ubt_batch_rhs(ubt_batch *batch, int prec, void *b, void *stream) {

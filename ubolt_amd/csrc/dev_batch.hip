@@ -93,6 +93,60 @@ __global__ void __launch_bounds__(kT) k_bmaxpy(int64_t n, int k, const T *__rest
   }
 }
 
+// Projections of the Arnoldi step j on the stored basis (V_i = s_i W_i, the
+// vector w in the scale a_p = s_j (first pass, w = A M^{-1} W_j) or 1 (second
+// pass, w already the true vector)): d_i = W_i . w summed over problem p's
+// chunks, h_i = s_i a_p d_i (the Hessenberg entry V_i . (a_p w)), and the
+// update coefficient c_i = -s_i h_i of W_i in a_p w - sum_i h_i V_i
+__global__ void k_bproj_final(int k, const int *__restrict__ cfirst, const double *__restrict__ part,
+                              const double *__restrict__ vs, int j, int first_pass, double *__restrict__ h,
+                              double *__restrict__ c) {
+  const int p = blockIdx.x, i = blockIdx.y;
+  if (i >= k) return;
+  double t = 0.0;
+  for (int ch = cfirst[p] + (int)threadIdx.x; ch < cfirst[p + 1]; ch += 64) t += part[(int64_t)ch * kS + i];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (threadIdx.x == 0) {
+    const double si = vs[(int64_t)p * kS + i], ap = first_pass ? vs[(int64_t)p * kS + j] : 1.0;
+    const double hi = si * ap * t;
+    h[(int64_t)p * kS + i] = hi;
+    c[(int64_t)p * kS + i] = -si * hi;
+  }
+}
+
+// w = a_p w + sum_i c[p][i] V[i] over each chunk of problem p (a_p = vs[p][j]
+// when vs is given, else 1), and with part, each chunk's sum of the new w^2
+template <class T>
+__global__ void __launch_bounds__(kT) k_bgs(int64_t n, int k, const T *__restrict__ V, const double *__restrict__ coef,
+                                            const double *__restrict__ vs, int j, T *__restrict__ w,
+                                            const int64_t *__restrict__ cstart, const int *__restrict__ clen,
+                                            const int *__restrict__ cprob, double *__restrict__ part) {
+  const int ch = blockIdx.x;
+  const int64_t s0 = cstart[ch];
+  const int len = clen[ch];
+  const int p = cprob[ch];
+  const double *h = coef + (int64_t)p * kS;
+  const double a = vs ? vs[(int64_t)p * kS + j] : 1.0;
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < len; i += kT) {
+    double v = a * (double)w[s0 + i];
+    for (int q = 0; q < k; q++) v += h[q] * (double)V[(int64_t)q * n + s0 + i];
+    const T o = (T)v;
+    w[s0 + i] = o;
+    ss += (double)o * (double)o;
+  }
+  if (!part) return;
+  __shared__ double sh[kT / 64];
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kT / 64; q++) t += sh[q];
+    part[(int64_t)ch * kS] = t;
+  }
+}
+
 // v = w * (d[p] != 0 ? 1 / d[p] : 0) (inv) or v = w * d[p] (!inv), converting precision
 template <class Ti, class To>
 __global__ void __launch_bounds__(kT) k_bscale(const Ti *__restrict__ w, const double *__restrict__ d, int inv,
@@ -130,12 +184,13 @@ template <class T> __global__ void k_axpby_b(int64_t n, double a, const T *__res
 
 // cycle start: g = beta e_0, done where beta <= tol (or the problem is idle)
 __global__ void k_bstart(int np, int m, const double *__restrict__ beta, const double *__restrict__ tol,
-                         const int *__restrict__ live, double *__restrict__ g, int *__restrict__ done,
-                         int *__restrict__ jdone) {
+                         const int *__restrict__ live, double *__restrict__ g, double *__restrict__ vs,
+                         int *__restrict__ done, int *__restrict__ jdone) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   for (int i = 0; i < kS; i++) g[(int64_t)p * kS + i] = 0.0;
   g[(int64_t)p * kS] = beta[p];
+  vs[(int64_t)p * kS] = beta[p] > 0.0 ? 1.0 / beta[p] : 0.0;  // W_0 is the residual itself
   const bool d = !live[p] || !(beta[p] > tol[p]);
   done[p] = d ? 1 : 0;
   jdone[p] = d ? 0 : m;
@@ -145,10 +200,14 @@ __global__ void k_bstart(int np, int m, const double *__restrict__ beta, const d
 __global__ void k_bgivens(int np, int j, const double *__restrict__ h1, const double *__restrict__ h2,
                           const double *__restrict__ hn, double *__restrict__ H, double *__restrict__ cs,
                           double *__restrict__ sn, double *__restrict__ g, double *__restrict__ res,
-                          const double *__restrict__ tol, int *__restrict__ done, int *__restrict__ jdone,
-                          int *__restrict__ its) {
+                          const double *__restrict__ tol, double *__restrict__ vs, int *__restrict__ done,
+                          int *__restrict__ jdone, int *__restrict__ its) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np || done[p]) return;
+  if (p >= np) return;
+  // W_{j+1} is the orthogonalised vector itself, of norm h_{j+1,j} (a
+  // finished problem keeps a finite scale too: its span still runs, inert)
+  vs[(int64_t)p * kS + j + 1] = hn[p] > 0.0 ? 1.0 / hn[p] : 0.0;
+  if (done[p]) return;
   double *Hp = H + (int64_t)p * kS * kS, *c = cs + (int64_t)p * kS, *s = sn + (int64_t)p * kS,
          *gp = g + (int64_t)p * kS;
   for (int i = 0; i <= j; i++) Hp[i * kS + j] = h1[(int64_t)p * kS + i] + (h2 ? h2[(int64_t)p * kS + i] : 0.0);
@@ -175,7 +234,7 @@ __global__ void k_bgivens(int np, int j, const double *__restrict__ h1, const do
 
 // y = H^{-1} g on the leading jdone x jdone triangle; zero past it
 __global__ void k_btri(int np, const double *__restrict__ H, const double *__restrict__ g, const int *__restrict__ jdone,
-                       double *__restrict__ y) {
+                       const double *__restrict__ vs, double *__restrict__ y) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double *Hp = H + (int64_t)p * kS * kS, *gp = g + (int64_t)p * kS;
@@ -190,6 +249,8 @@ __global__ void k_btri(int np, const double *__restrict__ H, const double *__res
     for (int k = i + 1; k < jd; k++) v -= Hp[i * kS + k] * yp[k];
     yp[i] = Hp[i * kS + i] != 0.0 ? v / Hp[i * kS + i] : 0.0;
   }
+  // Z_i was formed from W_i = V_i / s_i: its coefficient carries s_i
+  for (int i = 0; i < jd; i++) yp[i] *= vs[(int64_t)p * kS + i];
 }
 
 unsigned grid_n(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kT - 1) / kT, 8192)); }
@@ -498,6 +559,13 @@ void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const
     batch_sweep_launch<float>(*this, (const float *)r, (float *)psi, s);
 }
 
+bool Batch::sweep_exact(int prec) const {
+  if (box_sweep[prec == UBT_FP64 ? 0 : 1].ok) return true;  // uniform all-vacuum boxes: no cycle, no reflection
+  for (size_t p = 0; p < ops.size(); p++)
+    if (active[p] && !sweeps[p]->exact) return false;
+  return true;
+}
+
 void Batch::rhs(int prec, void *b, hipStream_t s) const {
   const size_t ts = prec == UBT_FP64 ? 8 : 4;
   for (size_t p = 0; p < ops.size(); p++) {
@@ -603,7 +671,6 @@ void BatchSolver::create(Batch *b, const SolverOptions &o) {
   } else {
     V32.alloc(sizeof(float) * n * (m + 1));
     Z32.alloc(sizeof(float) * n * m);
-    w32.alloc(sizeof(float) * n);
     t32.alloc(sizeof(float) * n);
     u32.alloc(sizeof(float) * n);
     d32.alloc(sizeof(float) * n);
@@ -615,7 +682,8 @@ void BatchSolver::create(Batch *b, const SolverOptions &o) {
   cs.alloc(sizeof(double) * P * kS);
   sn.alloc(sizeof(double) * P * kS);
   gv.alloc(sizeof(double) * P * kS);
-  hc.alloc(sizeof(double) * P * kS * 2);
+  hc.alloc(sizeof(double) * P * kS * 4);  // h and c of both Gram-Schmidt passes
+  vsc.alloc(sizeof(double) * P * kS);
   yv.alloc(sizeof(double) * P * kS);
   nrm.alloc(sizeof(double) * P);
   tol.alloc(sizeof(double) * P);
@@ -656,14 +724,18 @@ void BatchSolver::refresh(hipStream_t s) {
   xs_tag = batch->xs_tag();
 }
 
-// y = M^{-1} r: element blocks, then the sweep on the L-residual
+// y = M^{-1} r: element blocks, then the sweep on the L-residual. Where the
+// sweep is L's exact inverse S the composite is the sweep alone:
+// y + S (r - L y) = S r for any first stage y, so the blocks, the residual
+// apply and the two vector updates are left out (the same result up to
+// rounding)
 template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
   T *t = sizeof(T) == 8 ? t64.as<T>() : t32.as<T>();
   T *u = sizeof(T) == 8 ? u64.as<T>() : u32.as<T>();
   const int prec = prec_id<T>();
   const unsigned gn = grid_n(n);
   bool first = true;
-  if (opt.pc_blocks) {
+  if (opt.pc_blocks && !(opt.pc_sweep && batch->sweep_exact(prec))) {
     batch->blocks_apply(prec, r, y, s);
     first = false;
   }
@@ -687,19 +759,18 @@ template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
 template <class T> void BatchSolver::cycle_start(const T *b, const T *x, hipStream_t s) {
   const int m = opt.restart, prec = prec_id<T>();
   T *V = sizeof(T) == 8 ? V64.as<T>() : V32.as<T>();
-  T *w = sizeof(T) == 8 ? w64.as<T>() : w32.as<T>();
   const int64_t *cst = chunk_start.as<int64_t>();
-  const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
+  const int *cln = chunk_len.as<int>(), *cfp = cfirst.as<int>();
   int *dn = done.as<int>(), *jd = dn + np, *live = dn + 3 * np;
   double *hn = nrm.as<double>(), *pt = part.as<double>();
   const unsigned gn = grid_n(n), gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
-  batch->mult(prec, x, w, true, s);
-  hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, w);
-  hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
+  // W_0 = b - A x in V's first slot, s_0 = 1 / beta
+  batch->mult(prec, x, V, true, s);
+  hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, V);
+  hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)V, (const T *)V, cst, cln, pt);
   hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
-  hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, V, cst, cln, cpr);
   hipLaunchKernelGGL(k_bstart, dim3(gp), dim3(64), 0, s, np, m, (const double *)hn, (const double *)tol.as<double>(),
-                     (const int *)live, gv.as<double>(), dn, jd);
+                     (const int *)live, gv.as<double>(), vsc.as<double>(), dn, jd);
   UBT_HIP(hipGetLastError());
 }
 
@@ -707,33 +778,38 @@ template <class T> void BatchSolver::cycle_steps(int j0, int j1, hipStream_t s) 
   const int prec = prec_id<T>();
   T *V = sizeof(T) == 8 ? V64.as<T>() : V32.as<T>();
   T *Z = sizeof(T) == 8 ? Z64.as<T>() : Z32.as<T>();
-  T *w = sizeof(T) == 8 ? w64.as<T>() : w32.as<T>();
   const int64_t *cst = chunk_start.as<int64_t>();
   const int *cln = chunk_len.as<int>(), *cpr = chunk_prob.as<int>(), *cfp = cfirst.as<int>();
   int *dn = done.as<int>(), *jd = dn + np, *its = dn + 2 * np;
-  double *h1 = hc.as<double>(), *h2 = h1 + (size_t)np * kS, *hn = nrm.as<double>(), *pt = part.as<double>();
+  const size_t ps = (size_t)np * kS;
+  double *h1 = hc.as<double>(), *h2 = h1 + ps, *c1 = h1 + 2 * ps, *c2 = h1 + 3 * ps;
+  double *hn = nrm.as<double>(), *pt = part.as<double>(), *vs = vsc.as<double>();
   const unsigned gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
+  const bool two = opt.gs_passes > 1;
   for (int j = j0; j < j1; j++) {
-    T *vj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *vn = V + (int64_t)(j + 1) * n;
-    apply_pc<T>(vj, zj, s);
+    // W_{j+1} is built in place in V's slot j + 1: A M^{-1} W_j, then the
+    // projections removed and the scale s_j applied in one pass (the last
+    // pass also sums its squares for h_{j+1,j})
+    T *wj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *w = V + (int64_t)(j + 1) * n;
+    apply_pc<T>(wj, zj, s);
     batch->mult(prec, zj, w, true, s);
     hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
-    hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h1, kS, 0);
-    hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h1, kS, -1.0, w,
-                       cst, cln, cpr, (const int *)nullptr);
-    if (opt.gs_passes > 1) {  // classical Gram-Schmidt twice
+    hipLaunchKernelGGL(k_bproj_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt,
+                       (const double *)vs, j, 1, h1, c1);
+    hipLaunchKernelGGL(k_bgs<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)c1,
+                       (const double *)vs, j, w, cst, cln, cpr, two ? nullptr : pt);
+    if (two) {  // classical Gram-Schmidt twice
       hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
-      hipLaunchKernelGGL(k_bdot_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt, h2, kS, 0);
-      hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)h2, kS, -1.0,
-                         w, cst, cln, cpr, (const int *)nullptr);
+      hipLaunchKernelGGL(k_bproj_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt,
+                         (const double *)vs, j, 0, h2, c2);
+      hipLaunchKernelGGL(k_bgs<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)c2,
+                         (const double *)nullptr, 0, w, cst, cln, cpr, pt);
     }
-    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)w, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
-    hipLaunchKernelGGL((k_bscale<T, T>), dim3(nc), dim3(kT), 0, s, (const T *)w, (const double *)hn, 1, vn, cst, cln, cpr);
     hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1,
-                       (const double *)(opt.gs_passes > 1 ? h2 : nullptr),
+                       (const double *)(two ? h2 : nullptr),
                        (const double *)hn, H.as<double>(), cs.as<double>(), sn.as<double>(), gv.as<double>(),
-                       res.as<double>(), (const double *)tol.as<double>(), dn, jd, its);
+                       res.as<double>(), (const double *)tol.as<double>(), vs, dn, jd, its);
   }
   UBT_HIP(hipGetLastError());
 }
@@ -746,7 +822,7 @@ template <class T> void BatchSolver::cycle_end(T *x, hipStream_t s) {
   const int *jd = done.as<int>() + np;
   const unsigned gp = (unsigned)((np + 63) / 64), nc = (unsigned)n_chunks;
   hipLaunchKernelGGL(k_btri, dim3(gp), dim3(64), 0, s, np, (const double *)H.as<double>(),
-                     (const double *)gv.as<double>(), jd, yv.as<double>());
+                     (const double *)gv.as<double>(), jd, (const double *)vsc.as<double>(), yv.as<double>());
   hipLaunchKernelGGL(k_bmaxpy<T>, dim3(nc), dim3(kT), 0, s, n, m, (const T *)Z, (const double *)yv.as<double>(), kS, 1.0,
                      x, cst, cln, cpr, jd);
   UBT_HIP(hipGetLastError());

@@ -208,7 +208,10 @@ template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s) {
     fn((const T *)t, u);
     hipLaunchKernelGGL(k_axpby<T>, dim3(grid_for(n)), dim3(kThreads), 0, s, n, 1.0, (const T *)u, 1.0, y);
   };
-  if (opt.pc_blocks) stage([&](const T *a, T *b) { blocks.apply(prec, a, b, s, transpose); });
+  // where the sweep is L's exact inverse S and the stages correct with L, the
+  // first two stages are the sweep alone: y + S (r - L y) = S r for any y
+  const bool fold = opt.pc_sweep && sweep.exact && !amat;
+  if (opt.pc_blocks && !fold) stage([&](const T *a, T *b) { blocks.apply(prec, a, b, s, transpose); });
   if (opt.pc_sweep) stage([&](const T *a, T *b) { sweep.apply(prec, a, b, s, transpose); });
   if (with_dsa) stage([&](const T *a, T *b) { dsa->apply(prec, a, b, s, transpose); });
   if (first) hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for(n)), dim3(kThreads), 0, s, n, 1.0, r, y);

@@ -788,6 +788,7 @@ void Sweep::create(Operator *o, bool keep_host, bool lds_traces) {
     }
     max_level = std::max(max_level, lvl);
   }
+  exact = n_lagged == 0 && std::none_of(D.face_nb.begin(), D.face_nb.end(), [](int v) { return v < -1; });
   // per direction (one workgroup each): the items in generation order
   // (direction, then level), and each direction's level boundaries
   const int64_t n_items = (int64_t)item_cell.size();

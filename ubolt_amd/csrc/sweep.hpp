@@ -29,6 +29,9 @@ struct Sweep {
   Operator *op = nullptr;
   int n_levels = 0;                 // the most levels of any direction
   int64_t n_lagged = 0;             // upwind edges lagged to break cycles
+  // psi = L^{-1} r exactly: a DG schedule with no lagged edge and no
+  // reflective face (whose coupling the sweep leaves out)
+  bool exact = false;
   int threads = 64;                 // DG: workgroup size (the widest level, at most 1024)
   // DG, per direction (one workgroup each): items (cell, angle, upwind face
   // mask) in level order, level boundaries lvl[base[g] .. base[g + 1]) of
