@@ -37,13 +37,16 @@ template <class T> constexpr int kBoxSweepAngles = sizeof(T) == 8 ? 4 : 8;
 // diagonals whose r and sigma_t are in flight while one is solved
 template <class T> constexpr int kBoxSweepDepth = sizeof(T) == 8 ? 2 : 1;
 struct BoxSweepGroup {
-  int item;      // the problem's entry in the strip body's work list
+  int item;      // the problem's entry in the strip body's work list (-1: an idle slot)
   int q, k0;     // the quadrant, the group's first position among its angles
   int angle[8];  // the quadrant's angles (-1: none)
+  int ipt;       // cells per thread on the longest diagonal (1, 2, 4 or 8)
+  int64_t rbase; // the group's first value in the quadrant-major copies
 };
 // r and psi pass through a quadrant-major copy, row (q, cell, basis) of the
-// quadrant's A / 4 angles contiguous, so the sweep's loads and stores of one
-// quadrant are whole lines
+// quadrant's A / 4 angles contiguous. The groups of one (problem, quadrant)
+// read neighbouring parts of the same lines, so they are placed on one XCD
+// (workgroup i runs on XCD i mod 8) and share its L2.
 struct BoxSweepPlan {
   bool ok = false;   // every active problem is such a box (else the record sweep runs)
   int n_wg = 0, nx_max = 0;

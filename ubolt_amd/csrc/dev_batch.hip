@@ -428,14 +428,23 @@ void Batch::build_lists() {
 // a uniform all-vacuum 2D DG1 box (and the strip body's layout, whose work
 // items the sweep reads, in place), at most 256 cells along x (the LDS
 // diagonal); one workgroup per problem, quadrant and group of the quadrant's
-// angles, the problems with the most diagonals first
+// angles. The groups of one (problem, quadrant) share lines of the
+// quadrant-major copies, so they go to one XCD: the sets, longest march
+// first, are dealt to the XCD with the least work so far, and workgroup
+// i = 8 j + x is the j-th of XCD x's list (idle slots pad the shorter lists)
 void Batch::build_box_sweep() {
+  constexpr int kXcd = 8;
   for (int pi = 0; pi < 2; pi++) {
     BoxSweepPlan &w = box_sweep[pi];
     const int AG = pi == 0 ? kBoxSweepAngles<double> : kBoxSweepAngles<float>;
     w.ok = dim == 2 && nb == 3 && body_ok[pi][3] && box_sweep_enabled;
     w.nx_max = 0;
-    std::vector<std::pair<int, BoxSweepGroup>> grp;  // (diagonals, group)
+    struct Set {
+      int diagonals;
+      double work;
+      std::vector<BoxSweepGroup> g;
+    };
+    std::vector<Set> sets;
     std::vector<int> qpos;
     std::vector<int64_t> qoff(1, 0);
     int item = 0;
@@ -447,6 +456,10 @@ void Batch::build_box_sweep() {
       if (!w.ok) break;
       w.nx_max = std::max(w.nx_max, o.box_n[0]);
       const int Aq = q.n_angles / 4;
+      const int64_t nc = o.disc->n_cells;
+      // a diagonal holds at most min(nx, ny) cells, 256 / AG per pass
+      const int need = (std::min(o.box_n[0], o.box_n[1]) + 256 / AG - 1) / (256 / AG);
+      const int ipt = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
       std::vector<int> pos(q.n_angles, -1);
       for (int sx = 0; sx < 2; sx++)
         for (int sy = 0; sy < 2; sy++) {
@@ -459,14 +472,20 @@ void Batch::build_box_sweep() {
             break;
           }
           for (int j = 0; j < Aq; j++) pos[an[j]] = quad * Aq + j;
+          Set st;
+          st.diagonals = o.box_n[0] + o.box_n[1] - 1;
+          st.work = (double)nc * Aq;
           for (size_t k0 = 0; k0 < an.size(); k0 += AG) {
             BoxSweepGroup gw;
             gw.item = item;
             gw.q = quad;
             gw.k0 = (int)k0;
             for (int k = 0; k < 8; k++) gw.angle[k] = k < AG && k0 + k < an.size() ? an[k0 + k] : -1;
-            grp.push_back({o.box_n[0] + o.box_n[1] - 1, gw});
+            gw.ipt = ipt;
+            gw.rbase = off[p] + (int64_t)quad * nc * 3 * Aq + (int64_t)k0;
+            st.g.push_back(gw);
           }
+          sets.push_back(std::move(st));
         }
       qpos.insert(qpos.end(), pos.begin(), pos.end());
       qoff.push_back(qoff.back() + q.n_angles);
@@ -476,9 +495,21 @@ void Batch::build_box_sweep() {
       w.n_wg = 0;
       continue;
     }
-    std::stable_sort(grp.begin(), grp.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
-    std::vector<BoxSweepGroup> g;
-    for (const auto &e : grp) g.push_back(e.second);
+    std::stable_sort(sets.begin(), sets.end(), [](const Set &x, const Set &y) { return x.diagonals > y.diagonals; });
+    std::vector<std::vector<BoxSweepGroup>> lists(kXcd);
+    std::vector<double> load(kXcd, 0.0);
+    for (const Set &st : sets) {
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += st.work;
+      lists[x].insert(lists[x].end(), st.g.begin(), st.g.end());
+    }
+    size_t len = 0;
+    for (const auto &l : lists) len = std::max(len, l.size());
+    BoxSweepGroup idle{};
+    idle.item = -1;
+    std::vector<BoxSweepGroup> g(len * kXcd, idle);
+    for (int x = 0; x < kXcd; x++)
+      for (size_t j = 0; j < lists[x].size(); j++) g[j * kXcd + x] = lists[x][j];
     w.n_wg = (int)g.size();
     w.groups.upload(g);
     w.qpos.upload(qpos);

@@ -3291,38 +3291,38 @@ namespace {
 // inflow corner). Cells of a diagonal are independent; their upwind traces
 // are the previous diagonal's results, held in LDS by i' (x upwind at
 // i' - 1, y upwind at i'), so the only global traffic is r in, psi out and
-// sigma_t. A thread solves up to AG cells of a diagonal (AG angle lanes per
-// cell); the next diagonal's r and sigma_t are requested before the current
-// one is solved, into the other of two register sets (the loop is unrolled
-// by two so neither set is copied), and every load is unconditional on a
-// clamped cell.
+// sigma_t, r and psi in the quadrant-major copies (addressed by 32-bit
+// offsets from the group's base). A thread solves up to IPT cells of a
+// diagonal (AG angle lanes per cell); the next DEPTH diagonals' r and
+// sigma_t are requested before the current one is solved, into register
+// sets in rotation (the loop is unrolled by the rotation so no set is
+// copied). Every load is unconditional - past the last diagonal the fetch
+// repeats it, a cell past a diagonal's end reads its first cell - so at
+// least one diagonal's loads always follow a set and the compiler's wait
+// for that set never covers the prefetches behind it.
 // ---------------------------------------------------------------------------
-template <class T, int AG, int DEPTH>
-__global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__restrict__ r, T *__restrict__ psi,
-                                                            const BatchItem<T> *__restrict__ items,
-                                                            const BoxSweepGroup *__restrict__ groups,
-                                                            int nx_max) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int NB = 3, SL = kBoxSweepThreads / AG, IPT = 256 / SL;
-  T *ring = reinterpret_cast<T *>(smem);
-  const BoxSweepGroup &gw = groups[blockIdx.x];
-  const BatchItem<T> &it = items[gw.item];
-  const OpArgs<T> &g = it.g;
-  const BoxArgs &bx = it.box;
+template <class T, int AG, int DEPTH, int IPT>
+__device__ __forceinline__ void box_march(const T *__restrict__ rg, T *__restrict__ pg, const OpArgs<T> &g,
+                                          const BoxArgs &bx, const BoxSweepGroup &gw, T *ring, int nx_max) {
+  constexpr int NB = 3, SL = kBoxSweepThreads / AG;
   const int k = threadIdx.x % AG, slot = threadIdx.x / AG;
   const int a0 = gw.angle[k];
   const bool va = a0 >= 0;
   const int a = va ? a0 : gw.angle[0];
-  const int A = g.A, nx = bx.n[0], ny = bx.n[1];
-  // the quadrant-major copies: row (q, cell, basis) of the A / 4 quadrant
-  // angles, this lane's at position k0 + k
-  const int Aq = A / 4, kq = gw.k0 + (va ? k : 0);
-  const T *rp = r + it.off + (int64_t)gw.q * g.n_cells * NB * Aq + kq;
-  T *pp = psi + it.off + (int64_t)gw.q * g.n_cells * NB * Aq + kq;
+  const int nx = bx.n[0], ny = bx.n[1];
+  const int Aq = g.A / 4, NA = NB * Aq;
+  // a lane past the quadrant's angles reads lane 0's values (and stores
+  // nothing)
+  const int lane = va ? k : 0;
+  // q = 2 sx + sy, sx = 0 for omega_x > 0 (the march runs up x from i = 0),
+  // sy likewise for y
+  const bool px = (gw.q >> 1) == 0, py = (gw.q & 1) == 0;
+  auto cell_of = [&](int ip, int jp) -> unsigned {
+    const int i = px ? ip : nx - 1 - ip, j = py ? jp : ny - 1 - jp;
+    return (unsigned)(j * nx + i);
+  };
   BoxConst<T, 2, NB> kc;
   box_consts<T, 2, NB>(g, bx, a, kc);
-  const bool px = kc.in_side[0] == 0, py = kc.in_side[1] == 0;
-  const int NA = NB * Aq;
   // the block sigma_t I + M solved in closed form (Cayley-Hamilton for 3 x 3):
   // adj(s I + M) = s^2 I + s (tr M I - M) + adj M,
   // det(s I + M) = s^3 + tr M s^2 + c2 s + det M (c2: the principal 2 x 2
@@ -3341,37 +3341,43 @@ __global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__r
   adjM[2][2] = M[0][0] * M[1][1] - M[0][1] * M[1][0];
   const T c2 = adjM[0][0] + adjM[1][1] + adjM[2][2];
   const T detM = M[0][0] * adjM[0][0] + M[0][1] * adjM[1][0] + M[0][2] * adjM[2][0];
-  auto cell_of = [&](int ip, int d) -> int64_t {
-    const int jp = d - ip;
-    const int i = px ? ip : nx - 1 - ip, j = py ? jp : ny - 1 - jp;
-    return (int64_t)j * nx + i;
-  };
-  // the cells of diagonal d this thread solves: ip = lo + slot + m SL
+  const int n_diag = nx + ny - 1;
+  // the cells of diagonal d this thread solves: i' = lo + e, e = slot + m SL
   struct Set {
     T r[IPT][NB], st[IPT];
   };
   auto fetch = [&](int d, Set &s) {
+    d = d < n_diag ? d : n_diag - 1;
     const int lo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, hi = d < nx - 1 ? d : nx - 1;
+    const int n = hi - lo;
 #pragma unroll
     for (int m = 0; m < IPT; m++) {
-      int ip = lo + slot + m * SL;
-      ip = ip <= hi ? ip : lo;  // clamped: a valid cell, its values discarded
-      const int64_t c = cell_of(ip, d);
+      int e = slot + m * SL;
+      e = e <= n ? e : 0;  // clamped: a valid cell, its values discarded
+      const unsigned c = cell_of(lo + e, d - lo - e);
 #pragma unroll
-      for (int b = 0; b < NB; b++) s.r[m][b] = ldg(rp + c * NA + b * Aq);
+      for (int b = 0; b < NB; b++) s.r[m][b] = ldg(rg + (c * NA + b * Aq + lane));
       s.st[m] = ldg(g.sigt + c);
     }
   };
-  const int n_diag = nx + ny - 1;
+  // every loaded value is read once at the start of its solve (into sink,
+  // stored on a path never taken), so no load of a set is still in flight
+  // when its registers are reused - the clamped cells past a diagonal's end
+  // would otherwise leave loads whose completion is first known at the
+  // reuse, a wait for every load in flight
+  T sink = T(0);
   auto solve = [&](int d, const Set &s) {
     const int lo = d - (ny - 1) > 0 ? d - (ny - 1) : 0, hi = d < nx - 1 ? d : nx - 1;
+    const int n = hi - lo;
     const T *prev = ring + (size_t)((d + 1) & 1) * nx_max * AG * NB;
     T *cur = ring + (size_t)(d & 1) * nx_max * AG * NB;
 #pragma unroll
+    for (int m = 0; m < IPT; m++) sink += s.r[m][0] + s.r[m][1] + s.r[m][2] + s.st[m];
+#pragma unroll
     for (int m = 0; m < IPT; m++) {
-      const int ip = lo + slot + m * SL;
-      if (ip > hi) break;
-      const int jp = d - ip;
+      const int e = slot + m * SL;
+      if (e > n) break;
+      const int ip = lo + e, jp = d - ip;
       T hx[NB], hy[NB], v[NB];
 #pragma unroll
       for (int b = 0; b < NB; b++) {
@@ -3385,27 +3391,25 @@ __global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__r
         for (int q = 0; q < NB; q++) w -= kc.U[0][b][q] * hx[q] + kc.U[1][b][q] * hy[q];
         v[b] = w;
       }
-      const T sg = s.st[m];
-      const T inv_det = T(1) / (((sg + trM) * sg + c2) * sg + detM);
+      const T sgm = s.st[m];
+      const T inv_det = T(1) / (((sgm + trM) * sgm + c2) * sgm + detM);
       T x[NB];
 #pragma unroll
       for (int p = 0; p < NB; p++) {
         T w = T(0);
 #pragma unroll
         for (int q = 0; q < NB; q++) {
-          const T adj = sg * ((p == q ? sg + trM : T(0)) - M[p][q]) + adjM[p][q];
+          const T adj = sgm * ((p == q ? sgm + trM : T(0)) - M[p][q]) + adjM[p][q];
           w += adj * v[q];
         }
         x[p] = w * inv_det;
       }
 #pragma unroll
-      for (int b = 0; b < NB; b++) v[b] = x[b];
-#pragma unroll
-      for (int b = 0; b < NB; b++) cur[(ip * AG + k) * NB + b] = v[b];
+      for (int b = 0; b < NB; b++) cur[(ip * AG + k) * NB + b] = x[b];
       if (va) {
-        const int64_t c = cell_of(ip, d);
+        const unsigned c = cell_of(ip, jp);
 #pragma unroll
-        for (int b = 0; b < NB; b++) pp[c * NA + b * Aq] = v[b];
+        for (int b = 0; b < NB; b++) pg[c * NA + b * Aq + k] = x[b];
       }
     }
     __syncthreads();
@@ -3416,26 +3420,48 @@ __global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__r
     Set s0, s1;
     fetch(0, s0);
     for (int d = 0; d < n_diag; d += 2) {
-      if (d + 1 < n_diag) fetch(d + 1, s1);
+      fetch(d + 1, s1);
       solve(d, s0);
       if (d + 1 >= n_diag) break;
-      if (d + 2 < n_diag) fetch(d + 2, s0);
+      fetch(d + 2, s0);
       solve(d + 1, s1);
     }
   } else {
     Set s0, s1, s2;
     fetch(0, s0);
-    if (1 < n_diag) fetch(1, s1);
+    fetch(1, s1);
     for (int d = 0; d < n_diag; d += 3) {
-      if (d + 2 < n_diag) fetch(d + 2, s2);
+      fetch(d + 2, s2);
       solve(d, s0);
       if (d + 1 >= n_diag) break;
-      if (d + 3 < n_diag) fetch(d + 3, s0);
+      fetch(d + 3, s0);
       solve(d + 1, s1);
       if (d + 2 >= n_diag) break;
-      if (d + 4 < n_diag) fetch(d + 4, s1);
+      fetch(d + 4, s1);
       solve(d + 2, s2);
     }
+  }
+  if (gw.item == -2) ring[threadIdx.x] = sink;  // never: item is -1 (idle) or an index
+}
+
+template <class T, int AG, int DEPTH>
+__global__ void __launch_bounds__(kBoxSweepThreads) sweep_box_batch(const T *__restrict__ r, T *__restrict__ psi,
+                                                            const BatchItem<T> *__restrict__ items,
+                                                            const BoxSweepGroup *__restrict__ groups,
+                                                            int nx_max) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *ring = reinterpret_cast<T *>(smem);
+  const BoxSweepGroup &gw = groups[blockIdx.x];
+  if (gw.item < 0) return;  // an idle slot of a shorter XCD list
+  const BatchItem<T> &it = items[gw.item];
+  const T *rg = r + gw.rbase;
+  T *pg = psi + gw.rbase;
+  // the cells per thread the group's box needs (uniform over the workgroup)
+  switch (gw.ipt) {
+    case 1: box_march<T, AG, DEPTH, 1>(rg, pg, it.g, it.box, gw, ring, nx_max); break;
+    case 2: box_march<T, AG, DEPTH, 2>(rg, pg, it.g, it.box, gw, ring, nx_max); break;
+    case 4: box_march<T, AG, DEPTH, (AG < 4 ? AG : 4)>(rg, pg, it.g, it.box, gw, ring, nx_max); break;
+    default: box_march<T, AG, DEPTH, AG>(rg, pg, it.g, it.box, gw, ring, nx_max); break;
   }
 }
 
@@ -3482,8 +3508,7 @@ void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
   hipLaunchKernelGGL((quad_reorder<T, true>), dim3(grid), dim3(256), 0, s, r, rq, items, n_items, w.qpos.as<int>(),
                      w.qoff.as<int64_t>());
   hipLaunchKernelGGL((sweep_box_batch<T, AG, kBoxSweepDepth<T>>), dim3(w.n_wg), dim3(kBoxSweepThreads), lds, s,
-                     (const T *)rq, pq,
-                     items, w.groups.as<BoxSweepGroup>(), w.nx_max);
+                     (const T *)rq, pq, items, w.groups.as<BoxSweepGroup>(), w.nx_max);
   hipLaunchKernelGGL((quad_reorder<T, false>), dim3(grid), dim3(256), 0, s, (const T *)pq, psi, items, n_items,
                      w.qpos.as<int>(), w.qoff.as<int64_t>());
   UBT_HIP(hipGetLastError());
