@@ -28,33 +28,65 @@ constexpr int kS = 32;         // per-problem stride of the coefficient arrays (
 constexpr int kChunk = 16384;  // rows per reduction chunk
 constexpr int kT = 256;
 
-template <class T>
-__global__ void __launch_bounds__(kT) k_bdot(int64_t n, int k, const T *__restrict__ V, const T *__restrict__ w,
+// 16 bytes of T: the unit a thread loads and stores
+template <class T> struct alignas(16) Vec16 {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+};
+
+// a chunk of x (x the whole vector) moves in 16-byte units when its start is
+// on a 16-byte boundary and its length a whole number of units (always on
+// these layouts: 3 x angles rows per cell, vectors a multiple of that apart);
+// otherwise row by row
+template <class T> __device__ bool units_ok(const T *x, int64_t s0, int len) {
+  return ((reinterpret_cast<uintptr_t>(x + s0) & 15) == 0) && len % Vec16<T>::N == 0;
+}
+
+// part[ch][j] = V_j . w over chunk ch for the K vectors (K a template
+// constant, so every load of a unit - w and the K vectors - is
+// unconditional and issued together)
+template <class T, int K>
+__global__ void __launch_bounds__(kT) k_bdot(int64_t n, const T *__restrict__ V, const T *__restrict__ w,
                                              const int64_t *__restrict__ cstart, const int *__restrict__ clen,
                                              double *__restrict__ part) {
+  constexpr int VW = Vec16<T>::N;
   const int ch = blockIdx.x;
   const int64_t s0 = cstart[ch];
   const int len = clen[ch];
-  double acc[kS];
+  double acc[K];
 #pragma unroll
-  for (int j = 0; j < kS; j++) acc[j] = 0.0;
-  for (int i = threadIdx.x; i < len; i += kT) {
-    const double wi = (double)w[s0 + i];
+  for (int j = 0; j < K; j++) acc[j] = 0.0;
+  bool vec = units_ok(w, s0, len);
 #pragma unroll
-    for (int j = 0; j < kS; j++)
-      if (j < k) acc[j] += (double)V[(int64_t)j * n + s0 + i] * wi;
+  for (int j = 0; j < K; j++) vec = vec && units_ok(V + (int64_t)j * n, s0, len);
+  if (vec) {
+    for (int i = threadIdx.x * VW; i < len; i += kT * VW) {
+      const Vec16<T> wu = *reinterpret_cast<const Vec16<T> *>(w + s0 + i);
+      Vec16<T> vu[K];
+#pragma unroll
+      for (int j = 0; j < K; j++) vu[j] = *reinterpret_cast<const Vec16<T> *>(V + (int64_t)j * n + s0 + i);
+#pragma unroll
+      for (int j = 0; j < K; j++)
+#pragma unroll
+        for (int e = 0; e < VW; e++) acc[j] += (double)vu[j].v[e] * (double)wu.v[e];
+    }
+  } else {
+    for (int i = threadIdx.x; i < len; i += kT) {
+      const double wi = (double)w[s0 + i];
+#pragma unroll
+      for (int j = 0; j < K; j++) acc[j] += (double)V[(int64_t)j * n + s0 + i] * wi;
+    }
   }
-  __shared__ double sh[kS][kT / 64];
+  __shared__ double sh[K][kT / 64];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
-  for (int j = 0; j < kS; j++) {
-    if (j >= k) break;
+  for (int j = 0; j < K; j++) {
     double v = acc[j];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if (l == 0) sh[j][wv] = v;
   }
   __syncthreads();
-  if ((int)threadIdx.x < k) {
+  if ((int)threadIdx.x < K) {
     double t = 0.0;
     for (int q = 0; q < kT / 64; q++) t += sh[threadIdx.x][q];
     part[(int64_t)ch * kS + threadIdx.x] = t;
@@ -114,26 +146,53 @@ __global__ void k_bproj_final(int k, const int *__restrict__ cfirst, const doubl
   }
 }
 
-// w = a_p w + sum_i c[p][i] V[i] over each chunk of problem p (a_p = vs[p][j]
-// when vs is given, else 1), and with part, each chunk's sum of the new w^2
-template <class T>
-__global__ void __launch_bounds__(kT) k_bgs(int64_t n, int k, const T *__restrict__ V, const double *__restrict__ coef,
+// w = a_p w + sum_i c[p][i] V[i], i < K, over each chunk of problem p (a_p =
+// vs[p][j] when vs is given, else 1), and with part, each chunk's sum of the
+// new w^2; K a template constant, so a unit's loads are issued together
+template <class T, int K>
+__global__ void __launch_bounds__(kT) k_bgs(int64_t n, const T *__restrict__ V, const double *__restrict__ coef,
                                             const double *__restrict__ vs, int j, T *__restrict__ w,
                                             const int64_t *__restrict__ cstart, const int *__restrict__ clen,
                                             const int *__restrict__ cprob, double *__restrict__ part) {
+  constexpr int VW = Vec16<T>::N;
   const int ch = blockIdx.x;
   const int64_t s0 = cstart[ch];
   const int len = clen[ch];
   const int p = cprob[ch];
-  const double *h = coef + (int64_t)p * kS;
+  double h[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) h[q] = coef[(int64_t)p * kS + q];
   const double a = vs ? vs[(int64_t)p * kS + j] : 1.0;
   double ss = 0.0;
-  for (int i = threadIdx.x; i < len; i += kT) {
-    double v = a * (double)w[s0 + i];
-    for (int q = 0; q < k; q++) v += h[q] * (double)V[(int64_t)q * n + s0 + i];
-    const T o = (T)v;
-    w[s0 + i] = o;
-    ss += (double)o * (double)o;
+  bool vec = units_ok((const T *)w, s0, len);
+#pragma unroll
+  for (int q = 0; q < K; q++) vec = vec && units_ok(V + (int64_t)q * n, s0, len);
+  if (vec) {
+    for (int i = threadIdx.x * VW; i < len; i += kT * VW) {
+      const Vec16<T> wu = *reinterpret_cast<const Vec16<T> *>(w + s0 + i);
+      Vec16<T> vu[K];
+#pragma unroll
+      for (int q = 0; q < K; q++) vu[q] = *reinterpret_cast<const Vec16<T> *>(V + (int64_t)q * n + s0 + i);
+      Vec16<T> o;
+#pragma unroll
+      for (int e = 0; e < VW; e++) {
+        double v = a * (double)wu.v[e];
+#pragma unroll
+        for (int q = 0; q < K; q++) v += h[q] * (double)vu[q].v[e];
+        o.v[e] = (T)v;
+        ss += (double)o.v[e] * (double)o.v[e];
+      }
+      *reinterpret_cast<Vec16<T> *>(w + s0 + i) = o;
+    }
+  } else {
+    for (int i = threadIdx.x; i < len; i += kT) {
+      double v = a * (double)w[s0 + i];
+#pragma unroll
+      for (int q = 0; q < K; q++) v += h[q] * (double)V[(int64_t)q * n + s0 + i];
+      const T o = (T)v;
+      w[s0 + i] = o;
+      ss += (double)o * (double)o;
+    }
   }
   if (!part) return;
   __shared__ double sh[kT / 64];
@@ -144,6 +203,35 @@ __global__ void __launch_bounds__(kT) k_bgs(int64_t n, int k, const T *__restric
     double t = 0.0;
     for (int q = 0; q < kT / 64; q++) t += sh[q];
     part[(int64_t)ch * kS] = t;
+  }
+}
+
+// launchers: the vector count k (1 .. kS - 1) picks the instance
+#define UBT_K_CASES(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) \
+  X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31)
+
+template <class T>
+void launch_bdot(int k, unsigned nc, hipStream_t s, int64_t n, const T *V, const T *w, const int64_t *cst,
+                 const int *cln, double *pt) {
+  switch (k) {
+#define UBT_BDOT(K) \
+  case K: hipLaunchKernelGGL((k_bdot<T, K>), dim3(nc), dim3(kT), 0, s, n, V, w, cst, cln, pt); break;
+    UBT_K_CASES(UBT_BDOT)
+#undef UBT_BDOT
+    default: fail(UBT_ERR_ARG_OUTOFRANGE, "%d Krylov vectors (1 to %d)", k, kS - 1);
+  }
+}
+
+template <class T>
+void launch_bgs(int k, unsigned nc, hipStream_t s, int64_t n, const T *V, const double *coef, const double *vs, int j,
+                T *w, const int64_t *cst, const int *cln, const int *cpr, double *pt) {
+  switch (k) {
+#define UBT_BGS(K) \
+  case K: hipLaunchKernelGGL((k_bgs<T, K>), dim3(nc), dim3(kT), 0, s, n, V, coef, vs, j, w, cst, cln, cpr, pt); break;
+    UBT_K_CASES(UBT_BGS)
+#undef UBT_BGS
+    default: fail(UBT_ERR_ARG_OUTOFRANGE, "%d Krylov vectors (1 to %d)", k, kS - 1);
   }
 }
 
@@ -798,7 +886,7 @@ template <class T> void BatchSolver::cycle_start(const T *b, const T *x, hipStre
   // W_0 = b - A x in V's first slot, s_0 = 1 / beta
   batch->mult(prec, x, V, true, s);
   hipLaunchKernelGGL(k_axpby_b<T>, dim3(gn), dim3(kT), 0, s, n, 1.0, b, -1.0, V);
-  hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, 1, (const T *)V, (const T *)V, cst, cln, pt);
+  launch_bdot<T>(1, nc, s, n, (const T *)V, (const T *)V, cst, cln, pt);
   hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
   hipLaunchKernelGGL(k_bstart, dim3(gp), dim3(64), 0, s, np, m, (const double *)hn, (const double *)tol.as<double>(),
                      (const int *)live, gv.as<double>(), vsc.as<double>(), dn, jd);
@@ -824,17 +912,17 @@ template <class T> void BatchSolver::cycle_steps(int j0, int j1, hipStream_t s) 
     T *wj = V + (int64_t)j * n, *zj = Z + (int64_t)j * n, *w = V + (int64_t)(j + 1) * n;
     apply_pc<T>(wj, zj, s);
     batch->mult(prec, zj, w, true, s);
-    hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
+    launch_bdot<T>(j + 1, nc, s, n, (const T *)V, (const T *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bproj_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt,
                        (const double *)vs, j, 1, h1, c1);
-    hipLaunchKernelGGL(k_bgs<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)c1,
-                       (const double *)vs, j, w, cst, cln, cpr, two ? nullptr : pt);
+    launch_bgs<T>(j + 1, nc, s, n, (const T *)V, (const double *)c1, (const double *)vs, j, w, cst, cln, cpr,
+                  two ? nullptr : pt);
     if (two) {  // classical Gram-Schmidt twice
-      hipLaunchKernelGGL(k_bdot<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const T *)w, cst, cln, pt);
+      launch_bdot<T>(j + 1, nc, s, n, (const T *)V, (const T *)w, cst, cln, pt);
       hipLaunchKernelGGL(k_bproj_final, dim3(np, j + 1), dim3(64), 0, s, j + 1, cfp, (const double *)pt,
                          (const double *)vs, j, 0, h2, c2);
-      hipLaunchKernelGGL(k_bgs<T>, dim3(nc), dim3(kT), 0, s, n, j + 1, (const T *)V, (const double *)c2,
-                         (const double *)nullptr, 0, w, cst, cln, cpr, pt);
+      launch_bgs<T>(j + 1, nc, s, n, (const T *)V, (const double *)c2, (const double *)nullptr, 0, w, cst, cln,
+                    cpr, pt);
     }
     hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, s, 1, cfp, (const double *)pt, hn, 1, 1);
     hipLaunchKernelGGL(k_bgivens, dim3(gp), dim3(64), 0, s, np, j, (const double *)h1,
@@ -946,7 +1034,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
   rel_residual.assign(np, 0.0);
   UBT_HIP(hipMemsetAsync(done.as<int>() + 2 * np, 0, sizeof(int) * np, cap));  // iteration counters
   // ||b|| per problem
-  hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, b, b, cst, cln, pt);
+  launch_bdot<double>(1, nc, cap, n, b, b, cst, cln, pt);
   hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, bnorm.as<double>(), 1, 1);
   std::vector<double> bn(np), rn(np), tl(np);
   std::vector<int> live(np);
@@ -963,7 +1051,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
     double *w = w64.as<double>();
     batch->mult(UBT_FP64, xi, w, true, cap);
     hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, w);
-    hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)w, (const double *)w, cst, cln, pt);
+    launch_bdot<double>(1, nc, cap, n, (const double *)w, (const double *)w, cst, cln, pt);
     hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, nrm.as<double>(), 1, 1);
     hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, (const double *)xi, 0.0, x);
     UBT_HIP(hipMemcpyAsync(rn.data(), nrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
@@ -975,7 +1063,7 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
     for (outer = 0; outer <= opt.max_outer; outer++) {
       batch->mult(UBT_FP64, x, r, true, cap);
       hipLaunchKernelGGL(k_axpby_b<double>, dim3(gn), dim3(kT), 0, cap, n, 1.0, b, -1.0, r);
-      hipLaunchKernelGGL(k_bdot<double>, dim3(nc), dim3(kT), 0, cap, n, 1, (const double *)r, (const double *)r, cst, cln, pt);
+      launch_bdot<double>(1, nc, cap, n, (const double *)r, (const double *)r, cst, cln, pt);
       hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(64), 0, cap, 1, cfp, (const double *)pt, onrm.as<double>(), 1, 1);
       UBT_HIP(hipMemcpyAsync(rn.data(), onrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
       UBT_HIP(hipStreamSynchronize(cap));
