@@ -6,6 +6,7 @@
 // share the discretisation type (dimension, basis, faces per cell).
 #pragma once
 
+#include "fgmres.hpp"
 #include "solver.hpp"
 
 namespace ubt {
@@ -123,44 +124,24 @@ bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *re
                        int64_t *n_tiles, BoxArgs *box);
 constexpr int kBatchBoxRows = 4;  // rows per thread of the box body
 
-struct BatchSolver {
+// The batch's solver: the device FGMRES engine over the batch's problems,
+// the batch's persistent apply as the operator and the composite
+// preconditioner (element blocks, then the sweep on the L-residual - the
+// sweep alone where it inverts L exactly)
+struct BatchSolver : KrylovSystem {
   Batch *batch = nullptr;
   SolverOptions opt;
-  int64_t n = 0;
-  int np = 0;
-  // chunking of the batch vector for the per-problem reductions
-  int n_chunks = 0;
-  DevBuf chunk_start, chunk_len, chunk_prob;
-  DevBuf V64, Z64, V32, Z32, w64, t64, u64, w32, t32, u32, d32, b32, r64;
-  DevBuf cfirst;              // per problem: its first chunk (prefix, np + 1)
-  DevBuf part, H, cs, sn, gv, hc, yv, nrm, tol, res, bnorm;
-  // per problem x kS: the scale s_i of each stored basis vector. The basis is
-  // kept as W_i with V_i = s_i W_i (W_0 the residual, W_{j+1} the
-  // orthogonalised vector before its normalisation, s = 1 / its norm), so no
-  // pass over the vector only to normalise it
-  DevBuf vsc;
-  DevBuf onrm;                // per problem: the outer residual norm (refinement scale)
-  DevBuf done;                // int per problem x 4: done, steps this cycle, iterations, live
-  int gen = -1;               // the batch work lists the graphs were captured on
-  uint64_t xs_tag = 0;        // the members' cross-section generations the records and graphs follow
-  std::vector<int> iterations, reason;
-  std::vector<double> rel_residual;
-  int outer = 0, cycles = 0;
-  bool use_graph = true;
-  static constexpr int kStepsPerCheck = 4;       // Arnoldi steps per graph between flag reads
-  std::vector<hipGraphExec_t> graphs[2];         // per precision: start, step blocks, end
-  hipStream_t cap = nullptr;
-  ~BatchSolver();
-  void drop_graphs();
+  Fgmres eng;
+  DevBuf t64, u64, t32, u32;  // the composite preconditioner's work vectors
+  int gen = -1;               // the batch work lists the solver was refreshed on
+  uint64_t xs_tag = 0;        // the members' cross-section generations the records follow
 
   void create(Batch *b, const SolverOptions &o);
   void refresh(hipStream_t s);
   void solve(const double *b, double *x, hipStream_t s);
   template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
-  template <class T> void cycle_start(const T *b, const T *x, hipStream_t s);
-  template <class T> void cycle_steps(int j0, int j1, hipStream_t s);
-  template <class T> void cycle_end(T *x, hipStream_t s);
-  template <class T> void fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, hipStream_t s);
+  void mult(int prec, const void *x, void *y, hipStream_t s) override;
+  void pc(int prec, const void *r, void *y, hipStream_t s) override;
 };
 
 }  // namespace ubt

@@ -628,8 +628,8 @@ int ubt_solver_apply_pc(ubt_solver *solver, int prec, const void *r, void *y, vo
     UBT_CHECK(r != y, UBT_ERR_ARG_IDN, "the preconditioner apply needs distinct r and y");
     auto &s = solver->s;
     s.transpose = false;
-    UBT_CHECK(prec == UBT_FP64 ? s.V64.ptr != nullptr : s.V32.ptr != nullptr, UBT_ERR_ARG_INCOMP,
-              "this solver holds %s work vectors", s.opt.mixed ? "FP32" : "FP64");
+    UBT_CHECK(s.has_precision(prec), UBT_ERR_ARG_INCOMP, "this solver holds %s work vectors",
+              s.opt.mixed ? "FP32" : "FP64");
     if (prec == UBT_FP64)
       s.apply_pc<double>((const double *)r, (double *)y, (hipStream_t)stream);
     else
@@ -819,7 +819,7 @@ int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, dou
                             int *outer, int *cycles) {
   return guard([&] {
     NEED(solver);
-    const auto &s = solver->s;
+    const auto &s = solver->s.eng;
     for (size_t p = 0; p < s.iterations.size(); p++) {
       if (iterations) iterations[p] = s.iterations[p];
       if (rel_residual) rel_residual[p] = s.rel_residual[p];
@@ -832,8 +832,8 @@ int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, dou
 int ubt_batch_solver_set_graph(ubt_batch_solver *solver, int on) {
   return guard([&] {
     NEED(solver);
-    solver->s.use_graph = on != 0;
-    solver->s.drop_graphs();
+    solver->s.eng.use_graph = on != 0;
+    solver->s.eng.drop_graphs();
   });
 }
 int ubt_batch_solver_destroy(ubt_batch_solver *solver) {

@@ -2931,7 +2931,7 @@ void Operator::tune() {
   DevBuf xb, yb;
   xb.alloc(sizeof(double) * n);
   yb.alloc(sizeof(double) * n);
-  UBT_HIP(hipMemsetAsync(xb.ptr, 0, sizeof(double) * n, s));
+  vec_fill<double>(n, 1.0, xb.as<double>(), s);  // tuned on a non-zero input
   for (int p = 0; p < 2; p++) {
     float best = 1e30f;
     int best_v = 0;
