@@ -175,6 +175,14 @@ int ubt_disc_create_structured(const ubt_quad *quad, int dim, const int *n_cells
  * build_dg1_geometry (src/unstructured_dgk.kokkos.cxx:224-821). */
 int ubt_disc_create_dg(const ubt_quad *quad, const ubt_mesh *mesh, int order, const ubt_bc *bcs,
                        int n_bcs, ubt_disc **disc);
+/* As ubt_disc_create_dg, with the per-cell and per-face geometry - fan
+ * moments, normals, distances and, at order 1, the orthonormal basis and the
+ * face matrices - computed by kernels on the current device (on stream), as
+ * the reference builds it in device kernels (build_dg1_geometry,
+ * src/unstructured_dgk.kokkos.cxx:29-218,706-821); the topology and the
+ * boundary classification stay on the host. Same tables, same errors. */
+int ubt_disc_create_dg_device(const ubt_quad *quad, const ubt_mesh *mesh, int order, const ubt_bc *bcs,
+                              int n_bcs, void *stream, ubt_disc **disc);
 /* Sizes of the phase space (reference PhaseSpace, include/ubolt/phase_space.hpp:34-87). */
 int ubt_disc_size(const ubt_disc *disc, int *dim, int *n_cells, int *n_basis, int *n_angles,
                   int *faces_per_cell);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "ubt_mesh_size": [P, PI, PI, PI],
     "ubt_mesh_destroy": [P],
     "ubt_disc_create_structured": [P, I, PI, PD, C.POINTER(Bc), I, PP],
+    "ubt_disc_create_dg_device": [P, P, I, P, I, P, PP],
     "ubt_disc_create_dg": [P, P, I, C.POINTER(Bc), I, PP],
     "ubt_disc_size": [P, PI, PI, PI, PI, PI],
     "ubt_disc_get": [P, C.c_char_p, P, C.POINTER(C.c_size_t)],

@@ -116,10 +116,14 @@ class Disc(_Handle):
         return self
 
     @classmethod
-    def dg(cls, quad, mesh, order, bcs=None):
+    def dg(cls, quad, mesh, order, bcs=None, device=False):
+        """device: the per-cell and per-face geometry computed by GPU kernels"""
         self = cls()
         arr, nb = bc_array(bcs)
-        call("ubt_disc_create_dg", quad.h, mesh.h, order, arr, nb, C.byref(self.h))
+        if device:
+            call("ubt_disc_create_dg_device", quad.h, mesh.h, order, arr, nb, None, C.byref(self.h))
+        else:
+            call("ubt_disc_create_dg", quad.h, mesh.h, order, arr, nb, C.byref(self.h))
         self.kind, self.quad, self.mesh, self.order = "dg", quad, mesh, order
         self._sizes()
         return self

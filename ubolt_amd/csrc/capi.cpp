@@ -223,6 +223,17 @@ int ubt_disc_create_dg(const ubt_quad *quad, const ubt_mesh *mesh, int order, co
     *disc = h.release();
   });
 }
+int ubt_disc_create_dg_device(const ubt_quad *quad, const ubt_mesh *mesh, int order, const ubt_bc *bcs, int n_bcs,
+                              void *stream, ubt_disc **disc) {
+  return guard([&] {
+    NEED(quad);
+    NEED(mesh);
+    NEED(disc);
+    auto h = std::make_unique<ubt_disc_s>();
+    h->d = ubt::make_unstructured_device(quad->q, mesh->m, order, bc_table(bcs, n_bcs), (hipStream_t)stream);
+    *disc = h.release();
+  });
+}
 int ubt_disc_size(const ubt_disc *disc, int *dim, int *n_cells, int *n_basis, int *n_angles, int *faces_per_cell) {
   return guard([&] {
     NEED(disc);

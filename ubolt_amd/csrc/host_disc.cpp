@@ -13,6 +13,7 @@
 #include <array>
 #include <cmath>
 
+#include "geometry.hpp"
 #include "ubt_internal.hpp"
 
 namespace ubt {
@@ -45,51 +46,6 @@ Key face_key(const int *v, int n) {
   for (int i = 0; i < n; i++) k[i] = v[i];
   std::sort(k.begin(), k.begin() + n);
   return k;
-}
-
-// Moments of a region about a reference point, accumulated simplex by simplex:
-// m0 = int 1, m1 = int y, m2 = int y y^T for y = x - reference
-struct Moments {
-  double m0 = 0.0, m1[3] = {0, 0, 0}, m2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  // The exact moments of a uniform d-simplex with vertices y[0..d]:
-  //   int y = |S| mean(y_k),  int y y^T = |S|/((d+1)(d+2)) (sum y_k y_k^T + s s^T), s = sum y_k
-  void add(int d, double measure, const double (*y)[3]) {
-    double s[3] = {0, 0, 0};
-    for (int k = 0; k <= d; k++)
-      for (int a = 0; a < 3; a++) s[a] += y[k][a];
-    m0 += measure;
-    for (int a = 0; a < 3; a++) m1[a] += measure * s[a] / (d + 1);
-    const double c = measure / ((d + 1) * (d + 2));
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) {
-        double o = s[a] * s[b];
-        for (int k = 0; k <= d; k++) o += y[k][a] * y[k][b];
-        m2[3 * a + b] += c * o;
-      }
-  }
-  // centroid and the second moment about it
-  void central(const double ref[3], double cen[3], double C[9]) const {
-    for (int a = 0; a < 3; a++) cen[a] = ref[a] + m1[a] / m0;
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++) C[3 * a + b] = m2[3 * a + b] - m1[a] * m1[b] / m0;
-  }
-};
-
-inline void sub3(const double *a, const double *b, double *o) {
-  for (int d = 0; d < 3; d++) o[d] = a[d] - b[d];
-}
-inline void cross3(const double *a, const double *b, double *o) {
-  o[0] = a[1] * b[2] - a[2] * b[1];
-  o[1] = a[2] * b[0] - a[0] * b[2];
-  o[2] = a[0] * b[1] - a[1] * b[0];
-}
-inline double norm3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
-
-int dominant_axis(const double *n, int dim) {
-  int best = 0;
-  for (int d = 1; d < dim; d++)
-    if (std::fabs(n[d]) > std::fabs(n[best])) best = d;
-  return best;
 }
 
 // Tangential coordinates of a point on a face normal to `axis`: the other axes, ascending
@@ -246,7 +202,7 @@ std::shared_ptr<Disc> make_structured(const Quadrature &q, int dim, const int n[
 // ---------------------------------------------------------------------------
 // Unstructured DG0 / DG1
 // ---------------------------------------------------------------------------
-std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int order, const BCTable &bcs) {
+std::shared_ptr<Disc> dg_begin(const Quadrature &q, const Mesh &m, int order, const BCTable &bcs) {
   UBT_CHECK(order == 0 || order == 1, UBT_ERR_ARG_OUTOFRANGE, "the unstructured backend is DG0 or DG1, got order %d",
             order);
   UBT_CHECK(q.dim == m.dim, UBT_ERR_ARG_INCOMP, "a %dD quadrature was handed to a %dD mesh", q.dim, m.dim);
@@ -262,10 +218,9 @@ std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int 
   G.n_angles = q.n_angles;
   G.n_cells = m.n_cells;
   G.nf = m.faces_per_cell;
-  const int dim = m.dim, nf = G.nf, nc = m.n_cells, nb = G.nb;
-  const size_t slots = (size_t)nc * nf;
-  UBT_CHECK((int64_t)nc * nb * q.n_angles < (int64_t)1 << 40, UBT_ERR_ARG_OUTOFRANGE, "phase space too large");
-
+  const int nc = m.n_cells;
+  const size_t slots = (size_t)nc * G.nf;
+  UBT_CHECK((int64_t)nc * G.nb * q.n_angles < (int64_t)1 << 40, UBT_ERR_ARG_OUTOFRANGE, "phase space too large");
   G.volume.assign(nc, 0.0);
   G.centroid.assign(3 * (size_t)nc, 0.0);
   G.face_nA.assign(3 * slots, 0.0);
@@ -276,113 +231,70 @@ std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int 
   G.face_label = m.face_label;
   G.face_axis.assign(slots, 0);
   G.face_inflow.assign(slots, 0.0);
-  std::vector<double> central(9 * (size_t)nc), face_C(9 * slots);
+  if (order == 0) G.face_basis_value.assign(slots, 1.0);
+  return D;
+}
 
-  auto X = [&](int v) { return &m.coords[3 * (size_t)v]; };
-
-  // Per cell: fan moments off the vertex average (inside any star-shaped cell)
-  for (int c = 0; c < nc; c++) {
-    const int *cv = &m.cell_verts[(size_t)c * m.verts_per_cell];
-    double cref[3] = {0, 0, 0};
-    for (int i = 0; i < m.verts_per_cell; i++)
-      for (int d = 0; d < 3; d++) cref[d] += X(cv[i])[d] / m.verts_per_cell;
-    Moments mc;
-    for (int f = 0; f < nf; f++) {
-      const size_t k = (size_t)c * nf + f;
-      const int fnv = m.face_nverts[k];
-      const int *fv = &m.face_verts[4 * k];
-      if (dim == 2) {
-        double y[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        sub3(X(fv[0]), cref, y[1]);
-        sub3(X(fv[1]), cref, y[2]);
-        mc.add(2, 0.5 * std::fabs(y[1][0] * y[2][1] - y[1][1] * y[2][0]), y);
-      } else {
-        double fref[3] = {0, 0, 0};
-        for (int i = 0; i < fnv; i++)
-          for (int d = 0; d < 3; d++) fref[d] += X(fv[i])[d] / fnv;
-        for (int e = 0; e < fnv; e++) {
-          double y[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-          sub3(fref, cref, y[1]);
-          sub3(X(fv[e]), cref, y[2]);
-          sub3(X(fv[(e + 1) % fnv]), cref, y[3]);
-          double cr[3];
-          cross3(y[2], y[3], cr);
-          const double det = y[1][0] * cr[0] + y[1][1] * cr[1] + y[1][2] * cr[2];
-          mc.add(3, std::fabs(det) / 6.0, y);
-        }
-      }
-    }
-    UBT_CHECK(mc.m0 > 0.0, UBT_ERR_ARG_WRONG, "cell %d has non-positive volume %g", c, mc.m0);
-    G.volume[c] = mc.m0;
-    mc.central(cref, &G.centroid[3 * (size_t)c], &central[9 * (size_t)c]);
+void dg_geometry_error(int what, int c, int f) {
+  switch (what) {
+    case 1: fail(UBT_ERR_ARG_WRONG, "cell %d has non-positive volume", c);
+    case 2: fail(UBT_ERR_ARG_WRONG, "face %d of cell %d has non-positive area", f, c);
+    case 3: fail(UBT_ERR_ARG_WRONG, "face %d of cell %d has no normal", f, c);
+    case 4: fail(UBT_ERR_ARG_WRONG, "the centroid of cell %d is not inside its face %d", c, f);
+    case 5: fail(UBT_ERR_ARG_WRONG, "the centroid of cell %d is not inside its face", c);
+    case 6: fail(UBT_ERR_ARG_WRONG, "cell %d has a degenerate second moment", c);
+    default: fail(UBT_ERR_PLIB, "geometry error %d at face %d of cell %d", what, f, c);
   }
+}
 
-  // Per (cell, face): area, centroid, central second moment, outward nA
+void dg_geometry_host(Disc &G, const Mesh &m, std::vector<double> &central, std::vector<double> &face_C) {
+  const int dim = m.dim, nf = G.nf, nc = m.n_cells;
+  const size_t slots = (size_t)nc * nf;
+  central.assign(9 * (size_t)nc, 0.0);
+  face_C.assign(9 * slots, 0.0);
   for (int c = 0; c < nc; c++) {
-    const double *cen = &G.centroid[3 * (size_t)c];
+    const size_t k0 = (size_t)c * nf;
+    if (!geo::cell_moments(dim, nf, m.verts_per_cell, &m.cell_verts[(size_t)c * m.verts_per_cell],
+                           &m.face_nverts[k0], &m.face_verts[4 * k0], m.coords.data(), &G.volume[c],
+                           &G.centroid[3 * (size_t)c], &central[9 * (size_t)c]))
+      dg_geometry_error(1, c, 0);
+  }
+  for (int c = 0; c < nc; c++)
     for (int f = 0; f < nf; f++) {
       const size_t k = (size_t)c * nf + f;
-      const int fnv = m.face_nverts[k];
-      const int *fv = &m.face_verts[4 * k];
-      double fref[3] = {0, 0, 0}, nA[3] = {0, 0, 0};
-      for (int i = 0; i < fnv; i++)
-        for (int d = 0; d < 3; d++) fref[d] += X(fv[i])[d] / fnv;
-      Moments mf;
-      if (dim == 2) {
-        double y[2][3], e[3];
-        sub3(X(fv[0]), fref, y[0]);
-        sub3(X(fv[1]), fref, y[1]);
-        sub3(X(fv[1]), X(fv[0]), e);
-        const double L = norm3(e);
-        mf.add(1, L, y);
-        nA[0] = e[1];
-        nA[1] = -e[0];
-      } else {
-        for (int e = 0; e < fnv; e++) {
-          double y[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, cr[3];
-          sub3(X(fv[e]), fref, y[1]);
-          sub3(X(fv[(e + 1) % fnv]), fref, y[2]);
-          cross3(y[1], y[2], cr);
-          mf.add(2, 0.5 * norm3(cr), y);
-          for (int d = 0; d < 3; d++) nA[d] += 0.5 * cr[d];
-        }
-      }
-      const double area = mf.m0;
-      UBT_CHECK(area > 0.0, UBT_ERR_ARG_WRONG, "face %d of cell %d has non-positive area", f, c);
-      double *xf = &G.face_centroid[3 * k];
-      mf.central(fref, xf, &face_C[9 * k]);
-      const double nrm = norm3(nA);
-      UBT_CHECK(nrm > 0.0, UBT_ERR_ARG_WRONG, "face %d of cell %d has no normal", f, c);
-      double out = 0.0;
-      for (int d = 0; d < dim; d++) out += nA[d] * (xf[d] - cen[d]);
-      const double sgn = out < 0.0 ? -1.0 : 1.0;
-      for (int d = 0; d < dim; d++) G.face_nA[3 * k + d] = sgn * area * nA[d] / nrm;
-      G.face_area[k] = area;
-      G.face_axis[k] = dominant_axis(&G.face_nA[3 * k], dim);
-      double dist = 0.0;
-      for (int d = 0; d < dim; d++) dist += G.face_nA[3 * k + d] / area * (xf[d] - cen[d]);
-      UBT_CHECK(dist > 0.0, UBT_ERR_ARG_WRONG, "the centroid of cell %d is not inside its face %d", c, f);
-      G.face_distance[2 * k] = dist;
+      const int e = geo::face_geometry(dim, m.face_nverts[k], &m.face_verts[4 * k], m.coords.data(),
+                                       &G.centroid[3 * (size_t)c], &G.face_area[k], &G.face_centroid[3 * k],
+                                       &face_C[9 * k], &G.face_nA[3 * k], &G.face_axis[k], &G.face_distance[2 * k]);
+      if (e != geo::kFaceOk) dg_geometry_error(1 + e, c, f);
       const int nbr = m.face_neighbour[k];
-      if (nbr >= 0) {
-        G.face_nb[k] = nbr;
-        double dn = 0.0;
-        for (int d = 0; d < dim; d++) dn += G.face_nA[3 * k + d] / area * (G.centroid[3 * (size_t)nbr + d] - xf[d]);
-        UBT_CHECK(dn > 0.0, UBT_ERR_ARG_WRONG, "the centroid of cell %d is not inside its face", nbr);
-        G.face_distance[2 * k + 1] = dn;
-      } else {
-        const BCFace bc = bcs.get(G.face_label[k]);
-        if (bc.type == UBT_BC_REFLECT) {
-          UBT_CHECK(std::fabs(G.face_nA[3 * k + G.face_axis[k]]) >= (1.0 - 1e-10) * area, UBT_ERR_ARG_WRONG,
-                    "face %d of cell %d (label %d) is reflective but not axis-aligned", f, c, G.face_label[k]);
-          G.face_nb[k] = -2 - G.face_axis[k];
-        } else {
-          G.face_nb[k] = -1;
-          double t[2] = {0, 0};
-          const int nt = tangential(xf, G.face_axis[k], dim, t);
-          if (in_window(bc, t, nt)) G.face_inflow[k] = bc.inflow / q.sum_weights;
-        }
-      }
+      if (nbr < 0) continue;
+      const double dn = geo::neighbour_distance(dim, &G.face_nA[3 * k], G.face_area[k], &G.centroid[3 * (size_t)nbr],
+                                                &G.face_centroid[3 * k]);
+      if (!(dn > 0.0)) dg_geometry_error(5, nbr, f);
+      G.face_distance[2 * k + 1] = dn;
+    }
+}
+
+void dg_boundaries(Disc &G, const Mesh &m, const BCTable &bcs, const Quadrature &q) {
+  const int dim = m.dim;
+  const size_t slots = (size_t)m.n_cells * G.nf;
+  for (size_t k = 0; k < slots; k++) {
+    const int nbr = m.face_neighbour[k];
+    const int c = (int)(k / G.nf), f = (int)(k % G.nf);
+    if (nbr >= 0) {
+      G.face_nb[k] = nbr;
+      continue;
+    }
+    const BCFace bc = bcs.get(G.face_label[k]);
+    if (bc.type == UBT_BC_REFLECT) {
+      UBT_CHECK(std::fabs(G.face_nA[3 * k + G.face_axis[k]]) >= (1.0 - 1e-10) * G.face_area[k], UBT_ERR_ARG_WRONG,
+                "face %d of cell %d (label %d) is reflective but not axis-aligned", f, c, G.face_label[k]);
+      G.face_nb[k] = -2 - G.face_axis[k];
+    } else {
+      G.face_nb[k] = -1;
+      double t[2] = {0, 0};
+      const int nt = tangential(&G.face_centroid[3 * k], G.face_axis[k], dim, t);
+      if (in_window(bc, t, nt)) G.face_inflow[k] = bc.inflow / q.sum_weights;
     }
   }
   // Every label the BC table names must be on some boundary face
@@ -392,77 +304,46 @@ std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int 
     UBT_CHECK(seen, UBT_ERR_ARG_WRONG, "a boundary condition was given for label %d but no boundary face carries it",
               e.first);
   }
+}
 
-  if (order == 1) {
-    // beta rows: gradients of phi_1..phi_dim, from M = central / V = L L^T, beta = L^{-1}
-    G.basis_beta.assign(9 * (size_t)nc, 0.0);
-    for (int c = 0; c < nc; c++) {
-      double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      const double *C = &central[9 * (size_t)c];
-      for (int a = 0; a < dim; a++)
-        for (int b = 0; b <= a; b++) {
-          double s = C[3 * a + b] / G.volume[c];
-          for (int k = 0; k < b; k++) s -= L[3 * a + k] * L[3 * b + k];
-          if (a == b) {
-            UBT_CHECK(s > 0.0, UBT_ERR_ARG_WRONG, "cell %d has a degenerate second moment", c);
-            L[3 * a + a] = std::sqrt(s);
-          } else {
-            L[3 * a + b] = s / L[3 * b + b];
-          }
-        }
-      double *beta = &G.basis_beta[9 * (size_t)c];
-      for (int col = 0; col < dim; col++)
-        for (int a = col; a < dim; a++) {
-          double s = (a == col) ? 1.0 : 0.0;
-          for (int k = col; k < a; k++) s -= L[3 * a + k] * beta[3 * k + col];
-          beta[3 * a + col] = s / L[3 * a + a];
-        }
+void dg_basis_host(Disc &G, const std::vector<double> &central, const std::vector<double> &face_C) {
+  const int dim = G.dim, nf = G.nf, nc = G.n_cells, nb = G.nb;
+  const size_t slots = (size_t)nc * nf;
+  G.basis_beta.assign(9 * (size_t)nc, 0.0);
+  for (int c = 0; c < nc; c++)
+    if (!geo::basis_beta(dim, &central[9 * (size_t)c], G.volume[c], &G.basis_beta[9 * (size_t)c]))
+      dg_geometry_error(6, c, 0);
+  G.face_own.assign(slots * nb * nb, 0.0);
+  G.face_up.assign(slots * nb * nb, 0.0);
+  G.face_basis_value.assign(slots * nb, 0.0);
+  G.face_nb_grad.assign(slots * nb * 3, 0.0);
+  G.basis_grad.assign((size_t)nc * nb * 3, 0.0);
+  for (int c = 0; c < nc; c++) {
+    const double *beta = &G.basis_beta[9 * (size_t)c], *cen = &G.centroid[3 * (size_t)c];
+    for (int i = 0; i < nb; i++)
+      for (int d = 0; d < 3; d++) G.basis_grad[((size_t)c * nb + i) * 3 + d] = geo::basis_grad(beta, i, d);
+    for (int f = 0; f < nf; f++) {
+      const size_t k = (size_t)c * nf + f;
+      const int nbr = G.face_nb[k];
+      const int up = nbr >= 0 ? nbr : (nbr <= -2 ? c : -1);
+      if (nbr >= 0)
+        for (int j = 0; j < nb; j++)
+          for (int d = 0; d < 3; d++)
+            G.face_nb_grad[(k * nb + j) * 3 + d] = geo::basis_grad(&G.basis_beta[9 * (size_t)nbr], j, d);
+      geo::face_matrices(nb, beta, cen, up >= 0 ? &G.basis_beta[9 * (size_t)up] : nullptr,
+                         up >= 0 ? &G.centroid[3 * (size_t)up] : nullptr, &G.face_centroid[3 * k], &face_C[9 * k],
+                         G.face_area[k], G.volume[c], &G.face_own[k * nb * nb], &G.face_up[k * nb * nb],
+                         &G.face_basis_value[k * nb]);
     }
-    auto value = [&](int cc, int i, const double *x) {
-      if (i == 0) return 1.0;
-      double v = 0.0;
-      for (int d = 0; d < 3; d++) v += G.basis_beta[9 * (size_t)cc + 3 * (i - 1) + d] * (x[d] - G.centroid[3 * (size_t)cc + d]);
-      return v;
-    };
-    auto grad = [&](int cc, int i, int d) { return i == 0 ? 0.0 : G.basis_beta[9 * (size_t)cc + 3 * (i - 1) + d]; };
-    G.face_own.assign(slots * nb * nb, 0.0);
-    G.face_up.assign(slots * nb * nb, 0.0);
-    G.face_basis_value.assign(slots * nb, 0.0);
-    G.face_nb_grad.assign(slots * nb * 3, 0.0);
-    G.basis_grad.assign((size_t)nc * nb * 3, 0.0);
-    for (int c = 0; c < nc; c++) {
-      for (int i = 0; i < nb; i++)
-        for (int d = 0; d < 3; d++) G.basis_grad[((size_t)c * nb + i) * 3 + d] = grad(c, i, d);
-      for (int f = 0; f < nf; f++) {
-        const size_t k = (size_t)c * nf + f;
-        const double *xf = &G.face_centroid[3 * k], *Cf = &face_C[9 * k];
-        const double A = G.face_area[k];
-        const int nbr = G.face_nb[k];
-        const int up = nbr >= 0 ? nbr : (nbr <= -2 ? c : -1);
-        if (nbr >= 0)
-          for (int j = 0; j < nb; j++)
-            for (int d = 0; d < 3; d++) G.face_nb_grad[(k * nb + j) * 3 + d] = grad(nbr, j, d);
-        const double scale = 1.0 / (G.volume[c] * A);
-        for (int i = 0; i < nb; i++) {
-          const double vi = value(c, i, xf);
-          G.face_basis_value[k * nb + i] = vi;
-          for (int j = 0; j < nb; j++) {
-            double own = A * vi * value(c, j, xf);
-            for (int a = 0; a < 3; a++)
-              for (int b = 0; b < 3; b++) own += grad(c, i, a) * Cf[3 * a + b] * grad(c, j, b);
-            G.face_own[(k * nb + i) * nb + j] = own * scale;
-            if (up < 0) continue;
-            double cr = A * vi * value(up, j, xf);
-            for (int a = 0; a < 3; a++)
-              for (int b = 0; b < 3; b++) cr += grad(c, i, a) * Cf[3 * a + b] * grad(up, j, b);
-            G.face_up[(k * nb + i) * nb + j] = cr * scale;
-          }
-        }
-      }
-    }
-  } else {
-    G.face_basis_value.assign(slots, 1.0);
   }
+}
+
+std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int order, const BCTable &bcs) {
+  auto D = dg_begin(q, m, order, bcs);
+  std::vector<double> central, face_C;
+  dg_geometry_host(*D, m, central, face_C);
+  dg_boundaries(*D, m, bcs, q);
+  if (order == 1) dg_basis_host(*D, central, face_C);
   return D;
 }
 

@@ -196,5 +196,18 @@ std::shared_ptr<Disc> make_structured(const Quadrature &q, int dim, const int n[
                                       const double len[3], const BCTable &bcs);
 std::shared_ptr<Disc> make_unstructured(const Quadrature &q, const Mesh &m, int order,
                                         const BCTable &bcs);
+// The same discretisation with the per-cell and per-face geometry (moments,
+// normals, distances; at order 1 the basis and the face matrices) computed by
+// HIP kernels on the current device and copied into the host tables
+std::shared_ptr<Disc> make_unstructured_device(const Quadrature &q, const Mesh &m, int order, const BCTable &bcs,
+                                               hipStream_t s);
+// the construction's shared stages: the discretisation before any geometry,
+// a geometry error by code (1 volume, 2 area, 3 normal, 4 a centroid outside
+// its face, 5 a neighbour's centroid outside it, 6 a degenerate second
+// moment) of cell c / face f, the boundary classification (neighbours,
+// vacuum with its inflow window, reflective axes) from the face geometry
+std::shared_ptr<Disc> dg_begin(const Quadrature &q, const Mesh &m, int order, const BCTable &bcs);
+[[noreturn]] void dg_geometry_error(int what, int c, int f);
+void dg_boundaries(Disc &G, const Mesh &m, const BCTable &bcs, const Quadrature &q);
 
 }  // namespace ubt
