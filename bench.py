@@ -27,6 +27,10 @@ below.
 Delaunay triangulation of the unit square, 20,000 triangles, S8, one group):
 its FP32 apply, FP64 apply and time to 1e-8; it prints its own JSON line.
 
+`--moments` times the angular integral (ubt_moments: the one dense step, on
+MFMA) over the 128^3 S8 structured box, 1 and 4 moments, FP64 and FP32; it
+prints its own JSON line.
+
 `--sensitivity` runs the adjoint workload instead (3D DG1 hexes 64^3, S6, 4
 downscatter groups, cross sections piecewise constant on 8^3-cell blocks):
 gradients of the volume integral of the group-summed scalar flux over
@@ -81,6 +85,8 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true", help="skip the single-problem secondary keys")
     ap.add_argument("--sensitivity", action="store_true", help="run the adjoint / sensitivity workload")
     ap.add_argument("--delaunay", action="store_true", help="run the single-problem Delaunay workload alone")
+    ap.add_argument("--moments", action="store_true",
+                    help="time the angular integral (ubt_moments, MFMA) on the 128^3 S8 structured box")
     ap.add_argument("--batch-n", type=int, default=64, help="problems in the ensemble")
     ap.add_argument("--sens-n", type=int, default=64, help="hexes per side of the sensitivity workload")
     ap.add_argument("--sens-cells", type=int, default=32, help="finite-difference checks (one cell each)")
@@ -462,13 +468,48 @@ def ensemble(args):
     if not args.no_secondary:
         out["single_problem"] = delaunay(args, secondary=True)
     if args.copy_ref:
-        src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        copy_reference()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def copy_reference():
+    """a plain 64 MiB device copy, 5 times: the known-size reference the HBM
+    traffic counters are scaled by"""
+    import torch
+    src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def moments(args):
+    """The angular integral phi = sum_a w_a psi_a (ubt_moments, the one dense
+    step, on MFMA) over every node of the 128^3 S8 structured box (2.1 M
+    nodes x 80 angles), M = 1 and 4 moments, FP64 and FP32 psi: psi read once
+    and phi written once are the algorithmic bytes."""
+    import torch
+    import ubolt_amd as u
+    from ubolt_amd import problems as P
+    prob = P.fd_cube_multigroup(n=128, sn=8, G=1)
+    op = prob.operator()
+    d = prob.disc
+    stream = torch.cuda.current_stream()
+    out = {"metric": "angular integral GB/s (algorithmic)", "workload": "fd_box_128x128x128_s8_moments",
+           "n_nodes": d.n_nodes, "n_angles": d.n_angles, "steps": args.steps, "runs": []}
+    for prec, dt in ((u.FP64, torch.float64), (u.FP32, torch.float32)):
+        psi = torch.rand(d.n_rows, dtype=dt, device="cuda")
+        for M in (1, 4):
+            phi = torch.zeros(d.n_nodes * M, dtype=torch.float64 if prec == u.FP64 else torch.float32, device="cuda")
+            sec, _ = time_applies(lambda x, y: op.moments(x, y, M), psi, phi, args.steps, args.warmup, stream)
+            alg = d.n_rows * psi.element_size() + d.n_nodes * M * phi.element_size()
+            out["runs"].append({"prec": "fp64" if prec == u.FP64 else "fp32", "n_moments": M,
+                                "us": round(sec * 1e6, 2), "achieved_gbs": round(alg / sec / 1e9, 1),
+                                "frac_of_8tbs": round(alg / sec / 8e12, 3), "alg_bytes": alg})
+    if args.copy_ref:
+        copy_reference()
     print(json.dumps(out))
 
 
@@ -482,6 +523,8 @@ def main():
         sensitivity(args)
     elif args.delaunay:
         delaunay(args)
+    elif args.moments:
+        moments(args)
     else:
         ensemble(args)
 

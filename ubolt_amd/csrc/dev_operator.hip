@@ -1875,23 +1875,64 @@ __device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
 template <class T, class OutT>
 __global__ void __launch_bounds__(256) moments_mfma(const T *__restrict__ psi, OutT *__restrict__ phi,
                                                     const T *__restrict__ wext, int64_t n_nodes, int A,
-                                                    int Apad, int M, int phi_stride) {
+                                                    int Apad, int M, int phi_stride, int w_in_lds) {
   using V4 = typename std::conditional<sizeof(T) == 4, v4f, v4d>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   T *tile = reinterpret_cast<T *>(smem);  // 64 x Apad
+  // the weights (Apad x 16) beside the tile when they fit: each MFMA step
+  // then reads both operands from LDS instead of waiting on a cached load
+  T *wl = tile + 64 * Apad;
+  if (w_in_lds)
+    for (int e = threadIdx.x; e < Apad * 16; e += blockDim.x) wl[e] = wext[e];
   const int64_t node0 = (int64_t)blockIdx.x * 64;
   const int64_t nn = n_nodes - node0 < 64 ? n_nodes - node0 : 64;
-  // stage: contiguous psi[node0 .. node0+nn) x A, zero-padded columns
-  for (int e = threadIdx.x; e < 64 * Apad; e += blockDim.x) {
-    const int row = e / Apad, col = e - row * Apad;
-    tile[e] = (row < nn && col < A) ? psi[(node0 + row) * A + col] : T(0);
+  // stage: contiguous psi[node0 .. node0+nn) x A, zero-padded columns. With
+  // A a multiple of 4 (every level-symmetric set) the tile is psi's own run,
+  // copied in 16-byte words; otherwise a row index from a float reciprocal
+  // (exact here: e < 2^14, Apad < 2^9, so (e + 1/2) / Apad stays more than
+  // 2^-10 inside its integer interval)
+  if (Apad == A && ((reinterpret_cast<uintptr_t>(psi) & 15) == 0)) {
+    // rounds of U 16-byte loads per thread issued together (indices past the
+    // tile's values clamped to its first word, their values dropped), then
+    // written to LDS: a round trip to HBM per U words, not per word
+    constexpr int VW = 16 / sizeof(T), U = 8;
+    struct alignas(16) W16 {
+      T v[VW];
+    };
+    const int total = (int)(nn * A);  // a multiple of 4 values
+    const T *src = psi + node0 * A;
+    const int step = (int)blockDim.x * VW;
+    for (int base = (int)threadIdx.x * VW; base < 64 * A; base += U * step) {
+      W16 w[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = base + u * step;
+        w[u] = *reinterpret_cast<const W16 *>(src + (e < total ? e : 0));
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int e = base + u * step;
+        if (e >= 64 * A) break;
+        if (e >= total)
+#pragma unroll
+          for (int q = 0; q < VW; q++) w[u].v[q] = T(0);
+        *reinterpret_cast<W16 *>(tile + e) = w[u];
+      }
+    }
+  } else {
+    const float inv = 1.0f / (float)Apad;
+    for (int e = threadIdx.x; e < 64 * Apad; e += blockDim.x) {
+      const int row = (int)(((float)e + 0.5f) * inv), col = e - row * Apad;
+      tile[e] = (row < nn && col < A) ? psi[(node0 + row) * A + col] : T(0);
+    }
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int i = l & 15, kq = l >> 4;
   V4 acc = {T(0), T(0), T(0), T(0)};
   const T *trow = tile + (wave * 16 + i) * Apad;
-  for (int k0 = 0; k0 < Apad; k0 += 4) acc = mfma16x16x4(trow[k0 + kq], wext[(k0 + kq) * 16 + i], acc);
+  const T *wsrc = w_in_lds ? wl : wext;
+  for (int k0 = 0; k0 < Apad; k0 += 4) acc = mfma16x16x4(trow[k0 + kq], wsrc[(k0 + kq) * 16 + i], acc);
   const int col = i;
   if (col < M) {
 #pragma unroll
@@ -2992,10 +3033,12 @@ void launch_moments(const Operator &op, const PrecTables &t, const T *psi, OutT 
   const int A = D.n_angles, Apad = (A + 3) / 4 * 4;
   const int64_t N = D.n_nodes();
   const unsigned grid = (unsigned)((N + 63) / 64);
-  const size_t lds = (size_t)64 * Apad * sizeof(T);
+  size_t lds = (size_t)64 * Apad * sizeof(T);
   UBT_CHECK(lds <= 65536, UBT_ERR_SUP, "the moments tile of %d angles needs %zu bytes of LDS", A, lds);
+  const int w_in_lds = lds + (size_t)16 * Apad * sizeof(T) <= 65536 ? 1 : 0;
+  if (w_in_lds) lds += (size_t)16 * Apad * sizeof(T);
   hipLaunchKernelGGL((moments_mfma<T, OutT>), dim3(grid), dim3(256), lds, s, psi, phi,
-                     unit ? t.wunit.as<T>() : t.wext.as<T>(), N, A, Apad, M, stride);
+                     unit ? t.wunit.as<T>() : t.wext.as<T>(), N, A, Apad, M, stride, w_in_lds);
   UBT_HIP(hipGetLastError());
 }
 
