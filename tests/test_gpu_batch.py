@@ -277,3 +277,38 @@ def test_batch_pc_folds_to_exact_sweep(gpu):
     b2.set_group(0)
     info = b2.sweep_info()
     assert not info["exact_fp64"] and not info["exact_fp32"] and not info["box_fp64"]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_solver_quadrant_order(gpu, mixed):
+    """On a batch of uniform all-vacuum boxes the batch solver keeps its Krylov
+    vectors in the box sweep's quadrant-major order (the box apply reads and
+    writes it, the sweep needs no reorder); with the record sweep it runs in
+    the reference order. Both solves against the oracle's matrices and each
+    other, iteration for iteration"""
+    probs = P.batch_quads(n_problems=4, seed=31, side=(6, 40), sns=(4, 8, 12), groups=(1, 1))
+    ops = [p.operator() for p in probs]
+    b = u.Batch(ops)
+    b.set_group(0)
+    rhs = b.vec()
+    b.rhs(rhs)
+    out = {}
+    for box in (True, False):
+        b.set_sweep(box)
+        sol = u.BatchSolver(b, rtol=1e-8 if mixed else 1e-11, max_it=600, restart=20, mixed=mixed, gs_passes=1)
+        sol.refresh()
+        x = b.vec()
+        res = sol.solve(rhs, x)
+        torch.cuda.synchronize()
+        out[box] = (to_host(x), res)
+        print(f"[case] box sweep {box} mixed {mixed}: iterations {list(res['iterations'])}", flush=True)
+    b.set_sweep(True)
+    bh = to_host(rhs)
+    (xq, rq), (xn, rn) = out[True], out[False]
+    assert list(rq["iterations"]) == list(rn["iterations"])
+    for p, prob in enumerate(probs):
+        sl = slice(b.offsets[p], b.offsets[p + 1])
+        A = oracle_full(prob)
+        tq = np.linalg.norm(bh[sl] - A @ xq[sl]) / np.linalg.norm(bh[sl])
+        assert rq["reason"][p] == 1 and tq <= (1.05e-8 if mixed else 1e-10), (p, tq)
+        assert rel_max(xq[sl], xn[sl]) <= (1e-6 if mixed else 1e-9), p

@@ -90,10 +90,15 @@ struct Batch {
 
   void create(const std::vector<Operator *> &o);
   void set_group(int g);       // problems without group g go inactive
-  void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const;
+  // quad: x and y in the box sweep's quadrant-major order (box batches)
+  void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s, bool quad = false) const;
   void blocks_setup(hipStream_t s);  // singular check of every active problem's blocks
   void blocks_apply(int prec, const void *x, void *y, hipStream_t s) const;
-  void sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const;
+  void sweep_apply(int prec, const void *r, void *psi, hipStream_t s, bool staged = false) const;
+  // the solver may run in the box sweep's quadrant-major order: every active
+  // problem a box of the box sweep and the box apply body applicable
+  bool quad_ready(int prec) const { const int pi = prec == UBT_FP64 ? 0 : 1; return box_sweep[pi].ok && body_ok[pi][2]; }
+  void quad_order(int prec, const void *src, void *dst, bool to_q, hipStream_t s) const;
   // the sweep inverts every active problem's L exactly (the box sweep, or
   // record sweeps with no lagged or reflective coupling)
   bool sweep_exact(int prec) const;
@@ -113,11 +118,15 @@ int sweep_slots(int nf, int nfu);
 void batch_records_launch(const Batch &b, int pi, hipStream_t s);
 
 // launchers (dev_operator.hip, dev_sweep.hip)
+// quad: x and y in the quadrant-major order of the box sweep (box body only)
 template <class T>
-void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s);
+void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s, bool quad = false);
 template <class T> void batch_blocks_launch(const Batch &b, const T *x, T *y, hipStream_t s);
 template <class T> void batch_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
-template <class T> void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s);
+// staged: r and psi already in the quadrant-major order (no reorder either side)
+template <class T> void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s, bool staged = false);
+// reference order <-> quadrant-major order of every active problem's rows
+template <class T> void batch_quad_order(const Batch &b, const T *src, T *dst, bool to_q, hipStream_t s);
 // per-problem tile size, LDS and tile count of a body (the box body also
 // fills its layout); false where it does not apply
 bool batch_body_layout(const Operator &op, int prec, int body, int *cpb, int *red_lanes, size_t *lds,
@@ -133,6 +142,11 @@ struct BatchSolver : KrylovSystem {
   SolverOptions opt;
   Fgmres eng;
   DevBuf t64, u64, t32, u32;  // the composite preconditioner's work vectors
+  // on a box-sweep batch the Krylov vectors live in the sweep's quadrant-major
+  // order (the box apply reads and writes it, the sweep needs no reorder):
+  // b and x pass through bq, xq once per solve
+  bool quad = false;
+  DevBuf bq, xq;
   int gen = -1;               // the batch work lists the solver was refreshed on
   uint64_t xs_tag = 0;        // the members' cross-section generations the records follow
 

@@ -303,15 +303,24 @@ void Batch::sweep_records(hipStream_t s) const {
   }
 }
 
-void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s) const {
+void Batch::mult(int prec, const void *x, void *y, bool scatter, hipStream_t s, bool quad) const {
   UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
   UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the batched apply needs distinct x and y");
+  // the quadrant-major order has the box body only
   if (prec == UBT_FP64)
-    batch_apply_launch<double>(*this, body[0], (const double *)x, (double *)y, scatter, s);
+    batch_apply_launch<double>(*this, quad ? 2 : body[0], (const double *)x, (double *)y, scatter, s, quad);
   else if (prec == UBT_FP32)
-    batch_apply_launch<float>(*this, body[1], (const float *)x, (float *)y, scatter, s);
+    batch_apply_launch<float>(*this, quad ? 2 : body[1], (const float *)x, (float *)y, scatter, s, quad);
   else
     fail(UBT_ERR_ARG_OUTOFRANGE, "unknown precision %d", prec);
+}
+
+void Batch::quad_order(int prec, const void *src, void *dst, bool to_q, hipStream_t s) const {
+  UBT_CHECK(quad_ready(prec), UBT_ERR_ARG_WRONGSTATE, "the batch has no quadrant-major order");
+  if (prec == UBT_FP64)
+    batch_quad_order<double>(*this, (const double *)src, (double *)dst, to_q, s);
+  else
+    batch_quad_order<float>(*this, (const float *)src, (float *)dst, to_q, s);
 }
 
 void Batch::blocks_setup(hipStream_t s) {
@@ -330,13 +339,14 @@ void Batch::blocks_apply(int prec, const void *x, void *y, hipStream_t s) const 
     batch_blocks_launch<float>(*this, (const float *)x, (float *)y, s);
 }
 
-void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s) const {
+void Batch::sweep_apply(int prec, const void *r, void *psi, hipStream_t s, bool staged) const {
   const int pi = prec == UBT_FP64 ? 0 : 1;
+  UBT_CHECK(!staged || box_sweep[pi].ok, UBT_ERR_ARG_WRONGSTATE, "a staged sweep needs the box sweep");
   if (box_sweep[pi].ok) {
     if (prec == UBT_FP64)
-      batch_box_sweep_launch<double>(*this, (const double *)r, (double *)psi, s);
+      batch_box_sweep_launch<double>(*this, (const double *)r, (double *)psi, s, staged);
     else
-      batch_box_sweep_launch<float>(*this, (const float *)r, (float *)psi, s);
+      batch_box_sweep_launch<float>(*this, (const float *)r, (float *)psi, s, staged);
     return;
   }
   sweep_records(s);
@@ -446,9 +456,16 @@ void BatchSolver::refresh(hipStream_t s) {
   batch->sweep_records(s);
   gen = batch->gen;
   xs_tag = batch->xs_tag();
+  // the quadrant-major order: the preconditioner is then the box sweep alone
+  // (it inverts every problem's L exactly), so no block stage is needed
+  quad = opt.pc_sweep && batch->quad_ready(UBT_FP64) && (!opt.mixed || batch->quad_ready(UBT_FP32));
+  if (quad && bq.bytes < sizeof(double) * (size_t)batch->n_rows) {
+    bq.alloc(sizeof(double) * (size_t)batch->n_rows);
+    xq.alloc(sizeof(double) * (size_t)batch->n_rows);
+  }
 }
 
-void BatchSolver::mult(int prec, const void *x, void *y, hipStream_t s) { batch->mult(prec, x, y, true, s); }
+void BatchSolver::mult(int prec, const void *x, void *y, hipStream_t s) { batch->mult(prec, x, y, true, s, quad); }
 
 void BatchSolver::pc(int prec, const void *r, void *y, hipStream_t s) {
   if (prec == UBT_FP64)
@@ -467,6 +484,10 @@ template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
   T *u = sizeof(T) == 8 ? u64.as<T>() : u32.as<T>();
   const int prec = prec_id<T>();
   const int64_t n = batch->n_rows;
+  if (quad) {  // the box sweep on vectors already in its order
+    batch->sweep_apply(prec, r, y, s, true);
+    return;
+  }
   bool first = true;
   if (opt.pc_blocks && !(opt.pc_sweep && batch->sweep_exact(prec))) {
     batch->blocks_apply(prec, r, y, s);
@@ -498,7 +519,15 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
     batch->sweep_records(s);
     xs_tag = tag;
   }
-  eng.solve(b, x, tag * 0x100000001B3ull + (uint64_t)gen, s);
+  const uint64_t state = (tag * 0x100000001B3ull + (uint64_t)gen) * 2 + (quad ? 1 : 0);
+  if (!quad) {
+    eng.solve(b, x, state, s);
+    return;
+  }
+  UBT_CHECK(b && x && b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
+  batch->quad_order(UBT_FP64, b, bq.ptr, true, s);
+  eng.solve(bq.as<double>(), xq.as<double>(), state, s);
+  batch->quad_order(UBT_FP64, xq.ptr, x, false, s);
 }
 
 template void BatchSolver::apply_pc<double>(const double *, double *, hipStream_t);

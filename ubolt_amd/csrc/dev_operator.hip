@@ -1093,7 +1093,7 @@ __device__ inline void seg_wsum(const T *red, const T *ws, int nseg, int len, T 
 template <class T, int DIM, int NB, int K, bool SCATTER>
 __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
                                          const BoxArgs &bx, const BoxConst<T, DIM, NB> &kc, int64_t t, T *buf,
-                                         T *phi, const T *ws, int lc, int a) {
+                                         T *phi, const T *ws, int lc, int a, int64_t RS, int BS, int64_t aoff) {
   constexpr int R = DIM - 1;
   const int A = g.A, W = bx.W;
   const int nx = bx.n[0], ny = bx.n[1], nR = bx.n[R];
@@ -1107,7 +1107,9 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
   auto cell = [&](int ii, int jj, int rr) -> int64_t {
     return DIM == 2 ? (int64_t)rr * nx + ii : ((int64_t)rr * ny + jj) * nx + ii;
   };
-  auto row_of = [&](int64_t c) { return c * NB * A + a; };
+  // value (cell c, basis b) of this thread's angle at row_of(c) + b BS: the
+  // reference order (RS = NB A, BS = A, aoff = a) or the quadrant-major one
+  auto row_of = [&](int64_t c) { return c * RS + aoff; };
   T p[K][NB], hx[K][NB], hy[K][NB], hr[NB], st[K], ss[K];
   bool okc[K];
 #pragma unroll
@@ -1115,7 +1117,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     const bool v = col && r0 + k < nR;
     const int64_t c = v ? cell(i, j, r0 + k) : 0;
 #pragma unroll
-    for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+    for (int b = 0; b < NB; b++) p[k][b] = v ? x[row_of(c) + (int64_t)b * BS] : T(0);
     st[k] = v ? ldg(g.sigt + c) : T(0);
     ss[k] = v ? ldg(g.sigs + c) : T(0);
     okc[k] = v && (bx.ok == nullptr || ldg(bx.ok + c));
@@ -1126,7 +1128,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     const bool v = col && rh >= 0 && rh < nR;
     const int64_t c = v ? cell(i, j, rh) : 0;
 #pragma unroll
-    for (int b = 0; b < NB; b++) hr[b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+    for (int b = 0; b < NB; b++) hr[b] = v ? x[row_of(c) + (int64_t)b * BS] : T(0);
   }
   // the x upwind column: another lane of the tile, or a halo load at its edge
   const int ix = kc.in_side[0] == 0 ? i - 1 : i + 1;
@@ -1137,7 +1139,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     const bool v = xhalo && r0 + k < nR;
     const int64_t c = v ? cell(ix, j, r0 + k) : 0;
 #pragma unroll
-    for (int b = 0; b < NB; b++) hx[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+    for (int b = 0; b < NB; b++) hx[k][b] = v ? x[row_of(c) + (int64_t)b * BS] : T(0);
   }
   // 3D: the y upwind row, a global load
   const int jn = DIM == 3 ? (kc.in_side[DIM == 3 ? 1 : 0] == 0 ? j - 1 : j + 1) : 0;
@@ -1147,7 +1149,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
     const bool v = yin && r0 + k < nR;
     const int64_t c = v ? cell(i, jn, r0 + k) : 0;
 #pragma unroll
-    for (int b = 0; b < NB; b++) hy[k][b] = v ? x[row_of(c) + (int64_t)b * A] : T(0);
+    for (int b = 0; b < NB; b++) hy[k][b] = v ? x[row_of(c) + (int64_t)b * BS] : T(0);
   }
   if (lane) {
 #pragma unroll
@@ -1203,7 +1205,7 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
       for (int b = 0; b < NB; b++) acc[b] -= sc * phi[(lc * K + k) * NB + b];
     }
 #pragma unroll
-    for (int b = 0; b < NB; b++) y[row_of(c) + (int64_t)b * A] = acc[b];
+    for (int b = 0; b < NB; b++) y[row_of(c) + (int64_t)b * BS] = acc[b];
   }
 }
 
@@ -1218,7 +1220,7 @@ __host__ __device__ inline size_t box_lds_reals(int A, int W, int K, int nb) {
 // Runs tiles [t0, t1) of one problem; every thread of the workgroup calls it.
 template <class T, int DIM, int NB, int K, bool SCATTER>
 __device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
-                                        const BoxArgs &bx, int64_t t0, int64_t t1) {
+                                        const BoxArgs &bx, int64_t t0, int64_t t1, const int *qp = nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int A = g.A, W = bx.W;
   const int lc = threadIdx.x / A;
@@ -1229,7 +1231,18 @@ __device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__
   for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = ldg(g.w + e);
   BoxConst<T, DIM, NB> kc;
   box_consts<T, DIM, NB>(g, bx, a, kc);
-  for (int64_t t = t0; t < t1; t++) box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb, phib, ws, lc, a);
+  // vectors in the reference order, or (qp: the problem's angle positions
+  // q A / 4 + kq) in the quadrant-major order (q, cell, basis, kq)
+  int64_t RS = (int64_t)NB * A, aoff = a;
+  int BS = A;
+  if (qp) {
+    const int Aq = A / 4, qk = ldg(qp + a), q = qk / Aq;
+    RS = (int64_t)NB * Aq;
+    BS = Aq;
+    aoff = (int64_t)q * g.n_cells * NB * Aq + (qk - q * Aq);
+  }
+  for (int64_t t = t0; t < t1; t++)
+    box_tile<T, DIM, NB, K, SCATTER>(x, y, g, bx, kc, t, bufb, phib, ws, lc, a, RS, BS, aoff);
 }
 
 // One problem: a persistent grid (a multiple of 8 workgroups), each walking one
@@ -1492,7 +1505,9 @@ template <class T, int DIM, int NB, int K, bool SCATTER>
 __global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_box(const T *__restrict__ x, T *__restrict__ y,
                                                           const BatchItem<T> *__restrict__ items,
                                                           const int64_t *__restrict__ tile_off, int64_t total,
-                                                          const int64_t *__restrict__ edge_off, int64_t n_edge) {
+                                                          const int64_t *__restrict__ edge_off, int64_t n_edge,
+                                                          const int *__restrict__ qpos,
+                                                          const int64_t *__restrict__ qoff) {
   const int64_t G = gridDim.x;
   const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
   const int64_t chunk = (total + G - 1) / G;
@@ -1507,7 +1522,7 @@ __global__ void __launch_bounds__(kBoxThreads) dg_apply_batch_box(const T *__res
     first = false;
     const BatchItem<T> &it = items[p];
     box_run<T, DIM, NB, K, SCATTER>(x + it.off, y + it.off, it.g, it.box, t - tile_off[p],
-                                    e - tile_off[p]);
+                                    e - tile_off[p], qpos ? qpos + qoff[p] : nullptr);
     t = e;
   }
   const int64_t ce = (n_edge + G - 1) / G;
@@ -3249,8 +3264,12 @@ bool batch_body_layout(const Operator &op, int prec, int body, int *cpb_out, int
 }
 
 template <class T>
-void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s) {
+void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter, hipStream_t s, bool quad) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
+  UBT_CHECK(!quad || (body == 2 && b.box_sweep[pi].ok), UBT_ERR_PLIB,
+            "the quadrant-major apply needs the box body on a box-sweep batch");
+  const int *qpos = quad ? b.box_sweep[pi].qpos.as<int>() : nullptr;
+  const int64_t *qoff = quad ? b.box_sweep[pi].qoff.as<int64_t>() : nullptr;
   const int64_t total = b.n_tiles[pi][body];
   if (total == 0) return;
   int64_t g = std::min<int64_t>(total, (int64_t)b.n_cu * b.persist[pi]);
@@ -3282,10 +3301,10 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   if (!launched && b.dim == DIM && b.nb == NB && body == 2) {                                          \
     if (scatter)                                                                                       \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, true>), dim3(grid),             \
-                         dim3(kBoxThreads), lds, s, x, y, items, toff, total, eoff, n_edge);           \
+                         dim3(kBoxThreads), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
     else                                                                                               \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, false>), dim3(grid),            \
-                         dim3(kBoxThreads), lds, s, x, y, items, toff, total, eoff, n_edge);           \
+                         dim3(kBoxThreads), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
     launched = true;                                                                                   \
   }
   BB(2, 3) BB(3, 4)
@@ -3539,29 +3558,49 @@ __global__ void __launch_bounds__(256) quad_reorder(const T *__restrict__ src, T
 }
 
 template <class T>
-void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s) {
+void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s, bool staged) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   const BoxSweepPlan &w = b.box_sweep[pi];
   constexpr int AG = kBoxSweepAngles<T>;
   const size_t lds = (size_t)2 * w.nx_max * AG * 3 * sizeof(T);
   const BatchItem<T> *items = b.items[pi][3].as<BatchItem<T>>();
   const int n_items = b.n_items[pi][3];
+  if (staged) {  // r and psi are already in the quadrant-major order
+    hipLaunchKernelGGL((sweep_box_batch<T, AG, kBoxSweepDepth<T>>), dim3(w.n_wg), dim3(kBoxSweepThreads), lds, s, r,
+                       psi, items, w.groups.as<BoxSweepGroup>(), w.nx_max);
+    UBT_HIP(hipGetLastError());
+    return;
+  }
   T *rq = w.rq.as<T>(), *pq = w.pq.as<T>();
-  const unsigned grid = (unsigned)(b.n_cu * 8);
-  hipLaunchKernelGGL((quad_reorder<T, true>), dim3(grid), dim3(256), 0, s, r, rq, items, n_items, w.qpos.as<int>(),
-                     w.qoff.as<int64_t>());
+  batch_quad_order<T>(b, r, rq, true, s);
   hipLaunchKernelGGL((sweep_box_batch<T, AG, kBoxSweepDepth<T>>), dim3(w.n_wg), dim3(kBoxSweepThreads), lds, s,
                      (const T *)rq, pq, items, w.groups.as<BoxSweepGroup>(), w.nx_max);
-  hipLaunchKernelGGL((quad_reorder<T, false>), dim3(grid), dim3(256), 0, s, (const T *)pq, psi, items, n_items,
-                     w.qpos.as<int>(), w.qoff.as<int64_t>());
+  batch_quad_order<T>(b, (const T *)pq, psi, false, s);
   UBT_HIP(hipGetLastError());
 }
 
-template void batch_box_sweep_launch<double>(const Batch &, const double *, double *, hipStream_t);
-template void batch_box_sweep_launch<float>(const Batch &, const float *, float *, hipStream_t);
+template <class T> void batch_quad_order(const Batch &b, const T *src, T *dst, bool to_q, hipStream_t s) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const BoxSweepPlan &w = b.box_sweep[pi];
+  const BatchItem<T> *items = b.items[pi][3].as<BatchItem<T>>();
+  const int n_items = b.n_items[pi][3];
+  const unsigned grid = (unsigned)(b.n_cu * 8);
+  if (to_q)
+    hipLaunchKernelGGL((quad_reorder<T, true>), dim3(grid), dim3(256), 0, s, src, dst, items, n_items,
+                       w.qpos.as<int>(), w.qoff.as<int64_t>());
+  else
+    hipLaunchKernelGGL((quad_reorder<T, false>), dim3(grid), dim3(256), 0, s, src, dst, items, n_items,
+                       w.qpos.as<int>(), w.qoff.as<int64_t>());
+  UBT_HIP(hipGetLastError());
+}
 
-template void batch_apply_launch<double>(const Batch &, int, const double *, double *, bool, hipStream_t);
-template void batch_apply_launch<float>(const Batch &, int, const float *, float *, bool, hipStream_t);
+template void batch_box_sweep_launch<double>(const Batch &, const double *, double *, hipStream_t, bool);
+template void batch_box_sweep_launch<float>(const Batch &, const float *, float *, hipStream_t, bool);
+template void batch_quad_order<double>(const Batch &, const double *, double *, bool, hipStream_t);
+template void batch_quad_order<float>(const Batch &, const float *, float *, bool, hipStream_t);
+
+template void batch_apply_launch<double>(const Batch &, int, const double *, double *, bool, hipStream_t, bool);
+template void batch_apply_launch<float>(const Batch &, int, const float *, float *, bool, hipStream_t, bool);
 template void batch_blocks_launch<double>(const Batch &, const double *, double *, hipStream_t);
 template void batch_blocks_launch<float>(const Batch &, const float *, float *, hipStream_t);
 
