@@ -399,6 +399,53 @@ int ubt_solver_result(const ubt_solver *solver, int *iterations, int *outer, dou
                       double *history, int max_history, int *n_history);
 int ubt_solver_destroy(ubt_solver *solver);
 
+/* ------------------------------------------------------------- group sweep
+ * The multigroup driver of the reference's test driver
+ * (tests/transportk.kokkos.cxx:925-1208): the Gauss-Seidel sweep over the
+ * groups - per group the rhs (inflow, source, the transfers out of the
+ * cached scalar flux of the groups above), the solve, the scalar flux cached
+ * for the groups below - and, with upscatter, the outer iteration over the
+ * thermal block g_up .. G-1 (g_up: the first group anything upscatters into,
+ * from the cross sections) until the estimated error change * rho /
+ * (1 - rho) of its scalar flux is below outer_rtol; a thermal solve after the
+ * first sweep starts from its last psi at max(inner rtol, inner_gamma x the
+ * last change), and the iteration stops only on a sweep at the inner rtol.
+ * Options replace -outer_rtol, -outer_max_it, -outer_inner_gamma and, under
+ * upscatter, -ksp_rtol (tests/transportk.kokkos.cxx:606-647); the moments
+ * store's reconstruction (-outer_store moments) is not offered: psi is kept
+ * per group. */
+typedef struct {
+  double outer_rtol;   /* e.g. 1e-8 */
+  int max_outer;       /* sweeps, the first included, e.g. 1000 */
+  double inner_gamma;  /* [0, 1), e.g. 0.01; 0: every solve at the inner rtol */
+  double inner_rtol;   /* every solve's rtol under upscatter; 0: 1e-10. Without
+                        * upscatter the solver's own rtol */
+} ubt_group_options;
+
+typedef struct {
+  int g_up;            /* first thermal group (n_groups: no upscatter, one sweep) */
+  int sweeps;          /* the first included */
+  int status;          /* 1 converged; -1 a solve did not converge (the sweep
+                        * stopped at that group); -2 max_outer sweeps ran out */
+  double change;       /* the last sweep's ||phi_new - phi_old|| / ||phi_new|| over the thermal block */
+  double rho;          /* its ratio to the sweep before's (0 before the third sweep) */
+  double err_est;      /* change * rho / (1 - rho); infinite with no rho below 1 */
+  int64_t iterations;  /* Krylov iterations over every solve */
+} ubt_group_result;
+
+/* Every group of the solver's operator: psi (FP64 device memory) receives
+ * n_groups angular fluxes of n_rows each, group g at g * n_rows. Leaves the
+ * operator (and the DSA handle) at the last group solved, every group's
+ * scalar flux cached (ubt_set_scalar_flux). options NULL = defaults. Returns
+ * UBT_OK also when not converged: read ubt_solver_groups_result. Replaces
+ * the group loop of tests/transportk.kokkos.cxx:1019-1208. */
+int ubt_solver_solve_groups(ubt_solver *solver, const ubt_group_options *options, double *psi, void *stream);
+/* the last group sweep; per group (arrays of n_groups, each may be NULL):
+ * Krylov iterations summed over the sweeps, the last solve's relative
+ * residual and reason */
+int ubt_solver_groups_result(const ubt_solver *solver, ubt_group_result *result, int *iterations,
+                             double *rel_residual, int *reason);
+
 /* ---------------------------------------------------------------- batches
  * Many independent problems driven as one (no reference counterpart: the
  * reference solves one problem per process; this is how a caller runs a
@@ -462,6 +509,19 @@ int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, dou
 /* 1 (default): replay the restart cycles as captured HIP graphs (captured on
  * the first solve after a refresh); 0: launch every kernel from the host */
 int ubt_batch_solver_set_graph(ubt_batch_solver *solver, int on);
+/* Every group of every problem (ubt_solver_solve_groups per problem, the
+ * problems' solves of one group batched, each problem with its own outer
+ * test; a problem whose iteration has ended sits out the later sweeps): psi
+ * receives G_max batch vectors (G_max the most groups of any problem), group
+ * g at g * n_rows; a problem with fewer groups leaves its span of the later
+ * vectors zero. Leaves the batch at the last group swept. No reference
+ * counterpart (the reference sweeps one problem per process). */
+int ubt_batch_solver_solve_groups(ubt_batch_solver *solver, const ubt_group_options *options, double *psi,
+                                  void *stream);
+/* per problem: result[n]; iterations, rel_residual, reason [n * G_max],
+ * problem p's group g at p * G_max + g (each may be NULL) */
+int ubt_batch_solver_groups_result(const ubt_batch_solver *solver, ubt_group_result *result, int *iterations,
+                                   double *rel_residual, int *reason);
 int ubt_batch_solver_destroy(ubt_batch_solver *solver);
 
 #ifdef __cplusplus

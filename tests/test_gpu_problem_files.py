@@ -29,8 +29,9 @@ def test_problem_file_solve(gpu, name):
     prob = pf.problem
     print(f"[case] {name}: {prob.disc.n_cells} cells, nb {prob.disc.nb}, {prob.disc.n_angles} angles, "
           f"{prob.n_groups} groups", flush=True)
-    psi, results = pfm.solve(pf, rtol=1e-12, max_it=800)
+    psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800)
     torch.cuda.synchronize()
+    assert rep["status"] == 1 and rep["sweeps"] == 1 and rep["g_up"] == prob.n_groups, rep
     cache = []
     for g in range(prob.n_groups):
         want = oracle_solve(prob, oracle_rhs(prob, g, cache), g)
@@ -38,9 +39,9 @@ def test_problem_file_solve(gpu, name):
         cache.append(phw)
         ph = oracle.moments(to_host(psi[g]), prob.quad.w)
         err = np.linalg.norm(ph - phw) / np.linalg.norm(phw)
-        res = results[g][2]
-        print(f"[case] {name} group {g}: its {res['iterations']}, scalar flux rel L2 vs oracle {err:.2e}", flush=True)
-        assert res["reason"] == 1 and err <= 1e-8, (g, err)
+        its = rep["group_iterations"][g]
+        print(f"[case] {name} group {g}: its {its}, scalar flux rel L2 vs oracle {err:.2e}", flush=True)
+        assert rep["group_reason"][g] == 1 and err <= 1e-8, (g, err)
 
 
 @pytest.mark.parametrize("name", ["box_up3_layers", "plex_box_up3_layers_dg1"])
@@ -50,12 +51,13 @@ def test_problem_file_upscatter(gpu, name):
     pf = pfm.load(os.path.join(PROBLEMS, name + ".json"))
     assert pf.upscatter
     prob = pf.problem
-    psi, results = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-11, max_outer=200)
+    psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-11, max_outer=200)
     torch.cuda.synchronize()
+    assert rep["status"] == 1, rep
     G, W = prob.n_groups, prob.quad.sum_weights
     _, ss, _ = xs_arrays(prob)
     phis = [oracle.moments(to_host(p), prob.quad.w) for p in psi]
-    outers = max(r[0] for r in results) + 1
+    outers = rep["sweeps"]
     for g in range(G):
         b = oracle_rhs(prob, g, phis[:g])
         for gf in range(g + 1, G):

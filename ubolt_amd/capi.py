@@ -33,6 +33,16 @@ class SolverOptions(C.Structure):
                 ("pc_dsa", C.c_int), ("use_amat", C.c_int), ("gs_passes", C.c_int)]
 
 
+class GroupOptions(C.Structure):
+    _fields_ = [("outer_rtol", C.c_double), ("max_outer", C.c_int), ("inner_gamma", C.c_double),
+                ("inner_rtol", C.c_double)]
+
+
+class GroupResult(C.Structure):
+    _fields_ = [("g_up", C.c_int), ("sweeps", C.c_int), ("status", C.c_int), ("change", C.c_double),
+                ("rho", C.c_double), ("err_est", C.c_double), ("iterations", C.c_int64)]
+
+
 class Bc(C.Structure):
     _fields_ = [("label", C.c_int), ("type", C.c_int), ("inflow", C.c_double), ("n_window_pairs", C.c_int),
                 ("window", C.c_double * 4)]
@@ -113,6 +123,8 @@ SIGNATURES = {
     "ubt_solver_apply_pc": [P, I, P, P, P],
     "ubt_solver_result": [P, PI, PI, PD, PI, PD, I, PI],
     "ubt_solver_destroy": [P],
+    "ubt_solver_solve_groups": [P, C.POINTER(GroupOptions), P, P],
+    "ubt_solver_groups_result": [P, C.POINTER(GroupResult), PI, PD, PI],
     "ubt_batch_create": [PP, I, PP],
     "ubt_batch_size": [P, PI64, PI],
     "ubt_batch_offsets": [P, PI64],
@@ -133,6 +145,8 @@ SIGNATURES = {
     "ubt_batch_solver_result": [P, PI, PD, PI, PI, PI],
     "ubt_batch_solver_set_graph": [P, I],
     "ubt_batch_solver_destroy": [P],
+    "ubt_batch_solver_solve_groups": [P, C.POINTER(GroupOptions), P, P],
+    "ubt_batch_solver_groups_result": [P, C.POINTER(GroupResult), PI, PD, PI],
 }
 
 _lib = None

@@ -347,6 +347,26 @@ class Solver(_Handle):
         call("ubt_solver_apply_pc", self.h, Operator._prec(r), dptr(r), dptr(y), stream_ptr(stream))
         return y
 
+    def solve_groups(self, psi=None, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01, inner_rtol=0.0, stream=None):
+        """every group by the reference driver's group sweep (and, with
+        upscatter, its outer iteration): psi (n_groups x n_rows FP64, made if
+        None) receives every group's angular flux; returns (psi, report)"""
+        G, n = self.op.n_groups, self.op.n_rows
+        if psi is None:
+            psi = torch.zeros((G, n), dtype=torch.float64, device="cuda")
+        assert psi.dtype == torch.float64 and psi.numel() == G * n and psi.is_contiguous()
+        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol)
+        call("ubt_solver_solve_groups", self.h, C.byref(o), dptr(psi), stream_ptr(stream))
+        return psi, self.groups_result()
+
+    def groups_result(self):
+        G = self.op.n_groups
+        r = capi.GroupResult()
+        it, rs = (C.c_int * G)(), (C.c_int * G)()
+        rr = (C.c_double * G)()
+        call("ubt_solver_groups_result", self.h, C.byref(r), it, rr, rs)
+        return _group_report(r, list(it), list(rr), list(rs))
+
     def result(self):
         it, outer, reason, nh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         rr = C.c_double()
@@ -355,6 +375,12 @@ class Solver(_Handle):
              C.byref(nh))
         return {"iterations": it.value, "outer": outer.value, "rel_residual": rr.value, "reason": reason.value,
                 "history": hist[:min(nh.value, 4096)].copy()}
+
+
+def _group_report(r, its, res, reason):
+    return {"g_up": r.g_up, "sweeps": r.sweeps, "status": r.status, "change": r.change, "rho": r.rho,
+            "err_est": r.err_est, "iterations": r.iterations, "group_iterations": its, "group_rel_residual": res,
+            "group_reason": reason}
 
 
 class Batch(_Handle):
@@ -448,6 +474,29 @@ class BatchSolver(_Handle):
     def solve(self, b, x, stream=None):
         call("ubt_batch_solver_solve", self.h, dptr(b), dptr(x), stream_ptr(stream))
         return self.result()
+
+    def solve_groups(self, psi=None, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01, inner_rtol=0.0, stream=None):
+        """every group of every problem (the group sweep per problem, one
+        group's solves batched): psi (G_max x n_rows FP64, made if None);
+        returns (psi, per-problem reports)"""
+        G = max(o.n_groups for o in self.batch.ops)
+        n = self.batch.n_rows
+        if psi is None:
+            psi = torch.zeros((G, n), dtype=torch.float64, device="cuda")
+        assert psi.dtype == torch.float64 and psi.numel() == G * n and psi.is_contiguous()
+        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol)
+        call("ubt_batch_solver_solve_groups", self.h, C.byref(o), dptr(psi), stream_ptr(stream))
+        return psi, self.groups_result()
+
+    def groups_result(self):
+        G = max(o.n_groups for o in self.batch.ops)
+        n = self.batch.n_problems
+        r = (capi.GroupResult * n)()
+        it, rs = (C.c_int * (n * G))(), (C.c_int * (n * G))()
+        rr = (C.c_double * (n * G))()
+        call("ubt_batch_solver_groups_result", self.h, r, it, rr, rs)
+        return [_group_report(r[p], list(it[p * G:(p + 1) * G]), list(rr[p * G:(p + 1) * G]), list(rs[p * G:(p + 1) * G]))
+                for p in range(n)]
 
     def result(self):
         n = self.batch.n_problems

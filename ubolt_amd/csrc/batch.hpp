@@ -30,6 +30,7 @@ struct BatchSweep {
   int64_t n_items = 0, widest = 0;
   mutable DevBuf rec, rnb;
   mutable uint64_t rec_tag = ~0ull;
+  std::vector<char> lists_for;  // the active set the lists were built for (empty: none built)
 };
 // The batched sweep of uniform all-vacuum 2D DG1 boxes (no records): one
 // workgroup per problem, quadrant and group of kBoxSweepAngles of its angles
@@ -85,11 +86,16 @@ struct Batch {
   // block inverse and sweep work lists
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems
   int64_t n_blocks = 0;
-  BatchSweep sweep[2];         // per precision: its own grouping and records
+  // per precision: its own grouping and records (built when the record
+  // sweep first runs on an active set)
+  mutable BatchSweep sweep[2];
+  mutable std::vector<SweepLists> sched[2];  // per precision, per problem: the regrouped schedule, once computed
   BoxSweepPlan box_sweep[2];   // per precision: the record-free sweep of box batches
 
   void create(const std::vector<Operator *> &o);
-  void set_group(int g);       // problems without group g go inactive
+  // problems without group g go inactive, and with `want` (n flags) those it
+  // leaves out
+  void set_group(int g, const std::vector<char> &want = {});
   // quad: x and y in the box sweep's quadrant-major order (box batches)
   void mult(int prec, const void *x, void *y, bool scatter, hipStream_t s, bool quad = false) const;
   void blocks_setup(hipStream_t s);  // singular check of every active problem's blocks
@@ -110,6 +116,8 @@ struct Batch {
   void tune();
   void build_lists();
   void build_box_sweep();
+  void record_lists(int pi) const;  // the record sweep's work list for the active set
+  std::vector<char> lists_for;  // the active set the block prefix, sweep lists and box plan were built for
   bool box_sweep_enabled = true;  // false: the record sweep on every batch
 };
 
@@ -149,10 +157,13 @@ struct BatchSolver : KrylovSystem {
   DevBuf bq, xq;
   int gen = -1;               // the batch work lists the solver was refreshed on
   uint64_t xs_tag = 0;        // the members' cross-section generations the records follow
+  std::vector<GroupReport> groups;  // per problem: the last group sweep (solve_groups)
 
   void create(Batch *b, const SolverOptions &o);
   void refresh(hipStream_t s);
-  void solve(const double *b, double *x, hipStream_t s);
+  // guess: start from x as given (else from zero); rtol (n values, may be
+  // null): per-problem tolerances in place of opt.rtol
+  void solve(const double *b, double *x, hipStream_t s, bool guess = false, const double *rtol = nullptr);
   template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
   void mult(int prec, const void *x, void *y, hipStream_t s) override;
   void pc(int prec, const void *r, void *y, hipStream_t s) override;

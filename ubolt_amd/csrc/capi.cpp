@@ -89,6 +89,38 @@ template <class V> void copy_out(const std::vector<V> &v, void *out, size_t *nby
   *nbytes = n;
 }
 
+ubt::GroupOptions group_options(const ubt_group_options *o) {
+  ubt::GroupOptions g;
+  if (o) {
+    g.outer_rtol = o->outer_rtol;
+    g.max_outer = o->max_outer;
+    g.inner_gamma = o->inner_gamma;
+    g.inner_rtol = o->inner_rtol;
+  }
+  return g;
+}
+void group_result(const ubt::GroupReport &r, ubt_group_result *out, int *its, double *res, int *reason, int stride) {
+  if (out) {
+    out->g_up = r.g_up;
+    out->sweeps = r.sweeps;
+    out->status = r.status;
+    out->change = r.change;
+    out->rho = r.rho;
+    out->err_est = r.err_est;
+    out->iterations = r.iterations;
+  }
+  for (size_t g = 0; g < r.group_iterations.size(); g++) {
+    if (its) its[g] = r.group_iterations[g];
+    if (res) res[g] = r.group_residual[g];
+    if (reason) reason[g] = r.group_reason[g];
+  }
+  for (int g = (int)r.group_iterations.size(); g < stride; g++) {
+    if (its) its[g] = 0;
+    if (res) res[g] = 0.0;
+    if (reason) reason[g] = 0;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -840,6 +872,41 @@ int ubt_batch_solver_result(const ubt_batch_solver *solver, int *iterations, dou
     if (cycles) *cycles = s.cycles;
   });
 }
+
+int ubt_solver_solve_groups(ubt_solver *solver, const ubt_group_options *options, double *psi, void *stream) {
+  return guard([&] {
+    NEED(solver);
+    ubt::solve_groups(solver->s, group_options(options), psi, (hipStream_t)stream, solver->s.groups);
+  });
+}
+int ubt_solver_groups_result(const ubt_solver *solver, ubt_group_result *result, int *iterations,
+                             double *rel_residual, int *reason) {
+  return guard([&] {
+    NEED(solver);
+    const auto &r = solver->s.groups;
+    group_result(r, result, iterations, rel_residual, reason, (int)r.group_iterations.size());
+  });
+}
+int ubt_batch_solver_solve_groups(ubt_batch_solver *solver, const ubt_group_options *options, double *psi,
+                                  void *stream) {
+  return guard([&] {
+    NEED(solver);
+    ubt::solve_groups(solver->s, group_options(options), psi, (hipStream_t)stream, solver->s.groups);
+  });
+}
+int ubt_batch_solver_groups_result(const ubt_batch_solver *solver, ubt_group_result *result, int *iterations,
+                                   double *rel_residual, int *reason) {
+  return guard([&] {
+    NEED(solver);
+    const auto &rs = solver->s.groups;
+    int G = 0;
+    for (const auto *op : solver->s.batch->ops) G = std::max(G, op->n_groups);
+    for (size_t p = 0; p < rs.size(); p++)
+      group_result(rs[p], result ? result + p : nullptr, iterations ? iterations + p * G : nullptr,
+                   rel_residual ? rel_residual + p * G : nullptr, reason ? reason + p * G : nullptr, G);
+  });
+}
+
 int ubt_batch_solver_set_graph(ubt_batch_solver *solver, int on) {
   return guard([&] {
     NEED(solver);

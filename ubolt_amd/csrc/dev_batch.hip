@@ -49,12 +49,12 @@ void Batch::create(const std::vector<Operator *> &o) {
   tune();
 }
 
-void Batch::set_group(int g) {
+void Batch::set_group(int g, const std::vector<char> &want) {
   UBT_CHECK(g >= 0, UBT_ERR_ARG_OUTOFRANGE, "group %d is negative", g);
   bool any = false;
   active.assign(ops.size(), 0);
   for (size_t p = 0; p < ops.size(); p++) {
-    if (ops[p]->n_groups > g) {
+    if (ops[p]->n_groups > g && (want.empty() || want[p])) {
       active[p] = 1;
       ops[p]->group = g;
       any = true;
@@ -113,28 +113,48 @@ void Batch::build_lists() {
   UBT_CHECK(body_ok[0][0] && body_ok[1][0], UBT_ERR_SUP, "a problem of the batch has more than 256 angles");
   for (int pi = 0; pi < 2; pi++)
     if (!body_ok[pi][body[pi]]) body[pi] = 0;
+  // what follows depends on the active set alone (not on the group's cross
+  // sections): kept while the set is unchanged, so a group change costs the
+  // apply work lists and the sweep's per-group records only
+  if (active == lists_for) {
+    sweep_records(nullptr);
+    UBT_HIP(hipStreamSynchronize(nullptr));
+    return;
+  }
+  lists_for = active;
   // element blocks: prefix of cells x angles over the active problems
   std::vector<int64_t> blk(1, 0);
   for (size_t p = 0; p < ops.size(); p++)
     if (active[p]) blk.push_back(blk.back() + (int64_t)ops[p]->disc->n_cells * ops[p]->disc->n_angles);
   bblk.upload(blk);
   n_blocks = blk.back();
-  // sweep, per precision: one workgroup per direction group of every active
-  // problem (kBatchSweepGroup directions of one octant), the problems'
-  // schedules concatenated; the workgroups are listed longest first (most
-  // levels), so the groups that bound the sweep are dispatched before the
-  // short ones
+  build_box_sweep();
+  // the record sweep's lists are built when it first runs on this set
+  for (int pi = 0; pi < 2; pi++) sweep[pi].lists_for.clear();
+  sweep_records(nullptr);
+  UBT_HIP(hipStreamSynchronize(nullptr));
+}
+
+// The record sweep's work list in precision pi: one workgroup per direction
+// group of every active problem (kBatchSweepGroup directions of one octant),
+// the problems' schedules concatenated; the workgroups are listed longest
+// first (most levels), so the groups that bound the sweep are dispatched
+// before the short ones. Each problem's regrouped schedule is computed once
+// and kept; the list is rebuilt when the active set changes.
+void Batch::record_lists(int pi) const {
+  BatchSweep &w = sweep[pi];
+  if (w.lists_for == active) return;
   int nfu = 1;
   for (size_t p = 0; p < ops.size(); p++)
     if (active[p]) nfu = std::max(nfu, sweeps[p]->nfu);
-  for (int pi = 0; pi < 2; pi++) {
-    BatchSweep &w = sweep[pi];
+  std::vector<SweepLists> &lists = sched[pi];
+  if (lists.size() != ops.size()) lists.assign(ops.size(), SweepLists{});
+  {
     struct Grp {
       int q, g, nlev;
       size_t p;
     };
     std::vector<Grp> grps;
-    std::vector<SweepLists> lists(ops.size());
     std::vector<int64_t> item0(ops.size(), 0);
     std::vector<int> ce, an;
     std::vector<unsigned char> up;
@@ -143,7 +163,7 @@ void Batch::build_lists() {
     for (size_t p = 0; p < ops.size(); p++) {
       if (!active[p]) continue;
       SweepLists &L = lists[p];
-      group_schedule(*sweeps[p], kBatchSweepGroup[pi], L);
+      if (L.base.empty()) group_schedule(*sweeps[p], kBatchSweepGroup[pi], L);
       item0[p] = (int64_t)ce.size();
       ce.insert(ce.end(), L.cell.begin(), L.cell.end());
       an.insert(an.end(), L.angle.begin(), L.angle.end());
@@ -175,9 +195,7 @@ void Batch::build_lists() {
     w.base.upload(bs);
     w.rec_tag = ~0ull;
   }
-  build_box_sweep();
-  sweep_records(nullptr);
-  UBT_HIP(hipStreamSynchronize(nullptr));
+  w.lists_for = active;
 }
 
 // The record-free box sweep's work list, per precision: every active problem
@@ -289,6 +307,7 @@ void Batch::sweep_records(hipStream_t s) const {
   for (int pi = 0; pi < 2; pi++) {
     const BatchSweep &w = sweep[pi];
     if (box_sweep[pi].ok) continue;  // the box sweep forms its blocks on the fly
+    record_lists(pi);
     if (w.rec_tag == tag || w.n_items == 0) continue;
     // 64-item blocks (see the sweep's record layout)
     const size_t blocks64 = (size_t)((w.n_items + 63) / 64) * 64;
@@ -507,7 +526,7 @@ template <class T> void BatchSolver::apply_pc(const T *r, T *y, hipStream_t s) {
   if (first) vec_axpby<T>(n, 1.0, r, 0.0, y, s);
 }
 
-void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
+void BatchSolver::solve(const double *b, double *x, hipStream_t s, bool guess, const double *rtol) {
   UBT_CHECK(eng.n_chunks > 0 && gen == batch->gen, UBT_ERR_ARG_WRONGSTATE,
             "refresh the batch solver after setting the batch group");
   // a member's cross sections changed since the refresh (ubt_op_set_xs):
@@ -521,12 +540,13 @@ void BatchSolver::solve(const double *b, double *x, hipStream_t s) {
   }
   const uint64_t state = (tag * 0x100000001B3ull + (uint64_t)gen) * 2 + (quad ? 1 : 0);
   if (!quad) {
-    eng.solve(b, x, state, s);
+    eng.solve(b, x, state, s, guess, rtol);
     return;
   }
   UBT_CHECK(b && x && b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
   batch->quad_order(UBT_FP64, b, bq.ptr, true, s);
-  eng.solve(bq.as<double>(), xq.as<double>(), state, s);
+  if (guess) batch->quad_order(UBT_FP64, x, xq.ptr, true, s);
+  eng.solve(bq.as<double>(), xq.as<double>(), state, s, guess, rtol);
   batch->quad_order(UBT_FP64, xq.ptr, x, false, s);
 }
 

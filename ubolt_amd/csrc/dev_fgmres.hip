@@ -586,7 +586,7 @@ void Fgmres::fgmres(const T *b, T *x, const double *tol_abs, int max_cycles, boo
   }
 }
 
-void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s) {
+void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s, bool guess, const double *rtol) {
   UBT_CHECK(b && x, UBT_ERR_ARG_NULL, "b and x are required");
   UBT_CHECK(b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
   UBT_CHECK(n_chunks > 0, UBT_ERR_ARG_WRONGSTATE, "no problem takes part in the solve");
@@ -611,14 +611,22 @@ void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s
   // ||b|| per problem
   launch_bdot<double>(1, nc, cap, n, b, b, cst, cln, pt);
   hipLaunchKernelGGL(k_bdot_final, dim3(np, 1), dim3(kT), 0, cap, 1, cfp, (const double *)pt, n_chunks, bnorm.as<double>(), 1, 1);
-  std::vector<double> bn(np), rn(np), tl(np);
+  std::vector<double> bn(np), rn(np), tl(np), rt(np, opt.rtol);
+  if (rtol)
+    for (int p = 0; p < np; p++) {
+      UBT_CHECK(rtol[p] > 0.0, UBT_ERR_ARG_OUTOFRANGE, "rtol of problem %d must be positive, got %g", p, rtol[p]);
+      rt[p] = rtol[p];
+    }
   UBT_HIP(hipMemcpyAsync(bn.data(), bnorm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
   UBT_HIP(hipStreamSynchronize(cap));
   if (!opt.mixed) {
     double *bi = b32.as<double>(), *xi = d32.as<double>();
     vec_axpby<double>(n, 1.0, b, 0.0, bi, cap);
-    vec_fill<double>(n, 0.0, xi, cap);
-    for (int p = 0; p < np; p++) tl[p] = opt.rtol * bn[p];
+    if (guess)
+      vec_axpby<double>(n, 1.0, (const double *)x, 0.0, xi, cap);
+    else
+      vec_fill<double>(n, 0.0, xi, cap);
+    for (int p = 0; p < np; p++) tl[p] = rt[p] * bn[p];
     fgmres<double>(bi, xi, tl.data(), max_cycles, true, cap);
     // the last start's true residual norms
     UBT_HIP(hipMemcpyAsync(rn.data(), nrm.as<double>(), sizeof(double) * np, hipMemcpyDeviceToHost, cap));
@@ -627,7 +635,7 @@ void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s
   } else {
     double *r = r64.as<double>();
     float *bf = b32.as<float>(), *d = d32.as<float>();
-    vec_fill<double>(n, 0.0, x, cap);
+    if (!guess) vec_fill<double>(n, 0.0, x, cap);
     for (outer = 0; outer <= opt.max_outer; outer++) {
       sys->mult(UBT_FP64, x, r, cap);
       vec_axpby<double>(n, 1.0, b, -1.0, r, cap);
@@ -640,7 +648,7 @@ void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s
       bool all = true;
       int its_max = 0;
       for (int p = 0; p < np; p++) {
-        const bool conv = !live[p] || rn[p] <= opt.rtol * bn[p];
+        const bool conv = !live[p] || rn[p] <= rt[p] * bn[p];
         all = all && conv;
         tl[p] = conv ? 1e300 : opt.inner_rtol;  // a converged problem sits the inner solve out
         its_max = std::max(its_max, it[p]);
@@ -664,7 +672,7 @@ void Fgmres::solve(const double *b, double *x, uint64_t state_tag, hipStream_t s
   for (int p = 0; p < np; p++) {
     iterations[p] = it[p];
     rel_residual[p] = live[p] ? (bn[p] > 0.0 ? rn[p] / bn[p] : 0.0) : 0.0;
-    reason[p] = !live[p] ? 0 : (rel_residual[p] <= opt.rtol ? 1 : -1);
+    reason[p] = !live[p] ? 0 : (rel_residual[p] <= rt[p] ? 1 : -1);
   }
   if (!opt.mixed && hist_cap > 0 && np >= 1) {
     const int nh = std::min(it[0], hist_cap);

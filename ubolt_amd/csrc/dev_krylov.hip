@@ -96,7 +96,7 @@ template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s) {
   if (first) vec_axpby<T>(n, 1.0, r, 0.0, y, s);
 }
 
-void Solver::solve(const double *b, double *x, hipStream_t s, bool adj) {
+void Solver::solve(const double *b, double *x, hipStream_t s, bool adj, bool guess) {
   UBT_CHECK(b && x, UBT_ERR_ARG_NULL, "b and x are required");
   UBT_CHECK(b != x, UBT_ERR_ARG_IDN, "the solve needs distinct b and x");
   UBT_CHECK(op->xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
@@ -124,7 +124,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s, bool adj) {
   tag = tag * 0x100000001B3ull + (uint64_t)(op->group + 1);
   tag = tag * 0x100000001B3ull + (adj ? 2u : 1u);
   tag = tag * 0x100000001B3ull + (uint64_t)(with_dsa ? dsa->group + 2 : 0);
-  eng.solve(b, x, tag, s);
+  eng.solve(b, x, tag, s, guess);
   iterations = eng.iterations[0];
   outer = eng.outer;
   rel_residual = eng.rel_residual[0];

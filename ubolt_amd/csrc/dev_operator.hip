@@ -14,6 +14,7 @@
 // contiguous run for the angles that see it upwind.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <unordered_map>
 
@@ -2916,6 +2917,14 @@ void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s,
   op.t32.sigt.upload_as<float>(st);
   op.t32.sigs.upload_as<float>(ss);
   op.sigs_d.upload(ss);
+  // the first group anything upscatters into (reference
+  // tests/transportk.kokkos.cxx:953-962, over cells instead of materials)
+  op.g_up = (int)G;
+  for (size_t gf = 1; gf < G; gf++)
+    for (size_t gt = 0; gt < gf && (int)gt < op.g_up; gt++) {
+      const double *v = ss.data() + (gf * G + gt) * nc;
+      if (std::any_of(v, v + nc, [](double x) { return x != 0.0; })) op.g_up = (int)gt;
+    }
   const bool first = !op.xs_set;
   if (source) {
     std::vector<double> q(source, source + G * nc);
@@ -3057,6 +3066,58 @@ void launch_moments(const Operator &op, const PrecTables &t, const T *psi, OutT 
   UBT_HIP(hipGetLastError());
 }
 
+// The outer iteration's change measure: cache = fresh while summing
+// (fresh - cache)^2 and fresh^2, per block, then one block adds the partials
+// in block order (deterministic) to sums[0..1]
+__global__ void __launch_bounds__(256) measure_overwrite(const double *__restrict__ fresh, double *__restrict__ cache,
+                                                         int64_t n, double *__restrict__ part) {
+  __shared__ double sd[256], sq[256];
+  double d = 0.0, q = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double v = fresh[i], o = cache[i];
+    d += (v - o) * (v - o);
+    q += v * v;
+    cache[i] = v;
+  }
+  sd[threadIdx.x] = d;
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sd[threadIdx.x] += sd[threadIdx.x + k];
+      sq[threadIdx.x] += sq[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sd[0];
+    part[2 * blockIdx.x + 1] = sq[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) measure_final(const double *__restrict__ part, int nblk, double *sums) {
+  __shared__ double sd[256], sq[256];
+  double d = 0.0, q = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    d += part[2 * i];
+    q += part[2 * i + 1];
+  }
+  sd[threadIdx.x] = d;
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      sd[threadIdx.x] += sd[threadIdx.x + k];
+      sq[threadIdx.x] += sq[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[0] += sd[0];
+    sums[1] += sq[0];
+  }
+}
+
 RhsArgs make_rhs_args(const Operator &op, int g, int gf_lo, int gf_hi, int skip) {
   const Disc &D = *op.disc;
   RhsArgs r{};
@@ -3112,13 +3173,34 @@ void Operator::moments_f64(int prec, const void *psi, double *phi, hipStream_t s
     launch_moments<float, double>(*this, t32, (const float *)psi, phi, 1, 1, s, unit);
 }
 
-void Operator::set_scalar_flux(int g, int prec, const void *psi, hipStream_t s) {
+void Operator::set_scalar_flux(int g, int prec, const void *psi, hipStream_t s, double *sums) {
   UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
-  double *dst = phi_cache.as<double>() + (int64_t)g * disc->n_nodes();
+  const int64_t nodes = disc->n_nodes();
+  double *cache = phi_cache.as<double>() + (int64_t)g * nodes;
+  // measured: integrate into the scratch, then overwrite the cache from it
+  // while summing the change (reference GroupTransfer::set_scalar_flux,
+  // src/multigroupk.kokkos.cxx:266-299)
+  if (sums && phi_tmp.bytes < sizeof(double) * (size_t)nodes) phi_tmp.alloc(sizeof(double) * (size_t)nodes);
+  double *dst = sums ? phi_tmp.as<double>() : cache;
   if (prec == UBT_FP64)
     launch_moments<double, double>(*this, t64, (const double *)psi, dst, 1, 1, s);
   else
     launch_moments<float, double>(*this, t32, (const float *)psi, dst, 1, 1, s);
+  if (sums) {
+    if (!measure_part.bytes) measure_part.alloc(sizeof(double) * 2 * kMeasureBlocks);
+    const unsigned nblk = (unsigned)std::min<int64_t>(kMeasureBlocks, (nodes + 255) / 256);
+    hipLaunchKernelGGL(measure_overwrite, dim3(nblk), dim3(256), 0, s, (const double *)dst, cache, nodes,
+                       measure_part.as<double>());
+    hipLaunchKernelGGL(measure_final, dim3(1), dim3(256), 0, s, (const double *)measure_part.as<double>(), (int)nblk, sums);
+    UBT_HIP(hipGetLastError());
+  }
+  phi_set[g] = 1;
+}
+
+void Operator::cache_zero_flux(int g, hipStream_t s) {
+  UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
+  const int64_t nodes = disc->n_nodes();
+  UBT_HIP(hipMemsetAsync(phi_cache.as<double>() + (int64_t)g * nodes, 0, sizeof(double) * (size_t)nodes, s));
   phi_set[g] = 1;
 }
 
@@ -3141,13 +3223,14 @@ void Operator::scatter_source(int gf, int gt, int prec, void *b, hipStream_t s) 
   UBT_HIP(hipGetLastError());
 }
 
-void Operator::rhs(int g, int prec, void *b, hipStream_t s) const {
+void Operator::rhs(int g, int prec, void *b, hipStream_t s, bool all_groups) const {
   UBT_CHECK(xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
   UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
-  for (int gf = 0; gf < g; gf++)
-    UBT_CHECK(phi_set[gf], UBT_ERR_ARG_WRONGSTATE,
+  const int g_end = all_groups ? n_groups : g;
+  for (int gf = 0; gf < g_end; gf++)
+    UBT_CHECK(gf == g || phi_set[gf], UBT_ERR_ARG_WRONGSTATE,
               "no scalar flux cached for group %d - set it when that group is solved", gf);
-  RhsArgs r = make_rhs_args(*this, g, 0, g, g);
+  RhsArgs r = make_rhs_args(*this, g, 0, g_end, g);
   const int64_t rows = disc->n_rows();
   const unsigned grid = (unsigned)((rows + 255) / 256);
   if (prec == UBT_FP64)
@@ -3564,7 +3647,6 @@ void batch_box_sweep_launch(const Batch &b, const T *r, T *psi, hipStream_t s, b
   constexpr int AG = kBoxSweepAngles<T>;
   const size_t lds = (size_t)2 * w.nx_max * AG * 3 * sizeof(T);
   const BatchItem<T> *items = b.items[pi][3].as<BatchItem<T>>();
-  const int n_items = b.n_items[pi][3];
   if (staged) {  // r and psi are already in the quadrant-major order
     hipLaunchKernelGGL((sweep_box_batch<T, AG, kBoxSweepDepth<T>>), dim3(w.n_wg), dim3(kBoxSweepThreads), lds, s, r,
                        psi, items, w.groups.as<BoxSweepGroup>(), w.nx_max);

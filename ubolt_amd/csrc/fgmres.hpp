@@ -89,9 +89,12 @@ struct Fgmres {
   void create(KrylovSystem *system, int64_t rows, int problems, const SolverOptions &o, int hist_cap);
   // the problems' row spans (np + 1 offsets) and which take part; drops the graphs
   void set_problems(const std::vector<int64_t> &off, const std::vector<char> &active, hipStream_t s);
-  // x = A^{-1} b per problem from x = 0, b and x FP64; graphs captured under
-  // another tag are dropped first (the system's state they replay changed)
-  void solve(const double *b, double *x, uint64_t state_tag, hipStream_t s);
+  // x = A^{-1} b per problem, b and x FP64, from x = 0 or (guess) from x as
+  // given; rtol (np values, may be null) replaces opt.rtol per problem.
+  // Graphs captured under another tag are dropped first (the system's state
+  // they replay changed)
+  void solve(const double *b, double *x, uint64_t state_tag, hipStream_t s, bool guess = false,
+             const double *rtol = nullptr);
   template <class T> void cycle_start(const T *b, const T *x, hipStream_t s);
   template <class T> void cycle_steps(int j0, int j1, hipStream_t s);
   template <class T> void cycle_end(T *x, hipStream_t s);

@@ -84,6 +84,8 @@ struct BoxArgs {
 
 // the strip-march unit's band length along the last axis
 constexpr int kStripRows = 64;
+// blocks of the scalar-flux change reduction (Operator::set_scalar_flux)
+constexpr int kMeasureBlocks = 512;
 
 struct Operator {
   std::shared_ptr<Disc> disc;
@@ -120,6 +122,11 @@ struct Operator {
   DevBuf sigs_d;              // double, G*G*cells (transfer sources)
   DevBuf phi_cache;           // double, G*nodes
   std::vector<char> phi_set;
+  // the first group anything upscatters into: the smallest g_to of a nonzero
+  // sigma_s[g_from][g_to], g_from > g_to, in any cell (n_groups without upscatter)
+  int g_up = 1;
+  DevBuf phi_tmp;             // double, nodes: a measured scalar flux before it overwrites the cache
+  DevBuf measure_part;        // double, 2 per reduction block
   DevBuf flag;                // int scratch for setup checks
 
   template <class T> OpArgs<T> args() const;
@@ -131,9 +138,15 @@ struct Operator {
   // phi (FP64, one moment) from psi of either precision
   // unit: the unweighted sum over angles (the transposed prolongation)
   void moments_f64(int prec, const void *psi, double *phi, hipStream_t s, bool unit = false) const;
-  void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s);
+  // sums (device, 2 doubles) or null: adds ||phi_new - phi_old||^2 and
+  // ||phi_new||^2 over the group's nodes (the outer iteration's change)
+  void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s, double *sums = nullptr);
+  // a zero scalar flux cached for group g
+  void cache_zero_flux(int g, hipStream_t s);
   void scatter_source(int g_from, int g_to, int prec, void *b, hipStream_t s) const;
-  void rhs(int g, int prec, void *b, hipStream_t s) const;
+  // all_groups: transfers from every group but g (a thermal group under
+  // upscatter), else from the groups below g
+  void rhs(int g, int prec, void *b, hipStream_t s, bool all_groups = false) const;
   // time every applicable apply variant on scratch vectors and keep the fastest
   void tune();
 };

@@ -6,6 +6,7 @@
 
 #include "dsa.hpp"
 #include "fgmres.hpp"
+#include "groups.hpp"
 #include "sweep.hpp"
 
 namespace ubt {
@@ -26,12 +27,15 @@ struct Solver : KrylovSystem {
   int iterations = 0, outer = 0, reason = 0;
   double rel_residual = 0.0;
   std::vector<double> history;  // relative residual estimate per Krylov iteration (per refinement step when mixed)
+  GroupReport groups;           // the last group sweep (solve_groups)
 
   void create(Operator *o, DSA *d, const SolverOptions &opts);
   void refresh(hipStream_t s);  // per group: block-inverse check
   bool transpose = false;  // the current solve is with A^T (the adjoint)
   // x = A^{-1} b, or (adj) x = A^{-T} b with every preconditioner stage transposed
-  void solve(const double *b, double *x, hipStream_t s, bool adj = false);
+  // guess: start from x as given (else from zero)
+  void solve(const double *b, double *x, hipStream_t s, bool adj = false, bool guess = false);
+  void set_rtol(double rtol) { opt.rtol = eng.opt.rtol = rtol; }
   // the preconditioner's work vectors exist in this precision
   bool has_precision(int prec) const { return prec == UBT_FP64 ? t64.ptr != nullptr : t32.ptr != nullptr; }
   template <class T> void apply_pc(const T *r, T *y, hipStream_t s);

@@ -429,59 +429,24 @@ def load(path):
     return ProblemFile(path)
 
 
-def solve(pf, rtol=1e-8, mixed=False, dsa=False, max_it=500, outer_rtol=1e-8, max_outer=100, stream=None):
-    """The reference driver's group sweep (tests/transportk.kokkos.cxx:1036-1156)
-    on the GPU: per group the rhs (ghost inflow + source + transfers from the
-    groups solved so far), a right-preconditioned FGMRES solve, the scalar flux
-    cached for the groups below. With upscatter the thermal groups (from the
-    first group anything upscatters into) are swept again, the upscatter
-    transfers from the previous pass added, until the scalar flux changes by at
-    most outer_rtol (src/multigroupk.kokkos.cxx:373-498). Returns the angular
-    flux per group (device tensors) and the solver results."""
-    import torch
+def solve(pf, rtol=None, mixed=False, dsa=False, max_it=500, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01,
+          stream=None):
+    """The reference driver's group sweep (tests/transportk.kokkos.cxx:925-1208)
+    in the library (ubt_solver_solve_groups): per group the rhs (ghost inflow +
+    source + transfers from the groups solved so far), a right-preconditioned
+    FGMRES solve, the scalar flux cached for the groups below; with upscatter
+    the thermal block swept again until the estimated error of its scalar flux
+    is below outer_rtol. rtol None: the solves run at 1e-8, or under upscatter
+    at the driver's 1e-10; a given rtol is every solve's (as -ksp_rtol).
+    Returns the angular flux per group (device tensors) and the sweep's
+    report."""
     from . import Solver, DSA
 
     prob = pf.problem
-    G = prob.n_groups
     op = prob.operator()
     d = DSA(op) if dsa else None
-    sol = Solver(op, d, rtol=rtol, max_it=max_it, mixed=mixed, pc_dsa=dsa)
-    ss = prob.sigma_s
-    up_from = [[gf for gf in range(g + 1, G) if np.any(ss[gf, g] != 0)] for g in range(G)]
-    first_thermal = min([g for g in range(G) if up_from[g]], default=G)
-    psi = [None] * G
-    results = []
-    phi_prev = None
-    for outer in range(max_outer if first_thermal < G else 1):
-        groups = range(G) if outer == 0 else range(first_thermal, G)
-        for g in groups:
-            op.set_group(g)
-            if d is not None:
-                d.set_group(g)
-            sol.refresh()
-            b = op.vec()
-            op.rhs(g, b, stream)
-            for gf in up_from[g]:
-                if psi[gf] is not None:
-                    op.scatter_source(gf, g, b, stream)
-            x = op.vec() if psi[g] is None else psi[g].clone()
-            res = sol.solve(b, x, stream)
-            results.append((outer, g, res))
-            op.set_scalar_flux(g, x, stream)
-            psi[g] = x
-        if first_thermal >= G:
-            break
-        phi = torch.cat([_phi(op, psi[g]) for g in range(first_thermal, G)])
-        if phi_prev is not None:
-            change = float(torch.linalg.norm(phi - phi_prev) / torch.linalg.norm(phi))
-            if change <= outer_rtol:
-                break
-        phi_prev = phi
-    return psi, results
+    sol = Solver(op, d, rtol=1e-8 if rtol is None else rtol, max_it=max_it, mixed=mixed, pc_dsa=dsa)
+    psi, report = sol.solve_groups(outer_rtol=outer_rtol, max_outer=max_outer, inner_gamma=inner_gamma,
+                                   inner_rtol=0.0 if rtol is None else rtol, stream=stream)
+    return [psi[g] for g in range(prob.n_groups)], report
 
-
-def _phi(op, psi):
-    import torch
-    phi = torch.zeros(op.n_nodes, dtype=torch.float64, device="cuda")
-    op.moments(psi, phi)
-    return phi

@@ -80,15 +80,13 @@ class TransportModel:
     def forward_solve(self, sigma_t, sigma_s, source):
         """psi [G, rows] and phi [G, nodes] (FP64, device)"""
         self._set(sigma_t, sigma_s, source)
-        G, n = self.G, self.disc.n_rows
-        psi = torch.zeros(G, n, dtype=torch.float64, device="cuda")
-        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        G = self.G
+        # the library's group sweep: downscatter only, so one forward sweep
+        psi, rep = self.solver.solve_groups()
         self.last_results = []
         for g in range(G):
-            self._group(g)
-            self.op.rhs(g, b)
-            self._check(self.solver.solve(b, psi[g]), "forward", g)
-            self.op.set_scalar_flux(g, psi[g])
+            self._check({"iterations": rep["group_iterations"][g], "rel_residual": rep["group_rel_residual"][g],
+                         "reason": rep["group_reason"][g]}, "forward", g)
         phi = (psi.view(G, -1, self.A) * self.w).sum(-1)
         return psi, phi
 
