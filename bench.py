@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--delaunay", action="store_true", help="run the single-problem Delaunay workload alone")
     ap.add_argument("--moments", action="store_true",
                     help="time the angular integral (ubt_moments, MFMA) on the 128^3 S8 structured box")
+    ap.add_argument("--fd", action="store_true",
+                    help="time every 3D FD apply kernel on the 128^3 S8 structured box")
     ap.add_argument("--batch-n", type=int, default=64, help="problems in the ensemble")
     ap.add_argument("--sens-n", type=int, default=64, help="hexes per side of the sensitivity workload")
     ap.add_argument("--sens-cells", type=int, default=32, help="finite-difference checks (one cell each)")
@@ -524,6 +526,39 @@ def moments(args):
     print(json.dumps(out))
 
 
+def fd_apply(args):
+    """The 3D FD apply (streaming + removal + scatter, one group) over the
+    128^3 S8 structured box (2.1 M cells x 80 angles): every applicable apply
+    kernel timed in FP32 and FP64; algorithmic bytes = psi read once + y
+    written once."""
+    import torch
+    import ubolt_amd as u
+    from ubolt_amd import problems as P
+    prob = P.fd_cube_multigroup(n=128, sn=8, G=1)
+    op = prob.operator()
+    n = prob.disc.n_rows
+    tuning = op.tuning()
+    stream = torch.cuda.current_stream()
+    out = {"metric": "FD apply GB/s (algorithmic)", "workload": "fd_box_128x128x128_s8_g1", "n_dof": n,
+           "steps": args.steps, "tuned": {"fp64": tuning["variant_fp64"], "fp32": tuning["variant_fp32"]},
+           "runs": []}
+    nv = len(tuning["us"]) // 2
+    for prec, dt, pi in ((u.FP64, torch.float64, 0), (u.FP32, torch.float32, 1)):
+        x = torch.rand(n, dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        for v in range(nv):
+            if tuning["us"][pi * nv + v] <= 0:
+                continue
+            op.set_variant(prec, v)
+            sec, _ = time_applies(op.mult, x, y, args.steps, args.warmup, stream)
+            alg = 2 * n * x.element_size()
+            out["runs"].append({"prec": "fp64" if pi == 0 else "fp32", "variant": v, "us": round(sec * 1e6, 1),
+                                "achieved_gbs": round(alg / sec / 1e9, 1), "frac_of_8tbs": round(alg / sec / 8e12, 3)})
+        op.set_variant(prec, -1)
+        del x, y
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -536,6 +571,8 @@ def main():
         delaunay(args)
     elif args.moments:
         moments(args)
+    elif args.fd:
+        fd_apply(args)
     else:
         ensemble(args)
 

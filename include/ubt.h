@@ -255,14 +255,17 @@ int ubt_op_diagonal(ubt_op *op, int prec, void *d, void *stream);
  * interior class folded per angle into three small matrices, 2 or 4 rows
  * per thread in registers), 10 the strip march (a uniform DG1 box: a
  * workgroup walks a strip of cells along the last axis with the neighbouring
- * rows in registers); -1 re-runs the autotuner, which times every applicable
+ * rows in registers), 11 and 12 the z-march (3D FD: a workgroup walks
+ * chunks of 32 planes along z with the z upwind plane in registers and the
+ * x neighbours through LDS, 4 or 8 cells per lane column); -1 re-runs the
+ * autotuner, which times every applicable
  * kernel on scratch vectors when the cross sections are first set and keeps
  * the fastest. A variant that does not apply falls back to 0. No reference
  * counterpart (the reference has one Kokkos SpMV); exposed so the tests can
  * check every kernel. */
 int ubt_op_set_variant(ubt_op *op, int prec, int variant);
-/* variant[2] (FP64, FP32) in use and us[22] the tuned times (FP64 variants
- * 0-10, then FP32 variants 0-10; 0 where not applicable). */
+/* variant[2] (FP64, FP32) in use and us[26] the tuned times (FP64 variants
+ * 0-12, then FP32 variants 0-12; 0 where not applicable). */
 int ubt_op_tuning(const ubt_op *op, int *variant, float *us);
 int ubt_op_destroy(ubt_op *op);
 

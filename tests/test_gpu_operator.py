@@ -76,9 +76,10 @@ def built(request):
 
 # tile, wavefront per cell, multi-cell tile (2 and 4 cells per thread), pencil on
 # a structured box (8, 16, 32 cells per workgroup), four-angle tile, box tile
-# (2 and 4 rows per thread); a variant that does not apply to a case runs the
+# (2 and 4 rows per thread), strip march, z-march on a 3D FD box (4 and 8
+# cells per lane column); a variant that does not apply to a case runs the
 # tile kernel
-VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10]
+VARIANTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -130,7 +131,7 @@ def test_tuner_reports(gpu, built):
     op.set_variant(core.FP64, -1)
     t = op.tuning()
     print(f"{name}: tuned {t}")
-    assert 0 <= t["variant_fp64"] < 11 and 0 <= t["variant_fp32"] < 11 and t["us"][0] > 0
+    assert 0 <= t["variant_fp64"] < 13 and 0 <= t["variant_fp32"] < 13 and t["us"][0] > 0
 
 
 def test_diagonal(gpu, built):
@@ -251,10 +252,10 @@ def test_box_tile_apply(gpu, case):
     prob = BOX_CASES[case]()
     op = prob.operator()
     us = op.tuning()["us"]
-    assert all(us[v] > 0 for v in (8, 9, 19, 20)), f"box tile not applicable: {us}"
+    assert all(us[v] > 0 for v in (8, 9, 21, 22)), f"box tile not applicable: {us}"
     # the strip march holds one halo element (nb x angles of a cell) per thread
     strip = prob.disc.nb * prob.disc.n_angles <= 256
-    assert (us[10] > 0 and us[21] > 0) == strip, f"strip march applicability: {us}"
+    assert (us[10] > 0 and us[23] > 0) == strip, f"strip march applicability: {us}"
     x = np.random.default_rng(4).standard_normal(prob.disc.n_rows)
     want = oracle_mult(prob, x)
     want_l = oracle_mult(prob, x, scatter=False)
@@ -268,4 +269,47 @@ def test_box_tile_apply(gpu, case):
         torch.cuda.synchronize()
         e64, e32, el = rel_max(to_host(y), want), rel_max(to_host(y32), want), rel_max(to_host(yl), want_l)
         print(f"{case} variant {v}: fp64 {e64:.2e} fp32 {e32:.2e} unscattered {el:.2e}")
+        assert e64 <= FP64_TOL and el <= FP64_TOL and e32 <= FP32_TOL
+
+
+# the z-march (3D FD): runs of lanes that end inside the box's last x tile,
+# several z chunks with a partial last one, every face reflective or with
+# inflow, S4 (ten lane columns per workgroup), S8 (three) and S12 (one)
+ZM_CASES = {
+    "fd3d_zm_s4_vac": lambda: _fd3d_shape([13, 7, 70], 4, None),
+    "fd3d_zm_s8_reflect": lambda: _fd3d_shape([13, 9, 37], 8, {1: "reflect", 2: "reflect", 3: "reflect",
+                                                                4: "reflect", 5: "reflect", 6: "reflect"}),
+    "fd3d_zm_s12_mixed": lambda: _fd3d_shape([5, 6, 33], 12, {2: "reflect", 3: {"type": "vacuum", "inflow": 1.5},
+                                                              5: "reflect"}),
+}
+
+
+def _fd3d_shape(n, sn, bcs):
+    q = core.Quadrature(3, sn)
+    d = core.Disc.structured(q, n, [1.0, 0.7, 2.1], bcs if bcs is not None else P.VACUUM_3D)
+    rng = np.random.default_rng(17)
+    nc = d.n_cells
+    st = rng.uniform(0.5, 5.0, nc)
+    return P.Problem("fd3d_zm", q, d, st, rng.uniform(0.1, 0.9, nc) * st, np.ones(nc))
+
+
+@pytest.mark.parametrize("case", sorted(ZM_CASES))
+def test_fd_zmarch_apply(gpu, case):
+    prob = ZM_CASES[case]()
+    op = prob.operator()
+    us = op.tuning()["us"]
+    assert all(us[v] > 0 for v in (11, 12, 24, 25)), f"z-march not applicable: {us}"
+    x = np.random.default_rng(5).standard_normal(prob.disc.n_rows)
+    want = oracle_mult(prob, x)
+    want_l = oracle_mult(prob, x, scatter=False)
+    for v in (11, 12):
+        op.set_variant(core.FP64, v)
+        op.set_variant(core.FP32, v)
+        y, y32, yl = op.vec(), op.vec(core.FP32), op.vec()
+        op.mult(to_dev(x), y)
+        op.mult(to_dev(x, torch.float32), y32)
+        op.mult_unscattered(to_dev(x), yl)
+        torch.cuda.synchronize()
+        e64, e32, el = rel_max(to_host(y), want), rel_max(to_host(y32), want), rel_max(to_host(yl), want_l)
+        print(f"[case] {case} variant {v}: fp64 {e64:.2e} fp32 {e32:.2e} unscattered {el:.2e}", flush=True)
         assert e64 <= FP64_TOL and el <= FP64_TOL and e32 <= FP32_TOL

@@ -174,12 +174,13 @@ class Operator(_Handle):
     def set_variant(self, prec, variant):
         """apply kernel: 0 tile, 1 wavefront per cell, 2/3 multi-cell tile (2/4 cells per thread),
         4/5/6 pencil on a structured box (8/16/32 cells per workgroup), 7 four-angle tile, 8/9 box tile on a
-        uniform DG1 box (2/4 rows per thread), 10 strip march on a uniform DG1 box, -1 re-tune"""
+        uniform DG1 box (2/4 rows per thread), 10 strip march on a uniform DG1 box, 11/12 z-march on a 3D FD box
+        (4/8 cells per lane column), -1 re-tune"""
         call("ubt_op_set_variant", self.h, prec, variant)
 
     def tuning(self):
         v = (C.c_int * 2)()
-        us = (C.c_float * 22)()
+        us = (C.c_float * 26)()
         call("ubt_op_tuning", self.h, v, us)
         return {"variant_fp64": v[0], "variant_fp32": v[1], "us": [round(x, 2) for x in us]}
 

@@ -1091,6 +1091,161 @@ __device__ inline void seg_wsum(const T *red, const T *ws, int nseg, int len, T 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Z-march structured apply (3D FD; reference src/termsk.kokkos.cxx:70-98
+// streaming, :480-509 removal, :584-649 scatter). A workgroup owns a run of
+// XR = R * CPT cells along x at one y index and walks a chunk of zc planes
+// along z; a thread (lane column, angle) holds CPT consecutive cells of each
+// plane. Planes are loaded two ahead, so the z upwind trace - the previous
+// plane's own trace for a positive cosine, the next plane's for a negative
+// one - is already in registers; the x upwind trace is a neighbouring cell of
+// the lane's run, or of the neighbouring lane column through the LDS copy the
+// moments are summed from (one global halo load per plane at the run's
+// inflow edge); the y upwind trace is a load of the neighbouring row, which
+// that row's workgroup reads as its own trace at about the same time, so it
+// comes through L2. psi is read from HBM about once (plus one plane per chunk
+// on each side). Every load is clamped into the grid, so none sits under a
+// branch; a missing neighbour's weight is zero.
+// ---------------------------------------------------------------------------
+template <class T, int CPT> struct ZPlane {
+  T p[CPT], yu[CPT];
+  T hx;
+  T sg;  // one cross section of the plane's run (threads below 2 XR), staged through LDS
+};
+
+template <class T, int CPT, bool SCATTER>
+__global__ void __launch_bounds__(256) fd_apply_zm(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g, int zc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int A = g.A, R = g.cpb, XR = R * CPT;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int tiles_x = (nx + XR - 1) / XR, chunks = (nz + zc - 1) / zc;
+  const int64_t n_tiles = (int64_t)tiles_x * ny * chunks;
+  const int64_t t = xcd_tile(n_tiles);
+  if (t >= n_tiles) return;
+  T *ws = reinterpret_cast<T *>(smem);
+  T *buf = ws + ((A + 1) & ~1);
+  T *phi = buf + (int64_t)XR * A;
+  T *sgl = phi + XR;  // [2 planes][sigma_t, sigma_s][XR], by the plane's parity
+  for (int e = threadIdx.x; e < A; e += blockDim.x) ws[e] = ldg(g.w + e);
+  const int tx = (int)(t % tiles_x);
+  const int64_t rest = t / tiles_x;
+  const int j = (int)(rest % ny), ch = (int)(rest / ny);
+  const int z0 = ch * zc, z1 = z0 + zc < nz ? z0 + zc : nz;
+  const int lc = (int)threadIdx.x / A;
+  const bool lane = lc < R;
+  const int a = lane ? (int)threadIdx.x - lc * A : 0;
+  const int xb = tx * XR + (lane ? lc : 0) * CPT;
+  T cs[3], cd[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    cs[d] = ldg(g.omega + 3 * a + d);
+    cd[d] = fabs(cs[d]) * g.invh[d];
+  }
+  const bool xp = cs[0] > T(0), yp = cs[1] > T(0), zp = cs[2] > T(0);
+  const int ju = yp ? j - 1 : j + 1;
+  const bool yin = cs[1] != T(0) && ju >= 0 && ju < ny;
+  const int jl = yin ? ju : j;
+  int xc[CPT];
+  bool xv[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    xv[k] = lane && xb + k < nx;
+    xc[k] = xb + k < nx ? xb + k : nx - 1;
+  }
+  // the cell beyond the run on its x inflow side: a global load where it lies
+  // outside the workgroup, else the neighbouring lane column's cell in LDS
+  const int xh = xp ? xb - 1 : xb + CPT;
+  const bool xhin = cs[0] != T(0) && xh >= 0 && xh < nx;
+  const bool xhalo = xhin && (xp ? lc == 0 : lc == R - 1);
+  const int xhc = xhalo ? xh : xc[0];
+  const int nbl = xp ? lc - 1 : lc + 1;
+  const int nbk = xp ? CPT - 1 : 0;
+  // a plane's rows start at the (uniform) plane base; the lane's offsets
+  // inside a plane are the same on every plane and fit 32 bits (the
+  // launcher checks), so every load is a scalar base plus a 32-bit offset
+  const int64_t prow = (int64_t)nx * ny * A, pcell = (int64_t)nx * ny;
+  constexpr unsigned TS = sizeof(T);
+  unsigned op[CPT], oy[CPT];  // byte offsets in a plane
+#pragma unroll
+  for (int k = 0; k < CPT; k++) {
+    op[k] = (unsigned)((((int64_t)j * nx + xc[k]) * A + a) * TS);
+    oy[k] = (unsigned)((((int64_t)jl * nx + xc[k]) * A + a) * TS);
+  }
+  const unsigned oh = (unsigned)((((int64_t)j * nx + xhc) * A + a) * TS);
+  // the cross sections of the run: thread i < XR loads sigma_t of cell i,
+  // XR <= i < 2 XR sigma_s (threads past that load sigma_t of the run's
+  // first cell, unused)
+  const int si = (int)threadIdx.x < 2 * XR ? (int)threadIdx.x : 0;
+  const int sx0 = tx * XR + (si < XR ? si : si - XR);
+  const unsigned os = (unsigned)(((int64_t)j * nx + (sx0 < nx ? sx0 : nx - 1)) * TS);
+  const T *sgp = SCATTER && si >= XR ? g.sigs : g.sigt;
+  auto at = [](const T *base, unsigned off) -> T {
+    return ldg(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off));
+  };
+  auto load = [&](ZPlane<T, CPT> &s, int z) {
+    const T *xz = x + (int64_t)z * prow;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+      s.p[k] = at(xz, op[k]);
+      s.yu[k] = at(xz, oy[k]);
+    }
+    s.hx = at(xz, oh);
+    s.sg = at(sgp + (int64_t)z * pcell, os);
+  };
+  T prv[CPT];
+  {
+    const T *xq = x + (int64_t)(z0 > 0 ? z0 - 1 : 0) * prow;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) prv[k] = at(xq, op[k]);
+  }
+  ZPlane<T, CPT> s0, s1, s2;
+  load(s0, z0);
+  load(s1, z0 + 1 < nz ? z0 + 1 : nz - 1);
+  const T diag = cd[0] + cd[1] + cd[2];
+  for (int z = z0; z < z1; z++) {
+    load(s2, z + 2 < nz ? z + 2 : nz - 1);
+    if (lane) {
+#pragma unroll
+      for (int k = 0; k < CPT; k++) buf[(lc * CPT + k) * A + a] = s0.p[k];
+    }
+    // the plane's cross sections, in the half of the staging its parity owns:
+    // the half written two planes ago is no longer read (every thread has
+    // passed this plane's first barrier)
+    T *sgz = sgl + (z & 1) * 2 * XR;
+    if ((int)threadIdx.x < 2 * XR) sgz[threadIdx.x] = s0.sg;
+    __syncthreads();
+    T xl = s0.hx;
+    if (lane && !xhalo && nbl >= 0 && nbl < R) xl = buf[(nbl * CPT + nbk) * A + a];
+    if (SCATTER) seg_wsum(buf, ws, XR, A, phi, g.red_lanes);
+    __syncthreads();
+    const bool zin = cs[2] != T(0) && (zp ? z > 0 : z < nz - 1);
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+      if (!xv[k]) continue;
+      const int ck = lc * CPT + k;
+      T v = (sgz[ck] + diag) * s0.p[k];
+      const bool xi = cs[0] != T(0) && (xp ? xc[k] > 0 : xc[k] < nx - 1);
+      const T ux = xp ? (k > 0 ? s0.p[k > 0 ? k - 1 : 0] : xl) : (k < CPT - 1 ? s0.p[k < CPT - 1 ? k + 1 : 0] : xl);
+      if (xi) v -= cd[0] * ux;
+      if (yin) v -= cd[1] * s0.yu[k];
+      if (zin) v -= cd[2] * (zp ? prv[k] : s1.p[k]);
+      if (SCATTER) v -= sgz[XR + ck] * g.inv_sum_w * phi[ck];
+      // reflective faces: the cell's own reflected angle, on a branch only
+      // boundary cells enter
+      const T *xc0 = x + (int64_t)z * prow + (op[k] / TS - a);
+      const bool bx = cs[0] != T(0) && !xi, by = cs[1] != T(0) && !yin, bz = cs[2] != T(0) && !zin;
+      if (bx && g.face_type[0][xp ? 0 : 1] == kReflect) v -= cd[0] * xc0[g.reflect[3 * a]];
+      if (by && g.face_type[1][yp ? 0 : 1] == kReflect) v -= cd[1] * xc0[g.reflect[3 * a + 1]];
+      if (bz && g.face_type[2][zp ? 0 : 1] == kReflect) v -= cd[2] * xc0[g.reflect[3 * a + 2]];
+      *reinterpret_cast<T *>(reinterpret_cast<char *>(y + (int64_t)z * prow) + op[k]) = v;
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; k++) prv[k] = s0.p[k];
+    s0 = s1;
+    s1 = s2;
+  }
+}
+
 template <class T, int DIM, int NB, int K, bool SCATTER>
 __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict__ y, const OpArgs<T> &g,
                                          const BoxArgs &bx, const BoxConst<T, DIM, NB> &kc, int64_t t, T *buf,
@@ -2454,6 +2609,42 @@ bool launch_apply_fd_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool 
   return false;
 }
 
+// The z-march apply (3D FD): planes walked in chunks of 32 per workgroup,
+// 4 (cpt8 false) or 8 cells per lane column
+template <class T>
+bool launch_apply_fd_zm(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, bool cpt8,
+                        bool dry = false) {
+  const Disc &D = *op.disc;
+  const int A = D.n_angles;
+  if (D.kind != DISC_FD || D.dim != 3 || A > 256) return false;
+  const int CPT = cpt8 ? 8 : 4;
+  const int R = 256 / A, XR = R * CPT;
+  const int threads = ((R * A + 63) / 64) * 64;
+  const size_t lds = (size_t)(((A + 1) & ~1) + XR * A + XR + 4 * XR) * sizeof(T);
+  // a plane's rows are addressed with 32-bit offsets
+  const int64_t prow = (int64_t)D.n[0] * D.n[1] * A;
+  if (lds > 65536 || 2 * XR > threads || prow * (int64_t)sizeof(T) >= ((int64_t)1 << 31)) return false;
+  if (dry) return true;
+  int S = pow2_floor(std::max(1, threads / XR));
+  S = std::min(std::min(S, 64), pow2_floor(A));
+  a.cpb = R;
+  a.red_lanes = std::max(1, S);
+  const int nz = D.n[2];
+  const int zc = std::min(nz, 32);
+  const int64_t n_tiles = (int64_t)((D.n[0] + XR - 1) / XR) * D.n[1] * ((nz + zc - 1) / zc);
+  const unsigned grid = (unsigned)(8 * ((n_tiles + 7) / 8));
+#define ZM(C)                                                                                                \
+  if (CPT == C) {                                                                                            \
+    if (scatter)                                                                                             \
+      hipLaunchKernelGGL((fd_apply_zm<T, C, true>), dim3(grid), dim3(threads), lds, s, x, y, a, zc);         \
+    else                                                                                                     \
+      hipLaunchKernelGGL((fd_apply_zm<T, C, false>), dim3(grid), dim3(threads), lds, s, x, y, a, zc);        \
+  }
+  ZM(4) ZM(8)
+#undef ZM
+  return true;
+}
+
 template <class T>
 bool launch_apply_mc(const Operator &op, OpArgs<T> a, const T *x, T *y, bool scatter, hipStream_t s, int cpt) {
   const Disc &D = *op.disc;
@@ -2512,6 +2703,10 @@ void launch_apply(const Operator &op, const OpArgs<T> &a0, const T *x, T *y, boo
     return;
   }
   if (!adj && (var == 2 || var == 3) && launch_apply_mc<T>(op, a, x, y, scatter, s, var == 2 ? 2 : 4)) {
+    UBT_HIP(hipGetLastError());
+    return;
+  }
+  if (!adj && (var == 11 || var == 12) && launch_apply_fd_zm<T>(op, a, x, y, scatter, s, var == 12)) {
     UBT_HIP(hipGetLastError());
     return;
   }
@@ -2971,6 +3166,9 @@ void Operator::tune() {
     if (v == 10)
       return p == 0 ? launch_apply_strip<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
                     : launch_apply_strip<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);
+    if (v == 11 || v == 12)
+      return p == 0 ? launch_apply_fd_zm<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, v == 12, true)
+                    : launch_apply_fd_zm<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, v == 12, true);
     if (v == 7)
       return p == 0 ? launch_apply_q4<double>(*this, args<double>(), nullptr, nullptr, true, nullptr, true)
                     : launch_apply_q4<float>(*this, args<float>(), nullptr, nullptr, true, nullptr, true);

@@ -51,8 +51,9 @@ struct PrecTables {
 // 2 and 3 the multi-cell tile with 2 and 4 cells per thread, 4-6 the pencil
 // kernel of a structured box marching 8, 16 or 32 cells per workgroup, 7 the
 // four-angle tile, 8 and 9 the box tile with 2 and 4 rows per thread, 10 the
-// strip march on a uniform DG1 box
-constexpr int kApplyVariants = 11;
+// strip march on a uniform DG1 box, 11 and 12 the z-march on a 3D FD box
+// (4 or 8 cells per lane column)
+constexpr int kApplyVariants = 13;
 constexpr int kPencilMaxClasses = 64;
 
 // The box-tile kernels' view of a uniform DG1 box: a tile is W columns along
