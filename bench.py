@@ -488,14 +488,21 @@ def ensemble(args):
 
 
 def copy_reference():
-    """a plain 64 MiB device copy, 5 times: the known-size reference the HBM
-    traffic counters are scaled by"""
+    """a plain 1 GiB device copy from a cold source, 3 times: the known-size
+    reference the HBM traffic counters are checked against. Before each copy
+    another 1 GiB buffer is written, so no part of the source is left in the
+    L2 or the Infinity Cache (a small or just-written source is partly read
+    from cache, and the copy then under-reports the counters' scale)."""
     import torch
-    src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    n = 256 * 1024 * 1024
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(5):
+    flush = torch.empty_like(src)
+    for _ in range(3):
+        flush.fill_(2.0)
         dst.copy_(src)
     torch.cuda.synchronize()
+    del src, dst, flush
 
 
 def moments(args):
@@ -556,6 +563,8 @@ def fd_apply(args):
                                 "achieved_gbs": round(alg / sec / 1e9, 1), "frac_of_8tbs": round(alg / sec / 8e12, 3)})
         op.set_variant(prec, -1)
         del x, y
+    if args.copy_ref:
+        copy_reference()
     print(json.dumps(out))
 
 

@@ -1253,10 +1253,19 @@ __device__ __forceinline__ void box_tile(const T *__restrict__ x, T *__restrict_
   constexpr int R = DIM - 1;
   const int A = g.A, W = bx.W;
   const int nx = bx.n[0], ny = bx.n[1], nR = bx.n[R];
-  const int tx = (int)(t % bx.tiles[0]);
-  const int64_t rest = t / bx.tiles[0];
-  const int j = DIM == 3 ? (int)(rest % bx.tiles[1]) : 0;
-  const int r0 = (int)(DIM == 3 ? rest / bx.tiles[1] : rest) * K;
+  // tiles walk bands of kBoxBand row blocks along the register axis, the
+  // blocks of a band one after another at each x (and y) position: the row
+  // two blocks share is read by the second while the first's read of it is
+  // still in L2, so only the band's two outer halo rows come from HBM
+  const int64_t per_band = (int64_t)bx.tiles[0] * bx.tiles[1] * kBoxBand;
+  const int band = (int)(t / per_band);
+  int64_t idx = t - (int64_t)band * per_band;
+  const int nbh = bx.tiles[2] - band * kBoxBand < kBoxBand ? bx.tiles[2] - band * kBoxBand : kBoxBand;
+  const int rbi = (int)(idx % nbh);
+  idx /= nbh;
+  const int tx = (int)(idx % bx.tiles[0]);
+  const int j = DIM == 3 ? (int)(idx / bx.tiles[0]) : 0;
+  const int r0 = (band * kBoxBand + rbi) * K;
   const int i = tx * W + lc;
   const bool lane = lc < W;
   const bool col = lane && i < nx;

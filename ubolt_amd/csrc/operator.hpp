@@ -58,12 +58,14 @@ constexpr int kPencilMaxClasses = 64;
 
 // The box-tile kernels' view of a uniform DG1 box: a tile is W columns along
 // x (thread = column, angle) by K rows along the last axis held in
-// registers, at one y index in 3D. Tiles are numbered x fastest, then y,
-// then the register axis.
+// registers, at one y index in 3D. Tiles are numbered in bands of kBoxBand
+// row blocks: within a band the blocks of one x (and y) position are
+// consecutive, then x, then y.
 // The cells on the box's boundary (and any of another class) are left to a
 // second phase of the same launch that runs the tile kernel's body on their
 // list.
 constexpr int kBoxThreads = 256;  // box-tile workgroup: W columns x A angles fill at most this
+constexpr int kBoxBand = 2;  // row blocks per band of the box-tile walk (1, 2, 4 measured: 2 and 4 equal, 2-3% ahead of 1)
 
 struct BoxArgs {
   int n[3];         // cells per axis
