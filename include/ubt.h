@@ -282,7 +282,7 @@ int ubt_set_scalar_flux(ubt_op *op, int g, int prec, const void *psi, void *stre
  * Replaces GroupTransfer::add_transfer (src/multigroupk.kokkos.cxx:306-364). */
 int ubt_scatter_source(ubt_op *op, int g_from, int g_to, int prec, void *b, void *stream);
 /* b = ghost inflow + isotropic source (basis 0) + the transfers from every
- * group g' < g, in one launch: the driver's rhs for group g
+ * group g' < g, in one launch: the reference test driver's rhs for group g
  * (tests/transportk.kokkos.cxx:1058-1065; UboltFillInflow
  * src/discretisationk.kokkos.cxx:187-206; add_isotropic
  * src/material_speck.kokkos.cxx:164-187). */

@@ -7,10 +7,11 @@ the Gauss-Seidel group sweep and the upscatter outer iteration.
   sigma_t,g phi_g - sum_g' sigma_s(g' -> g) phi_g' = S_g, which the outer
   iteration must reach (the reference's -check_inf_medium oracle) within the
   reference's pinned sweep count (tests/Makefile: -outer_max_it 27);
-* the *_up3_layers regressions at the driver's defaults (outer_rtol 1e-8,
-  inner rtol 1e-10, inner_gamma 0.01) converge within the reference's pinned
-  sweeps (16, 15 at inner_gamma 0, 17 for DG1) to a flux whose multigroup
-  residual with the oracle's matrices is at the outer tolerance;
+* the *_up3_layers regressions at the reference test driver's defaults
+  (outer_rtol 1e-8, inner rtol 1e-10, inner_gamma 0.01) converge within the
+  reference's pinned sweeps (16, 15 at inner_gamma 0, 17 for DG1) to a flux
+  whose multigroup residual with the oracle's matrices is at the outer
+  tolerance;
 * the outer_rtol below the inner rtol is refused like the reference's;
 * a batch of problems of different group counts, with and without
   upscatter, gives every problem the flux and sweep count of its own

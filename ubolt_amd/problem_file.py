@@ -437,7 +437,8 @@ def solve(pf, rtol=None, mixed=False, dsa=False, max_it=500, outer_rtol=1e-8, ma
     FGMRES solve, the scalar flux cached for the groups below; with upscatter
     the thermal block swept again until the estimated error of its scalar flux
     is below outer_rtol. rtol None: the solves run at 1e-8, or under upscatter
-    at the driver's 1e-10; a given rtol is every solve's (as -ksp_rtol).
+    at the reference test driver's 1e-10; a given rtol is every solve's (as
+    -ksp_rtol).
     Returns the angular flux per group (device tensors) and the sweep's
     report."""
     from . import Solver, DSA
