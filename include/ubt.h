@@ -281,6 +281,11 @@ int ubt_set_scalar_flux(ubt_op *op, int g, int prec, const void *psi, void *stre
 /* b += sigma_s(g_from -> g_to) phi_{g_from} / sum_weights on every row.
  * Replaces GroupTransfer::add_transfer (src/multigroupk.kokkos.cxx:306-364). */
 int ubt_scatter_source(ubt_op *op, int g_from, int g_to, int prec, void *b, void *stream);
+/* phi (FP64 device memory, n_nodes) = the scalar flux cached for group g (the
+ * cache ubt_set_scalar_flux fills; the group sweep leaves every group's
+ * there). Reads the reference's GroupTransfer::phi_ cache
+ * (src/multigroupk.kokkos.cxx:266-299), as its flux output does. */
+int ubt_op_scalar_flux(const ubt_op *op, int g, double *phi, void *stream);
 /* b = ghost inflow + isotropic source (basis 0) + the transfers from every
  * group g' < g, in one launch: the reference test driver's rhs for group g
  * (tests/transportk.kokkos.cxx:1058-1065; UboltFillInflow
@@ -413,16 +418,23 @@ int ubt_solver_destroy(ubt_solver *solver);
  * (1 - rho) of its scalar flux is below outer_rtol; a thermal solve after the
  * first sweep starts from its last psi at max(inner rtol, inner_gamma x the
  * last change), and the iteration stops only on a sweep at the inner rtol.
- * Options replace -outer_rtol, -outer_max_it, -outer_inner_gamma and, under
- * upscatter, -ksp_rtol (tests/transportk.kokkos.cxx:606-647); the moments
- * store's reconstruction (-outer_store moments) is not offered: psi is kept
- * per group. */
+ * Options replace -outer_rtol, -outer_max_it, -outer_inner_gamma, under
+ * upscatter -ksp_rtol, and -outer_store / -outer_recon
+ * (tests/transportk.kokkos.cxx:606-680). */
 typedef struct {
   double outer_rtol;   /* e.g. 1e-8 */
   int max_outer;       /* sweeps, the first included, e.g. 1000 */
   double inner_gamma;  /* [0, 1), e.g. 0.01; 0: every solve at the inner rtol */
   double inner_rtol;   /* every solve's rtol under upscatter; 0: 1e-10. Without
                         * upscatter the solver's own rtol */
+  int store_moments;   /* 0: psi holds every group's angular flux (the angular
+                        * store); 1: the moments store - psi is ONE vector every
+                        * group's solve reuses, a thermal group keeps its scalar
+                        * flux and current, and after the first sweep its solve
+                        * starts from their P1 expansion (single problems only) */
+  int recon_steps;     /* moments store: at most this many safeguarded
+                        * Richardson steps on L_g refine the expansion, to the
+                        * solve's own rtol (0: the expansion alone; e.g. 10) */
 } ubt_group_options;
 
 typedef struct {
@@ -434,10 +446,12 @@ typedef struct {
   double rho;          /* its ratio to the sweep before's (0 before the third sweep) */
   double err_est;      /* change * rho / (1 - rho); infinite with no rho below 1 */
   int64_t iterations;  /* Krylov iterations over every solve */
+  int64_t recon_steps; /* the moments store's Richardson steps */
 } ubt_group_result;
 
 /* Every group of the solver's operator: psi (FP64 device memory) receives
- * n_groups angular fluxes of n_rows each, group g at g * n_rows. Leaves the
+ * n_groups angular fluxes of n_rows each, group g at g * n_rows (the moments
+ * store: one vector of n_rows, the last group solved). Leaves the
  * operator (and the DSA handle) at the last group solved, every group's
  * scalar flux cached (ubt_set_scalar_flux). options NULL = defaults. Returns
  * UBT_OK also when not converged: read ubt_solver_groups_result. Replaces

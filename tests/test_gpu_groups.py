@@ -177,3 +177,61 @@ def test_batch_group_sweep_matches_single(gpu):
         assert rb["status"] == 1 and rs["status"] == 1
         assert rb["g_up"] == rs["g_up"] and abs(rb["sweeps"] - rs["sweeps"]) <= 1
         assert max(diffs) <= 1e-7, diffs
+
+
+# the moments store (-outer_store moments): one psi, each thermal group's
+# solve after the first sweep warm-started from the P1 expansion of its
+# scalar flux and current (recon_steps 0, -outer_recon none) or that
+# expansion refined by Richardson steps on L_g (-outer_recon tol). Every solve
+# still stops on its own true residual, so the answer is the angular store's
+# (reference docs: the infinite-medium files stay the exact oracle and the
+# sweep counts match the angular recipes)
+@pytest.mark.parametrize("recon", [0, 10])
+def test_moments_store_infinite_medium(gpu, recon):
+    pf = load("box_up2_inf_medium")
+    prob = pf.problem
+    st, ss, q = xs_arrays(prob)
+    want = np.linalg.solve(np.diag(st[:, 0]) - ss[:, :, 0].T, q[:, 0])
+    psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-11, max_outer=27, store_moments=True,
+                         recon_steps=recon)
+    torch.cuda.synchronize()
+    print(f"[case] moments store, recon {recon}: {rep['sweeps']} sweeps, {rep['iterations']} inner iterations, "
+          f"{rep['recon_steps']} reconstruction steps", flush=True)
+    assert rep["status"] == 1 and rep["sweeps"] <= 27, rep
+    assert len(psi) == 1 and psi[0].numel() == prob.disc.n_rows
+    assert (rep["recon_steps"] > 0) == (recon > 0)
+    for g in range(prob.n_groups):
+        phi = to_host(rep["operator"].scalar_flux(g))
+        err = np.abs(phi - want[g]).max() / abs(want[g])
+        print(f"[case] group {g}: phi {want[g]:.6f}, max error {err:.2e}", flush=True)
+        assert err <= 1e-9, (g, err)
+
+
+@pytest.mark.parametrize("name,pin", [("box_up3_layers", 16), ("plex_box_up3_layers_dg1", 17)])
+def test_moments_store_matches_angular_store(gpu, name, pin):
+    pf = load(name)
+    G = pf.problem.n_groups
+    _, ra = pfm.solve(pf, max_it=800)
+    phi_a = [to_host(ra["operator"].scalar_flux(g)) for g in range(G)]
+    for recon in (0, 10):
+        _, rm = pfm.solve(pf, max_it=800, store_moments=True, recon_steps=recon)
+        torch.cuda.synchronize()
+        diff = max(np.linalg.norm(to_host(rm["operator"].scalar_flux(g)) - phi_a[g]) / np.linalg.norm(phi_a[g])
+                   for g in range(G))
+        print(f"[case] {name} recon {recon}: moments store {rm['sweeps']} sweeps / {rm['iterations']} inner "
+              f"iterations ({rm['recon_steps']} reconstruction steps), angular store {ra['sweeps']} / "
+              f"{ra['iterations']}; scalar flux difference {diff:.2e}", flush=True)
+        assert rm["status"] == 1 and rm["sweeps"] <= pin, rm
+        assert diff <= 1e-7, diff
+
+
+def test_moments_store_refused_on_a_batch(gpu):
+    import ctypes as C
+    from ubolt_amd import capi
+    ops = [load("plex_box_up3_layers_dg1").problem.operator()]
+    bs = u.BatchSolver(u.Batch(ops), rtol=1e-10)
+    G, n = ops[0].n_groups, bs.batch.n_rows
+    o = capi.GroupOptions(1e-8, 100, 0.01, 0.0, 1, 10)
+    psi = torch.zeros((G, n), dtype=torch.float64, device="cuda")
+    rc = capi.lib().ubt_batch_solver_solve_groups(bs.h, C.byref(o), C.c_void_p(psi.data_ptr()), None)
+    assert capi.STATUS.get(rc) == "ERR_SUP"

@@ -35,12 +35,13 @@ class SolverOptions(C.Structure):
 
 class GroupOptions(C.Structure):
     _fields_ = [("outer_rtol", C.c_double), ("max_outer", C.c_int), ("inner_gamma", C.c_double),
-                ("inner_rtol", C.c_double)]
+                ("inner_rtol", C.c_double), ("store_moments", C.c_int), ("recon_steps", C.c_int)]
 
 
 class GroupResult(C.Structure):
     _fields_ = [("g_up", C.c_int), ("sweeps", C.c_int), ("status", C.c_int), ("change", C.c_double),
-                ("rho", C.c_double), ("err_est", C.c_double), ("iterations", C.c_int64)]
+                ("rho", C.c_double), ("err_est", C.c_double), ("iterations", C.c_int64),
+                ("recon_steps", C.c_int64)]
 
 
 class Bc(C.Structure):
@@ -124,6 +125,7 @@ SIGNATURES = {
     "ubt_solver_result": [P, PI, PI, PD, PI, PD, I, PI],
     "ubt_solver_destroy": [P],
     "ubt_solver_solve_groups": [P, C.POINTER(GroupOptions), P, P],
+    "ubt_op_scalar_flux": [P, I, P, P],
     "ubt_solver_groups_result": [P, C.POINTER(GroupResult), PI, PD, PI],
     "ubt_batch_create": [PP, I, PP],
     "ubt_batch_size": [P, PI64, PI],

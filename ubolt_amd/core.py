@@ -171,6 +171,12 @@ class Operator(_Handle):
         call("ubt_op_set_group", self.h, g)
         self.group = g
 
+    def scalar_flux(self, g, stream=None):
+        """the scalar flux cached for group g (FP64, n_nodes)"""
+        phi = torch.empty(self.n_nodes, dtype=torch.float64, device="cuda")
+        call("ubt_op_scalar_flux", self.h, g, dptr(phi), stream_ptr(stream))
+        return phi
+
     def set_variant(self, prec, variant):
         """apply kernel: 0 tile, 1 wavefront per cell, 2/3 multi-cell tile (2/4 cells per thread),
         4/5/6 pencil on a structured box (8/16/32 cells per workgroup), 7 four-angle tile, 8/9 box tile on a
@@ -348,15 +354,19 @@ class Solver(_Handle):
         call("ubt_solver_apply_pc", self.h, Operator._prec(r), dptr(r), dptr(y), stream_ptr(stream))
         return y
 
-    def solve_groups(self, psi=None, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01, inner_rtol=0.0, stream=None):
+    def solve_groups(self, psi=None, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01, inner_rtol=0.0,
+                     store_moments=False, recon_steps=10, stream=None):
         """every group by the reference driver's group sweep (and, with
         upscatter, its outer iteration): psi (n_groups x n_rows FP64, made if
-        None) receives every group's angular flux; returns (psi, report)"""
+        None) receives every group's angular flux; with store_moments psi is
+        one vector (n_rows) and every group's scalar flux is read with
+        Operator.scalar_flux; returns (psi, report)"""
         G, n = self.op.n_groups, self.op.n_rows
+        shape = (n,) if store_moments else (G, n)
         if psi is None:
-            psi = torch.zeros((G, n), dtype=torch.float64, device="cuda")
-        assert psi.dtype == torch.float64 and psi.numel() == G * n and psi.is_contiguous()
-        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol)
+            psi = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        assert psi.dtype == torch.float64 and psi.numel() == (n if store_moments else G * n) and psi.is_contiguous()
+        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol, int(store_moments), recon_steps)
         call("ubt_solver_solve_groups", self.h, C.byref(o), dptr(psi), stream_ptr(stream))
         return psi, self.groups_result()
 
@@ -380,8 +390,8 @@ class Solver(_Handle):
 
 def _group_report(r, its, res, reason):
     return {"g_up": r.g_up, "sweeps": r.sweeps, "status": r.status, "change": r.change, "rho": r.rho,
-            "err_est": r.err_est, "iterations": r.iterations, "group_iterations": its, "group_rel_residual": res,
-            "group_reason": reason}
+            "err_est": r.err_est, "iterations": r.iterations, "recon_steps": r.recon_steps,
+            "group_iterations": its, "group_rel_residual": res, "group_reason": reason}
 
 
 class Batch(_Handle):
@@ -485,7 +495,7 @@ class BatchSolver(_Handle):
         if psi is None:
             psi = torch.zeros((G, n), dtype=torch.float64, device="cuda")
         assert psi.dtype == torch.float64 and psi.numel() == G * n and psi.is_contiguous()
-        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol)
+        o = capi.GroupOptions(outer_rtol, max_outer, inner_gamma, inner_rtol, 0, 0)
         call("ubt_batch_solver_solve_groups", self.h, C.byref(o), dptr(psi), stream_ptr(stream))
         return psi, self.groups_result()
 

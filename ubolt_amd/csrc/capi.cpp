@@ -96,6 +96,8 @@ ubt::GroupOptions group_options(const ubt_group_options *o) {
     g.max_outer = o->max_outer;
     g.inner_gamma = o->inner_gamma;
     g.inner_rtol = o->inner_rtol;
+    g.store_moments = o->store_moments;
+    g.recon_steps = o->recon_steps;
   }
   return g;
 }
@@ -108,6 +110,7 @@ void group_result(const ubt::GroupReport &r, ubt_group_result *out, int *its, do
     out->rho = r.rho;
     out->err_est = r.err_est;
     out->iterations = r.iterations;
+    out->recon_steps = r.recon_steps;
   }
   for (size_t g = 0; g < r.group_iterations.size(); g++) {
     if (its) its[g] = r.group_iterations[g];
@@ -438,6 +441,18 @@ int ubt_scatter_source(ubt_op *op, int g_from, int g_to, int prec, void *b, void
     NEED(op);
     NEED(b);
     op->op->scatter_source(g_from, g_to, prec, b, (hipStream_t)stream);
+  });
+}
+int ubt_op_scalar_flux(const ubt_op *op, int g, double *phi, void *stream) {
+  return guard([&] {
+    NEED(op);
+    UBT_CHECK(phi, UBT_ERR_ARG_NULL, "phi is required");
+    const auto &o = *op->op;
+    UBT_CHECK(g >= 0 && g < o.n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, o.n_groups);
+    UBT_CHECK(o.phi_set[g], UBT_ERR_ARG_WRONGSTATE, "no scalar flux cached for group %d", g);
+    const int64_t nodes = o.disc->n_nodes();
+    UBT_HIP(hipMemcpyAsync(phi, o.phi_cache.as<double>() + (int64_t)g * nodes, sizeof(double) * (size_t)nodes,
+                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
   });
 }
 int ubt_rhs(ubt_op *op, int g, int prec, void *b, void *stream) {

@@ -69,10 +69,10 @@ void Solver::pc(int prec, const void *r, void *y, hipStream_t s) {
     apply_pc<float>((const float *)r, (float *)y, s);
 }
 
-template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s) {
+template <class T> void Solver::apply_pc(const T *r, T *y, hipStream_t s, bool no_dsa) {
   T *t = sizeof(T) == 8 ? (T *)t64.ptr : (T *)t32.ptr;
   T *u = sizeof(T) == 8 ? (T *)u64.ptr : (T *)u32.ptr;
-  const bool with_dsa = dsa && opt.pc_dsa;
+  const bool with_dsa = dsa && opt.pc_dsa && !no_dsa;
   const bool amat = with_dsa && opt.use_amat;
   const int prec = prec_id<T>();
   bool first = true;
@@ -132,7 +132,7 @@ void Solver::solve(const double *b, double *x, hipStream_t s, bool adj, bool gue
   history = eng.history;
 }
 
-template void Solver::apply_pc<double>(const double *, double *, hipStream_t);
-template void Solver::apply_pc<float>(const float *, float *, hipStream_t);
+template void Solver::apply_pc<double>(const double *, double *, hipStream_t, bool);
+template void Solver::apply_pc<float>(const float *, float *, hipStream_t, bool);
 
 }  // namespace ubt

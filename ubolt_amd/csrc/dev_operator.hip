@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 #include <unordered_map>
 
@@ -3402,6 +3403,132 @@ void Operator::set_scalar_flux(int g, int prec, const void *psi, hipStream_t s, 
     UBT_HIP(hipGetLastError());
   }
   phi_set[g] = 1;
+}
+
+namespace {
+// psi(node, a) = phi(node) / W + sum_d p1(a, d) J(node, d), J the columns
+// 1 .. dim of the stored moments
+__global__ void expand_p1_kernel(double *__restrict__ psi, const double *__restrict__ phi,
+                                 const double *__restrict__ mom, const double *__restrict__ p1, int A, int dim,
+                                 double inv_w, int64_t rows) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const int64_t node = r / A;
+  const int a = (int)(r - node * A);
+  double v = phi[node] * inv_w;
+  for (int d = 0; d < dim; d++) v += p1[3 * a + d] * mom[node * (1 + dim) + 1 + d];
+  psi[r] = v;
+}
+
+// per block: the sum of x^2 (and a zero, the layout measure_final reads)
+__global__ void __launch_bounds__(256) sumsq_part(const double *__restrict__ x, int64_t n, double *__restrict__ part) {
+  __shared__ double sq[256];
+  double q = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) q += x[i] * x[i];
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sq[threadIdx.x] += sq[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sq[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+}  // namespace
+
+void Operator::store_moments(int g, const double *psi, hipStream_t s) {
+  UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
+  const int M = 1 + disc->dim;
+  const int64_t nodes = disc->n_nodes();
+  if (!mom_store.bytes) {
+    mom_store.alloc(sizeof(double) * (size_t)n_groups * nodes * M);
+    UBT_HIP(hipMemsetAsync(mom_store.ptr, 0, mom_store.bytes, s));
+  }
+  launch_moments<double, double>(*this, t64, psi, mom_store.as<double>() + (int64_t)g * nodes * M, M, M, s);
+}
+
+void Operator::zero_moments(int g, hipStream_t s) {
+  UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
+  const int M = 1 + disc->dim;
+  const int64_t nodes = disc->n_nodes();
+  if (!mom_store.bytes) mom_store.alloc(sizeof(double) * (size_t)n_groups * nodes * M);
+  UBT_HIP(hipMemsetAsync(mom_store.as<double>() + (int64_t)g * nodes * M, 0, sizeof(double) * (size_t)nodes * M, s));
+}
+
+void Operator::expand_p1(int g, double *psi, hipStream_t s) {
+  UBT_CHECK(g >= 0 && g < n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range, n_groups is %d", g, n_groups);
+  UBT_CHECK(phi_set[g] && mom_store.bytes, UBT_ERR_ARG_WRONGSTATE,
+            "no scalar flux and current stored for group %d to expand", g);
+  const Disc &D = *disc;
+  const int A = D.n_angles, dim = D.dim;
+  if (!p1.bytes) {
+    // T = sum_a w_a Omega_a Omega_a^T; the first moment must vanish for the
+    // expansion's scalar flux to be phi (reference multigroupk:384-433)
+    const Quadrature &q = D.quad;
+    double t[3][3] = {}, first[3] = {};
+    for (int a = 0; a < A; a++)
+      for (int d = 0; d < dim; d++) {
+        first[d] += q.w[a] * q.omega[3 * a + d];
+        for (int e = 0; e < dim; e++) t[d][e] += q.w[a] * q.omega[3 * a + d] * q.omega[3 * a + e];
+      }
+    for (int d = 0; d < dim; d++)
+      UBT_CHECK(std::fabs(first[d]) <= 1e-12 * std::fabs(q.sum_weights), UBT_ERR_ARG_INCOMP,
+                "the P1 expansion needs a quadrature whose weighted directions sum to zero: sum_a w_a Omega_a[%d] is %g",
+                d, first[d]);
+    // T^{-1} by Gauss-Jordan with partial pivoting
+    double inv[3][3] = {};
+    for (int d = 0; d < dim; d++) inv[d][d] = 1.0;
+    for (int k = 0; k < dim; k++) {
+      int piv = k;
+      for (int i = k + 1; i < dim; i++)
+        if (std::fabs(t[i][k]) > std::fabs(t[piv][k])) piv = i;
+      UBT_CHECK(t[piv][k] != 0.0, UBT_ERR_ARG_INCOMP, "the quadrature's second moment sum_a w_a Omega_a Omega_a^T is singular");
+      std::swap(t[k], t[piv]);
+      std::swap(inv[k], inv[piv]);
+      const double f = 1.0 / t[k][k];
+      for (int j = 0; j < dim; j++) {
+        t[k][j] *= f;
+        inv[k][j] *= f;
+      }
+      for (int i = 0; i < dim; i++) {
+        if (i == k) continue;
+        const double c = t[i][k];
+        for (int j = 0; j < dim; j++) {
+          t[i][j] -= c * t[k][j];
+          inv[i][j] -= c * inv[k][j];
+        }
+      }
+    }
+    std::vector<double> h(3 * (size_t)A, 0.0);
+    for (int a = 0; a < A; a++)
+      for (int d = 0; d < dim; d++)
+        for (int e = 0; e < dim; e++) h[3 * a + d] += inv[d][e] * q.omega[3 * a + e];
+    p1.upload(h);
+  }
+  const int64_t rows = D.n_rows(), nodes = D.n_nodes();
+  hipLaunchKernelGGL(expand_p1_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, psi,
+                     (const double *)phi_cache.as<double>() + (int64_t)g * nodes,
+                     (const double *)mom_store.as<double>() + (int64_t)g * nodes * (1 + dim),
+                     (const double *)p1.as<double>(), A, dim, 1.0 / D.quad.sum_weights, rows);
+  UBT_HIP(hipGetLastError());
+}
+
+double Operator::norm2(const double *x, hipStream_t s) {
+  const int64_t n = disc->n_rows();
+  if (!measure_part.bytes) measure_part.alloc(sizeof(double) * 2 * kMeasureBlocks);
+  if (!norm_sums.bytes) norm_sums.alloc(2 * sizeof(double));
+  double *sums = norm_sums.as<double>();
+  UBT_HIP(hipMemsetAsync(sums, 0, 2 * sizeof(double), s));
+  const unsigned nblk = (unsigned)std::min<int64_t>(kMeasureBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(sumsq_part, dim3(nblk), dim3(256), 0, s, x, n, measure_part.as<double>());
+  hipLaunchKernelGGL(measure_final, dim3(1), dim3(256), 0, s, (const double *)measure_part.as<double>(), (int)nblk, sums);
+  UBT_HIP(hipGetLastError());
+  double h[2];
+  UBT_HIP(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
+  UBT_HIP(hipStreamSynchronize(s));
+  return std::sqrt(h[0]);
 }
 
 void Operator::cache_zero_flux(int g, hipStream_t s) {

@@ -39,6 +39,13 @@ struct GroupOptions {
   int max_outer = 1000;       // sweeps over the thermal block, the first included
   double inner_gamma = 0.01;  // adaptive inner rtol factor, [0, 1); 0 runs every solve at the inner rtol
   double inner_rtol = 0.0;    // every solve's rtol under upscatter; 0: kUpscatterInnerRtol
+  // the moments store (single problem): psi is one vector every group's
+  // solve reuses; a thermal group's scalar flux and current are kept, and
+  // after the first sweep its solve starts from their P1 expansion refined by
+  // at most recon_steps safeguarded Richardson steps on L_g (0: the expansion
+  // alone)
+  int store_moments = 0;
+  int recon_steps = 10;
 };
 
 struct GroupReport {
@@ -47,6 +54,7 @@ struct GroupReport {
   int status = 0;             // 1 converged, -1 an inner solve failed, -2 out of sweeps
   double change = 0.0, rho = 0.0, err_est = 0.0;  // of the last sweep
   int64_t iterations = 0;     // over every solve
+  int64_t recon_steps = 0;    // Richardson steps of the moments store's reconstructions
   std::vector<int> group_iterations, group_reason;  // per group: summed over sweeps, of the last solve
   std::vector<double> group_residual;               // per group: the last solve's relative residual
 };

@@ -14,6 +14,7 @@
 namespace ubt {
 
 void check_group_options(const GroupOptions &o) {
+  UBT_CHECK(o.recon_steps >= 0, UBT_ERR_ARG_OUTOFRANGE, "recon_steps must not be negative, was given %d", o.recon_steps);
   UBT_CHECK(o.outer_rtol > 0.0, UBT_ERR_ARG_OUTOFRANGE, "outer_rtol must be positive, was given %g", o.outer_rtol);
   UBT_CHECK(o.max_outer >= 1, UBT_ERR_ARG_OUTOFRANGE, "max_outer must be at least 1, was given %d", o.max_outer);
   UBT_CHECK(o.inner_gamma >= 0.0 && o.inner_gamma < 1.0, UBT_ERR_ARG_OUTOFRANGE,
@@ -77,6 +78,45 @@ struct KeepRtol {
   ~KeepRtol() { *a = *b = v; }
 };
 
+// The moments store's warm start of group g (reference Reconstruct,
+// tests/transportk.kokkos.cxx:300-345): psi = the P1 expansion of the cached
+// scalar flux and the stored current, then Richardson steps on
+// L psi = b + S_gg phi_g (the within-group scatter of the stored flux),
+// psi += P_L r with P_L the removal and streaming stages, each kept only if it
+// lowers ||r||, until ||r|| <= eta ||b + S_gg phi_g|| or max_steps. Where the
+// sweep is L's exact inverse one step solves it. Returns the steps taken.
+static int reconstruct(Solver &sv, int g, const double *b, double *psi, int max_steps, double eta, double *r,
+                       double *t, double *dz, hipStream_t s) {
+  Operator &op = *sv.op;
+  const int64_t n = op.disc->n_rows();
+  op.expand_p1(g, psi, s);
+  if (max_steps == 0) return 0;
+  // r = b + S_gg phi_g - L psi
+  vec_axpby<double>(n, 1.0, b, 0.0, r, s);
+  op.scatter_source(g, g, UBT_FP64, r, s);
+  const double c_norm = op.norm2(r, s);
+  op.mult(UBT_FP64, psi, t, false, s);
+  vec_axpby<double>(n, -1.0, (const double *)t, 1.0, r, s);
+  double r_norm = op.norm2(r, s);
+  const double target = eta * c_norm;
+  int steps = 0;
+  while (steps < max_steps && r_norm > target) {
+    sv.apply_pc<double>(r, dz, s, true);
+    steps++;
+    // the trial iterate in t, so psi stays the best one if the step is refused
+    vec_axpby<double>(n, 1.0, (const double *)psi, 0.0, t, s);
+    vec_axpby<double>(n, 1.0, (const double *)dz, 1.0, t, s);
+    op.mult(UBT_FP64, t, r, false, s);
+    vec_axpby<double>(n, 1.0, b, -1.0, r, s);
+    op.scatter_source(g, g, UBT_FP64, r, s);
+    const double new_norm = op.norm2(r, s);
+    if (!(new_norm < r_norm)) break;
+    vec_axpby<double>(n, 1.0, (const double *)t, 0.0, psi, s);
+    r_norm = new_norm;
+  }
+  return steps;
+}
+
 void solve_groups(Solver &sv, const GroupOptions &o, double *psi, hipStream_t s, GroupReport &rep) {
   check_group_options(o);
   UBT_CHECK(psi, UBT_ERR_ARG_NULL, "psi is required");
@@ -88,15 +128,22 @@ void solve_groups(Solver &sv, const GroupOptions &o, double *psi, hipStream_t s,
   OuterTest t = make_test(o, up, sv.opt.rtol);
   reset_report(rep, G, g_up);
   const bool with_dsa = sv.dsa && sv.opt.pc_dsa;
+  // the moments store keeps one psi; without upscatter nothing is
+  // reconstructed, so it is the plain sweep over that one vector
+  const bool store = o.store_moments != 0;
   KeepRtol keep{&sv.opt.rtol, &sv.eng.opt.rtol, sv.opt.rtol};
-  DevBuf b, sums;
+  DevBuf b, sums, rw;
   b.alloc(sizeof(double) * (size_t)n);
   sums.alloc(sizeof(double) * 2);
-  UBT_HIP(hipMemsetAsync(psi, 0, sizeof(double) * (size_t)G * n, s));
+  if (store && up) rw.alloc(sizeof(double) * 3 * (size_t)n);  // the reconstruction's r, t and dz
+  UBT_HIP(hipMemsetAsync(psi, 0, sizeof(double) * (size_t)(store ? 1 : G) * n, s));
   // the first sweep reads every thermal group below the one it solves before
   // that group is solved: a zero flux cached for each (reference :964-976)
   if (up)
-    for (int g = g_up; g < G; g++) op.cache_zero_flux(g, s);
+    for (int g = g_up; g < G; g++) {
+      op.cache_zero_flux(g, s);
+      if (store) op.zero_moments(g, s);
+    }
   for (int outer = 0;; outer++) {
     t.begin_sweep(outer);
     UBT_HIP(hipMemsetAsync(sums.ptr, 0, sizeof(double) * 2, s));
@@ -106,9 +153,17 @@ void solve_groups(Solver &sv, const GroupOptions &o, double *psi, hipStream_t s,
       if (with_dsa) sv.dsa->set_group(g, s);
       sv.refresh(s);
       op.rhs(g, UBT_FP64, b.ptr, s, g >= g_up);
-      sv.set_rtol(t.rtol(g, g_up));
-      double *x = psi + (int64_t)g * n;
-      // from the second sweep every solve starts from the group's last psi
+      const double rtol = t.rtol(g, g_up);
+      sv.set_rtol(rtol);
+      double *x = store ? psi : psi + (int64_t)g * n;
+      // from the second sweep every solve starts from the group's last psi,
+      // or under the moments store from its reconstruction
+      if (store && outer >= 1) {
+        double *rr = rw.as<double>();
+        rep.recon_steps += reconstruct(sv, g, b.as<double>(), x, o.recon_steps, rtol, rr, rr + n, rr + 2 * n, s);
+      } else if (store) {
+        UBT_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)n, s));
+      }
       sv.solve(b.as<double>(), x, s, false, outer >= 1);
       rep.group_iterations[g] += sv.iterations;
       rep.iterations += sv.iterations;
@@ -119,6 +174,7 @@ void solve_groups(Solver &sv, const GroupOptions &o, double *psi, hipStream_t s,
         break;
       }
       op.set_scalar_flux(g, UBT_FP64, x, s, g >= g_up ? sums.as<double>() : nullptr);
+      if (store && up && g >= g_up) op.store_moments(g, x, s);
     }
     rep.sweeps = outer + 1;
     if (failed) {
@@ -143,6 +199,7 @@ void solve_groups(Solver &sv, const GroupOptions &o, double *psi, hipStream_t s,
 
 void solve_groups(BatchSolver &bs, const GroupOptions &o, double *psi, hipStream_t s, std::vector<GroupReport> &rep) {
   check_group_options(o);
+  UBT_CHECK(!o.store_moments, UBT_ERR_SUP, "the moments store is offered for a single problem's group sweep only");
   UBT_CHECK(psi, UBT_ERR_ARG_NULL, "psi is required");
   Batch &B = *bs.batch;
   const int np = (int)B.ops.size();

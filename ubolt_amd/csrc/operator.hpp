@@ -136,6 +136,11 @@ struct Operator {
   int g_up = 1;
   DevBuf phi_tmp;             // double, nodes: a measured scalar flux before it overwrites the cache
   DevBuf measure_part;        // double, 2 per reduction block
+  // the outer iteration's moments store: per group the moments {phi, J} of
+  // its last angular flux (G * nodes * (1 + dim)), and the P1 expansion's
+  // T^{-1} Omega_a per angle (3 per angle), T = sum_a w_a Omega_a Omega_a^T
+  DevBuf mom_store, p1;
+  DevBuf norm_sums;           // double, 2: norm2's result
   DevBuf flag;                // int scratch for setup checks
 
   template <class T> OpArgs<T> args() const;
@@ -152,6 +157,15 @@ struct Operator {
   void set_scalar_flux(int g, int prec, const void *psi, hipStream_t s, double *sums = nullptr);
   // a zero scalar flux cached for group g
   void cache_zero_flux(int g, hipStream_t s);
+  // the moments store: {phi, J} of group g from its angular flux (FP64), or
+  // zero; psi = phi_g / W + sum_d (T^{-1} Omega_a)_d J_g,d, the P1 expansion
+  // of the cached scalar flux and the stored current (reference
+  // GroupTransfer::enable_current / expand_p1, src/multigroupk.kokkos.cxx:373-498)
+  void store_moments(int g, const double *psi, hipStream_t s);
+  void zero_moments(int g, hipStream_t s);
+  void expand_p1(int g, double *psi, hipStream_t s);
+  // ||x||_2 over the rows (FP64; synchronises s)
+  double norm2(const double *x, hipStream_t s);
   void scatter_source(int g_from, int g_to, int prec, void *b, hipStream_t s) const;
   // all_groups: transfers from every group but g (a thermal group under
   // upscatter), else from the groups below g

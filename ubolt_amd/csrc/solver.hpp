@@ -38,7 +38,8 @@ struct Solver : KrylovSystem {
   void set_rtol(double rtol) { opt.rtol = eng.opt.rtol = rtol; }
   // the preconditioner's work vectors exist in this precision
   bool has_precision(int prec) const { return prec == UBT_FP64 ? t64.ptr != nullptr : t32.ptr != nullptr; }
-  template <class T> void apply_pc(const T *r, T *y, hipStream_t s);
+  // no_dsa: the removal and streaming stages alone (the L preconditioner)
+  template <class T> void apply_pc(const T *r, T *y, hipStream_t s, bool no_dsa = false);
   template <class T> void apply_op(const T *x, T *y, bool scatter, hipStream_t s);
   void mult(int prec, const void *x, void *y, hipStream_t s) override;
   void pc(int prec, const void *r, void *y, hipStream_t s) override;
