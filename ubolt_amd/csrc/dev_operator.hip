@@ -1415,7 +1415,7 @@ __device__ __forceinline__ void box_run(const T *__restrict__ x, T *__restrict__
 // contiguous run of tiles, ranked XCD-major so a run's halos stay in one L2,
 // then its share of the boundary-cell list through the tile kernel's body.
 template <class T, int DIM, int NB, int K, bool SCATTER>
-__global__ void __launch_bounds__(kBoxTileThreads<T>) dg_apply_box(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g,
+__global__ void __launch_bounds__((kBoxTileThreads<T, DIM>)) dg_apply_box(const T *__restrict__ x, T *__restrict__ y, OpArgs<T> g,
                                                     BoxArgs bx, int64_t total) {
   const int64_t G = gridDim.x;
   const int64_t rank = (int64_t)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
@@ -1668,7 +1668,7 @@ __global__ void __launch_bounds__(256) dg_apply_batch(const T *__restrict__ x, T
 // time (the per-angle matrices are refolded at each problem change), then
 // its contiguous share of every problem's boundary-cell tiles.
 template <class T, int DIM, int NB, int K, bool SCATTER>
-__global__ void __launch_bounds__(kBoxTileThreads<T>) dg_apply_batch_box(const T *__restrict__ x, T *__restrict__ y,
+__global__ void __launch_bounds__((kBoxTileThreads<T, DIM>)) dg_apply_batch_box(const T *__restrict__ x, T *__restrict__ y,
                                                           const BatchItem<T> *__restrict__ items,
                                                           const int64_t *__restrict__ tile_off, int64_t total,
                                                           const int64_t *__restrict__ edge_off, int64_t n_edge,
@@ -2449,7 +2449,7 @@ bool box_layout(const Operator &op, size_t ts, int K, BoxArgs &bx, int &threads,
   const Disc &D = *op.disc;
   const int A = D.n_angles, nb = D.nb, nf = D.nf, dim = D.dim;
   if (op.box_cls < 0 || A > 256) return false;
-  int W = box_tile_threads(ts) / A;
+  int W = box_tile_threads(ts, dim) / A;
   while (W > 1 && box_lds_reals(A, W, K, nb) * ts > 65536) W--;
   lds = box_lds_reals(A, W, K, nb) * ts;
   if (lds > 65536) return false;
@@ -3718,10 +3718,10 @@ void batch_apply_launch(const Batch &b, int body, const T *x, T *y, bool scatter
   if (!launched && b.dim == DIM && b.nb == NB && body == 2) {                                          \
     if (scatter)                                                                                       \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, true>), dim3(grid),             \
-                         dim3(kBoxTileThreads<T>), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
+                         dim3(kBoxTileThreads<T, DIM>), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
     else                                                                                               \
       hipLaunchKernelGGL((dg_apply_batch_box<T, DIM, NB, kBatchBoxRows, false>), dim3(grid),            \
-                         dim3(kBoxTileThreads<T>), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
+                         dim3(kBoxTileThreads<T, DIM>), lds, s, x, y, items, toff, total, eoff, n_edge, qpos, qoff); \
     launched = true;                                                                                   \
   }
   BB(2, 3) BB(3, 4)

@@ -65,12 +65,13 @@ constexpr int kPencilMaxClasses = 64;
 // second phase of the same launch that runs the tile kernel's body on their
 // list.
 constexpr int kBoxThreads = 256;  // strip-march workgroup: W columns x A angles fill at most this
-// box-tile workgroup, per precision: 1024 threads in FP32 (four times the
-// columns per tile of 256, a quarter of the halo columns per cell; on the
-// 64-problem ensemble 256 -> 512 -> 1024 ran 511 -> 501 -> 481 us), 256 in
-// FP64 (512 ran 3% slower there)
-template <class T> constexpr int kBoxTileThreads = sizeof(T) == 4 ? 1024 : 256;
-inline int box_tile_threads(size_t ts) { return ts == 4 ? 1024 : 256; }
+// box-tile workgroup: 1024 threads for 2D FP32 (four times the columns per
+// tile of 256, a quarter of the halo columns per cell; on the 64-problem
+// ensemble 256 -> 512 -> 1024 ran 511 -> 501 -> 481 us), 256 for FP64 (512
+// ran 3% slower there) and for 3D (whose tile holds more registers than
+// 1024-thread workgroups allow)
+template <class T, int DIM> constexpr int kBoxTileThreads = sizeof(T) == 4 && DIM == 2 ? 1024 : 256;
+inline int box_tile_threads(size_t ts, int dim) { return ts == 4 && dim == 2 ? 1024 : 256; }
 constexpr int kBoxBand = 2;  // row blocks per band of the box-tile walk (1, 2, 4 measured: 2 and 4 equal, 2-3% ahead of 1)
 
 struct BoxArgs {
