@@ -312,3 +312,24 @@ def test_batch_solver_quadrant_order(gpu, mixed):
         tq = np.linalg.norm(bh[sl] - A @ xq[sl]) / np.linalg.norm(bh[sl])
         assert rq["reason"][p] == 1 and tq <= (1.05e-8 if mixed else 1e-10), (p, tq)
         assert rel_max(xq[sl], xn[sl]) <= (1e-6 if mixed else 1e-9), p
+
+
+def test_batch_singular_block_names_problem(gpu):
+    """the batch's singular-block check (one counter per problem, read back
+    together) names the problem: on unit DG0 quads an S2 ordinate's outflow
+    sum is |mu| + |eta|, so sigma_t = -(|mu| + |eta|) zeroes cell 0's blocks"""
+    q = u.Quadrature(2, 2)
+    coords, cells = P.box_quads(3, 3, 3.0, 3.0)
+    ops = []
+    for k in range(2):
+        d = u.Disc.dg(q, u.Mesh(coords, cells), 0)
+        st = np.full(9, 1.0)
+        if k == 1:
+            st[0] = -(abs(q.omega[0, 0]) + abs(q.omega[0, 1]))
+        ops.append(P.Problem(f"p{k}", q, d, st, np.zeros(9), None).operator())
+    b = u.Batch(ops)
+    bs = u.BatchSolver(b, rtol=1e-8)
+    with pytest.raises(u.UbtError) as e:
+        bs.refresh()
+    assert e.value.status == "ERR_MAT_LU_ZRPVT", e.value
+    assert "problem 1" in str(e.value) and "of 36" in str(e.value), str(e.value)

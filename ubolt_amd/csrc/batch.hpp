@@ -85,6 +85,7 @@ struct Batch {
   int gen = 0;                 // bumped whenever the work lists are rebuilt
   // block inverse and sweep work lists
   DevBuf bblk;                 // blocks: prefix of cells x angles over the active problems
+  DevBuf block_cnt;            // int per problem: singular blocks found by blocks_setup
   int64_t n_blocks = 0;
   // per precision: its own grouping and records (built when the record
   // sweep first runs on an active set)

@@ -342,13 +342,22 @@ void Batch::quad_order(int prec, const void *src, void *dst, bool to_q, hipStrea
     batch_quad_order<float>(*this, (const float *)src, (float *)dst, to_q, s);
 }
 
+// every active problem's singular-block check, one counter each, read back
+// together (UBT_ERR_MAT_LU_ZRPVT naming the first problem with a singular
+// block)
 void Batch::blocks_setup(hipStream_t s) {
-  for (size_t p = 0; p < ops.size(); p++) {
-    if (!active[p]) continue;
-    BlockInverse bi;
-    bi.op = ops[p];
-    bi.setup(s);  // UBT_ERR_MAT_LU_ZRPVT on a singular block
-  }
+  const size_t np = ops.size();
+  if (block_cnt.bytes < sizeof(int) * np) block_cnt.alloc(sizeof(int) * np);
+  int *cnt = block_cnt.as<int>();
+  UBT_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * np, s));
+  for (size_t p = 0; p < np; p++)
+    if (active[p]) blockinv_check_launch(*ops[p], cnt + p, s);
+  std::vector<int> h(np);
+  UBT_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  UBT_HIP(hipStreamSynchronize(s));
+  for (size_t p = 0; p < np; p++)
+    UBT_CHECK(!active[p] || h[p] == 0, UBT_ERR_MAT_LU_ZRPVT, "%d of %lld element blocks of problem %zu are singular",
+              h[p], (long long)ops[p]->disc->n_cells * ops[p]->disc->n_angles, p);
 }
 
 void Batch::blocks_apply(int prec, const void *x, void *y, hipStream_t s) const {

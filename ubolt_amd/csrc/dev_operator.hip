@@ -3577,14 +3577,12 @@ void Operator::rhs(int g, int prec, void *b, hipStream_t s, bool all_groups) con
 // ---------------------------------------------------------------------------
 // Block inverse
 // ---------------------------------------------------------------------------
-void BlockInverse::setup(hipStream_t s) {
-  const Operator &o = *op;
+// counts o's singular element blocks into *cnt (which the caller zeroed)
+void blockinv_check_launch(const Operator &o, int *cnt, hipStream_t s) {
   UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
   const Disc &D = *o.disc;
   const int64_t nblk = (int64_t)D.n_cells * D.n_angles;
   const unsigned grid = (unsigned)((nblk + 255) / 256);
-  int *cnt = o.flag.as<int>();
-  UBT_HIP(hipMemsetAsync(cnt, 0, sizeof(int), s));
   const int fd = D.kind == DISC_FD;
   switch (D.nb) {
     case 1: hipLaunchKernelGGL(blockinv_check<1>, dim3(grid), dim3(256), 0, s, o.args<double>(), fd, cnt); break;
@@ -3593,6 +3591,15 @@ void BlockInverse::setup(hipStream_t s) {
     default: fail(UBT_ERR_SUP, "element blocks of %d basis functions are not supported", D.nb);
   }
   UBT_HIP(hipGetLastError());
+}
+
+void BlockInverse::setup(hipStream_t s) {
+  const Operator &o = *op;
+  UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections first");
+  const int64_t nblk = (int64_t)o.disc->n_cells * o.disc->n_angles;
+  int *cnt = o.flag.as<int>();
+  UBT_HIP(hipMemsetAsync(cnt, 0, sizeof(int), s));
+  blockinv_check_launch(o, cnt, s);
   int h = 0;
   UBT_HIP(hipMemcpyAsync(&h, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
   UBT_HIP(hipStreamSynchronize(s));

@@ -178,6 +178,9 @@ struct Operator {
 std::unique_ptr<Operator> make_operator(std::shared_ptr<Disc> disc, int n_groups);
 void operator_set_xs(Operator &op, const double *sigma_t, const double *sigma_s, const double *source);
 
+// counts o's singular element blocks into *cnt (zeroed by the caller) on s
+void blockinv_check_launch(const Operator &o, int *cnt, hipStream_t s);
+
 struct BlockInverse {
   Operator *op = nullptr;
   int group_ready = -1;
