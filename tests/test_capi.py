@@ -59,3 +59,36 @@ def test_op_create_needs_a_device():
     with pytest.raises(u.UbtError) as e:
         u.Operator(d, 1)
     assert e.value.status == "ERR_GPU"
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """the ctypes mirrors of the option and result structs have the header's
+    size and field offsets (a C program built with gcc reports them)"""
+    import ctypes as C
+    import subprocess
+
+    from ubolt_amd import capi
+
+    structs = {"ubt_bc": capi.Bc, "ubt_solver_options": capi.SolverOptions,
+               "ubt_group_options": capi.GroupOptions, "ubt_group_result": capi.GroupResult}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "ubt.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = {}
+    for line in out:
+        if line:
+            name, key, val = line.split()
+            want[(name, key)] = int(val)
+    for name, cls in structs.items():
+        assert C.sizeof(cls) == want[(name, "size")], name
+        for field, _ in cls._fields_:
+            assert getattr(cls, field).offset == want[(name, field)], (name, field)
