@@ -15,7 +15,10 @@ the Gauss-Seidel group sweep and the upscatter outer iteration.
 * the outer_rtol below the inner rtol is refused like the reference's;
 * a batch of problems of different group counts, with and without
   upscatter, gives every problem the flux and sweep count of its own
-  single-problem sweep.
+  single-problem sweep;
+* at gamma 0 with tight inner solves the library's sweep takes the oracle's
+  sweep count (oracle/groups.py, exact inner solves) with its change and
+  scalar flux.
 """
 import os
 
@@ -106,6 +109,30 @@ def test_outer_iteration_with_dsa_and_mixed(gpu):
               f"(plain {rb['iterations']}), psi difference {diff:.2e}", flush=True)
         assert rep["status"] == 1 and rep["sweeps"] <= 17, rep
         assert diff <= 1e-6, diff
+
+
+@pytest.mark.parametrize("name", ["box_up3_layers", "plex_box_up3_layers", "plex_box_up3_layers_dg1"])
+def test_outer_iteration_matches_oracle(gpu, name):
+    from oracle import groups as og
+    from test_oracle_groups import group_system
+    pf = load(name)
+    prob = pf.problem
+    A, b0, tr, mom, g_up = group_system(prob)
+    trace = []
+    _, phi_o, sweeps_o, status_o = og.group_sweep(A, b0, tr, mom, g_up, outer_rtol=1e-8, trace=trace)
+    assert status_o == 1
+    psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-8, inner_gamma=0.0)
+    torch.cuda.synchronize()
+    diff = max(np.abs(to_host(rep["operator"].scalar_flux(g)) - phi_o[g]).max() / np.abs(phi_o[g]).max()
+               for g in range(prob.n_groups))
+    change_o, rho_o, _ = trace[-1]
+    print(f"[case] {name}: library {rep['sweeps']} sweeps (change {rep['change']:.4e}, rho {rep['rho']:.4f}), "
+          f"oracle {sweeps_o} (change {change_o:.4e}, rho {rho_o:.4f}); scalar flux difference {diff:.2e}",
+          flush=True)
+    assert rep["status"] == 1 and rep["g_up"] == g_up and rep["sweeps"] == sweeps_o
+    assert abs(rep["change"] - change_o) <= 1e-2 * change_o
+    assert abs(rep["rho"] - rho_o) <= 1e-2 * rho_o
+    assert diff <= 1e-9, diff
 
 
 def test_outer_rtol_below_inner_is_refused(gpu):
