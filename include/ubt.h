@@ -330,8 +330,16 @@ int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power,
                         int inner_max_it, double inner_rtol, int inner_check_every);
 /* Assemble group g's diffusion operator from the operator's cross sections:
  * D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g, g). Replaces
- * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345). Void cells
- * (sigma_t <= 0) are refused with UBT_ERR_SUP. */
+ * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345) with its
+ * default void policy: void cells (sigma_t <= 0) are bridged - D = 1/(3
+ * (max(sigma_t, 0) + 1/L)), L = 4 V / S the voids' mean chord (V their
+ * volume, S their faces onto a non-void cell or a vacuum face), a DG1 face
+ * touching a void with the harmonic face D on both sides - when something is
+ * not void and a vacuum face or some absorption holds the operator; a group
+ * void everywhere is masked whole (ubt_dsa_apply then gives a zero
+ * correction, ubt_dsa_matvec and the solves UBT_ERR_ARG_WRONGSTATE). Voids
+ * that only masking could treat (-dsa_void_bridge 0, or no vacuum face and
+ * no absorption) are refused with UBT_ERR_SUP. */
 int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream);
 /* y = P D^{-1} R x: the 0th moment (times V on the plex), the inner solve, and
  * the isotropic correction / sum_weights on every row. Replaces

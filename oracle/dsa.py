@@ -1,8 +1,8 @@
 """DSA diffusion operators restated with scipy - TEST INFRASTRUCTURE ONLY.
 
 The reference's three DSA operators, assembled as sparse matrices cell by cell
-from the same formulas (no voids), and the DSA apply y = P D^{-1} R x with an
-exact inner solve:
+from the same formulas, and the DSA apply y = P D^{-1} R x with an exact inner
+solve:
 
 * structured star, per unit volume (src/dsa_structuredk.kokkos.cxx:183-285):
   harmonic face D, consistent-D blend with the axis half-range current,
@@ -10,11 +10,69 @@ exact inner solve:
 * DG0 two-point flux, volume-weighted (src/dsa_plexk.kokkos.cxx:288-348);
 * DG1 MIP interior penalty, volume-weighted (src/dsa_plexk.kokkos.cxx:365-475);
 helpers: src/dsa_operatork.hpp:110-147 (half-range current, blend).
-D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g).
+D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g). Voids (sigma_t <= 0) are
+bridged (src/dsak.kokkos.cxx:233-345, the default -dsa_void_bridge): D =
+1/(3 (max(sigma_t, 0) + 1/L)) with L = 4 V / S, V the voids' volume and S
+their faces onto a non-void cell or a vacuum face
+(src/dsa_structuredk.kokkos.cxx:127-170, src/dsa_plexk.kokkos.cxx:243-262),
+when something is not void and a vacuum face or some absorption holds the
+operator; a DG1 face touching a void takes the harmonic face D on both sides
+(src/dsa_plexk.kokkos.cxx:433-441). The masked alternative is not restated.
 """
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spl
+
+
+def structured_void_vs(dim, n, h, face_type, void):
+    """(V, S) of the void cells of a structured box"""
+    n = list(n[:dim])
+    vol = float(np.prod(h[:dim]))
+    strides = [1, n[0], n[0] * (n[1] if dim > 1 else 1)]
+    V = S = 0.0
+    for c in np.flatnonzero(void):
+        idx, cc = [], int(c)
+        for d in range(dim):
+            idx.append(cc % n[d])
+            cc //= n[d]
+        V += vol
+        for d in range(dim):
+            for side, step in ((0, -1), (1, 1)):
+                m = idx[d] + step
+                ends = (not void[c + step * strides[d]]) if 0 <= m < n[d] else face_type[d][side] == 0
+                if ends:
+                    S += vol / h[d]
+    return V, S
+
+
+def plex_void_vs(nf, face_nb, area, volume, void):
+    """(V, S) of the void cells of a mesh (face_nb: >= 0 a cell, -1 vacuum, < -1 reflective)"""
+    V = S = 0.0
+    for c in np.flatnonzero(void):
+        V += volume[c]
+        for f in range(nf):
+            k = c * nf + f
+            n = face_nb[k]
+            if (n >= 0 and not void[n]) or n == -1:
+                S += area[k]
+    return V, S
+
+
+def diffusion_coefficients(sigt, sigs, vs, any_vacuum):
+    """D per cell, the void mask and the mean chord L (None without voids)"""
+    void = ~(np.asarray(sigt) > 0.0)
+    D = np.empty(len(sigt))
+    D[~void] = 1.0 / (3.0 * sigt[~void])
+    if not void.any():
+        return D, void, None
+    if void.all():
+        raise ValueError("void everywhere: the operator is masked whole")
+    V, S = vs(void)
+    if not (any_vacuum or np.any(sigt - sigs > 0.0)) or S <= 0.0:
+        raise NotImplementedError("voids that cannot be bridged would be masked")
+    L = 4.0 * V / S
+    D[void] = 1.0 / (3.0 * (np.maximum(sigt[void], 0.0) + 1.0 / L))
+    return D, void, L
 
 
 def half_range(omega, w, W, n):
@@ -33,12 +91,14 @@ def structured(dim, n, h, face_type, omega, w, W, sigt, sigs, cons=True, p=1.5):
     hr = [half_range(omega, w, W, np.eye(3)[d]) for d in range(dim)]
     rows, cols, vals = [], [], []
     strides = [1, n[0], n[0] * (n[1] if dim > 1 else 1)]
+    any_vac = any(face_type[d][s] == 0 for d in range(dim) for s in range(2))
+    D, _, _ = diffusion_coefficients(sigt, sigs, lambda v: structured_void_vs(dim, n, h, face_type, v), any_vac)
     for c in range(cells):
         idx, cc = [], c
         for d in range(dim):
             idx.append(cc % n[d])
             cc //= n[d]
-        dc = 1.0 / (3.0 * sigt[c])
+        dc = D[c]
         diag = sigt[c] - sigs[c]
         for d in range(dim):
             hd = h[d]
@@ -46,7 +106,7 @@ def structured(dim, n, h, face_type, omega, w, W, sigt, sigs, cons=True, p=1.5):
                 m = idx[d] + step
                 if 0 <= m < n[d]:
                     nb = c + step * strides[d]
-                    dn = 1.0 / (3.0 * sigt[nb])
+                    dn = D[nb]
                     df = 2.0 * dc * dn / (dc + dn)
                     if cons:
                         df = blend(df, hr[d] * hd, p)
@@ -64,8 +124,10 @@ def dg0(nf, face_nb, face_nA, area, dist, volume, omega, w, W, sigt, sigs, cons=
     nA = face_nA.reshape(-1, 3)
     dist = dist.reshape(-1, 2)
     rows, cols, vals = [], [], []
+    D, _, _ = diffusion_coefficients(sigt, sigs, lambda v: plex_void_vs(nf, face_nb, area, volume, v),
+                                     bool(np.any(np.asarray(face_nb) == -1)))
     for c in range(cells):
-        dc = 1.0 / (3.0 * sigt[c])
+        dc = D[c]
         diag = volume[c] * (sigt[c] - sigs[c])
         for f in range(nf):
             k = c * nf + f
@@ -74,7 +136,7 @@ def dg0(nf, face_nb, face_nA, area, dist, volume, omega, w, W, sigt, sigs, cons=
             d_own = dist[k, 0]
             if face_nb[k] >= 0:
                 nb = face_nb[k]
-                dn = 1.0 / (3.0 * sigt[nb])
+                dn = D[nb]
                 tf = A / (d_own / dc + dist[k, 1] / dn)
                 if cons:
                     L = d_own + dist[k, 1]
@@ -96,8 +158,10 @@ def dg1(nb, nf, face_nb, face_nA, area, dist, volume, face_own, face_up, basis_g
     Fu = face_up.reshape(-1, nb, nb)
     G = basis_grad.reshape(cells, nb, 3)
     M = sp.lil_matrix((cells * nb, cells * nb))
+    D, void, L = diffusion_coefficients(sigt, sigs, lambda v: plex_void_vs(nf, face_nb, area, volume, v),
+                                        bool(np.any(np.asarray(face_nb) == -1)))
     for c in range(cells):
-        dc = 1.0 / (3.0 * sigt[c])
+        dc = D[c]
         sa = sigt[c] - sigs[c]
         V = volume[c]
         own = V * (dc * (G[c] @ G[c].T) + sa * np.eye(nb))
@@ -113,12 +177,14 @@ def dg1(nb, nf, face_nb, face_nA, area, dist, volume, face_own, face_up, basis_g
             hc = 2.0 * dist[k, 0]
             m = s * Fo[k][:, 0]  # int_f phi_i
             if n >= 0:
-                dn = 1.0 / (3.0 * sigt[n])
+                dcf, dn = dc, D[n]
+                if L is not None and (void[c] or void[n]):
+                    dcf = dn = 2.0 * dc * D[n] / (dc + D[n])
                 hn = 2.0 * dist[k, 1]
                 gn = G[n] @ nrm
-                kap = max(0.5 * C * (dc / hc + dn / hn), 0.25)
-                own += kap * s * Fo[k] - 0.5 * dc * np.outer(m, gc) - 0.5 * dc * np.outer(gc, s * Fo[k][0, :])
-                blk = -kap * s * Fu[k] - 0.5 * dn * np.outer(m, gn) + 0.5 * dc * np.outer(gc, s * Fu[k][0, :])
+                kap = max(0.5 * C * (dcf / hc + dn / hn), 0.25)
+                own += kap * s * Fo[k] - 0.5 * dcf * np.outer(m, gc) - 0.5 * dcf * np.outer(gc, s * Fo[k][0, :])
+                blk = -kap * s * Fu[k] - 0.5 * dn * np.outer(m, gn) + 0.5 * dcf * np.outer(gc, s * Fu[k][0, :])
                 M[c * nb:(c + 1) * nb, n * nb:(n + 1) * nb] = M[c * nb:(c + 1) * nb, n * nb:(n + 1) * nb] + blk
             else:
                 kap = max(C * dc / hc, 0.25)

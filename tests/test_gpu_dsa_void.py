@@ -1,0 +1,129 @@
+"""The DSA stage with voids on the GPU (reference src/dsak.kokkos.cxx:233-345,
+the default bridged policy; the reference's void files, tests/Makefile "DSA
+with VOIDS"):
+
+* the bridged diffusion operator of the structured, DG0 and DG1 void files
+  equals the oracle's (oracle/dsa.py, pinned by the hand-computed chords) to
+  1e-12, and its apply with a tight inner solve the oracle's exact apply;
+* on those files the DSA stage cuts the Krylov iterations the way the
+  reference's bridged pins do (box_void_channel 31 -> 6, cube_void_duct
+  22 -> 5, plex_box_void_channel 31 -> 6, its DG1 twin 37 -> 10; those counts
+  are with PCAIR, this build's stage is the sweep) and the solve keeps the
+  oracle's scalar flux;
+* a group void everywhere is masked whole: a zero correction, the solve the
+  same as without DSA; voids that only masking could treat are refused.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import ubolt_amd as u
+from helpers import oracle_dsa_apply, oracle_dsa_matrix, oracle_rhs, oracle_solve, rel_max, to_dev, to_host
+from ubolt_amd import problem_file as pfm
+from ubolt_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+PROBLEMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "problems")
+VOID_FILES = ["box_void_channel", "plex_box_void_channel", "plex_box_void_channel_dg1", "cube_void_duct"]
+
+
+def load(name):
+    return pfm.load(os.path.join(PROBLEMS, name + ".json")).problem
+
+
+@pytest.mark.parametrize("name", VOID_FILES)
+def test_bridged_matvec_and_apply(gpu, name):
+    prob = load(name)
+    op = prob.operator()
+    dsa = u.DSA(op, inner_max_it=3000, inner_rtol=1e-14)
+    dsa.set_group(0)
+    M, _ = oracle_dsa_matrix(prob)
+    x = np.random.default_rng(3).standard_normal(prob.disc.n_nodes)
+    y = torch.zeros(prob.disc.n_nodes, dtype=torch.float64, device="cuda")
+    dsa.matvec(to_dev(x), y)
+    xr = np.random.default_rng(4).standard_normal(prob.disc.n_rows)
+    ya = op.vec()
+    dsa.apply(to_dev(xr), ya)
+    torch.cuda.synchronize()
+    em, ea = rel_max(to_host(y), M @ x), rel_max(to_host(ya), oracle_dsa_apply(prob, xr))
+    print(f"[case] {name}: bridged matvec {em:.2e}, apply {ea:.2e}", flush=True)
+    assert em <= 1e-12 and ea <= 1e-9
+
+
+@pytest.mark.parametrize("name,ref_plain,ref_dsa", [("box_void_channel", 31, 6), ("plex_box_void_channel", 31, 6),
+                                                    ("plex_box_void_channel_dg1", 37, 10),
+                                                    ("cube_void_duct", 22, 5)])
+def test_bridged_dsa_cuts_iterations(gpu, name, ref_plain, ref_dsa):
+    prob = load(name)
+    op = prob.operator()
+    its, flux = {}, {}
+    for use in (False, True):
+        d = None
+        if use:
+            d = u.DSA(op, inner_max_it=200, inner_rtol=1e-6)
+            d.set_group(0)
+        sol = u.Solver(op, d, rtol=1e-10, max_it=500, pc_dsa=use)
+        sol.refresh()
+        b = op.vec()
+        op.rhs(0, b)
+        x = op.vec()
+        res = sol.solve(b, x)
+        torch.cuda.synchronize()
+        assert res["reason"] == 1
+        its[use] = res["iterations"]
+        flux[use] = oracle.moments(to_host(x), prob.quad.w)
+    want = oracle.moments(oracle_solve(prob, oracle_rhs(prob, 0, [])), prob.quad.w)
+    err = max(np.linalg.norm(flux[k] - want) / np.linalg.norm(want) for k in flux)
+    print(f"[case] {name}: FGMRES iterations without DSA {its[False]}, with bridged DSA {its[True]} (reference "
+          f"{ref_plain} -> {ref_dsa}); scalar flux vs oracle {err:.2e}", flush=True)
+    assert 2 * its[True] <= its[False]
+    assert err <= 1e-8
+
+
+def test_all_void_group_is_masked(gpu):
+    prob = P.fd_box_2d_central_source(n=6, sn=2)
+    op = prob.operator()
+    n = prob.disc.n_cells
+    op.set_xs(np.zeros(n), np.zeros(n), np.ones(n))
+    dsa = u.DSA(op)
+    dsa.set_group(0)
+    x = to_dev(np.random.default_rng(5).standard_normal(prob.disc.n_rows))
+    y = op.vec()
+    y.fill_(1.0)
+    dsa.apply(x, y)
+    torch.cuda.synchronize()
+    assert float(y.abs().max()) == 0.0
+    with pytest.raises(u.UbtError) as e:
+        dsa.matvec(torch.zeros(n, dtype=torch.float64, device="cuda"), torch.zeros(n, dtype=torch.float64,
+                                                                                      device="cuda"))
+    assert e.value.status == "ERR_ARG_WRONGSTATE"
+    got = {}
+    for use in (False, True):
+        sol = u.Solver(op, dsa if use else None, rtol=1e-10, max_it=200, pc_dsa=use)
+        sol.refresh()
+        b = op.vec()
+        op.rhs(0, b)
+        xs = op.vec()
+        res = sol.solve(b, xs)
+        torch.cuda.synchronize()
+        assert res["reason"] == 1
+        got[use] = (res["iterations"], to_host(xs))
+    assert got[True][0] == got[False][0]
+    assert rel_max(got[True][1], got[False][1]) <= 1e-12
+
+
+def test_unbridgeable_void_is_refused(gpu):
+    pf = load("box_50_inf_medium")  # every face reflective
+    op = pf.operator()
+    st = np.ones(pf.disc.n_cells)
+    st[7] = 0.0
+    op.set_xs(st, st.copy(), np.ones_like(st))  # and no absorption
+    dsa = u.DSA(op)
+    with pytest.raises(u.UbtError) as e:
+        dsa.set_group(0)
+    assert e.value.status == "ERR_SUP"
+    assert "cannot be bridged" in str(e.value)

@@ -14,8 +14,13 @@
 //    volume-weighted;
 // with D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g) and the consistent-D
 // blend (D^p + (m h)^p)^(1/p), p = 1.5, on FD and DG0 faces.
+// Voids (sigma_t <= 0; reference src/dsak.kokkos.cxx:233-345) are bridged: D
+// = 1/(3 (max(sigma_t, 0) + 1/L)) with L = 4 V / S the voids' mean chord, and
+// a DG1 face touching a void takes the harmonic face D on both sides; a group
+// void everywhere is masked whole (zero correction).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -30,6 +35,7 @@ namespace {
 struct AsmArgs {
   int kind, dim, nb, nf, ns, n_cells, cons;
   double p, C;
+  double inv_chord;           // 1 / L of the bridged voids, 0 without them
   const double *sigt, *sigs;  // current group
   const int *nbr;             // cells * ns
   // structured
@@ -46,11 +52,17 @@ __device__ inline double blend(double d, double m, double p) {
   return p == 1.0 ? d + m : pow(pow(d, p) + pow(m, p), 1.0 / p);
 }
 
+// D of a cell: 1 / (3 sigma_t); in a bridged void (sigma_t <= 0) a third of
+// the free flight, 1 / (3 (max(sigma_t, 0) + 1 / L)) (Wigner's rational form)
+__device__ inline double dcoef(double st, double inv_chord) {
+  return st > 0.0 ? 1.0 / (3.0 * st) : 1.0 / (3.0 * (fmax(st, 0.0) + inv_chord));
+}
+
 __global__ void asm_structured(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
   const double st = a.sigt[c];
-  const double dc = 1.0 / (3.0 * st);
+  const double dc = dcoef(st, a.inv_chord);
   double diag = st - a.sigs[c];
   for (int d = 0; d < a.dim; d++) {
     const double h = a.h[d];
@@ -59,7 +71,7 @@ __global__ void asm_structured(AsmArgs a) {
       const int n = a.nbr[c * a.ns + s];
       double o = 0.0;
       if (n >= 0) {
-        const double dn = 1.0 / (3.0 * a.sigt[n]);
+        const double dn = dcoef(a.sigt[n], a.inv_chord);
         double df = 2.0 * dc * dn / (dc + dn);
         if (a.cons) df = blend(df, a.hr[d] * h, a.p);
         o = -df / (h * h);
@@ -78,7 +90,7 @@ __global__ void asm_dg0(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
   const double st = a.sigt[c];
-  const double dc = 1.0 / (3.0 * st);
+  const double dc = dcoef(st, a.inv_chord);
   double diag = a.volume[c] * (st - a.sigs[c]);
   for (int f = 0; f < a.nf; f++) {
     const int64_t k = c * a.nf + f;
@@ -86,7 +98,7 @@ __global__ void asm_dg0(AsmArgs a) {
     const double A = a.area[k], d_own = a.dist[2 * k];
     double o = 0.0;
     if (n >= 0) {
-      const double dn = 1.0 / (3.0 * a.sigt[n]), d_nb = a.dist[2 * k + 1];
+      const double dn = dcoef(a.sigt[n], a.inv_chord), d_nb = a.dist[2 * k + 1];
       double tf = A / (d_own / dc + d_nb / dn);
       if (a.cons) {
         const double len = d_own + d_nb;
@@ -108,7 +120,7 @@ __global__ void asm_dg1(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
   const double st = a.sigt[c];
-  const double dc = 1.0 / (3.0 * st), sa = st - a.sigs[c];
+  const double dc = dcoef(st, a.inv_chord), sa = st - a.sigs[c];
   const double V = a.volume[c];
   const int64_t gc = a.gcls[c];
   double own[NB][NB];
@@ -140,13 +152,20 @@ __global__ void asm_dg1(AsmArgs a) {
     }
     const double hc = 2.0 * a.dist[2 * k];
     if (n >= 0) {
-      const double dn = 1.0 / (3.0 * a.sigt[n]), hn = 2.0 * a.dist[2 * k + 1];
-      const double kappa = fmax(0.5 * a.C * (dc / hc + dn / hn), 0.25);
+      const double hn = 2.0 * a.dist[2 * k + 1];
+      double dcf = dc, dn = dcoef(a.sigt[n], a.inv_chord);
+      // a face touching a bridged void: both sides take the face's harmonic
+      // D, in the averages and in kappa (the same from either cell)
+      if (a.inv_chord > 0.0 && (!(st > 0.0) || !(a.sigt[n] > 0.0))) {
+        dcf = 2.0 * dc * dn / (dc + dn);
+        dn = dcf;
+      }
+      const double kappa = fmax(0.5 * a.C * (dcf / hc + dn / hn), 0.25);
       for (int i = 0; i < NB; i++) {
         const double mi = scale * Fo[i * NB];
         for (int j = 0; j < NB; j++) {
-          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dc * gcv[j] * mi - 0.5 * dc * gcv[i] * scale * Fo[j];
-          nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dc * gcv[i] * scale * Fu[j];
+          own[i][j] += kappa * scale * Fo[i * NB + j] - 0.5 * dcf * gcv[j] * mi - 0.5 * dcf * gcv[i] * scale * Fo[j];
+          nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dcf * gcv[i] * scale * Fu[j];
         }
       }
     } else {
@@ -419,6 +438,80 @@ __global__ void void_census(const double *__restrict__ st, const double *__restr
   if (st[c] - ss[c] > 0.0) atomicAdd(&flags[3], 1);
 }
 
+// The voids' volume V and surface S (their faces onto a non-void cell or a
+// vacuum face) for the mean chord L = 4 V / S: per block in a fixed order,
+// then one block over the partials, so the sums do not depend on scheduling
+constexpr int kVoidBlocks = 1024;
+
+__global__ void __launch_bounds__(256) void_vs_partial(AsmArgs a, double *part) {
+  __shared__ double sv[256], sf[256];
+  double v = 0.0, f = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < a.n_cells; c += (int64_t)gridDim.x * 256) {
+    if (a.sigt[c] > 0.0) continue;
+    if (a.kind == DISC_FD) {
+      double vol = 1.0;
+      for (int d = 0; d < a.dim; d++) vol *= a.h[d];
+      v += vol;
+      for (int d = 0; d < a.dim; d++)
+        for (int side = 0; side < 2; side++) {
+          const int n = a.nbr[c * a.ns + 2 * d + side];
+          if (n >= 0 ? a.sigt[n] > 0.0 : a.vac[d][side] != 0) f += vol / a.h[d];
+        }
+    } else {
+      v += a.volume[c];
+      for (int k = 0; k < a.nf; k++) {
+        const int n = a.face_nb[c * a.nf + k];
+        if (n >= 0 ? a.sigt[n] > 0.0 : n == -1) f += a.area[c * a.nf + k];
+      }
+    }
+  }
+  sv[threadIdx.x] = v;
+  sf[threadIdx.x] = f;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sv[threadIdx.x] += sv[threadIdx.x + w];
+      sf[threadIdx.x] += sf[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sv[0];
+    part[2 * blockIdx.x + 1] = sf[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) void_vs_final(double *part, int nblk) {
+  __shared__ double sv[256], sf[256];
+  double v = 0.0, f = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) {
+    v += part[2 * b];
+    f += part[2 * b + 1];
+  }
+  sv[threadIdx.x] = v;
+  sf[threadIdx.x] = f;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sv[threadIdx.x] += sv[threadIdx.x + w];
+      sf[threadIdx.x] += sf[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * nblk] = sv[0];
+    part[2 * nblk + 1] = sf[0];
+  }
+}
+
+__global__ void clear_int(int *p) { *p = 0; }
+
+template <class T>
+__global__ void fill_zero(T *__restrict__ y, int64_t n) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) y[r] = T(0);
+}
+
 template <class T>
 __global__ void prolong(const double *__restrict__ sol, T *__restrict__ y, int64_t rows, int A, double inv_w) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -512,6 +605,7 @@ void DSA::create(Operator *o) {
   const int64_t nblk = (n_nodes + kCgThreads - 1) / kCgThreads;
   partial.alloc(sizeof(double) * nblk);
   scal.alloc(sizeof(double) * 8);
+  vsum.alloc(sizeof(double) * (2 * kVoidBlocks + 2));
   UBT_HIP(hipHostMalloc((void **)&flag_host, sizeof(double), hipHostMallocDefault));
 }
 
@@ -524,11 +618,10 @@ void DSA::set_group(int g, hipStream_t s) {
   const Disc &D = *o.disc;
   UBT_CHECK(o.xs_set, UBT_ERR_ARG_WRONGSTATE, "set the cross sections before the DSA group");
   UBT_CHECK(g >= 0 && g < o.n_groups, UBT_ERR_ARG_OUTOFRANGE, "group %d out of range", g);
-  // the void census: this operator has no void masking or bridging, so a
-  // non-positive sigma_t is refused, and a pure-Neumann absorption-free group
+  group = -1;  // until this group's operator is in place
   const double *st = o.t64.sigt.as<double>() + (int64_t)g * n_cells;
   const double *ssg = o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells;
-  // flags (one read back below, on the caller's stream): 0 singular diagonal
+  // flags (read back below, on the caller's stream): 0 singular diagonal
   // blocks, 1 void cells (sigma_t <= 0), 2 the first void cell, 3 absorbing
   // cells (sigma_a > 0)
   int *flags = o.flag.as<int>();
@@ -546,7 +639,7 @@ void DSA::set_group(int g, hipStream_t s) {
   a.p = opt.consistent_power;
   a.C = opt.mip_penalty;
   a.sigt = st;
-  a.sigs = o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells;
+  a.sigs = ssg;
   a.nbr = nbr.as<int>();
   for (int d = 0; d < 3; d++) {
     a.h[d] = D.h[d];
@@ -556,9 +649,7 @@ void DSA::set_group(int g, hipStream_t s) {
   }
   a.own = own.as<double>();
   a.off = off.as<double>();
-  if (D.kind == DISC_FD) {
-    hipLaunchKernelGGL(asm_structured, dim3(grid), dim3(256), 0, s, a);
-  } else {
+  if (D.kind != DISC_FD) {
     a.volume = o.volume_d.as<double>();
     a.area = o.face_area_d.as<double>();
     a.dist = o.face_dist_d.as<double>();
@@ -569,31 +660,69 @@ void DSA::set_group(int g, hipStream_t s) {
     a.grad = o.t64.grad.as<double>();
     a.face_nb = o.face_nb.as<int>();
     a.gcls = o.gcls.as<int>();
-    if (nb == 1)
+  }
+  // the operator and its block-Jacobi inverses (singular blocks counted in flags[0])
+  auto assemble = [&](double inv_chord) {
+    a.inv_chord = inv_chord;
+    if (D.kind == DISC_FD)
+      hipLaunchKernelGGL(asm_structured, dim3(grid), dim3(256), 0, s, a);
+    else if (nb == 1)
       hipLaunchKernelGGL(asm_dg0, dim3(grid), dim3(256), 0, s, a);
     else if (nb == 3)
       hipLaunchKernelGGL(asm_dg1<3>, dim3(grid), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL(asm_dg1<4>, dim3(grid), dim3(256), 0, s, a);
-  }
-  UBT_HIP(hipGetLastError());
-  int *bad = flags;
-  if (nb == 1)
-    hipLaunchKernelGGL(jacobi_invert<1>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
-  else if (nb == 3)
-    hipLaunchKernelGGL(jacobi_invert<3>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
-  else
-    hipLaunchKernelGGL(jacobi_invert<4>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, bad);
-  UBT_HIP(hipGetLastError());
+    UBT_HIP(hipGetLastError());
+    if (nb == 1)
+      hipLaunchKernelGGL(jacobi_invert<1>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, flags);
+    else if (nb == 3)
+      hipLaunchKernelGGL(jacobi_invert<3>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, flags);
+    else
+      hipLaunchKernelGGL(jacobi_invert<4>, dim3(grid), dim3(256), 0, s, own.as<double>(), jinv.as<double>(), n_cells, flags);
+    UBT_HIP(hipGetLastError());
+  };
+  // without voids (the common case) one assembly and one read-back
+  assemble(0.0);
   int h[4] = {0, 0, 0, 0};
   UBT_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
   UBT_HIP(hipStreamSynchronize(s));
-  // this operator has no void masking or bridging, so a non-positive sigma_t
-  // is refused, and so is an absorption-free group with no vacuum face
-  UBT_CHECK(h[1] == 0, UBT_ERR_SUP, "cell %d is a void in group %d (%d void cells): DSA void treatment is not "
-            "supported", h[2], g, h[1]);
-  UBT_CHECK(any_vacuum || h[3] > 0, UBT_ERR_ARG_WRONGSTATE,
-            "the DSA diffusion operator is singular: every face is reflective and sigma_a is zero everywhere");
+  n_void = h[1];
+  inv_chord = 0.0;
+  all_void = false;
+  if (n_void == n_cells) {
+    // void everywhere: nothing on the far side of a void to couple to, so the
+    // whole operator is masked and the correction is zero
+    all_void = true;
+    group = g;
+    return;
+  }
+  if (n_void > 0) {
+    // bridged: something is not void and a vacuum face or some absorption
+    // holds the operator (masking, the alternative, is not supported)
+    UBT_CHECK(any_vacuum || h[3] > 0, UBT_ERR_SUP,
+              "group %d has %d void cells (the first %d), every face reflective and no absorption: the voids cannot "
+              "be bridged, and DSA void masking is not supported", g, n_void, h[2]);
+    double *vs = vsum.as<double>();
+    const int nblk = (int)std::min<int64_t>(grid, kVoidBlocks);
+    hipLaunchKernelGGL(void_vs_partial, dim3(nblk), dim3(256), 0, s, a, vs);
+    hipLaunchKernelGGL(void_vs_final, dim3(1), dim3(256), 0, s, vs, nblk);
+    UBT_HIP(hipGetLastError());
+    double vsh[2] = {0.0, 0.0};
+    UBT_HIP(hipMemcpyAsync(vsh, vs + 2 * nblk, sizeof(vsh), hipMemcpyDeviceToHost, s));
+    UBT_HIP(hipStreamSynchronize(s));
+    UBT_CHECK(vsh[1] > 0.0, UBT_ERR_SUP,
+              "group %d: no face of its %d void cells ends a flight (a non-void neighbour or a vacuum face), so they "
+              "cannot be bridged, and DSA void masking is not supported", g, n_void);
+    const double chord = 4.0 * vsh[0] / vsh[1];
+    inv_chord = 1.0 / chord;
+    hipLaunchKernelGGL(clear_int, dim3(1), dim3(1), 0, s, flags);
+    assemble(inv_chord);
+    UBT_HIP(hipMemcpyAsync(h, flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    UBT_HIP(hipStreamSynchronize(s));
+  } else {
+    UBT_CHECK(any_vacuum || h[3] > 0, UBT_ERR_ARG_WRONGSTATE,
+              "the DSA diffusion operator is singular: every face is reflective and sigma_a is zero everywhere");
+  }
   UBT_CHECK(h[0] == 0, UBT_ERR_MAT_LU_ZRPVT, "%d diffusion diagonal blocks are singular", h[0]);
   group = g;
 }
@@ -612,6 +741,8 @@ static MvArgs mv_args(const DSA &d) {
 
 void DSA::matvec(const double *x, double *y, hipStream_t s) const {
   UBT_CHECK(group >= 0, UBT_ERR_ARG_WRONGSTATE, "set the DSA group first");
+  UBT_CHECK(!all_void, UBT_ERR_ARG_WRONGSTATE, "group %d is void everywhere: its diffusion operator is masked out",
+            group);
   const MvArgs m = mv_args(*this);
   const unsigned grid = (unsigned)((n_nodes + 255) / 256);
   if (nb == 1)
@@ -674,6 +805,8 @@ static void cg_dispatch(DSA &d, const double *b, const double *wgt, double *x, h
 void DSA::solve(const double *b, double *x, hipStream_t s) {
   UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
             "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
+  UBT_CHECK(!all_void, UBT_ERR_ARG_WRONGSTATE, "group %d is void everywhere: its diffusion operator is masked out",
+            group);
   cg_dispatch(*this, b, nullptr, x, s, own_work(*this));
 }
 
@@ -713,6 +846,8 @@ static double *scratch_on(int dev) {
 void DSA::solve_noalloc(const double *b, double *x, hipStream_t s) {
   UBT_CHECK(group == op->group, UBT_ERR_ARG_WRONGSTATE,
             "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
+  UBT_CHECK(!all_void, UBT_ERR_ARG_WRONGSTATE, "group %d is void everywhere: its diffusion operator is masked out",
+            group);
   UBT_CHECK(n_nodes <= kNoAllocNodes, UBT_ERR_ARG_SIZ,
             "the allocation-free inner solve takes at most %lld diffusion nodes, this operator has %lld",
             (long long)kNoAllocNodes, (long long)n_nodes);
@@ -728,6 +863,16 @@ void DSA::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose)
             "the DSA operator was set for group %d but the transport operator is at group %d", group, op->group);
   UBT_CHECK(x && y, UBT_ERR_ARG_NULL, "x and y are required");
   UBT_CHECK(x != y, UBT_ERR_ARG_IDN, "the DSA apply needs distinct x and y");
+  if (all_void) {  // the masked operator's correction: zero (a fill kernel, so the apply can be captured)
+    const int64_t rows = op->disc->n_rows();
+    const unsigned grid = (unsigned)((rows + 255) / 256);
+    if (prec == UBT_FP64)
+      hipLaunchKernelGGL(fill_zero<double>, dim3(grid), dim3(256), 0, s, (double *)y, rows);
+    else
+      hipLaunchKernelGGL(fill_zero<float>, dim3(grid), dim3(256), 0, s, (float *)y, rows);
+    UBT_HIP(hipGetLastError());
+    return;
+  }
   // R: the 0th moment per node (matrix cores), weighted inside the CG init;
   // the adjoint sums unweighted and moves the node weight to the prolongation
   op->moments_f64(prec, x, rhs.as<double>(), s, transpose);

@@ -32,6 +32,16 @@ struct DSA {
   double half_range[3] = {0, 0, 0};
   int vacuum[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   bool any_vacuum = false;
+  // the group's void policy (void: sigma_t <= 0). Voids are bridged - kept in
+  // the operator with D = 1 / (3 (max(sigma_t, 0) + 1 / L)), L = 4 V / S the
+  // voids' mean chord (V their volume, S their faces onto a non-void cell or
+  // a vacuum face) - when something is not void and a vacuum face or some
+  // absorption holds the operator; a group void everywhere is masked whole
+  // (a zero correction)
+  int n_void = 0;
+  double inv_chord = 0.0;    // 1 / L, 0 without bridged voids
+  bool all_void = false;
+  DevBuf vsum;               // double: per-block (V, S) partials, then the two sums
   DevBuf nbr;                // int, cells * ns: neighbour cell per slot, -1 none
   DevBuf own, off, jinv;     // double: cells*nb*nb, cells*ns*nb*nb, cells*nb*nb
   DevBuf weight;             // double, nodes: V (plex) or 1 (structured)
