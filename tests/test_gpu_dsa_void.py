@@ -10,6 +10,10 @@ with VOIDS"):
   22 -> 5, plex_box_void_channel 31 -> 6, its DG1 twin 37 -> 10; those counts
   are with PCAIR, this build's stage is the sweep) and the solve keeps the
   oracle's scalar flux;
+* the four-group decades files with a void channel in every group and a
+  block void in the top group only run the group sweep with DSA to the
+  oracle's scalar flux, DSA cutting the thick groups' iterations (reference
+  pin with DSA: 7 per group, tests/Makefile);
 * a group void everywhere is masked whole: a zero correction, the solve the
   same as without DSA; voids that only masking could treat are refused.
 """
@@ -82,6 +86,24 @@ def test_bridged_dsa_cuts_iterations(gpu, name, ref_plain, ref_dsa):
           f"{ref_plain} -> {ref_dsa}); scalar flux vs oracle {err:.2e}", flush=True)
     assert 2 * its[True] <= its[False]
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize("name", ["box_decades4_void", "plex_decades4_void"])
+def test_decades_voids_group_sweep_with_dsa(gpu, name):
+    pf = pfm.load(os.path.join(PROBLEMS, name + ".json"))
+    prob = pf.problem
+    psi0, r0 = pfm.solve(pf, rtol=1e-10, max_it=800)
+    psi1, r1 = pfm.solve(pf, rtol=1e-10, max_it=800, dsa=True)
+    torch.cuda.synchronize()
+    cache, errs = [], []
+    for g in range(prob.n_groups):
+        want = oracle.moments(oracle_solve(prob, oracle_rhs(prob, g, cache), g), prob.quad.w)
+        cache.append(want)
+        errs.append(np.linalg.norm(oracle.moments(to_host(psi1[g]), prob.quad.w) - want) / np.linalg.norm(want))
+    print(f"[case] {name}: per-group iterations without DSA {r0['group_iterations']}, with bridged DSA "
+          f"{r1['group_iterations']} (reference pin with DSA 7); scalar flux vs oracle {max(errs):.2e}", flush=True)
+    assert r1["status"] == 1 and max(errs) <= 1e-8
+    assert max(r1["group_iterations"]) < max(r0["group_iterations"])
 
 
 def test_all_void_group_is_masked(gpu):
