@@ -328,18 +328,24 @@ int ubt_dsa_create(ubt_op *op, ubt_dsa **dsa);
  * early (a host synchronisation); 0 never synchronises (capturable). */
 int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power, double mip_penalty,
                         int inner_max_it, double inner_rtol, int inner_check_every);
+/* The void policy, the reference's -dsa_void_bridge, -dsa_void_sigma_t and
+ * -dsa_void_d (src/dsak.kokkos.cxx:181-183; defaults 1, 0, 0): a cell is void
+ * where sigma_t <= void_sigma_t; bridge != 0 bridges voids where the operator
+ * allows it, 0 masks them; void_d > 0 fixes a bridged void's D. Takes effect
+ * at the next ubt_dsa_set_group. */
+int ubt_dsa_set_void_policy(ubt_dsa *dsa, int bridge, double void_sigma_t, double void_d);
 /* Assemble group g's diffusion operator from the operator's cross sections:
  * D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g, g). Replaces
- * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345) with its
- * default void policy: void cells (sigma_t <= 0) are bridged - D = 1/(3
- * (max(sigma_t, 0) + 1/L)), L = 4 V / S the voids' mean chord (V their
- * volume, S their faces onto a non-void cell or a vacuum face), a DG1 face
- * touching a void with the harmonic face D on both sides - when something is
- * not void and a vacuum face or some absorption holds the operator; a group
- * void everywhere is masked whole (ubt_dsa_apply then gives a zero
- * correction, ubt_dsa_matvec and the solves UBT_ERR_ARG_WRONGSTATE). Voids
- * that only masking could treat (-dsa_void_bridge 0, or no vacuum face and
- * no absorption) are refused with UBT_ERR_SUP. */
+ * DSAPrecon::set_group / assemble (src/dsak.kokkos.cxx:216-345) with its void
+ * policy: void cells are bridged - D = void_d or 1/(3 (max(sigma_t, 0) +
+ * 1/L)), L = 4 V / S the voids' mean chord (V their volume, S their faces onto
+ * a non-void cell or a vacuum face), a DG1 face touching a void with the
+ * harmonic face D on both sides - when bridging is on, something is not void,
+ * a vacuum face or some absorption holds the operator and S > 0; otherwise
+ * masked - identity rows (V times identity blocks on the plex) coupled to
+ * nothing, a face into one a vacuum face, nothing restricted or corrected
+ * there. A group void everywhere is masked whole (ubt_dsa_apply then gives a
+ * zero correction, ubt_dsa_matvec and the solves UBT_ERR_ARG_WRONGSTATE). */
 int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream);
 /* y = P D^{-1} R x: the 0th moment (times V on the plex), the inner solve, and
  * the isotropic correction / sum_weights on every row. Replaces

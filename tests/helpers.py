@@ -72,27 +72,49 @@ def to_host(t):
     return t.detach().double().cpu().numpy()
 
 
-def oracle_dsa_matrix(prob, g=0, cons=True, p=1.5, C=4.0):
+def oracle_dsa_matrix(prob, g=0, cons=True, p=1.5, C=4.0, **void):
     """the reference DSA diffusion matrix of group g (oracle/dsa.py) and the
-    restriction weight per node"""
+    restriction weight per node (zero on masked void nodes); void: the void
+    policy's options (bridge, void_sigma_t, void_d)"""
     from oracle import dsa as odsa
     d, q = prob.disc, prob.quad
     st, ss, _ = xs_arrays(prob)
     if d.kind == "fd":
         ft = d.get("bc_face").reshape(3, 2, 7)[:, :, 0].astype(int)
-        M = odsa.structured(d.dim, d.get("n"), d.get("h"), ft, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p)
-        return M, np.ones(d.n_nodes)
-    args = (d.nf, d.get("face_nb"), d.get("face_nA"), d.get("face_area"), d.get("face_distance"), d.get("volume"))
-    if d.nb == 1:
-        M = odsa.dg0(*args, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p)
+        M = odsa.structured(d.dim, d.get("n"), d.get("h"), ft, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p,
+                            **void)
+        wgt = np.ones(d.n_nodes)
     else:
-        M = odsa.dg1(d.nb, *args, d.get("face_own"), d.get("face_up"), d.get("basis_grad"), st[g], ss[g, g], C)
-    return M, np.repeat(d.get("volume"), d.nb)
+        args = (d.nf, d.get("face_nb"), d.get("face_nA"), d.get("face_area"), d.get("face_distance"), d.get("volume"))
+        if d.nb == 1:
+            M = odsa.dg0(*args, q.omega, q.w, q.sum_weights, st[g], ss[g, g], cons, p, **void)
+        else:
+            M = odsa.dg1(d.nb, *args, d.get("face_own"), d.get("face_up"), d.get("basis_grad"), st[g], ss[g, g], C,
+                         **void)
+        wgt = np.repeat(d.get("volume"), d.nb)
+    return M, wgt * np.repeat(oracle_dsa_unmasked(prob, g, **void), d.nb)
 
 
-def oracle_dsa_apply(prob, x, g=0):
+def oracle_dsa_unmasked(prob, g=0, **void):
+    """1 per cell, 0 on the cells the void policy masks"""
     from oracle import dsa as odsa
-    M, wgt = oracle_dsa_matrix(prob, g)
+    d = prob.disc
+    st, ss, _ = xs_arrays(prob)
+    if d.kind == "fd":
+        ft = d.get("bc_face").reshape(3, 2, 7)[:, :, 0].astype(int)
+        vs = lambda v: odsa.structured_void_vs(d.dim, d.get("n"), d.get("h"), ft, v)
+        any_vac = bool(np.any(ft[:d.dim] == 0))
+    else:
+        fnb = d.get("face_nb")
+        vs = lambda v: odsa.plex_void_vs(d.nf, fnb, d.get("face_area"), d.get("volume"), v)
+        any_vac = bool(np.any(fnb == -1))
+    _, vd, _, masked = odsa.diffusion_coefficients(st[g], ss[g, g], vs, any_vac, **void)
+    return np.where(vd & masked, 0.0, 1.0)
+
+
+def oracle_dsa_apply(prob, x, g=0, **void):
+    from oracle import dsa as odsa
+    M, wgt = oracle_dsa_matrix(prob, g, **void)
     return odsa.apply(M, wgt, prob.quad.n_angles, prob.quad.w, prob.quad.sum_weights, x)
 
 

@@ -1,21 +1,22 @@
 """The DSA stage with voids on the GPU (reference src/dsak.kokkos.cxx:233-345,
-the default bridged policy; the reference's void files, tests/Makefile "DSA
-with VOIDS"):
+bridged by default, masked with -dsa_void_bridge 0; the reference's void
+files, tests/Makefile "DSA with VOIDS"):
 
-* the bridged diffusion operator of the structured, DG0 and DG1 void files
-  equals the oracle's (oracle/dsa.py, pinned by the hand-computed chords) to
-  1e-12, and its apply with a tight inner solve the oracle's exact apply;
+* the bridged and the masked diffusion operators of the structured, DG0 and
+  DG1 void files equal the oracle's (oracle/dsa.py, pinned by the
+  hand-computed chords and the masked strip = vacuum face identity) to 1e-12,
+  and their applies with a tight inner solve the oracle's exact apply;
 * on those files the DSA stage cuts the Krylov iterations the way the
-  reference's bridged pins do (box_void_channel 31 -> 6, cube_void_duct
-  22 -> 5, plex_box_void_channel 31 -> 6, its DG1 twin 37 -> 10; those counts
-  are with PCAIR, this build's stage is the sweep) and the solve keeps the
-  oracle's scalar flux;
+  reference's pins do (bridged / masked: box_void_channel 31 -> 6 / 8,
+  cube_void_duct 22 -> 5 / 7, plex_box_void_channel 31 -> 6 / 8, its DG1 twin
+  37 -> 10 / 11; those counts are with PCAIR, this build's stage is the
+  sweep) and the solve keeps the oracle's scalar flux;
 * the four-group decades files with a void channel in every group and a
   block void in the top group only run the group sweep with DSA to the
   oracle's scalar flux, DSA cutting the thick groups' iterations (reference
   pin with DSA: 7 per group, tests/Makefile);
 * a group void everywhere is masked whole: a zero correction, the solve the
-  same as without DSA; voids that only masking could treat are refused.
+  same as without DSA; voids that cannot be bridged fall back to the mask.
 """
 import os
 
@@ -33,19 +34,23 @@ pytestmark = pytest.mark.gpu
 
 PROBLEMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "problems")
 VOID_FILES = ["box_void_channel", "plex_box_void_channel", "plex_box_void_channel_dg1", "cube_void_duct"]
+# the reference's FGMRES counts without DSA, with bridged and with masked DSA
+REF_COUNTS = {"box_void_channel": (31, 6, 8), "plex_box_void_channel": (31, 6, 8),
+              "plex_box_void_channel_dg1": (37, 10, 11), "cube_void_duct": (22, 5, 7)}
 
 
 def load(name):
     return pfm.load(os.path.join(PROBLEMS, name + ".json")).problem
 
 
+@pytest.mark.parametrize("bridge", [True, False])
 @pytest.mark.parametrize("name", VOID_FILES)
-def test_bridged_matvec_and_apply(gpu, name):
+def test_void_matvec_and_apply(gpu, name, bridge):
     prob = load(name)
     op = prob.operator()
-    dsa = u.DSA(op, inner_max_it=3000, inner_rtol=1e-14)
+    dsa = u.DSA(op, inner_max_it=3000, inner_rtol=1e-14, void_bridge=bridge)
     dsa.set_group(0)
-    M, _ = oracle_dsa_matrix(prob)
+    M, _ = oracle_dsa_matrix(prob, bridge=bridge)
     x = np.random.default_rng(3).standard_normal(prob.disc.n_nodes)
     y = torch.zeros(prob.disc.n_nodes, dtype=torch.float64, device="cuda")
     dsa.matvec(to_dev(x), y)
@@ -53,22 +58,22 @@ def test_bridged_matvec_and_apply(gpu, name):
     ya = op.vec()
     dsa.apply(to_dev(xr), ya)
     torch.cuda.synchronize()
-    em, ea = rel_max(to_host(y), M @ x), rel_max(to_host(ya), oracle_dsa_apply(prob, xr))
-    print(f"[case] {name}: bridged matvec {em:.2e}, apply {ea:.2e}", flush=True)
+    em, ea = rel_max(to_host(y), M @ x), rel_max(to_host(ya), oracle_dsa_apply(prob, xr, bridge=bridge))
+    print(f"[case] {name} {'bridged' if bridge else 'masked'}: matvec {em:.2e}, apply {ea:.2e}", flush=True)
     assert em <= 1e-12 and ea <= 1e-9
 
 
-@pytest.mark.parametrize("name,ref_plain,ref_dsa", [("box_void_channel", 31, 6), ("plex_box_void_channel", 31, 6),
-                                                    ("plex_box_void_channel_dg1", 37, 10),
-                                                    ("cube_void_duct", 22, 5)])
-def test_bridged_dsa_cuts_iterations(gpu, name, ref_plain, ref_dsa):
+@pytest.mark.parametrize("bridge", [True, False])
+@pytest.mark.parametrize("name", VOID_FILES)
+def test_void_dsa_cuts_iterations(gpu, name, bridge):
+    ref_plain, ref_b, ref_m = REF_COUNTS[name]
     prob = load(name)
     op = prob.operator()
     its, flux = {}, {}
     for use in (False, True):
         d = None
         if use:
-            d = u.DSA(op, inner_max_it=200, inner_rtol=1e-6)
+            d = u.DSA(op, inner_max_it=200, inner_rtol=1e-6, void_bridge=bridge)
             d.set_group(0)
         sol = u.Solver(op, d, rtol=1e-10, max_it=500, pc_dsa=use)
         sol.refresh()
@@ -82,8 +87,9 @@ def test_bridged_dsa_cuts_iterations(gpu, name, ref_plain, ref_dsa):
         flux[use] = oracle.moments(to_host(x), prob.quad.w)
     want = oracle.moments(oracle_solve(prob, oracle_rhs(prob, 0, [])), prob.quad.w)
     err = max(np.linalg.norm(flux[k] - want) / np.linalg.norm(want) for k in flux)
-    print(f"[case] {name}: FGMRES iterations without DSA {its[False]}, with bridged DSA {its[True]} (reference "
-          f"{ref_plain} -> {ref_dsa}); scalar flux vs oracle {err:.2e}", flush=True)
+    kind = "bridged" if bridge else "masked"
+    print(f"[case] {name}: FGMRES iterations without DSA {its[False]}, with {kind} DSA {its[True]} (reference "
+          f"{ref_plain} -> {ref_b if bridge else ref_m}); scalar flux vs oracle {err:.2e}", flush=True)
     assert 2 * its[True] <= its[False]
     assert err <= 1e-8
 
@@ -138,14 +144,19 @@ def test_all_void_group_is_masked(gpu):
     assert rel_max(got[True][1], got[False][1]) <= 1e-12
 
 
-def test_unbridgeable_void_is_refused(gpu):
+def test_unbridgeable_void_falls_back_to_mask(gpu):
     pf = load("box_50_inf_medium")  # every face reflective
     op = pf.operator()
     st = np.ones(pf.disc.n_cells)
     st[7] = 0.0
     op.set_xs(st, st.copy(), np.ones_like(st))  # and no absorption
     dsa = u.DSA(op)
-    with pytest.raises(u.UbtError) as e:
-        dsa.set_group(0)
-    assert e.value.status == "ERR_SUP"
-    assert "cannot be bridged" in str(e.value)
+    dsa.set_group(0)
+    prob = P.Problem("refl", pf.quad, pf.disc, st, st.copy(), None)
+    M, _ = oracle_dsa_matrix(prob)
+    x = np.random.default_rng(6).standard_normal(pf.disc.n_nodes)
+    y = torch.zeros(pf.disc.n_nodes, dtype=torch.float64, device="cuda")
+    dsa.matvec(to_dev(x), y)
+    torch.cuda.synchronize()
+    assert rel_max(to_host(y), M @ x) <= 1e-12
+    assert float(M[7, 7]) == 1.0

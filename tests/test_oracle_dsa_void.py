@@ -42,7 +42,7 @@ def coefficients(prob):
         fnb = d.get("face_nb")
         vs = lambda v: odsa.plex_void_vs(d.nf, fnb, d.get("face_area"), d.get("volume"), v)
         any_vac = bool(np.any(fnb == -1))
-    return odsa.diffusion_coefficients(st[0], ss[0, 0], vs, any_vac)
+    return odsa.diffusion_coefficients(st[0], ss[0, 0], vs, any_vac)[:3]
 
 
 @pytest.mark.parametrize("name,V,S", [("box_void_channel", 2.1, 8.2), ("plex_box_void_channel", 2.1, 8.2),
@@ -86,9 +86,9 @@ def test_dg1_void_faces_symmetric_positive_definite():
     d = prob.disc
     real = odsa.diffusion_coefficients
 
-    def no_face_rule(*a):
-        D, void, L = real(*a)
-        return D, void, None  # the face rule sees no bridged voids
+    def no_face_rule(*a, **k):
+        D, void, L, masked = real(*a, **k)
+        return D, void, None, masked  # the face rule sees no bridged voids
     odsa.diffusion_coefficients = no_face_rule
     try:
         Mw = odsa.dg1(d.nb, d.nf, d.get("face_nb"), d.get("face_nA"), d.get("face_area"), d.get("face_distance"),
@@ -103,10 +103,42 @@ def test_all_void_and_unbridgeable():
     n = prob.disc.n_cells
     with pytest.raises(ValueError):
         oracle_dsa_matrix(P.Problem("void", prob.quad, prob.disc, np.zeros(n), np.zeros(n), None))
-    # every face reflective, no absorption: only masking could treat the void
+    # every face reflective, no absorption: the void falls back to the mask
     pf = load("box_50_inf_medium")
     d = pf.disc
     st = np.ones(d.n_cells)
     st[7] = 0.0
-    with pytest.raises(NotImplementedError):
-        oracle_dsa_matrix(P.Problem("refl", pf.quad, d, st, st.copy(), None))
+    M, wgt = oracle_dsa_matrix(P.Problem("refl", pf.quad, d, st, st.copy(), None))
+    assert wgt[7] == 0.0 and wgt.sum() == d.n_cells - 1
+    assert M[7, 7] == 1.0 and M.getrow(7).nnz == 1 and M.getcol(7).nnz == 1
+
+
+def masked_strip_vs_narrower_box(kind):
+    """a box whose first column is a masked void against the same box one
+    column narrower with a vacuum face there: the non-void block must equal
+    the narrower box's operator (a face into a masked void is a vacuum face)"""
+    from ubolt_amd import core as uc
+    n, q = 6, uc.Quadrature(2, 4)
+    h = 1.0 / n
+    rng = np.random.default_rng(7)
+    st_in = rng.uniform(0.5, 3.0, (n, n - 1))
+    st = np.concatenate([np.zeros((n, 1)), st_in], axis=1).ravel()
+    def disc(nx):
+        if kind == "fd":
+            return uc.Disc.structured(q, [nx, n], [nx * h, 1.0], None)
+        coords, cells = P.box_quads(nx, n, lx=nx * h, ly=1.0)
+        return uc.Disc.dg(q, uc.Mesh(coords, cells), 0 if kind == "dg0" else 1, None)
+    Mw, wgt = oracle_dsa_matrix(P.Problem("w", q, disc(n), st, 0.5 * st, None), bridge=False)
+    Mn, _ = oracle_dsa_matrix(P.Problem("n", q, disc(n - 1), st_in.ravel(), 0.5 * st_in.ravel(), None))
+    return Mw, wgt, Mn, n
+
+
+@pytest.mark.parametrize("kind", ["fd", "dg0"])
+def test_masked_strip_is_a_vacuum_face(kind):
+    Mw, wgt, Mn, n = masked_strip_vs_narrower_box(kind)
+    keep = np.flatnonzero(wgt > 0)
+    assert keep.size == n * (n - 1)
+    sub = Mw.tocsr()[keep][:, keep]
+    assert abs(sub - Mn).max() <= 1e-12 * abs(Mn).max()
+    drop = np.flatnonzero(wgt == 0)
+    assert abs(Mw.tocsr()[drop][:, keep]).max() == 0.0

@@ -102,6 +102,7 @@ SIGNATURES = {
     "ubt_blockinv_destroy": [P],
     "ubt_dsa_create": [P, PP],
     "ubt_dsa_set_options": [P, I, D, D, I, D, I],
+    "ubt_dsa_set_void_policy": [P, I, D, D],
     "ubt_dsa_set_group": [P, I, P],
     "ubt_dsa_apply": [P, I, P, P, P],
     "ubt_dsa_apply_transpose": [P, I, P, P, P],

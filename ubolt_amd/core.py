@@ -267,11 +267,16 @@ class DSA(_Handle):
     _destroy = "ubt_dsa_destroy"
 
     def __init__(self, op, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200,
-                 inner_rtol=1e-6, inner_check_every=8):
+                 inner_rtol=1e-6, inner_check_every=8, void_bridge=True, void_sigma_t=0.0, void_d=0.0):
         super().__init__()
         self.op = op
         call("ubt_dsa_create", op.h, C.byref(self.h))
         self.set_options(consistent_d, consistent_power, mip_penalty, inner_max_it, inner_rtol, inner_check_every)
+        self.set_void_policy(void_bridge, void_sigma_t, void_d)
+
+    def set_void_policy(self, bridge=True, void_sigma_t=0.0, void_d=0.0):
+        """-dsa_void_bridge, -dsa_void_sigma_t, -dsa_void_d; effective at the next set_group"""
+        call("ubt_dsa_set_void_policy", self.h, int(bridge), void_sigma_t, void_d)
 
     def set_options(self, consistent_d=True, consistent_power=1.5, mip_penalty=4.0, inner_max_it=200, inner_rtol=1e-6,
                     inner_check_every=8):

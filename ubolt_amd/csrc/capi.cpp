@@ -524,6 +524,16 @@ int ubt_dsa_set_options(ubt_dsa *dsa, int consistent_d, double consistent_power,
     dsa->d.opt.check_every = inner_check_every;
   });
 }
+int ubt_dsa_set_void_policy(ubt_dsa *dsa, int bridge, double void_sigma_t, double void_d) {
+  return guard([&] {
+    NEED(dsa);
+    UBT_CHECK(void_sigma_t >= 0.0 && void_d >= 0.0, UBT_ERR_ARG_OUTOFRANGE,
+              "the void threshold and the void D must not be negative, got %g and %g", void_sigma_t, void_d);
+    dsa->d.opt.void_bridge = bridge != 0;
+    dsa->d.opt.void_sigma_t = void_sigma_t;
+    dsa->d.opt.void_d = void_d;
+  });
+}
 int ubt_dsa_set_group(ubt_dsa *dsa, int g, void *stream) {
   return guard([&] {
     NEED(dsa);

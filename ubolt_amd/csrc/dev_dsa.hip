@@ -14,10 +14,14 @@
 //    volume-weighted;
 // with D = 1/(3 sigma_t), sigma_a = sigma_t - sigma_s(g,g) and the consistent-D
 // blend (D^p + (m h)^p)^(1/p), p = 1.5, on FD and DG0 faces.
-// Voids (sigma_t <= 0; reference src/dsak.kokkos.cxx:233-345) are bridged: D
-// = 1/(3 (max(sigma_t, 0) + 1/L)) with L = 4 V / S the voids' mean chord, and
-// a DG1 face touching a void takes the harmonic face D on both sides; a group
-// void everywhere is masked whole (zero correction).
+// Voids (sigma_t <= a threshold, 0 by default; reference
+// src/dsak.kokkos.cxx:233-345) are bridged where the operator allows it: D =
+// 1/(3 (max(sigma_t, 0) + 1/L)) with L = 4 V / S the voids' mean chord (or a
+// given D), a DG1 face touching a void with the harmonic face D on both
+// sides. Otherwise, or when bridging is off, they are masked: identity rows
+// (V times identity blocks on the plex) coupled to nothing, a face into one a
+// vacuum face, and a zero restriction / correction weight on their nodes. A
+// group void everywhere is masked whole (zero correction).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,7 +39,11 @@ namespace {
 struct AsmArgs {
   int kind, dim, nb, nf, ns, n_cells, cons;
   double p, C;
-  double inv_chord;           // 1 / L of the bridged voids, 0 without them
+  // the void policy: a cell is void where sigma_t <= void_st; bridged voids
+  // take D = void_d, or 1/(3 (max(sigma_t, 0) + inv_chord)); masked voids are
+  // identity rows and a face into one is a vacuum face
+  double void_st, void_d, inv_chord;
+  int bridged, masked;
   const double *sigt, *sigs;  // current group
   const int *nbr;             // cells * ns
   // structured
@@ -52,17 +60,27 @@ __device__ inline double blend(double d, double m, double p) {
   return p == 1.0 ? d + m : pow(pow(d, p) + pow(m, p), 1.0 / p);
 }
 
-// D of a cell: 1 / (3 sigma_t); in a bridged void (sigma_t <= 0) a third of
+__device__ inline bool is_void(const AsmArgs &a, int64_t c) { return !(a.sigt[c] > a.void_st); }
+__device__ inline bool is_masked(const AsmArgs &a, int64_t c) { return a.masked && is_void(a, c); }
+
+// D of a cell: 1 / (3 sigma_t); in a bridged void the given D or a third of
 // the free flight, 1 / (3 (max(sigma_t, 0) + 1 / L)) (Wigner's rational form)
-__device__ inline double dcoef(double st, double inv_chord) {
-  return st > 0.0 ? 1.0 / (3.0 * st) : 1.0 / (3.0 * (fmax(st, 0.0) + inv_chord));
+__device__ inline double dcoef(const AsmArgs &a, int64_t c) {
+  const double st = a.sigt[c];
+  if (!a.bridged || st > a.void_st) return 1.0 / (3.0 * st);
+  return a.void_d > 0.0 ? a.void_d : 1.0 / (3.0 * (fmax(st, 0.0) + a.inv_chord));
 }
 
 __global__ void asm_structured(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
+  if (is_masked(a, c)) {  // an identity row, decoupled both ways
+    for (int k = 0; k < 2 * a.dim; k++) a.off[c * a.ns + k] = 0.0;
+    a.own[c] = 1.0;
+    return;
+  }
   const double st = a.sigt[c];
-  const double dc = dcoef(st, a.inv_chord);
+  const double dc = dcoef(a, c);
   double diag = st - a.sigs[c];
   for (int d = 0; d < a.dim; d++) {
     const double h = a.h[d];
@@ -70,13 +88,13 @@ __global__ void asm_structured(AsmArgs a) {
       const int s = 2 * d + side;
       const int n = a.nbr[c * a.ns + s];
       double o = 0.0;
-      if (n >= 0) {
-        const double dn = dcoef(a.sigt[n], a.inv_chord);
+      if (n >= 0 && !is_masked(a, n)) {
+        const double dn = dcoef(a, n);
         double df = 2.0 * dc * dn / (dc + dn);
         if (a.cons) df = blend(df, a.hr[d] * h, a.p);
         o = -df / (h * h);
         diag += df / (h * h);
-      } else if (a.vac[d][side]) {
+      } else if (n >= 0 || a.vac[d][side]) {  // a vacuum face, or one into a masked void
         const double db = a.cons ? blend(dc, a.hr[d] * h, a.p) : dc;
         diag += 1.0 / (h * (2.0 + h / (2.0 * db)));
       }
@@ -89,16 +107,21 @@ __global__ void asm_structured(AsmArgs a) {
 __global__ void asm_dg0(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
+  if (is_masked(a, c)) {  // V times an identity row, decoupled both ways
+    for (int f = 0; f < a.nf; f++) a.off[c * a.nf + f] = 0.0;
+    a.own[c] = a.volume[c];
+    return;
+  }
   const double st = a.sigt[c];
-  const double dc = dcoef(st, a.inv_chord);
+  const double dc = dcoef(a, c);
   double diag = a.volume[c] * (st - a.sigs[c]);
   for (int f = 0; f < a.nf; f++) {
     const int64_t k = c * a.nf + f;
     const int n = a.face_nb[k];
     const double A = a.area[k], d_own = a.dist[2 * k];
     double o = 0.0;
-    if (n >= 0) {
-      const double dn = dcoef(a.sigt[n], a.inv_chord), d_nb = a.dist[2 * k + 1];
+    if (n >= 0 && !is_masked(a, n)) {
+      const double dn = dcoef(a, n), d_nb = a.dist[2 * k + 1];
       double tf = A / (d_own / dc + d_nb / dn);
       if (a.cons) {
         const double len = d_own + d_nb;
@@ -106,7 +129,7 @@ __global__ void asm_dg0(AsmArgs a) {
       }
       o = -tf;
       diag += tf;
-    } else if (n == -1) {
+    } else if (n >= 0 || n == -1) {  // a vacuum face, or one into a masked void
       const double db = a.cons ? blend(dc, 2.0 * d_own * a.slot_hr[k], a.p) : dc;
       diag += A / (2.0 + d_own / db);
     }
@@ -119,9 +142,15 @@ template <int NB>
 __global__ void asm_dg1(AsmArgs a) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.n_cells) return;
-  const double st = a.sigt[c];
-  const double dc = dcoef(st, a.inv_chord), sa = st - a.sigs[c];
   const double V = a.volume[c];
+  if (is_masked(a, c)) {  // V times the identity block, every coupling zeroed
+    for (int e = 0; e < a.nf * NB * NB; e++) a.off[c * a.nf * NB * NB + e] = 0.0;
+    for (int i = 0; i < NB; i++)
+      for (int j = 0; j < NB; j++) a.own[(c * NB + i) * NB + j] = i == j ? V : 0.0;
+    return;
+  }
+  const double st = a.sigt[c];
+  const double dc = dcoef(a, c), sa = st - a.sigs[c];
   const int64_t gc = a.gcls[c];
   double own[NB][NB];
   for (int i = 0; i < NB; i++)
@@ -151,12 +180,12 @@ __global__ void asm_dg1(AsmArgs a) {
       }
     }
     const double hc = 2.0 * a.dist[2 * k];
-    if (n >= 0) {
+    if (n >= 0 && !is_masked(a, n)) {
       const double hn = 2.0 * a.dist[2 * k + 1];
-      double dcf = dc, dn = dcoef(a.sigt[n], a.inv_chord);
+      double dcf = dc, dn = dcoef(a, n);
       // a face touching a bridged void: both sides take the face's harmonic
       // D, in the averages and in kappa (the same from either cell)
-      if (a.inv_chord > 0.0 && (!(st > 0.0) || !(a.sigt[n] > 0.0))) {
+      if (a.bridged && (is_void(a, c) || is_void(a, n))) {
         dcf = 2.0 * dc * dn / (dc + dn);
         dn = dcf;
       }
@@ -168,7 +197,7 @@ __global__ void asm_dg1(AsmArgs a) {
           nbk[i * NB + j] = -kappa * scale * Fu[i * NB + j] - 0.5 * dn * gn[j] * mi + 0.5 * dcf * gcv[i] * scale * Fu[j];
         }
       }
-    } else {
+    } else {  // a vacuum face, or one into a masked void
       const double kappa = fmax(a.C * dc / hc, 0.25);
       for (int i = 0; i < NB; i++) {
         const double mi = scale * Fo[i * NB];
@@ -419,7 +448,7 @@ __global__ void cg_direction(int64_t N, const double *z, double *p, const double
   if (n < N) p[n] = z[n] + scal[3] * p[n];
 }
 
-// the void census of one group: void cells (sigma_t <= 0), the first of
+// the void census of one group: void cells (sigma_t <= the threshold), the first of
 // them, absorbing cells (sigma_t > sigma_s(g -> g))
 __global__ void census_init(int *flags) {
   flags[0] = 0;
@@ -428,10 +457,11 @@ __global__ void census_init(int *flags) {
   flags[3] = 0;
 }
 
-__global__ void void_census(const double *__restrict__ st, const double *__restrict__ ss, int64_t n, int *flags) {
+__global__ void void_census(const double *__restrict__ st, const double *__restrict__ ss, int64_t n, double void_st,
+                            int *flags) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  if (!(st[c] > 0.0)) {
+  if (!(st[c] > void_st)) {
     atomicAdd(&flags[1], 1);
     atomicMin(&flags[2], (int)c);
   }
@@ -447,7 +477,7 @@ __global__ void __launch_bounds__(256) void_vs_partial(AsmArgs a, double *part) 
   __shared__ double sv[256], sf[256];
   double v = 0.0, f = 0.0;
   for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < a.n_cells; c += (int64_t)gridDim.x * 256) {
-    if (a.sigt[c] > 0.0) continue;
+    if (!is_void(a, c)) continue;
     if (a.kind == DISC_FD) {
       double vol = 1.0;
       for (int d = 0; d < a.dim; d++) vol *= a.h[d];
@@ -455,13 +485,13 @@ __global__ void __launch_bounds__(256) void_vs_partial(AsmArgs a, double *part) 
       for (int d = 0; d < a.dim; d++)
         for (int side = 0; side < 2; side++) {
           const int n = a.nbr[c * a.ns + 2 * d + side];
-          if (n >= 0 ? a.sigt[n] > 0.0 : a.vac[d][side] != 0) f += vol / a.h[d];
+          if (n >= 0 ? !is_void(a, n) : a.vac[d][side] != 0) f += vol / a.h[d];
         }
     } else {
       v += a.volume[c];
       for (int k = 0; k < a.nf; k++) {
         const int n = a.face_nb[c * a.nf + k];
-        if (n >= 0 ? a.sigt[n] > 0.0 : n == -1) f += a.area[c * a.nf + k];
+        if (n >= 0 ? !is_void(a, n) : n == -1) f += a.area[c * a.nf + k];
       }
     }
   }
@@ -505,6 +535,13 @@ __global__ void __launch_bounds__(256) void_vs_final(double *part, int nblk) {
 }
 
 __global__ void clear_int(int *p) { *p = 0; }
+
+// the restriction / prolongation weight with the masked void nodes zeroed
+__global__ void mask_weight(const double *__restrict__ weight, const double *__restrict__ st, double void_st, int nb,
+                            int64_t n_nodes, double *__restrict__ wm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_nodes) wm[r] = st[r / nb] > void_st ? weight[r] : 0.0;
+}
 
 template <class T>
 __global__ void fill_zero(T *__restrict__ y, int64_t n) {
@@ -606,6 +643,7 @@ void DSA::create(Operator *o) {
   partial.alloc(sizeof(double) * nblk);
   scal.alloc(sizeof(double) * 8);
   vsum.alloc(sizeof(double) * (2 * kVoidBlocks + 2));
+  wmask.alloc(sizeof(double) * n_nodes);
   UBT_HIP(hipHostMalloc((void **)&flag_host, sizeof(double), hipHostMallocDefault));
 }
 
@@ -622,12 +660,12 @@ void DSA::set_group(int g, hipStream_t s) {
   const double *st = o.t64.sigt.as<double>() + (int64_t)g * n_cells;
   const double *ssg = o.t64.sigs.as<double>() + ((int64_t)g * o.n_groups + g) * n_cells;
   // flags (read back below, on the caller's stream): 0 singular diagonal
-  // blocks, 1 void cells (sigma_t <= 0), 2 the first void cell, 3 absorbing
-  // cells (sigma_a > 0)
+  // blocks, 1 void cells (sigma_t <= the void threshold), 2 the first void
+  // cell, 3 absorbing cells (sigma_a > 0)
   int *flags = o.flag.as<int>();
   const unsigned grid = (unsigned)((n_cells + 255) / 256);
   hipLaunchKernelGGL(census_init, dim3(1), dim3(1), 0, s, flags);
-  hipLaunchKernelGGL(void_census, dim3(grid), dim3(256), 0, s, st, ssg, n_cells, flags);
+  hipLaunchKernelGGL(void_census, dim3(grid), dim3(256), 0, s, st, ssg, n_cells, opt.void_sigma_t, flags);
   AsmArgs a{};
   a.kind = D.kind;
   a.dim = D.dim;
@@ -638,6 +676,8 @@ void DSA::set_group(int g, hipStream_t s) {
   a.cons = opt.consistent_d;
   a.p = opt.consistent_power;
   a.C = opt.mip_penalty;
+  a.void_st = opt.void_sigma_t;
+  a.void_d = opt.void_d;
   a.sigt = st;
   a.sigs = ssg;
   a.nbr = nbr.as<int>();
@@ -662,8 +702,7 @@ void DSA::set_group(int g, hipStream_t s) {
     a.gcls = o.gcls.as<int>();
   }
   // the operator and its block-Jacobi inverses (singular blocks counted in flags[0])
-  auto assemble = [&](double inv_chord) {
-    a.inv_chord = inv_chord;
+  auto assemble = [&]() {
     if (D.kind == DISC_FD)
       hipLaunchKernelGGL(asm_structured, dim3(grid), dim3(256), 0, s, a);
     else if (nb == 1)
@@ -682,13 +721,13 @@ void DSA::set_group(int g, hipStream_t s) {
     UBT_HIP(hipGetLastError());
   };
   // without voids (the common case) one assembly and one read-back
-  assemble(0.0);
+  assemble();
   int h[4] = {0, 0, 0, 0};
   UBT_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
   UBT_HIP(hipStreamSynchronize(s));
   n_void = h[1];
   inv_chord = 0.0;
-  all_void = false;
+  bridged = masked = all_void = false;
   if (n_void == n_cells) {
     // void everywhere: nothing on the far side of a void to couple to, so the
     // whole operator is masked and the correction is zero
@@ -697,26 +736,35 @@ void DSA::set_group(int g, hipStream_t s) {
     return;
   }
   if (n_void > 0) {
-    // bridged: something is not void and a vacuum face or some absorption
-    // holds the operator (masking, the alternative, is not supported)
-    UBT_CHECK(any_vacuum || h[3] > 0, UBT_ERR_SUP,
-              "group %d has %d void cells (the first %d), every face reflective and no absorption: the voids cannot "
-              "be bridged, and DSA void masking is not supported", g, n_void, h[2]);
-    double *vs = vsum.as<double>();
-    const int nblk = (int)std::min<int64_t>(grid, kVoidBlocks);
-    hipLaunchKernelGGL(void_vs_partial, dim3(nblk), dim3(256), 0, s, a, vs);
-    hipLaunchKernelGGL(void_vs_final, dim3(1), dim3(256), 0, s, vs, nblk);
-    UBT_HIP(hipGetLastError());
-    double vsh[2] = {0.0, 0.0};
-    UBT_HIP(hipMemcpyAsync(vsh, vs + 2 * nblk, sizeof(vsh), hipMemcpyDeviceToHost, s));
-    UBT_HIP(hipStreamSynchronize(s));
-    UBT_CHECK(vsh[1] > 0.0, UBT_ERR_SUP,
-              "group %d: no face of its %d void cells ends a flight (a non-void neighbour or a vacuum face), so they "
-              "cannot be bridged, and DSA void masking is not supported", g, n_void);
-    const double chord = 4.0 * vsh[0] / vsh[1];
-    inv_chord = 1.0 / chord;
+    // bridged when asked and something is not void and a vacuum face or some
+    // absorption holds the operator, and a face of the voids ends a flight;
+    // masked otherwise
+    if (opt.void_bridge && (any_vacuum || h[3] > 0)) {
+      double *vs = vsum.as<double>();
+      const int nblk = (int)std::min<int64_t>(grid, kVoidBlocks);
+      hipLaunchKernelGGL(void_vs_partial, dim3(nblk), dim3(256), 0, s, a, vs);
+      hipLaunchKernelGGL(void_vs_final, dim3(1), dim3(256), 0, s, vs, nblk);
+      UBT_HIP(hipGetLastError());
+      double vsh[2] = {0.0, 0.0};
+      UBT_HIP(hipMemcpyAsync(vsh, vs + 2 * nblk, sizeof(vsh), hipMemcpyDeviceToHost, s));
+      UBT_HIP(hipStreamSynchronize(s));
+      if (vsh[1] > 0.0) {
+        const double chord = 4.0 * vsh[0] / vsh[1];
+        inv_chord = 1.0 / chord;
+        bridged = true;
+      }
+    }
+    masked = !bridged;
+    a.bridged = bridged;
+    a.masked = masked;
+    a.inv_chord = inv_chord;
+    if (masked) {
+      const unsigned gn = (unsigned)((n_nodes + 255) / 256);
+      hipLaunchKernelGGL(mask_weight, dim3(gn), dim3(256), 0, s, weight.as<double>(), st, opt.void_sigma_t, nb,
+                         n_nodes, wmask.as<double>());
+    }
     hipLaunchKernelGGL(clear_int, dim3(1), dim3(1), 0, s, flags);
-    assemble(inv_chord);
+    assemble();
     UBT_HIP(hipMemcpyAsync(h, flags, sizeof(int), hipMemcpyDeviceToHost, s));
     UBT_HIP(hipStreamSynchronize(s));
   } else {
@@ -877,13 +925,15 @@ void DSA::apply(int prec, const void *x, void *y, hipStream_t s, bool transpose)
   // the adjoint sums unweighted and moves the node weight to the prolongation
   op->moments_f64(prec, x, rhs.as<double>(), s, transpose);
   double *xs = sol.as<double>();
-  const double *w = transpose ? nullptr : weight.as<double>();
+  // masked void nodes are neither restricted nor corrected: their weight is zero
+  const double *wn = masked ? wmask.as<double>() : weight.as<double>();
+  const double *w = transpose ? nullptr : wn;
   cg_dispatch(*this, rhs.as<double>(), w, xs, s, own_work(*this));
   const int64_t rows = op->disc->n_rows();
   const unsigned grid = (unsigned)((rows + 255) / 256);
   const double inv_w = 1.0 / op->disc->quad.sum_weights;
   if (transpose) {
-    const double *wa = op->t64.w.as<double>(), *wn = weight.as<double>();
+    const double *wa = op->t64.w.as<double>();
     if (prec == UBT_FP64)
       hipLaunchKernelGGL(prolong_t<double>, dim3(grid), dim3(256), 0, s, xs, wn, wa, (double *)y, rows,
                          op->disc->n_angles, inv_w);

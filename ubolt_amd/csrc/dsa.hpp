@@ -21,6 +21,10 @@ struct DSAOptions {
   int check_every = 8;       // read the converged flag on the host every this many
                              // iterations and stop launching (0: never - no host
                              // synchronisation, so the solve can be graph-captured)
+  // the void policy (-dsa_void_bridge, -dsa_void_sigma_t, -dsa_void_d)
+  int void_bridge = 1;       // bridge voids where the operator allows it; 0 masks them
+  double void_sigma_t = 0.0; // a cell is void where sigma_t <= this
+  double void_d = 0.0;       // a bridged void's D; 0: a third of the free flight
 };
 
 struct DSA {
@@ -32,16 +36,19 @@ struct DSA {
   double half_range[3] = {0, 0, 0};
   int vacuum[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   bool any_vacuum = false;
-  // the group's void policy (void: sigma_t <= 0). Voids are bridged - kept in
-  // the operator with D = 1 / (3 (max(sigma_t, 0) + 1 / L)), L = 4 V / S the
-  // voids' mean chord (V their volume, S their faces onto a non-void cell or
-  // a vacuum face) - when something is not void and a vacuum face or some
-  // absorption holds the operator; a group void everywhere is masked whole
-  // (a zero correction)
+  // the group's void policy (void: sigma_t <= opt.void_sigma_t). Voids are
+  // bridged - kept in the operator with D = 1 / (3 (max(sigma_t, 0) + 1 / L)),
+  // L = 4 V / S the voids' mean chord (V their volume, S their faces onto a
+  // non-void cell or a vacuum face), or opt.void_d - when bridging is on,
+  // something is not void and a vacuum face or some absorption holds the
+  // operator; otherwise masked: identity rows coupled to nothing, a face into
+  // them a vacuum face, nothing restricted or corrected there. A group void
+  // everywhere is masked whole (a zero correction)
   int n_void = 0;
   double inv_chord = 0.0;    // 1 / L, 0 without bridged voids
-  bool all_void = false;
+  bool bridged = false, masked = false, all_void = false;
   DevBuf vsum;               // double: per-block (V, S) partials, then the two sums
+  DevBuf wmask;              // double, nodes: weight, zero on masked void nodes
   DevBuf nbr;                // int, cells * ns: neighbour cell per slot, -1 none
   DevBuf own, off, jinv;     // double: cells*nb*nb, cells*ns*nb*nb, cells*nb*nb
   DevBuf weight;             // double, nodes: V (plex) or 1 (structured)
