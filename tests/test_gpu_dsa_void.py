@@ -63,6 +63,28 @@ def test_void_matvec_and_apply(gpu, name, bridge):
     assert em <= 1e-12 and ea <= 1e-9
 
 
+@pytest.mark.parametrize("name,g,opts", [("box_void_channel", 0, dict(void_d=0.05)),
+                                         ("plex_box_void_channel_dg1", 0, dict(void_d=0.05)),
+                                         ("box_decades4_void", 0, dict(void_sigma_t=0.2)),
+                                         ("plex_decades4_void", 0, dict(void_sigma_t=0.2, bridge=False))])
+def test_void_threshold_and_fixed_d(gpu, name, g, opts):
+    """-dsa_void_d fixes a bridged void's D; -dsa_void_sigma_t 0.2 makes the
+    decades files' thin top group (sigma_t 0.1) void outside its dense block"""
+    prob = load(name)
+    op = prob.operator()
+    dsa = u.DSA(op, void_bridge=opts.get("bridge", True), void_sigma_t=opts.get("void_sigma_t", 0.0),
+                void_d=opts.get("void_d", 0.0))
+    dsa.set_group(g)
+    M, _ = oracle_dsa_matrix(prob, g, **opts)
+    x = np.random.default_rng(8).standard_normal(prob.disc.n_nodes)
+    y = torch.zeros(prob.disc.n_nodes, dtype=torch.float64, device="cuda")
+    dsa.matvec(to_dev(x), y)
+    torch.cuda.synchronize()
+    em = rel_max(to_host(y), M @ x)
+    print(f"[case] {name} group {g} {opts}: matvec {em:.2e}", flush=True)
+    assert em <= 1e-12
+
+
 @pytest.mark.parametrize("bridge", [True, False])
 @pytest.mark.parametrize("name", VOID_FILES)
 def test_void_dsa_cuts_iterations(gpu, name, bridge):

@@ -430,7 +430,7 @@ def load(path):
 
 
 def solve(pf, rtol=None, mixed=False, dsa=False, max_it=500, outer_rtol=1e-8, max_outer=1000, inner_gamma=0.01,
-          store_moments=False, recon_steps=10, stream=None):
+          store_moments=False, recon_steps=10, dsa_void_bridge=True, stream=None):
     """The reference driver's group sweep (tests/transportk.kokkos.cxx:925-1208)
     in the library (ubt_solver_solve_groups): per group the rhs (ghost inflow +
     source + transfers from the groups solved so far), a right-preconditioned
@@ -443,12 +443,13 @@ def solve(pf, rtol=None, mixed=False, dsa=False, max_it=500, outer_rtol=1e-8, ma
     thermal group warm-started from the P1 expansion of its scalar flux and
     current, refined by at most recon_steps Richardson steps). Returns the
     angular flux per group (device tensors; under the moments store the one
-    vector, the last group solved) and the sweep's report."""
+    vector, the last group solved) and the sweep's report. dsa_void_bridge:
+    the reference's -dsa_void_bridge (False masks the voids)."""
     from . import Solver, DSA
 
     prob = pf.problem
     op = prob.operator()
-    d = DSA(op) if dsa else None
+    d = DSA(op, void_bridge=dsa_void_bridge) if dsa else None
     sol = Solver(op, d, rtol=1e-8 if rtol is None else rtol, max_it=max_it, mixed=mixed, pc_dsa=dsa)
     psi, report = sol.solve_groups(outer_rtol=outer_rtol, max_outer=max_outer, inner_gamma=inner_gamma,
                                    inner_rtol=0.0 if rtol is None else rtol, store_moments=store_moments,
