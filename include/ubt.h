@@ -360,8 +360,8 @@ int ubt_dsa_matvec(ubt_dsa *dsa, const double *x, double *y, void *stream);
 /* x = D_diff^{-1} b by the inner solve (x starts at zero). */
 int ubt_dsa_solve(ubt_dsa *dsa, const double *b, double *x, void *stream);
 /* x = D_diff^{-1} b exactly as ubt_dsa_solve, allocating nothing: the inner
- * CG's workspace (four node vectors, block partial sums, scalars) is a
- * file-scope device array of 2^21 nodes (64 MiB per GPU, held until the
+ * CG's workspace (five node vectors, block partial sums, scalars) is a
+ * file-scope device array of 2^21 nodes (80 MiB per GPU, held until the
  * process ends), so the call makes no hipMalloc/hipFree or other allocating
  * call. No counterpart in the reference: it is for callers that cannot
  * allocate (code under HIP graph capture - set the inner check interval to 0

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include <cmath>
@@ -281,6 +282,27 @@ __device__ inline double mv_row(const MvArgs &m, const double *__restrict__ x, i
   return v;
 }
 
+// (D p)(c, i) with p = z + beta p_old formed on the fly; pc: p at (c, i)
+template <int NB>
+__device__ inline double mv_row_dir(const MvArgs &m, const double *__restrict__ z, const double *__restrict__ po,
+                                    double beta, int64_t c, int i, double &pc) {
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const double pj = fma(beta, po[c * NB + j], z[c * NB + j]);
+    if (j == i) pc = pj;
+    v += m.own[(c * NB + i) * NB + j] * pj;
+  }
+  for (int s = 0; s < m.ns; s++) {
+    const int n = m.nbr[c * m.ns + s];
+    if (n < 0) continue;
+    const double *o = m.off + ((c * m.ns + s) * NB + i) * NB;
+#pragma unroll
+    for (int j = 0; j < NB; j++) v += o[j] * fma(beta, po[(int64_t)n * NB + j], z[(int64_t)n * NB + j]);
+  }
+  return v;
+}
+
 template <int NB>
 __global__ void dsa_matvec(MvArgs m, const double *__restrict__ x, double *__restrict__ y) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -291,7 +313,11 @@ __global__ void dsa_matvec(MvArgs m, const double *__restrict__ x, double *__res
 // ---------------------------------------------------------------------------
 // CG with device scalars. scal: 0 rz, 1 rz0, 2 alpha, 3 beta, 4 done, 5 its,
 // 6 rtol^2, 7 the reductions' ticket (an int). Partial sums per block,
-// reduced in fixed order by the block that finishes last.
+// reduced in fixed order by the block that finishes last. Two launches per
+// iteration: the new direction p = z + beta p_old is formed inside the
+// matvec (every thread forms it for the nodes it reads, with one fma, so all
+// agree bit for bit, and writes its own node's into the other p buffer),
+// then the update.
 // ---------------------------------------------------------------------------
 constexpr int kCgThreads = 256;
 
@@ -358,6 +384,7 @@ __global__ void cg_reduce_init(const double *partial, int nblk, double *scal, do
   *ticket = 0;
   scal[0] = s;
   scal[1] = s;
+  scal[3] = 0.0;  // the first direction is z
   scal[4] = (s == 0.0) ? 1.0 : 0.0;
   scal[5] = 0.0;
   scal[6] = rtol2;
@@ -388,17 +415,22 @@ __device__ inline bool last_block_total(double v, double *partial, int *ticket, 
   return true;
 }
 
-// q = D p and p.q; the last block forms alpha
+// p = z + beta p_old (beta 0 on the first iteration), q = D p and p.q; the
+// last block forms alpha
 template <int NB>
-__global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ p, double *__restrict__ q, double *partial,
-                              double *scal, int *ticket) {
+__global__ void cg_matvec_dot(MvArgs m, const double *__restrict__ z, const double *__restrict__ po,
+                              double *__restrict__ p, double *__restrict__ q, double *partial, double *scal,
+                              int *ticket) {
   if (scal[4] != 0.0) return;
+  const double beta = scal[3];
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (n < m.n_cells * NB) {
-    const double qq = mv_row<NB>(m, p, n / NB, (int)(n % NB));
+    double pn = 0.0;
+    const double qq = mv_row_dir<NB>(m, z, po, beta, n / NB, (int)(n % NB), pn);
+    p[n] = pn;
     q[n] = qq;
-    v = p[n] * qq;
+    v = pn * qq;
   }
   v = block_sum(v);
   double s;
@@ -440,12 +472,6 @@ __global__ void cg_update(MvArgs m, double *x, double *r, double *z, const doubl
     scal[5] += 1.0;
     if (s <= scal[6] * scal[1]) scal[4] = 1.0;
   }
-}
-
-__global__ void cg_direction(int64_t N, const double *z, double *p, const double *scal) {
-  if (scal[4] != 0.0) return;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < N) p[n] = z[n] + scal[3] * p[n];
 }
 
 // the void census of one group: void cells (sigma_t <= the threshold), the first of
@@ -638,6 +664,7 @@ void DSA::create(Operator *o) {
   r.alloc(nodes);
   z.alloc(nodes);
   p.alloc(nodes);
+  p2.alloc(nodes);
   q.alloc(nodes);
   const int64_t nblk = (n_nodes + kCgThreads - 1) / kCgThreads;
   partial.alloc(sizeof(double) * nblk);
@@ -802,16 +829,17 @@ void DSA::matvec(const double *x, double *y, hipStream_t s) const {
   UBT_HIP(hipGetLastError());
 }
 
-// The inner CG's workspace: four node vectors, the per-block partial sums
-// and the CG scalars (every entry is written before it is read)
+// The inner CG's workspace: five node vectors (r, z, the two direction
+// buffers, q), the per-block partial sums and the CG scalars (every entry is
+// written before it is read)
 struct CgWork {
-  double *r, *z, *p, *q, *part, *sc;
+  double *r, *z, *p, *p2, *q, *part, *sc;
   int *ticket;  // the last-block reductions' counter
 };
 
 static CgWork own_work(DSA &d) {
-  return {d.r.as<double>(), d.z.as<double>(), d.p.as<double>(), d.q.as<double>(), d.partial.as<double>(),
-          d.scal.as<double>(), reinterpret_cast<int *>(d.scal.as<double>() + 7)};
+  return {d.r.as<double>(), d.z.as<double>(), d.p.as<double>(), d.p2.as<double>(), d.q.as<double>(),
+          d.partial.as<double>(), d.scal.as<double>(), reinterpret_cast<int *>(d.scal.as<double>() + 7)};
 }
 
 template <int NB>
@@ -821,17 +849,17 @@ static void cg_run(DSA &d, const double *b, const double *wgt, double *x, hipStr
   const unsigned gnode = (unsigned)((N + kCgThreads - 1) / kCgThreads);
   const unsigned gcell = (unsigned)((d.n_cells + kCgThreads - 1) / kCgThreads);
   double *sc = w.sc, *part = w.part;
-  double *r = w.r, *z = w.z, *p = w.p, *q = w.q;
-  hipLaunchKernelGGL(cg_init<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, b, wgt, x, r, z, p, part);
+  double *r = w.r, *z = w.z, *po = w.p, *p = w.p2, *q = w.q;
+  hipLaunchKernelGGL(cg_init<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, b, wgt, x, r, z, po, part);
   hipLaunchKernelGGL(cg_reduce_init, dim3(1), dim3(64), 0, s, part, (int)gcell, sc, d.opt.rtol * d.opt.rtol,
                      w.ticket);
-  // three launches per iteration: the matvec and dot with alpha, the update
-  // and preconditioner with beta (each reduced by its last block), the new
-  // direction
+  // two launches per iteration: the new direction with the matvec and dot
+  // (alpha), the update and preconditioner (beta), each reduced by its last
+  // block; the two direction buffers swap roles every iteration
   for (int it = 0; it < d.opt.max_it; it++) {
-    hipLaunchKernelGGL(cg_matvec_dot<NB>, dim3(gnode), dim3(kCgThreads), 0, s, m, p, q, part, sc, w.ticket);
+    hipLaunchKernelGGL(cg_matvec_dot<NB>, dim3(gnode), dim3(kCgThreads), 0, s, m, z, po, p, q, part, sc, w.ticket);
     hipLaunchKernelGGL(cg_update<NB>, dim3(gcell), dim3(kCgThreads), 0, s, m, x, r, z, p, q, part, sc, w.ticket);
-    hipLaunchKernelGGL(cg_direction, dim3(gnode), dim3(kCgThreads), 0, s, N, z, p, sc);
+    std::swap(po, p);
     if (d.opt.check_every > 0 && (it + 1) % d.opt.check_every == 0 && it + 1 < d.opt.max_it) {
       UBT_HIP(hipMemcpyAsync(d.flag_host, sc + 4, sizeof(double), hipMemcpyDeviceToHost, s));
       UBT_HIP(hipStreamSynchronize(s));
@@ -868,7 +896,7 @@ void DSA::solve(const double *b, double *x, hipStream_t s) {
 // calls that overlap on one GPU race.
 // ---------------------------------------------------------------------------
 constexpr int64_t kNoAllocBlocks = (kNoAllocNodes + kCgThreads - 1) / kCgThreads;
-constexpr int64_t kNoAllocDoubles = 4 * kNoAllocNodes + kNoAllocBlocks + 8;
+constexpr int64_t kNoAllocDoubles = 5 * kNoAllocNodes + kNoAllocBlocks + 8;
 static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 static std::mutex scratch_mu;
@@ -901,8 +929,9 @@ void DSA::solve_noalloc(const double *b, double *x, hipStream_t s) {
             (long long)kNoAllocNodes, (long long)n_nodes);
   double *base = scratch_on(op->device);
   const int64_t N = n_nodes, nblk = (N + kCgThreads - 1) / kCgThreads;
-  double *sc = base + 4 * N + nblk;
-  const CgWork w = {base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, sc, reinterpret_cast<int *>(sc + 7)};
+  double *sc = base + 5 * N + nblk;
+  const CgWork w = {base, base + N, base + 2 * N, base + 3 * N, base + 4 * N, base + 5 * N, sc,
+                    reinterpret_cast<int *>(sc + 7)};
   cg_dispatch(*this, b, nullptr, x, s, w);
 }
 

@@ -9,7 +9,7 @@
 namespace ubt {
 
 // the most diffusion nodes the allocation-free inner solve takes (2^21: a
-// 128^3 structured box; its workspace is 4 node vectors, 64 MiB per GPU)
+// 128^3 structured box; its workspace is 5 node vectors, 80 MiB per GPU)
 constexpr int64_t kNoAllocNodes = int64_t(1) << 21;
 
 struct DSAOptions {
@@ -53,7 +53,7 @@ struct DSA {
   DevBuf own, off, jinv;     // double: cells*nb*nb, cells*ns*nb*nb, cells*nb*nb
   DevBuf weight;             // double, nodes: V (plex) or 1 (structured)
   DevBuf slot_hr;            // double, cells*ns: half-range current per face (DG0)
-  DevBuf rhs, sol, r, z, p, q, partial, scal;
+  DevBuf rhs, sol, r, z, p, p2, q, partial, scal;
   double *flag_host = nullptr;  // pinned: the converged flag read back every check_every iterations
   ~DSA();
   int last_iterations = 0;
