@@ -116,20 +116,23 @@ def test_void_dsa_cuts_iterations(gpu, name, bridge):
     assert err <= 1e-8
 
 
+@pytest.mark.parametrize("bridge", [True, False])
 @pytest.mark.parametrize("name", ["box_decades4_void", "plex_decades4_void"])
-def test_decades_voids_group_sweep_with_dsa(gpu, name):
+def test_decades_voids_group_sweep_with_dsa(gpu, name, bridge):
     pf = pfm.load(os.path.join(PROBLEMS, name + ".json"))
     prob = pf.problem
     psi0, r0 = pfm.solve(pf, rtol=1e-10, max_it=800)
-    psi1, r1 = pfm.solve(pf, rtol=1e-10, max_it=800, dsa=True)
+    psi1, r1 = pfm.solve(pf, rtol=1e-10, max_it=800, dsa=True, dsa_void_bridge=bridge)
     torch.cuda.synchronize()
     cache, errs = [], []
     for g in range(prob.n_groups):
         want = oracle.moments(oracle_solve(prob, oracle_rhs(prob, g, cache), g), prob.quad.w)
         cache.append(want)
         errs.append(np.linalg.norm(oracle.moments(to_host(psi1[g]), prob.quad.w) - want) / np.linalg.norm(want))
-    print(f"[case] {name}: per-group iterations without DSA {r0['group_iterations']}, with bridged DSA "
-          f"{r1['group_iterations']} (reference pin with DSA 7); scalar flux vs oracle {max(errs):.2e}", flush=True)
+    kind = "bridged" if bridge else "masked"
+    print(f"[case] {name}: per-group iterations without DSA {r0['group_iterations']}, with {kind} DSA "
+          f"{r1['group_iterations']} (reference pin with bridged DSA 7); scalar flux vs oracle {max(errs):.2e}",
+          flush=True)
     assert r1["status"] == 1 and max(errs) <= 1e-8
     assert max(r1["group_iterations"]) < max(r0["group_iterations"])
 
