@@ -111,28 +111,43 @@ def test_outer_iteration_with_dsa_and_mixed(gpu):
         assert diff <= 1e-6, diff
 
 
+@pytest.mark.parametrize("gamma", [0.0, 0.01])
 @pytest.mark.parametrize("name", ["box_up3_layers", "plex_box_up3_layers", "plex_box_up3_layers_dg1"])
-def test_outer_iteration_matches_oracle(gpu, name):
+def test_outer_iteration_matches_oracle(gpu, name, gamma):
+    """gamma 0: tight inner solves (rtol 1e-12) against the oracle's exact
+    ones; gamma 0.01: the reference test driver's defaults (inner rtol 1e-10)
+    against the oracle's warm-started right-preconditioned GMRES(30) with the
+    exact unscattered inverse, the library's FGMRES(30) with its sweep (exact
+    on these all-vacuum boxes)"""
+    import scipy.sparse.linalg as spl
     from oracle import groups as og
     from test_oracle_groups import group_system
+    from helpers import oracle_rows
     pf = load(name)
     prob = pf.problem
     A, b0, tr, mom, g_up = group_system(prob)
     trace = []
-    _, phi_o, sweeps_o, status_o = og.group_sweep(A, b0, tr, mom, g_up, outer_rtol=1e-8, trace=trace)
+    if gamma == 0.0:
+        _, phi_o, sweeps_o, status_o = og.group_sweep(A, b0, tr, mom, g_up, outer_rtol=1e-8, trace=trace)
+        psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-8, inner_gamma=0.0)
+    else:
+        L = [spl.splu(oracle_rows(prob, g).to_scipy().tocsc()) for g in range(prob.n_groups)]
+        inner = lambda g, b, x0, rtol: og.gmres_right(A[g], L[g].solve, b, x0, rtol)
+        _, phi_o, sweeps_o, status_o = og.group_sweep(A, b0, tr, mom, g_up, outer_rtol=1e-8, trace=trace,
+                                                      inner=inner, inner_gamma=gamma, inner_rtol=1e-10)
+        psi, rep = pfm.solve(pf, max_it=800, outer_rtol=1e-8, inner_gamma=gamma)
     assert status_o == 1
-    psi, rep = pfm.solve(pf, rtol=1e-12, max_it=800, outer_rtol=1e-8, inner_gamma=0.0)
     torch.cuda.synchronize()
     diff = max(np.abs(to_host(rep["operator"].scalar_flux(g)) - phi_o[g]).max() / np.abs(phi_o[g]).max()
                for g in range(prob.n_groups))
-    change_o, rho_o, _ = trace[-1]
-    print(f"[case] {name}: library {rep['sweeps']} sweeps (change {rep['change']:.4e}, rho {rep['rho']:.4f}), "
+    change_o, rho_o = trace[-1][:2]
+    print(f"[case] {name} gamma {gamma}: library {rep['sweeps']} sweeps (change {rep['change']:.4e}, rho {rep['rho']:.4f}), "
           f"oracle {sweeps_o} (change {change_o:.4e}, rho {rho_o:.4f}); scalar flux difference {diff:.2e}",
           flush=True)
     assert rep["status"] == 1 and rep["g_up"] == g_up and rep["sweeps"] == sweeps_o
     assert abs(rep["change"] - change_o) <= 1e-2 * change_o
     assert abs(rep["rho"] - rho_o) <= 1e-2 * rho_o
-    assert diff <= 1e-9, diff
+    assert diff <= (1e-9 if gamma == 0.0 else 1e-8), diff
 
 
 def test_outer_rtol_below_inner_is_refused(gpu):

@@ -8,7 +8,8 @@
   -outer_max_it 27);
 * box_up3_layers at gamma 0 converges within the reference's pin for that
   setting (tests/Makefile: -outer_inner_gamma 0 ... -outer_max_it 15) to a
-  flux with a small multigroup residual;
+  flux with a small multigroup residual, and at the default gamma 0.01 with
+  warm-started inexact inner solves within that recipe's pin (16);
 * a planted mistake - the thermal rhs without the upscatter transfers -
   misses the closed form.
 """
@@ -91,6 +92,50 @@ def test_up3_layers_within_gamma0_pin():
             if gf != g:
                 b += tr(gf, g, phi[gf])
         assert np.linalg.norm(b - A[g] @ psi[g]) <= 1e-7 * np.linalg.norm(b), g
+
+
+def test_up3_layers_adaptive_inner_within_pin():
+    """the adaptive inner tolerance (gamma 0.01, inner rtol 1e-10, the
+    reference test driver's defaults) with warm-started inexact inner solves
+    (right-preconditioned GMRES(30), the exact unscattered inverse as the
+    preconditioner - the sweep on this all-vacuum box):
+    within the reference's pin for that recipe (tests/Makefile: box_up3_layers
+    -outer_max_it 16), its last sweep at the inner rtol, the thermal rtols
+    loosened to gamma on the first sweep and tightening with the change"""
+    import scipy.sparse.linalg as spl
+    from helpers import oracle_rows
+    prob = pfm.load(os.path.join(PROBLEMS, "box_up3_layers.json")).problem
+    A, b0, tr, mom, g_up = group_system(prob)
+    L = [spl.splu(oracle_rows(prob, g).to_scipy().tocsc()) for g in range(prob.n_groups)]
+    inner = lambda g, b, x0, rtol: og.gmres_right(A[g], L[g].solve, b, x0, rtol)
+    trace = []
+    psi, phi, sweeps, status = og.group_sweep(A, b0, tr, mom, g_up, inner=inner, inner_gamma=0.01,
+                                              inner_rtol=1e-10, trace=trace)
+    rtols = [t[3] for t in trace]
+    print(f"[case] oracle box_up3_layers gamma 0.01: {sweeps} sweeps, thermal rtols {['%.1e' % r for r in rtols]}")
+    assert status == 1 and sweeps <= 16
+    assert rtols[0] == 0.01 and rtols[-1] == 1e-10
+    assert all(r1 <= r0 for r0, r1 in zip(rtols, rtols[1:]))
+    for g in range(prob.n_groups):
+        b = b0[g].copy()
+        for gf in range(prob.n_groups):
+            if gf != g:
+                b += tr(gf, g, phi[gf])
+        assert np.linalg.norm(b - A[g] @ psi[g]) <= 1e-7 * np.linalg.norm(b), g
+
+
+def test_gmres_right_matches_direct_solve():
+    import scipy.sparse.linalg as spl
+    from helpers import oracle_rows
+    prob = pfm.load(os.path.join(PROBLEMS, "box_up3_layers.json")).problem
+    A, b0, _, _, _ = group_system(prob)
+    g = 2
+    L = spl.splu(oracle_rows(prob, g).to_scipy().tocsc())
+    want = spl.splu(A[g].tocsc()).solve(b0[g])
+    for x0 in (None, 0.5 * want):
+        x = og.gmres_right(A[g], L.solve, b0[g], x0, 1e-12, restart=5)
+        assert np.linalg.norm(b0[g] - A[g] @ x) <= 1e-12 * np.linalg.norm(b0[g])
+        assert np.linalg.norm(x - want) <= 1e-9 * np.linalg.norm(want)
 
 
 def test_out_of_sweeps_status():
